@@ -1,0 +1,204 @@
+/*
+ * or_llg.c -- stochastic LLG integrators, restated from
+ *   source/Evolution/depondt.f90      depondt_evolve_first 50-193, _second 203-333
+ *   source/Evolution/midpoint.f90     smodeulermpt 34-181, modeulermpf 188-323
+ *   source/RNG/randomnumbers.f90      rannum scaling 625-771 (llg = 1 branch)
+ *   source/Evolution/updatemoments.f90 copym 48-95 (mompar = 0, initexc /= 'I')
+ * Spin-transfer torques are off (stt='N', do_she='N', do_sot='N'), so the
+ * torque accumulators are the literal 0.0 they start from.
+ */
+#include "uasd_oracle.h"
+#include <math.h>
+#include <stddef.h>
+
+#define V3(a, c, i, k, N) (a)[(c) + 3 * ((size_t)(i) + (size_t)(N) * (k))]
+#define S2(a, i, k, N) (a)[(size_t)(i) + (size_t)(N) * (k)]
+
+static inline void rodrigues(const double bd[3], double dt, double lam, const double e[3],
+                             double out[3])
+{
+  double lldamp = 1.0 / (1.0 + lam * lam);
+  double Bnorm = bd[0] * bd[0] + bd[1] * bd[1] + bd[2] * bd[2];
+  Bnorm = sqrt(Bnorm) + 1.0e-15;
+  double hx = bd[0] / Bnorm, hy = bd[1] / Bnorm, hz = bd[2] / Bnorm;
+  double v = Bnorm * dt * OR_GAMA * lldamp;
+  double cosv = cos(v), sinv = sin(v), u = 1.0 - cosv;
+  out[0] = hx * hx * u * e[0] + cosv * e[0] + hx * hy * u * e[1] - hz * sinv * e[1] +
+           hx * hz * u * e[2] + hy * sinv * e[2];
+  out[1] = hy * hx * u * e[0] + hz * sinv * e[0] + hy * hy * u * e[1] + cosv * e[1] +
+           hy * hz * u * e[2] - hx * sinv * e[2];
+  out[2] = hx * hz * u * e[0] - hy * sinv * e[0] + hz * hy * u * e[1] + hx * sinv * e[1] +
+           hz * hz * u * e[2] + cosv * e[2];
+}
+
+/* btherm holds N(0,1) deviates on entry (rng_gaussianP, depondt.f90:133) and
+ * the scaled thermal field on exit (depondt.f90:143-145). */
+void or_depondt_first(int N, int M, const double *lambda, const double *temp,
+                      double temprescale, double dt, const double *beff,
+                      double *btherm, const double *mmom, double *emom,
+                      double *emom2, double *emomM, double *b2eff,
+                      double *thermal_field)
+{
+  for (int k = 0; k < M; ++k)
+    for (int i = 0; i < N; ++i) {
+      double lam = lambda[i];
+      double Dp = (2.0 * lam * OR_K_BOLT) / (dt * OR_GAMA * OR_MUB);
+      double sigma = sqrt(Dp * temprescale * temp[i] / S2(mmom, i, k, N));
+      double bl[3], bd[3], e[3], mr[3];
+      for (int c = 0; c < 3; ++c) {
+        V3(btherm, c, i, k, N) = V3(btherm, c, i, k, N) * sigma;
+        bl[c] = V3(beff, c, i, k, N) + V3(btherm, c, i, k, N);
+        if (thermal_field) V3(thermal_field, c, i, k, N) = V3(btherm, c, i, k, N);
+        e[c] = V3(emom, c, i, k, N);
+      }
+      bd[0] = 0.0 + bl[0] + lam * e[1] * bl[2] - lam * e[2] * bl[1];
+      bd[1] = 0.0 + bl[1] + lam * e[2] * bl[0] - lam * e[0] * bl[2];
+      bd[2] = 0.0 + bl[2] + lam * e[0] * bl[1] - lam * e[1] * bl[0];
+      rodrigues(bd, dt, lam, e, mr);
+      double m = S2(mmom, i, k, N);
+      for (int c = 0; c < 3; ++c) {
+        V3(emom2, c, i, k, N) = e[c];
+        V3(emomM, c, i, k, N) = mr[c] * m;
+        V3(emom, c, i, k, N) = mr[c];
+        V3(b2eff, c, i, k, N) = bd[c];
+      }
+    }
+}
+
+void or_depondt_second(int N, int M, const double *lambda, double dt,
+                       const double *beff, const double *btherm,
+                       const double *b2eff, double *emom, double *emom2)
+{
+  for (int k = 0; k < M; ++k)
+    for (int i = 0; i < N; ++i) {
+      double lam = lambda[i];
+      double bl[3], bd[3], e[3], mr[3];
+      for (int c = 0; c < 3; ++c) {
+        bl[c] = V3(beff, c, i, k, N) + V3(btherm, c, i, k, N);
+        e[c] = V3(emom, c, i, k, N);
+      }
+      bd[0] = 0.0 + bl[0] + lam * e[1] * bl[2] - lam * e[2] * bl[1];
+      bd[1] = 0.0 + bl[1] + lam * e[2] * bl[0] - lam * e[0] * bl[2];
+      bd[2] = 0.0 + bl[2] + lam * e[0] * bl[1] - lam * e[1] * bl[0];
+      for (int c = 0; c < 3; ++c) bd[c] = 0.5 * bd[c] + 0.5 * V3(b2eff, c, i, k, N);
+      for (int c = 0; c < 3; ++c) {
+        e[c] = V3(emom2, c, i, k, N);
+        V3(emom, c, i, k, N) = e[c];
+      }
+      rodrigues(bd, dt, lam, e, mr);
+      for (int c = 0; c < 3; ++c) V3(emom2, c, i, k, N) = mr[c];
+    }
+}
+
+/* rannum, llg = 1: Dk = (lambda/(1+lambda**2)*k_bolt/gama/mub)*(gama/bn),
+ * D = Dk*mmomi*T*temprescale, sigma = sqrt(2D). ranv holds N(0,1) on entry. */
+void or_rannum_scale(int N, int M, int llg, const double *lambda, double bn,
+                     const double *mmomi, const double *temp, double temprescale,
+                     double *ranv)
+{
+  for (int k = 0; k < M; ++k)
+    for (int i = 0; i < N; ++i) {
+      double lam = lambda[i], Dk;
+      if (llg == 1)
+        Dk = (lam / (1.0 + lam * lam) * OR_K_BOLT / OR_GAMA / (OR_MUB)) * (OR_GAMA / bn);
+      else
+        Dk = (lam * OR_K_BOLT / OR_GAMA / (OR_MUB)) * (OR_GAMA / bn);
+      double D = Dk * S2(mmomi, i, k, N) * temp[i] * temprescale;
+      double sigma = sqrt(2.0 * D);
+      for (int c = 0; c < 3; ++c) V3(ranv, c, i, k, N) = V3(ranv, c, i, k, N) * sigma;
+    }
+}
+
+static inline void midpoint_solve(const double A[3], const double e[3], double et[3],
+                                  double *detAi)
+{
+  *detAi = 1.0 / (1.0 + (A[0] * A[0] + A[1] * A[1] + A[2] * A[2]));
+  double a2[3];
+  a2[0] = e[0] + e[1] * A[2] - e[2] * A[1];
+  a2[1] = e[1] + e[2] * A[0] - e[0] * A[2];
+  a2[2] = e[2] + e[0] * A[1] - e[1] * A[0];
+  et[0] = a2[0] * (1 + A[0] * A[0]) + a2[1] * (A[0] * A[1] + A[2]) + a2[2] * (A[0] * A[2] - A[1]);
+  et[1] = a2[0] * (A[1] * A[0] - A[2]) + a2[1] * (1 + A[1] * A[1]) + a2[2] * (A[1] * A[2] + A[0]);
+  et[2] = a2[0] * (A[2] * A[0] + A[1]) + a2[1] * (A[2] * A[1] - A[0]) + a2[2] * (1 + A[2] * A[2]);
+}
+
+static inline void midpoint_A(const double *landeg, double bn, const double *lambda,
+                              double deltat, int i, const double b[3], const double r[3],
+                              const double ep[3], double A[3], double s1out[3])
+{
+  double lam = lambda[i];
+  double lldamp = 1.0 / (1.0 + lam * lam);
+  double dt = deltat * bn * OR_GAMA * lldamp;
+  double sqrtdt = sqrt(dt);
+  double dtg = dt * landeg[i];
+  double sqrtdtg = sqrtdt * landeg[i];
+  double a1[3], s1[3];
+  a1[0] = -0.0 - b[0] - lam * (ep[1] * b[2] - ep[2] * b[1]);
+  a1[1] = -0.0 - b[1] - lam * (ep[2] * b[0] - ep[0] * b[2]);
+  a1[2] = -0.0 - b[2] - lam * (ep[0] * b[1] - ep[1] * b[0]);
+  s1[0] = -r[0] - lam * (ep[1] * r[2] - ep[2] * r[1]);
+  s1[1] = -r[1] - lam * (ep[2] * r[0] - ep[0] * r[2]);
+  s1[2] = -r[2] - lam * (ep[0] * r[1] - ep[1] * r[0]);
+  for (int c = 0; c < 3; ++c) A[c] = 0.5 * dtg * a1[c] + 0.5 * sqrtdtg * s1[c];
+  if (s1out)
+    for (int c = 0; c < 3; ++c) s1out[c] = s1[c];
+}
+
+void or_midpoint_first(int N, int M, const double *landeg, double bn,
+                       const double *lambda, const double *beff, const double *emom,
+                       double *emom2, double *emomM, const double *mmom,
+                       double deltat, const double *ranv, double *thermal_field)
+{
+  for (int i = 0; i < N; ++i)
+    for (int k = 0; k < M; ++k) {
+      double b[3], r[3], e[3], A[3], s1[3], et[3], detAi;
+      for (int c = 0; c < 3; ++c) {
+        b[c] = V3(beff, c, i, k, N);
+        r[c] = V3(ranv, c, i, k, N);
+        e[c] = V3(emom, c, i, k, N);
+      }
+      midpoint_A(landeg, bn, lambda, deltat, i, b, r, e, A, s1);
+      if (thermal_field)
+        for (int c = 0; c < 3; ++c) V3(thermal_field, c, i, k, N) = s1[c];
+      midpoint_solve(A, e, et, &detAi);
+      for (int c = 0; c < 3; ++c) et[c] = 0.5 * (e[c] + et[c] * detAi);
+      double m = S2(mmom, i, k, N);
+      for (int c = 0; c < 3; ++c) {
+        V3(emom2, c, i, k, N) = et[c];
+        V3(emomM, c, i, k, N) = et[c] * m;
+      }
+    }
+}
+
+void or_midpoint_second(int N, int M, const double *landeg, double bn,
+                        const double *lambda, const double *beff, const double *emom,
+                        double *emom2, double deltat, const double *ranv)
+{
+  for (int i = 0; i < N; ++i)
+    for (int k = 0; k < M; ++k) {
+      double b[3], r[3], e[3], ep[3], A[3], et[3], detAi;
+      for (int c = 0; c < 3; ++c) {
+        b[c] = V3(beff, c, i, k, N);
+        r[c] = V3(ranv, c, i, k, N);
+        e[c] = V3(emom, c, i, k, N);
+        ep[c] = V3(emom2, c, i, k, N);
+      }
+      midpoint_A(landeg, bn, lambda, deltat, i, b, r, ep, A, NULL);
+      midpoint_solve(A, e, et, &detAi);
+      for (int c = 0; c < 3; ++c) V3(emom2, c, i, k, N) = et[c] * detAi;
+    }
+}
+
+void or_moment_update(int N, int M, const double *mmom, double *emom,
+                      const double *emom2, double *emomM, double *mmomi)
+{
+  for (int k = 0; k < M; ++k)
+    for (int i = 0; i < N; ++i) {
+      double m = S2(mmom, i, k, N);
+      for (int c = 0; c < 3; ++c) {
+        V3(emom, c, i, k, N) = V3(emom2, c, i, k, N);
+        V3(emomM, c, i, k, N) = V3(emom2, c, i, k, N) * m;
+      }
+      if (mmomi) S2(mmomi, i, k, N) = 1.0 / m;
+    }
+}

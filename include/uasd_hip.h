@@ -1,0 +1,148 @@
+/*
+ * uasd_hip.h -- C-ABI of libuasd_hip.so, the MI355X (gfx950) spin-update
+ * engine that stands in for UppASD's hot path.
+ *
+ * Conventions
+ *  - Every entry point returns an int status (UASD_OK = 0, negative on
+ *    failure); uasd_last_error() gives the message of the calling thread's
+ *    last failure.  No entry point exits the process.
+ *  - Host arrays use the reference's Fortran layout and 1-based atom
+ *    indices, so a bind(C) interface can pass module arrays unchanged:
+ *      vec3 per site    (3,Natom,Mensemble)  e.g. emom, emomM, beff
+ *      per site/ens     (Natom,Mensemble)    e.g. mmom
+ *      neighbour list   (max_no_neigh,Natom) ham%nlist
+ *      couplings        (max_no_neigh,nHam)  ham%ncoup(:,:,1)
+ *  - A context owns device copies of the Hamiltonian and of the moments;
+ *    the *_host entry points copy host arrays in and out on every call
+ *    (drop-in semantics), the resident entry points (uasd_sd_steps,
+ *    uasd_mc_sweeps, uasd_measure) keep everything in HBM.
+ *  - Calls on one context are not thread safe; different contexts are.
+ */
+#ifndef UASD_HIP_H
+#define UASD_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UASD_API_VERSION 1
+
+enum {
+  UASD_OK = 0,
+  UASD_ERR_ARG = -1,          /* invalid argument / shape                      */
+  UASD_ERR_HIP = -2,          /* HIP runtime failure                           */
+  UASD_ERR_NOMEM = -3,        /* device allocation failed                      */
+  UASD_ERR_UNSUPPORTED = -4,  /* option outside the accelerated path           */
+  UASD_ERR_STATE = -5,        /* call order (e.g. stepping before set-up)      */
+  UASD_ERR_NODEVICE = -6      /* no usable gfx950 device                       */
+};
+
+enum { UASD_FP64 = 0, UASD_FP32 = 1 };
+
+/* Thermal-noise source for the LLG solvers. */
+enum {
+  UASD_NOISE_NONE = 0,        /* no noise drawn (the reference's T = 0 result) */
+  UASD_NOISE_PHILOX = 1,      /* counter-based Philox4x32-10, drawn in-kernel   */
+  UASD_NOISE_INJECTED = 2     /* caller-supplied N(0,1) array, scaled in-kernel */
+};
+
+/* Solvers (the reference's SDEalgh values, evolution.f90:134-217). */
+enum { UASD_SDE_MIDPOINT = 1, UASD_SDE_DEPONDT = 5 };
+
+typedef struct uasd_ctx uasd_ctx;
+
+const char *uasd_last_error(void);
+int uasd_version(void);
+int uasd_device_count(int *count);
+
+/* Context: Natom sites, Mensemble ensembles, FP64 or FP32 arithmetic.
+ * Replaces the state set-up of FortranData_Initiate / CudaMdSimulation::
+ * initiateMatrices (chelper.f90:166-188, gpu_files/cudaMdSimulation.cu:133). */
+int uasd_ctx_create(uasd_ctx **ctx, int natom, int mensemble, int precision);
+int uasd_ctx_destroy(uasd_ctx *ctx);
+/* Run on a caller stream (hipStream_t); NULL restores the context's own. */
+int uasd_ctx_set_stream(uasd_ctx *ctx, void *hip_stream);
+void *uasd_ctx_stream(uasd_ctx *ctx);
+int uasd_ctx_sync(uasd_ctx *ctx);
+
+/* ---- Hamiltonian: the ham_t arrays (hamiltoniandatatype.f90:14-100) ---- */
+/* aham (Natom) maps site -> Hamiltonian row (ham%aHam); NULL means row = site.
+ * Rows with identical couplings are stored once on the device. */
+int uasd_set_exchange(uasd_ctx *ctx, int max_no_neigh, int nham, const int *nlist,
+                      const int *nlistsize, const double *ncoup, const int *aham);
+/* dm_vect is (3,max_no_dmneigh,nHam) (ham%dm_vect). */
+int uasd_set_dm(uasd_ctx *ctx, int max_no_dmneigh, int nham, const int *dmlist,
+                const int *dmlistsize, const double *dm_vect, const int *aham);
+/* ham%taniso (Natom), eaniso (3,Natom), kaniso (2,Natom), sb (Natom). */
+int uasd_set_anisotropy(uasd_ctx *ctx, const int *taniso, const double *eaniso,
+                        const double *kaniso, const double *sb);
+/* external_field (3,Natom,Mensemble), or NULL to clear (calculatefields.f90:23-82). */
+int uasd_set_external_field(uasd_ctx *ctx, const double *field);
+
+/* ---- LLG parameters: lambda1_array, Temp_array, Landeg (Natom each) ---- */
+int uasd_set_llg(uasd_ctx *ctx, const double *lambda1, const double *temp,
+                 const double *landeg, double delta_t, double temprescale, double bn);
+/* Noise source; normals is (3,Natom,Mensemble) N(0,1) for UASD_NOISE_INJECTED
+ * and is used by the next evolve call (both halves of the step). */
+int uasd_set_noise(uasd_ctx *ctx, int mode, uint64_t seed, const double *normals);
+
+/* ---- moments ---- */
+/* emom (3,N,M) unit vectors and mmom (N,M); emomM = emom*mmom on the device. */
+int uasd_set_moments(uasd_ctx *ctx, const double *emom, const double *mmom);
+int uasd_get_moments(uasd_ctx *ctx, double *emom, double *emomM);
+
+/* ---- drop-in entry points, host arrays in and out on every call ---- */
+/* effective_field_full (hamiltonianactions.f90:108-252): beff from emomM;
+ * energy in mRy as the reference's field routine reports it. */
+int uasd_effective_field_host(uasd_ctx *ctx, const double *emomM, double *beff,
+                              double *energy);
+/* evolve_first (evolution.f90:38-273) for SDEalgh 1 and 5: reads beff, emom,
+ * mmom; writes emom2, emomM (and, for Depondt, b2eff and the updated emom). */
+int uasd_evolve_first_host(uasd_ctx *ctx, int sdealgh, const double *beff, double *emom,
+                           double *emom2, double *emomM, double *b2eff);
+/* evolve_second (evolution.f90:280-406) for SDEalgh 1 and 5. */
+int uasd_evolve_second_host(uasd_ctx *ctx, int sdealgh, const double *beff,
+                            const double *b2eff, double *emom, double *emom2);
+
+/* ---- device-resident hot path ---- */
+/* nsteps of the sd_mphase body (sd_driver.f90:668-765): field -> evolve_first
+ * -> field -> evolve_second -> moment_update, fused into two kernels/step. */
+int uasd_sd_steps(uasd_ctx *ctx, int sdealgh, int64_t nsteps);
+/* Field of the resident moments into a host array (NULL skips the copy). */
+int uasd_effective_field(uasd_ctx *ctx, double *beff, double *energy);
+/* nsweeps of Metropolis ('M') or heat-bath ('H') updates (mc_evolve,
+ * montecarlo.f90:44-273) over a colouring of the bond graph; temps (M) is the
+ * temperature of each ensemble. */
+int uasd_mc_sweeps(uasd_ctx *ctx, char mode, const double *temps, int64_t nsweeps,
+                   uint64_t seed);
+/* FP64 reductions: msum (3,M) = sum_i emomM(:,i,k); energy (M) or NULL. */
+int uasd_measure(uasd_ctx *ctx, double *msum, double *energy);
+
+/* ---- colouring used by the MC sweeps ---- */
+int uasd_get_coloring(uasd_ctx *ctx, int *ncolors, int *color_of_site);
+
+/* ---- per-kernel timing with HIP events on the context stream ---- */
+int uasd_set_kernel_timing(uasd_ctx *ctx, int enable);
+/* kind: 0 predictor, 1 corrector, 2 field, 3 MC colour pass */
+int uasd_get_kernel_timing(uasd_ctx *ctx, int kind, int64_t *launches, double *total_ms);
+
+/* ---- host set-up: the reference neighbour map, bit-exact ----
+ * setup_nm + setup_neighbour_hamiltonian (neighbourmap.f90:32-359,
+ * hamiltonianinit.f90:985-1157) for a Bravais lattice with basis.
+ * Shells as read_exchange leaves them: nn (NT), redcoord (NT,max_no_shells,3),
+ * nntype (NT,max_no_shells) or NULL, jc (hdim,NT,max_no_shells,nchmax,nchmax).
+ * Pass nlist == NULL to query max_no_neigh only.  ammom (NA,nchmax); achem
+ * (Natom) 1-based chemistry or NULL. */
+int uasd_build_pair_list(int NA, int NT, int N1, int N2, int N3, const double *C1,
+                         const double *C2, const double *C3, const double *bas,
+                         const int *atype, const char *bc, int sym, int max_no_shells,
+                         const int *nn, const double *redcoord, const int *nntype,
+                         int hdim, int nchmax, const double *jc, const double *ammom,
+                         const int *achem, int *max_no_neigh, int *nlist, int *nlistsize,
+                         double *ncoup);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -144,6 +144,8 @@ def _declare(L):
     L.or_midpoint_second.argtypes = [C.c_int, C.c_int, dp, C.c_double, dp, dp, dp, dp,
                                      C.c_double, dp]
     L.or_moment_update.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, vp]
+    L.or_sd_run.argtypes = [C.POINTER(Ham), C.c_int, C.c_int, C.c_int, C.c_long, C.c_uint64,
+                            C.c_double, dp, dp, dp, dp, dp, dp, C.c_uint64, C.c_int, vp]
     L.or_mc_delta.argtypes = [C.c_double]
     L.or_mc_delta.restype = C.c_double
     L.or_mc_energy_change.argtypes = [C.POINTER(Ham), C.c_int, dp, dp, dp, C.c_int, C.c_int,

@@ -10,6 +10,7 @@
 #include "uasd_oracle.h"
 #include <math.h>
 #include <stddef.h>
+#include <stdlib.h>
 
 #define V3(a, c, i, k, N) (a)[(c) + 3 * ((size_t)(i) + (size_t)(N) * (k))]
 #define S2(a, i, k, N) (a)[(size_t)(i) + (size_t)(N) * (k)]
@@ -201,4 +202,42 @@ void or_moment_update(int N, int M, const double *mmom, double *emom,
       }
       if (mmomi) S2(mmomi, i, k, N) = 1.0 / m;
     }
+}
+
+/* The sd_mphase step body (sd_driver.f90:668-765) for nsteps, with the
+ * thermal field drawn from the Philox counter stream the HIP kernels use
+ * (noise = 1) or absent (noise = 0).  Used as the CPU baseline and for long
+ * trajectory checks. */
+void or_sd_run(const or_ham *h, int N, int M, int alg, long nsteps, uint64_t step0, double dt,
+               const double *lam, const double *temp, const double *landeg,
+               const double *mmom, double *emom, double *emomM, uint64_t seed, int noise,
+               const double *ext)
+{
+  size_t n3 = (size_t)3 * N * M, n1 = (size_t)N * M;
+  double *beff = malloc(sizeof(double) * n3), *e2 = malloc(sizeof(double) * n3);
+  double *b2 = malloc(sizeof(double) * n3), *rv = malloc(sizeof(double) * n3);
+  double *mmomi = malloc(sizeof(double) * n1);
+  for (size_t x = 0; x < n1; ++x) mmomi[x] = 1.0 / mmom[x];
+  for (long s = 0; s < nsteps; ++s) {
+    uint64_t step = step0 + (uint64_t)s;
+    for (int k = 0; k < M; ++k)
+      for (int i = 0; i < N; ++i) {
+        double z[3] = {0.0, 0.0, 0.0};
+        if (noise == 1) or_noise3_f64(seed, 0x11u, step, (uint32_t)i, (uint32_t)k, z);
+        for (int c = 0; c < 3; ++c) V3(rv, c, i, k, N) = z[c];
+      }
+    or_effective_field(h, N, M, emomM, ext, beff, NULL, NULL, NULL);
+    if (alg == 1) {
+      or_rannum_scale(N, M, 1, lam, 1.0, mmomi, temp, 1.0, rv);
+      or_midpoint_first(N, M, landeg, 1.0, lam, beff, emom, e2, emomM, mmom, dt, rv, NULL);
+      or_effective_field(h, N, M, emomM, ext, beff, NULL, NULL, NULL);
+      or_midpoint_second(N, M, landeg, 1.0, lam, beff, emom, e2, dt, rv);
+    } else {
+      or_depondt_first(N, M, lam, temp, 1.0, dt, beff, rv, mmom, emom, e2, emomM, b2, NULL);
+      or_effective_field(h, N, M, emomM, ext, beff, NULL, NULL, NULL);
+      or_depondt_second(N, M, lam, dt, beff, rv, b2, emom, e2);
+    }
+    or_moment_update(N, M, mmom, emom, e2, emomM, mmomi);
+  }
+  free(beff); free(e2); free(b2); free(rv); free(mmomi);
 }

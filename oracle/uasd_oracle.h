@@ -150,6 +150,11 @@ void or_midpoint_second(int N, int M, const double *landeg, double bn,
                         double *emom2, double dt, const double *ranv);
 void or_moment_update(int N, int M, const double *mmom, double *emom,
                       const double *emom2, double *emomM, double *mmomi);
+/* nsteps of the sd_mphase body; noise 1 = Philox stream of the HIP kernels */
+void or_sd_run(const or_ham *h, int N, int M, int alg, long nsteps, uint64_t step0, double dt,
+               const double *lam, const double *temp, const double *landeg,
+               const double *mmom, double *emom, double *emomM, uint64_t seed, int noise,
+               const double *ext);
 
 /* magninit, initmag = 1 (magnetizationinit.f90:141-175); mmom is (N,M) */
 void or_magninit_random(or_mt *st, int NA, int N1, int N2, int N3, int M,

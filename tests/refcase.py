@@ -1,0 +1,141 @@
+"""Readers for the reference's regression-test inputs (committed copies under
+tests/golden/ref_cases/) and oracle-driven runs of those cases.
+
+The file formats follow the reference readers:
+  posfile  isite itype x y z                    (inputhandler_ext.f90:59-130, posfiletype C)
+  momfile  isite ichem |m| ex ey ez             (inputhandler_ext.f90:228-320)
+  jfile    isite jsite rx ry rz J               (inputhandler_ext.f90:444-618)
+  dmfile   isite jsite rx ry rz Dx Dy Dz        (inputhandler_ext.f90:1095-1251)
+  kfile    iat type K1 K2 ex ey ez ratio        (inputhandler_ext.f90:1069-1089)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CASES = os.path.join(HERE, "golden", "ref_cases")
+
+
+def golden(case):
+    with open(os.path.join(CASES, "golden.json")) as fh:
+        return json.load(fh)[case]
+
+
+def _rows(path):
+    out = []
+    with open(path) as fh:
+        for line in fh:
+            t = line.split()
+            if t and not t[0].startswith("#"):
+                out.append(t)
+    return out
+
+
+def _num(s):
+    return float(s.replace("d", "e").replace("D", "e"))
+
+
+def read_inpsd(path):
+    """keyword -> list of value tokens (first line), plus raw lines."""
+    kv, lines = {}, []
+    with open(path) as fh:
+        for line in fh:
+            t = line.split()
+            lines.append(t)
+            if t and t[0][0].isalpha():
+                kv.setdefault(t[0].lower(), t[1:])
+    return kv, lines
+
+
+def cell_of(lines):
+    for k, t in enumerate(lines):
+        if t and t[0].lower() == "cell":
+            rows = [t[1:4], lines[k + 1][0:3], lines[k + 2][0:3]]
+            return np.array([[_num(x) for x in r] for r in rows])
+    raise KeyError("cell")
+
+
+def read_case(case, base="inpsd.dat"):
+    d = os.path.join(CASES, case)
+    kv, lines = read_inpsd(os.path.join(d, base))
+    pos = _rows(os.path.join(d, "posfile"))
+    bas = np.array([[_num(x) for x in r[2:5]] for r in pos])
+    atype = np.array([int(r[1]) for r in pos], dtype=np.int32)
+    mom = _rows(os.path.join(d, "momfile"))
+    ammom = np.array([_num(r[2]) for r in mom])
+    edir = np.array([[_num(x) for x in r[3:6]] for r in mom])
+    edir = edir / np.linalg.norm(edir, axis=1)[:, None]
+    jname = kv.get("exchange", ["jfile"])[0]
+    jl = [(int(r[0]), int(r[1]), 1, 1, *[_num(x) for x in r[2:6]]) for r in
+          _rows(os.path.join(d, os.path.basename(jname)))]
+    out = dict(kv=kv, lines=lines, bas=bas, atype=atype, ammom=ammom, edir=edir, jlines=jl,
+               cell=cell_of(lines), n=tuple(int(x) for x in kv["ncell"][:3]),
+               bc=tuple(kv.get("bc", ["0", "0", "0"])[:3]),
+               sym=int(kv.get("sym", ["0"])[0]))
+    if "anisotropy" in kv:
+        k = _rows(os.path.join(d, os.path.basename(kv["anisotropy"][0])))
+        out["ktype"] = np.array([int(r[1]) for r in k], dtype=np.int32)
+        out["kvals"] = np.array([[_num(x) for x in r[2:8]] for r in k])
+    if "dm" in kv:
+        dml = _rows(os.path.join(d, os.path.basename(kv["dm"][0])))
+        out["dmlines"] = [(int(r[0]), int(r[1]), 1, 1, *[_num(x) for x in r[2:8]]) for r in dml]
+    return out
+
+
+class OracleSD:
+    """Oracle-driven restatement of sd_mphase for one lattice (single ensemble set)."""
+
+    def __init__(self, O, lat, pairs, ammom_sites, M=1, aniso=None, dm=None):
+        self.O, self.L = O, O.lib()
+        self.N, self.M = lat.natom, M
+        kw = {}
+        if aniso is not None:
+            kw.update(taniso=aniso[0], eaniso=aniso[1], kaniso=aniso[2], sb=aniso[3])
+        if dm is not None:
+            kw.update(dmlist=dm["nlist"], dmlistsize=dm["nlistsize"], dm_vect=dm["ncoup"])
+        self.ham = O.OracleHam(self.N, pairs["nlist"], pairs["nlistsize"], pairs["ncoup"], **kw)
+        self.mmom = np.ascontiguousarray(np.broadcast_to(ammom_sites, (M, self.N)), dtype=np.float64)
+        self.mmomi = 1.0 / self.mmom
+        self.e = np.zeros((M, self.N, 3))
+        self.eM = np.zeros((M, self.N, 3))
+        self.e2 = np.zeros_like(self.e)
+        self.b2 = np.zeros_like(self.e)
+        self.noise = np.zeros_like(self.e)
+
+    def random_init(self, st, lat):
+        self.L.or_magninit_random.argtypes = [C.POINTER(self.O.MT), C.c_int, C.c_int, C.c_int,
+                                              C.c_int, C.c_int, self.O.dp, self.O.dp, self.O.dp]
+        self.L.or_magninit_random(C.byref(st), lat.NA, *lat.n, self.M, self.mmom, self.e, self.eM)
+
+    def set_state(self, e):
+        self.e[...] = e
+        self.eM[...] = e * self.mmom[..., None]
+
+    def step(self, alg, dt, lam, temp, normals=None, landeg=None, ext=None):
+        """One sd_mphase step; normals = raw N(0,1) (M,N,3) or None for zero."""
+        N, M, L = self.N, self.M, self.L
+        lam = np.ascontiguousarray(np.broadcast_to(lam, (N,)), dtype=np.float64)
+        T = np.ascontiguousarray(np.broadcast_to(temp, (N,)), dtype=np.float64)
+        lg = np.ones(N) if landeg is None else np.ascontiguousarray(landeg, dtype=np.float64)
+        self.noise[...] = 0.0 if normals is None else normals
+        beff, _ = self.ham.field(self.eM, ext)
+        if alg == 1:
+            L.or_rannum_scale(N, M, 1, lam, 1.0, self.mmomi, T, 1.0, self.noise.reshape(-1))
+            L.or_midpoint_first(N, M, lg, 1.0, lam, beff, self.e, self.e2, self.eM, self.mmom, dt,
+                                self.noise, None)
+            beff, _ = self.ham.field(self.eM, ext)
+            L.or_midpoint_second(N, M, lg, 1.0, lam, beff, self.e, self.e2, dt, self.noise)
+        else:
+            L.or_depondt_first(N, M, lam, T, 1.0, dt, beff, self.noise, self.mmom, self.e, self.e2,
+                               self.eM, self.b2, None)
+            beff, _ = self.ham.field(self.eM, ext)
+            L.or_depondt_second(N, M, lam, dt, beff, self.noise, self.b2, self.e, self.e2)
+        L.or_moment_update(N, M, self.mmom, self.e, self.e2, self.eM, None)
+
+    def mavg(self):
+        m = self.eM.sum(axis=1) / self.N
+        return m

@@ -1,0 +1,91 @@
+"""The library's neighbour-map builder (uasd_build_pair_list, host code) is
+bit-identical to the oracle's restatement of setup_nm +
+setup_neighbour_hamiltonian, and both satisfy the closed-form shell counts."""
+import numpy as np
+import pytest
+
+import uppasd_amd as U
+from uppasd_amd import configs as CF
+from uppasd_amd.lattice import BCC_SHELLS
+
+
+def _oracle_pairs(O, lat, lines, sym, ammom, hdim=1, use_nntype=True, achem=None, nchmax=1):
+    latO = O.LatticeSpec(lat.cell, lat.bas, lat.atype, lat.n, lat.bc)
+    return O.build_pairs(latO, lines, sym, hdim=hdim, use_nntype=use_nntype, ammom=ammom,
+                         achem=achem, nchmax=nchmax, do_ralloy=1 if achem is not None else 0)
+
+
+def _same(a, b):
+    return a.shape == b.shape and a.tobytes() == b.tobytes()
+
+
+BCC_LINES = [(1, 1, 1, 1, *BCC_SHELLS[k], j) for k, j in enumerate(CF.bcc5_couplings())]
+
+
+@pytest.mark.parametrize("n,bc", [((6, 6, 6), "PPP"), ((4, 5, 3), "PPP"), ((4, 4, 4), "P0P"),
+                                  ((3, 1, 7), "00P")])
+def test_bcc5_bitexact(O, n, bc):
+    lat = U.bcc(n, tuple(bc))
+    nl, ns, nc = U.build_pair_list(lat, U.Shells.from_lines(lat, BCC_LINES), 1, np.full((2, 1), 2.2))
+    p = _oracle_pairs(O, lat, BCC_LINES, 1, np.full((2, 1), 2.2))
+    assert _same(nl, p["nlist"]) and _same(ns, p["nlistsize"]) and _same(nc, p["ncoup"])
+
+
+def test_bcc_shell_counts_and_symmetry():
+    """Shells of bcc hold 8/6/12/24/8 sites; J_ij m_i m_j is symmetric."""
+    sysd = CF.bcc_shells(8, [1.0, 2.0, 3.0, 4.0, 5.0])
+    nl, ns, nc = sysd["nlist"], sysd["nlistsize"], sysd["ncoup"]
+    assert np.all(ns == 58)
+    fc2 = 2.0 * 2.179872325e-21 / 9.274009994e-24
+    vals, counts = np.unique(np.round(nc[0] * 2.2 * 2.2 / fc2, 9), return_counts=True)
+    assert dict(zip(vals, counts)) == {1.0: 8, 2.0: 6, 3.0: 12, 4.0: 24, 5.0: 8}
+    N = nl.shape[0]
+    pairs = {}
+    for i in range(N):
+        for j in range(ns[i]):
+            pairs[(i, nl[i, j] - 1)] = nc[i, j]
+    for (i, j), v in pairs.items():
+        assert pairs[(j, i)] == v
+
+
+def test_bcc_neighbours_by_distance():
+    """Brute force: the 8 first-shell neighbours are the sites at distance sqrt(3)/2."""
+    lat = U.bcc((4, 4, 4))
+    sysd = CF.bcc_shells((4, 4, 4), [1.0])
+    NA, (N1, N2, N3) = lat.NA, lat.n
+    pos = np.array([lat.bas[i % NA] + [(i // NA) % N1, (i // NA // N1) % N2, i // NA // N1 // N2]
+                    for i in range(lat.natom)])
+    box = np.array([N1, N2, N3], dtype=float)
+    for i in (0, 1, 77):
+        d = pos - pos[i]
+        d -= box * np.round(d / box)
+        r = np.linalg.norm(d, axis=1)
+        want = set(np.nonzero(np.abs(r - np.sqrt(3) / 2) < 1e-9)[0] + 1)
+        assert set(sysd["nlist"][i, :sysd["nlistsize"][i]]) == want
+
+
+def test_triangular_dmi_and_hexagonal(O):
+    sysd = CF.triangular_dmi(8)
+    lat = sysd["lattice"]
+    p = _oracle_pairs(O, lat, sysd["jlines"], 0, np.ones((1, 1)))
+    assert _same(sysd["nlist"], p["nlist"]) and _same(sysd["ncoup"], p["ncoup"])
+    d = _oracle_pairs(O, lat, sysd["dmlines"], 0, np.ones((1, 1)), hdim=3, use_nntype=False)
+    assert _same(sysd["dmlist"], d["nlist"]) and _same(sysd["dm_vect"], d["ncoup"])
+    # one vector + the 24 hexagonal operations gives the same 6 neighbours
+    sh = U.Shells.from_lines(lat, [(1, 1, 1, 1, 1.0, 0.0, 0.0, 1.0)])
+    nl3, ns3, _ = U.build_pair_list(lat, sh, 3, np.ones((1, 1)))
+    assert np.all(ns3 == 6)
+    assert all(set(nl3[i]) == set(sysd["nlist"][i]) for i in range(lat.natom))
+
+
+def test_kagome_and_alloy(O):
+    sysd = CF.kagome_afm(5)
+    lat = sysd["lattice"]
+    p = _oracle_pairs(O, lat, sysd["jlines"], 0, np.ones((3, 1)))
+    assert _same(sysd["nlist"], p["nlist"]) and _same(sysd["ncoup"], p["ncoup"])
+    assert np.all(sysd["nlistsize"] == 4)
+    rng = np.random.default_rng(4)
+    occ = rng.integers(1, 3, 2 * 5 ** 3).astype(np.int32)
+    a = CF.bcc_alloy(5, occ)
+    q = _oracle_pairs(O, a["lattice"], a["lines"], 1, a["ammom"], achem=occ, nchmax=2)
+    assert _same(a["nlist"], q["nlist"]) and _same(a["ncoup"], q["ncoup"])

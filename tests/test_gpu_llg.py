@@ -1,0 +1,273 @@
+"""GPU parity of the field and LLG kernels against the FP64 oracle.
+
+Tolerances (FP64 path, -ffp-contract=off on both sides):
+  * field, single LLG half-steps: relative 1e-13 (the sums run in the same
+    order; only libm ulps of sin/cos/log/sqrt differ between host and device)
+  * T = 0 trajectories and injected-noise trajectories: max |e_gpu - e_cpu|
+    <= 1e-12 after the stated number of steps
+  * reference golden values (Solvers, bccFe): |delta| <= 1e-8, as the
+    reference's own 'similar' comparison
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import refcase as RC
+import uppasd_amd as U
+from uppasd_amd import configs as CF
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_system(O, sysd, M=1):
+    lat = sysd["lattice"]
+    latO = O.LatticeSpec(lat.cell, lat.bas, lat.atype, lat.n, lat.bc)
+    return latO
+
+
+def engine_for(sysd, M=1, precision="fp64"):
+    lat = sysd["lattice"]
+    eng = U.Engine(lat.natom, M, precision)
+    eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
+    if "dmlist" in sysd:
+        eng.set_dm(sysd["dmlist"], sysd["dmlistsize"], sysd["dm_vect"])
+    if "taniso" in sysd:
+        eng.set_anisotropy(sysd["taniso"], sysd["eaniso"], sysd["kaniso"], sysd["sb"])
+    if "ext" in sysd:
+        eng.set_external_field(sysd["ext"])
+    return eng
+
+
+def oracle_ham(O, sysd):
+    kw = {}
+    if "dmlist" in sysd:
+        kw.update(dmlist=sysd["dmlist"], dmlistsize=sysd["dmlistsize"], dm_vect=sysd["dm_vect"])
+    if "taniso" in sysd:
+        kw.update(taniso=sysd["taniso"], eaniso=sysd["eaniso"], kaniso=sysd["kaniso"], sb=sysd["sb"])
+    return O.OracleHam(sysd["lattice"].natom, sysd["nlist"], sysd["nlistsize"], sysd["ncoup"], **kw)
+
+
+def ext_array(sysd, M):
+    N = sysd["lattice"].natom
+    if "ext" not in sysd:
+        return None
+    return np.ascontiguousarray(np.broadcast_to(sysd["ext"], (M, N, 3)))
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.mark.parametrize("kind", ["bcc5", "tri_dmi"])
+def test_effective_field_matches_oracle(O, kind):
+    rng = np.random.default_rng(11)
+    sysd = CF.bcc_shells(6, CF.bcc5_couplings()) if kind == "bcc5" else CF.triangular_dmi(16)
+    M = 3
+    N = sysd["lattice"].natom
+    emom = CF.random_unit(N, rng, M)
+    emomM = emom * sysd["mmom"][None, :, None]
+    eng = engine_for(sysd, M)
+    beff, en = eng.effective_field(np.ascontiguousarray(emomM))
+    ham = oracle_ham(O, sysd)
+    bref, eref = ham.field(np.ascontiguousarray(emomM), ext_array(sysd, M))
+    assert rel(beff, bref) <= 1e-13
+    assert abs(en - eref) <= 1e-12 * abs(eref)
+
+
+def test_anisotropy_types_field(O):
+    """Uniaxial (1), cubic (2) and combined (7) anisotropy terms."""
+    rng = np.random.default_rng(3)
+    sysd = CF.bcc_shells(4, [1.0, 0.3])
+    N = sysd["lattice"].natom
+    t = rng.choice([1, 2, 7], size=N).astype(np.int32)
+    e = rng.normal(size=(N, 3))
+    e /= np.linalg.norm(e, axis=1)[:, None]
+    sysd.update(taniso=t, eaniso=e, kaniso=rng.normal(size=(N, 2)) * 30.0, sb=rng.random(N))
+    emomM = CF.random_unit(N, rng, 2) * 2.2
+    eng = engine_for(sysd, 2)
+    beff, en = eng.effective_field(np.ascontiguousarray(emomM))
+    bref, eref = oracle_ham(O, sysd).field(np.ascontiguousarray(emomM))
+    assert rel(beff, bref) <= 1e-13
+    assert abs(en - eref) <= 1e-12 * abs(eref)
+
+
+def _oracle_step(O, sim, alg, dt, lam, T, normals, ext):
+    sim.step(alg, dt, lam, T, normals=normals, ext=ext)
+
+
+@pytest.mark.parametrize("alg", [1, 5])
+def test_dropin_step_matches_oracle(O, alg):
+    """field -> evolve_first -> field -> evolve_second -> moment_update through
+    the host-array entry points, with an injected noise stream at 300 K."""
+    rng = np.random.default_rng(5)
+    sysd = CF.triangular_dmi(12)
+    M, N = 2, sysd["lattice"].natom
+    emom = np.ascontiguousarray(CF.random_unit(N, rng, M))
+    mm = np.ascontiguousarray(np.broadcast_to(sysd["mmom"], (M, N)))
+    eng = engine_for(sysd, M)
+    eng.set_moments(emom, mm)
+    lam, T, dt = 0.05, 300.0, 1e-16
+    eng.set_llg(lam, T, dt)
+    sim = RC.OracleSD(O, oracle_system(O, sysd), dict(nlist=sysd["nlist"],
+                      nlistsize=sysd["nlistsize"], ncoup=sysd["ncoup"]), sysd["mmom"], M=M,
+                      aniso=(sysd["taniso"], sysd["eaniso"], sysd["kaniso"], sysd["sb"]),
+                      dm=dict(nlist=sysd["dmlist"], nlistsize=sysd["dmlistsize"],
+                              ncoup=sysd["dm_vect"]))
+    sim.set_state(emom)
+    ext = ext_array(sysd, M)
+    e, eM = emom.copy(), emom * mm[..., None]
+    e2, b2 = np.zeros_like(e), np.zeros_like(e)
+    for it in range(5):
+        normals = rng.normal(size=(M, N, 3))
+        eng.set_noise(U.UASD_NOISE_INJECTED, 0, normals)
+        beff, _ = eng.effective_field(eM)
+        eng.evolve_first(alg, beff, e, e2, eM, b2)
+        beff, _ = eng.effective_field(eM)
+        eng.evolve_second(alg, beff, b2, e, e2)
+        U.moment_update(mm, e, e2, eM)
+        sim.step(alg, dt, lam, T, normals=normals.copy(), ext=ext)
+        assert np.max(np.abs(e - sim.e)) <= 1e-13, it
+
+
+@pytest.mark.parametrize("alg", [1, 5])
+def test_resident_T0_trajectory(O, alg):
+    """Deterministic T = 0 trajectory, fused kernels vs oracle, 500 steps."""
+    rng = np.random.default_rng(7)
+    sysd = CF.bcc_shells(4, CF.bcc5_couplings())
+    N = sysd["lattice"].natom
+    emom = np.ascontiguousarray(CF.random_unit(N, rng, 1))
+    eng = engine_for(sysd)
+    eng.set_moments(emom, sysd["mmom"][None])
+    eng.set_llg(0.1, 0.0, 1e-16)
+    eng.set_noise(U.UASD_NOISE_NONE)
+    sim = RC.OracleSD(O, oracle_system(O, sysd), sysd, sysd["mmom"])
+    sim.set_state(emom)
+    for _ in range(500):
+        sim.step(alg, 1e-16, 0.1, 0.0)
+    eng.sd_steps(alg, 500)
+    e, eM = eng.get_moments()
+    assert np.max(np.abs(e - sim.e)) <= 1e-12
+    assert np.max(np.abs(eM - sim.eM)) <= 1e-12 * 2.2
+
+
+@pytest.mark.parametrize("alg", [1, 5])
+def test_resident_injected_noise_trajectory(O, alg):
+    """Stochastic LLG at 300 K on an identical injected noise stream."""
+    rng = np.random.default_rng(9)
+    sysd = CF.bcc_shells(4, CF.bcc5_couplings())
+    M, N = 2, sysd["lattice"].natom
+    emom = np.ascontiguousarray(CF.random_unit(N, rng, M))
+    eng = engine_for(sysd, M)
+    eng.set_moments(emom, np.broadcast_to(sysd["mmom"], (M, N)))
+    eng.set_llg(0.1, 300.0, 1e-16)
+    sim = RC.OracleSD(O, oracle_system(O, sysd), sysd, sysd["mmom"], M=M)
+    sim.set_state(emom)
+    for _ in range(100):
+        normals = rng.normal(size=(M, N, 3))
+        eng.set_noise(U.UASD_NOISE_INJECTED, 0, normals)
+        eng.sd_steps(alg, 1)
+        sim.step(alg, 1e-16, 0.1, 300.0, normals=normals.copy())
+    e, _ = eng.get_moments()
+    assert np.max(np.abs(e - sim.e)) <= 1e-12
+
+
+def test_philox_noise_stream_matches_oracle(O):
+    """In-kernel Philox normals reproduce the oracle's restatement of the
+    same counter stream (libm ulps only)."""
+    rng = np.random.default_rng(13)
+    sysd = CF.bcc_shells(3, [1.34, 0.77])
+    M, N = 2, sysd["lattice"].natom
+    emom = np.ascontiguousarray(CF.random_unit(N, rng, M))
+    seed = 0x5EED1234ABCD
+    eng = engine_for(sysd, M)
+    eng.set_moments(emom, np.broadcast_to(sysd["mmom"], (M, N)))
+    eng.set_llg(0.1, 300.0, 1e-16)
+    eng.set_noise(U.UASD_NOISE_PHILOX, seed)
+    sim = RC.OracleSD(O, oracle_system(O, sysd), sysd, sysd["mmom"], M=M)
+    sim.set_state(emom)
+    nsteps = 20
+    for step in range(nsteps):
+        normals = np.zeros((M, N, 3))
+        for k in range(M):
+            for i in range(N):
+                normals[k, i] = O.noise3_f64(seed, 0x11, step, i, k)
+        sim.step(5, 1e-16, 0.1, 300.0, normals=normals)
+    eng.sd_steps(5, nsteps)
+    e, _ = eng.get_moments()
+    assert np.max(np.abs(e - sim.e)) <= 1e-11
+
+
+@pytest.mark.parametrize("solver,name", [(1, "Midpoint"), (5, "Depondt")])
+def test_solvers_golden_on_gpu(O, solver, name):
+    """The reference's Solvers chain (regulartests.yaml) on the device: the
+    random start comes from the oracle's MT19937 stream, T = 0."""
+    c = RC.read_case("Solvers", "inpsd.dat.base")
+    lat = U.Lattice(c["cell"], c["bas"], c["atype"], c["n"], c["bc"])
+    sh = U.Shells.from_lines(lat, c["jlines"])
+    nl, ns, nc = U.build_pair_list(lat, sh, c["sym"], c["ammom"].reshape(-1, 1))
+    latO = O.LatticeSpec(c["cell"], c["bas"], c["atype"], c["n"], c["bc"])
+    sim = RC.OracleSD(O, latO, dict(nlist=nl, nlistsize=ns, ncoup=nc), np.ones(lat.natom))
+    sim.random_init(O.mt(1), latO)
+    eng = U.Engine(lat.natom, 1)
+    eng.set_exchange(nl, ns, nc)
+    eng.set_moments(sim.e, np.ones((1, lat.natom)))
+    eng.set_llg(1.0, 0.0, 1.0e-15)
+    eng.set_noise(U.UASD_NOISE_NONE)
+    g = [x for x in RC.golden("Solvers") if x["type"] == name][0]
+    eng.sd_steps(solver, g["select"]["#_Iter."])
+    _, eM = eng.get_moments()
+    m = eM[0].sum(axis=0) / lat.natom
+    assert np.allclose([*m, np.linalg.norm(m)], g["expected"], atol=1e-8)
+
+
+def test_bccfe_sd_golden_on_gpu(O):
+    """bcc Fe at 500 K with the reference's ziggurat stream injected each
+    step: <m> after the initial phase and at measurement iteration 2700."""
+    c = RC.read_case("bccFe", "inpsd.dat.base")
+    lat = U.Lattice(c["cell"], c["bas"], c["atype"], c["n"], c["bc"])
+    sh = U.Shells.from_lines(lat, c["jlines"])
+    nl, ns, nc = U.build_pair_list(lat, sh, c["sym"], c["ammom"].reshape(-1, 1))
+    N = lat.natom
+    eng = U.Engine(N, 1)
+    eng.set_exchange(nl, ns, nc)
+    e0 = np.tile(c["edir"], (N // lat.NA, 1))[None]
+    eng.set_moments(np.ascontiguousarray(e0), np.full((1, N), 2.23))
+    z = O.zig(5)
+    noise = np.zeros((1, N, 3))
+    exp = {g["type"]: g for g in RC.golden("bccFe") if g["name"] == "bcc Fe (SD)"}
+
+    def run(nsteps, dt, stop=None):
+        eng.set_llg(0.5, 500.0, dt)
+        for s in range(nsteps):
+            if stop is not None and s == stop:
+                return
+            O.lib().or_zig_fill(ctypes.byref(z), noise.reshape(-1), noise.size)
+            eng.set_noise(U.UASD_NOISE_INJECTED, 0, noise)
+            eng.sd_steps(1, 1)
+
+    run(2000, 1.0e-16)
+    _, eM = eng.get_moments()
+    m = eM[0].sum(axis=0) / N
+    assert np.allclose([*m, np.linalg.norm(m)], exp["Initial phase"]["expected"], atol=1e-8)
+    run(3000, float(np.float32(1.0e-16)), stop=2700)
+    _, eM = eng.get_moments()
+    m = eM[0].sum(axis=0) / N
+    assert np.allclose([*m, np.linalg.norm(m)], exp["Magnetization"]["expected"], atol=1e-8)
+
+
+def test_measure_reductions(O):
+    rng = np.random.default_rng(21)
+    sysd = CF.triangular_dmi(10)
+    M, N = 4, sysd["lattice"].natom
+    emom = np.ascontiguousarray(CF.random_unit(N, rng, M))
+    eng = engine_for(sysd, M)
+    eng.set_moments(emom, np.broadcast_to(sysd["mmom"], (M, N)))
+    msum, en = eng.measure(with_energy=True)
+    eM = emom * sysd["mmom"][None, :, None]
+    assert np.allclose(msum, eM.sum(axis=1), rtol=1e-13, atol=1e-12)
+    ham = oracle_ham(O, sysd)
+    ext = ext_array(sysd, M)
+    for k in range(M):
+        _, ek = ham.field(np.ascontiguousarray(eM[k:k + 1]), None if ext is None else ext[k:k + 1])
+        assert abs(en[k] - ek) <= 1e-12 * abs(ek)

@@ -1,0 +1,121 @@
+"""The oracle against the reference's own golden values (tests/regulartests.yaml).
+
+These runs are stream-identical to the reference: MT19937 (mtprng quirk
+included), ziggurat normals, magninit's random start, the neighbour map and
+both LLG solvers.  A wrong sign, index or operand order in any of them moves
+the printed digits.  Tolerance: the yaml prints 9 significant digits; its
+'similar' comparison allows |delta| <= 1e-8.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import refcase as RC
+
+MUB, KB = 9.274009994e-24, 1.38064852e-23
+
+
+def _lattice(O, c):
+    return O.LatticeSpec(c["cell"], c["bas"], c["atype"], c["n"], c["bc"])
+
+
+def _pairs(O, c, lat):
+    return O.build_pairs(lat, c["jlines"], c["sym"], ammom=c["ammom"].reshape(-1, 1))
+
+
+def _sites(lat, per_basis):
+    return np.tile(np.asarray(per_basis, dtype=np.float64), lat.natom // lat.NA)
+
+
+def _expected(case, typ):
+    return [g for g in RC.golden(case) if g["type"] == typ]
+
+
+@pytest.mark.parametrize("solver,name", [(1, "Midpoint"), (5, "Depondt")])
+def test_solvers_chain_golden(O, solver, name):
+    c = RC.read_case("Solvers", "inpsd.dat.base")
+    lat = _lattice(O, c)
+    sim = RC.OracleSD(O, lat, _pairs(O, c, lat), _sites(lat, c["ammom"]))
+    st = O.mt(1)                       # tseed default 1
+    sim.random_init(st, lat)
+    z = O.zig(1)
+    g = _expected("Solvers", name)[0]
+    nstep = g["select"]["#_Iter."]
+    noise = np.zeros_like(sim.e)
+    for _ in range(nstep):
+        O.lib().or_zig_fill(ctypes.byref(z), noise.reshape(-1), noise.size)
+        sim.step(solver, 1.0e-15, 1.0, 0.0, normals=noise)
+    m = sim.mavg()[0]
+    got = [m[0], m[1], m[2], np.linalg.norm(m)]
+    assert np.allclose(got, g["expected"], rtol=0, atol=1e-8), (got, g["expected"])
+
+
+def test_bccfe_sd_golden(O):
+    """bcc Fe, 432 spins, midpoint at 500 K: 2000 initial-phase steps
+    (dt = 1.0d-16), then 3000 measurement steps with the default timestep
+    (the single-precision literal 1.0e-16).  Checks <m> at iterations 0 and
+    2700 and cumulant row 41 (<M>, <M^2>, <M^4>, U4, chi)."""
+    import ctypes
+    c = RC.read_case("bccFe", "inpsd.dat.base")
+    lat = _lattice(O, c)
+    sim = RC.OracleSD(O, lat, _pairs(O, c, lat), _sites(lat, c["ammom"]))
+    e0 = np.tile(c["edir"], (lat.natom // lat.NA, 1))
+    sim.set_state(e0[None])
+    z = O.zig(5)
+    noise = np.zeros_like(sim.e)
+
+    def step(T, dt, lam):
+        O.lib().or_zig_fill(ctypes.byref(z), noise.reshape(-1), noise.size)
+        sim.step(1, dt, lam, T, normals=noise)
+
+    for _ in range(2000):
+        step(500.0, 1.0e-16, 0.5)
+    exp = {g["type"]: g for g in RC.golden("bccFe") if g["name"] == "bcc Fe (SD)"}
+    m = sim.mavg()[0]
+    assert np.allclose([*m, np.linalg.norm(m)], exp["Initial phase"]["expected"], atol=1e-8)
+    dt = float(np.float32(1.0e-16))
+    w = tot = mm = mm2 = mm4 = 0.0
+    ncum, cumrow, m2700 = 0, None, None
+    N = lat.natom
+    for mstep in range(1, 3001):
+        if mstep - 1 == 2700:
+            m2700 = sim.mavg()[0].copy()
+        if mstep % 50 == 0:   # calc_and_print_cumulant, prn_averages.f90:919-1095
+            me = np.linalg.norm(sim.eM[0].sum(axis=0)) / N
+            w += 1.0
+            mm = (mm * tot + me * w) / (tot + w)
+            mm2 = (mm2 * tot + me ** 2 * w) / (tot + w)
+            mm4 = (mm4 * tot + (me ** 2) ** 2 * w) / (tot + w)
+            tot += w
+            ncum += 1
+            if ncum == 41:
+                U = 1 - (mm4 / 3 / mm2 ** 2)
+                chi = (mm2 - mm ** 2) * MUB ** 2 * N / KB ** 2 / 500.0
+                cumrow = [mm, mm2, mm4, U, chi]
+        step(500.0, dt, 0.5)
+    assert np.allclose([*m2700, np.linalg.norm(m2700)], exp["Magnetization"]["expected"], atol=1e-8)
+    want = exp["Cumulants"]["expected"][:5]
+    assert np.allclose(cumrow, want, rtol=1e-8, atol=1e-12), (cumrow, want)
+
+
+def test_heisstripe_anisotropy_golden(O):
+    """10x1x100 stripe, vacuum in x, uniaxial K1 = -0.02 mRy along y, T = 0
+    midpoint from a random start: <m> at iteration 1190 pins the anisotropy
+    field and the vacuum-boundary neighbour lists."""
+    import ctypes
+    c = RC.read_case("HeisStripe")
+    lat = _lattice(O, c)
+    pairs = _pairs(O, c, lat)
+    an = O.setup_anisotropies(lat, c["ktype"], c["kvals"], c["ammom"])
+    sim = RC.OracleSD(O, lat, pairs, _sites(lat, c["ammom"]), aniso=an)
+    st = O.mt(1)
+    sim.random_init(st, lat)
+    z = O.zig(1)
+    noise = np.zeros_like(sim.e)
+    g = [x for x in RC.golden("HeisStripe") if x["type"] == "Magnetization"][0]
+    for _ in range(g["select"]["#_Iter."]):
+        O.lib().or_zig_fill(ctypes.byref(z), noise.reshape(-1), noise.size)
+        sim.step(1, 1.0e-16, 0.10, 0.0, normals=noise)
+    m = sim.mavg()[0]
+    assert np.allclose([*m, np.linalg.norm(m)], g["expected"], atol=1e-8), (m, g["expected"])

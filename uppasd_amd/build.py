@@ -1,0 +1,61 @@
+"""Builds libuasd_hip.so in-tree for gfx950 (hipcc, no JIT cache).
+
+The FP64 kernels and the context are compiled with -ffp-contract=off so that
+every multiply and add rounds on its own, as the reference Fortran does; the
+FP32 kernels allow contraction.  Packed FP32 instructions are disabled for all
+device code (-packed-fp32-ops).  The host set-up (neighbour map builder) is
+plain C++ with OpenMP, built by g++.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libuasd_hip.so")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+ARCH = "gfx950"
+
+HIP_COMMON = [f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+              "-Wno-unused-command-line-argument"]
+UNITS = [
+    ("uasd_kernels_f64.hip", ["-ffp-contract=off"]),
+    ("uasd_kernels_f32.hip", ["-ffp-contract=fast"]),
+    ("uasd_ctx.hip", ["-ffp-contract=off"]),
+]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build step failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return r
+
+
+def build(verbose: bool = False) -> str:
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for src, extra in UNITS:
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        _run([HIPCC, *HIP_COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        objs.append(obj)
+    setup_obj = os.path.join(objdir, "uasd_setup.o")
+    _run(["g++", "-O2", "-fPIC", "-std=c++17", "-fopenmp", "-ffp-contract=off",
+          "-c", os.path.join(CSRC, "uasd_setup.cpp"), "-o", setup_obj], verbose)
+    objs.append(setup_obj)
+    tmp = LIB + ".tmp"
+    _run(["g++", "-shared", "-o", tmp, *objs, f"-L{ROCM}/lib", "-lamdhip64", "-fopenmp",
+          f"-Wl,-rpath,{ROCM}/lib"], verbose)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,1148 @@
+// uasd_ctx.hip -- context, uploads and the C-ABI of libuasd_hip.so.
+//
+// A context mirrors the state the reference keeps in module arrays (ham_t,
+// MomentData, FieldData, the LLG inputs) on one device, in the layout
+// described in uasd_kernels.hpp.  Uploads convert the Fortran layout
+// (3,Natom,Mensemble) to three planes indexed site*M+ens and compress the
+// per-site coupling rows of ham%ncoup / ham%dm_vect / the anisotropy arrays to
+// their distinct rows.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/uasd_hip.h"
+#include "uasd_launch.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      return fail(_e == hipErrorOutOfMemory ? UASD_ERR_NOMEM : UASD_ERR_HIP,          \
+                  std::string(#expr) + ": " + hipGetErrorString(_e));                  \
+  } while (0)
+
+// Owned device allocation.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  hipError_t ensure(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    release();
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, n);
+    if (e == hipSuccess) bytes = n;
+    else p = nullptr;
+    return e;
+  }
+  template <typename T>
+  T *as() const { return static_cast<T *>(p); }
+};
+
+// Hash of a byte string (FNV-1a) for deduplicating rows.
+struct RowKey {
+  std::string bytes;
+  bool operator==(const RowKey &o) const { return bytes == o.bytes; }
+};
+struct RowKeyHash {
+  size_t operator()(const RowKey &k) const {
+    uint64_t h = 1469598103934665603ULL;
+    for (unsigned char c : k.bytes) h = (h ^ c) * 1099511628211ULL;
+    return (size_t)h;
+  }
+};
+
+struct Timing {
+  int64_t launches = 0;
+  double ms = 0.0;
+};
+
+}  // namespace
+
+// ----------------------------------------------------------------- context
+struct uasd_ctx {
+  int N = 0, M = 0, prec = UASD_FP64, device = 0;
+  hipStream_t own_stream = nullptr, stream = nullptr;
+  size_t rsize = 8;  // sizeof(R)
+
+  // host-side descriptors kept for class building / colouring
+  int W = 0, ex_rows = 0;
+  std::vector<int> ex_row_of_site;  // (N)
+  std::vector<int> nbr_host;        // (W*N) slot-major, 0-based
+  std::vector<int> ex_len_host;     // (ex_rows)
+  int Wd = 0, dm_rows = 0;
+  std::vector<int> dm_row_of_site, dnbr_host, dm_len_host;
+  std::vector<int> an_row_of_site;
+  int an_rows = 0;
+  int ext_mode = 0;
+  std::vector<double> ext_u_host;  // (3*M)
+
+  // device buffers
+  DevBuf nbr, ex_len, cpl;
+  DevBuf dnbr, dm_len, dv;
+  DevBuf an_type, an_vals;
+  DevBuf cls, classes;
+  DevBuf ext_u, ext_full;
+  DevBuf spins;   // e(3) p(3) b2(3) planes
+  DevBuf mm;      // mmom per q
+  DevBuf field;   // beff planes for the unfused path
+  DevBuf scratch; // emomM planes for host field calls
+  DevBuf noise;   // injected N(0,1) planes
+  DevBuf llgarr;  // lambda, temp, lande (double, N each)
+  DevBuf energy;  // per q energy terms
+  DevBuf partial, redout;
+  DevBuf mcbuf;   // per-ensemble delta/beta/hbeta/ext + colour site lists
+
+  bool have_ex = false, have_state = false, have_llg = false;
+  bool classes_dirty = true;
+
+  // LLG parameters
+  double dt = 1e-16, temprescale = 1.0, bn = 1.0;
+  double lam0 = 0, temp0 = 0, lande0 = 1;
+  bool lam_uniform = true, temp_uniform = true, lande_uniform = true;
+  int noise_mode = UASD_NOISE_NONE;
+  uint64_t seed = 0;
+  uint64_t step = 0;      // LLG step counter (Philox counter)
+  uint64_t mc_sweep = 0;  // MC sweep counter
+
+  // colouring
+  int ncolors = 0;
+  std::vector<int> color_of_site, color_ptr, color_sites;
+  bool coloring_dirty = true;
+
+  // Per-kernel timing: an event pair is recorded around each launch without
+  // a host wait; pairs are resolved when the totals are read.
+  bool timing = false;
+  Timing timers[4];
+  std::vector<hipEvent_t> ev_pool;  // free events
+  struct Pending {
+    hipEvent_t a, b;
+    int kind;
+  };
+  std::vector<Pending> pending;
+  hipEvent_t cur = nullptr;
+};
+
+namespace {
+
+size_t plane_bytes(const uasd_ctx *c) { return (size_t)c->N * c->M * c->rsize; }
+
+template <typename R>
+R *plane(const uasd_ctx *c, const DevBuf &b, int k) {
+  return reinterpret_cast<R *>(static_cast<char *>(b.p) + (size_t)k * plane_bytes(c));
+}
+
+template <typename R>
+uasd::Spins<R> spins_at(const uasd_ctx *c, int first_plane) {
+  uasd::Spins<R> s;
+  s.x = plane<R>(c, c->spins, first_plane);
+  s.y = plane<R>(c, c->spins, first_plane + 1);
+  s.z = plane<R>(c, c->spins, first_plane + 2);
+  return s;
+}
+
+template <typename R>
+uasd::HamDev<R> ham_view(const uasd_ctx *c) {
+  uasd::HamDev<R> h{};
+  h.N = c->N;
+  h.M = c->M;
+  h.W = c->W;
+  h.ex_rows = c->ex_rows;
+  h.nbr = c->nbr.as<int>();
+  h.ex_len = c->ex_len.as<int>();
+  h.cpl = c->cpl.as<R>();
+  h.Wd = c->Wd;
+  h.dm_rows = c->dm_rows;
+  h.dnbr = c->dnbr.as<int>();
+  h.dm_len = c->dm_len.as<int>();
+  if (c->Wd > 0) {
+    size_t n = (size_t)c->Wd * c->dm_rows;
+    h.dvx = c->dv.as<R>();
+    h.dvy = h.dvx + n;
+    h.dvz = h.dvy + n;
+  }
+  if (c->an_rows > 0) {
+    h.an_type = c->an_type.as<int>();
+    const R *v = c->an_vals.as<R>();
+    size_t n = (size_t)c->an_rows;
+    h.an_ex = v;
+    h.an_ey = v + n;
+    h.an_ez = v + 2 * n;
+    h.an_k1 = v + 3 * n;
+    h.an_k2 = v + 4 * n;
+    h.an_sb = v + 5 * n;
+  }
+  h.cls = c->cls.as<int>();
+  h.classes = c->classes.as<uasd::SiteClass>();
+  h.ext_mode = c->ext_mode;
+  h.ext_u = c->ext_u.as<R>();
+  if (c->ext_mode == 2) {
+    h.extx = c->ext_full.as<R>();
+    h.exty = h.extx + (size_t)c->N * c->M;
+    h.extz = h.exty + (size_t)c->N * c->M;
+  }
+  return h;
+}
+
+template <typename R>
+int upload_converted(hipStream_t s, DevBuf &dst, const std::vector<double> &src) {
+  std::vector<R> tmp(src.begin(), src.end());
+  HIP_TRY(dst.ensure(std::max<size_t>(1, tmp.size()) * sizeof(R)));
+  if (!tmp.empty())
+    HIP_TRY(hipMemcpyAsync(dst.p, tmp.data(), tmp.size() * sizeof(R), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return UASD_OK;
+}
+
+int upload_real(uasd_ctx *c, DevBuf &dst, const std::vector<double> &src) {
+  return c->prec == UASD_FP64 ? upload_converted<double>(c->stream, dst, src)
+                              : upload_converted<float>(c->stream, dst, src);
+}
+
+template <typename T>
+int upload_raw(uasd_ctx *c, DevBuf &dst, const std::vector<T> &src) {
+  HIP_TRY(dst.ensure(std::max<size_t>(1, src.size()) * sizeof(T)));
+  if (!src.empty())
+    HIP_TRY(hipMemcpyAsync(dst.p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice,
+                           c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return UASD_OK;
+}
+
+// (3,N,M) Fortran array -> three planes indexed site*M+ens
+std::vector<double> to_planes(const double *a, int N, int M) {
+  std::vector<double> out((size_t)3 * N * M);
+  const size_t P = (size_t)N * M;
+  for (int k = 0; k < M; ++k)
+    for (int i = 0; i < N; ++i) {
+      const double *v = a + 3 * ((size_t)i + (size_t)N * k);
+      size_t q = (size_t)i * M + k;
+      out[q] = v[0];
+      out[P + q] = v[1];
+      out[2 * P + q] = v[2];
+    }
+  return out;
+}
+
+template <typename R>
+int download_planes(uasd_ctx *c, const DevBuf &b, int first_plane, std::vector<double> &out) {
+  const size_t P = (size_t)c->N * c->M;
+  std::vector<R> tmp(3 * P);
+  HIP_TRY(hipMemcpyAsync(tmp.data(), plane<R>(c, b, first_plane), 3 * P * sizeof(R),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  out.assign(tmp.begin(), tmp.end());
+  return UASD_OK;
+}
+
+// planes (3*P) -> Fortran (3,N,M); when m != null multiply by mmom (emomM)
+void from_planes(const std::vector<double> &pl, int N, int M, double *out,
+                 const std::vector<double> *m) {
+  const size_t P = (size_t)N * M;
+  for (int k = 0; k < M; ++k)
+    for (int i = 0; i < N; ++i) {
+      size_t q = (size_t)i * M + k;
+      double *v = out + 3 * ((size_t)i + (size_t)N * k);
+      for (int d = 0; d < 3; ++d) v[d] = m ? pl[d * P + q] * (*m)[q] : pl[d * P + q];
+    }
+}
+
+int rebuild_classes(uasd_ctx *c) {
+  if (!c->classes_dirty) return UASD_OK;
+  std::map<std::tuple<int, int, int>, int> ids;
+  std::vector<int> cls(c->N);
+  std::vector<uasd::SiteClass> table;
+  for (int i = 0; i < c->N; ++i) {
+    int er = c->ex_row_of_site.empty() ? 0 : c->ex_row_of_site[i];
+    int dr = c->dm_row_of_site.empty() ? -1 : c->dm_row_of_site[i];
+    int ar = c->an_row_of_site.empty() ? -1 : c->an_row_of_site[i];
+    auto key = std::make_tuple(er, dr, ar);
+    auto it = ids.find(key);
+    if (it == ids.end()) {
+      it = ids.emplace(key, (int)table.size()).first;
+      table.push_back(uasd::SiteClass{er, dr, ar, 0});
+    }
+    cls[i] = it->second;
+  }
+  int r = upload_raw(c, c->cls, cls);
+  if (r) return r;
+  r = upload_raw(c, c->classes, table);
+  if (r) return r;
+  c->classes_dirty = false;
+  return UASD_OK;
+}
+
+// Distinct rows of a per-site table.  row_ref[i] is the Hamiltonian row of
+// site i; `len` and `vals(width*hdim)` describe that row.
+struct RowSet {
+  std::vector<int> row_of_site;
+  std::vector<int> len;
+  std::vector<std::vector<double>> vals;
+};
+
+RowSet dedupe_rows(int N, const std::vector<int> &row_ref, const int *lenarr, const double *vals,
+                   int width, int hdim) {
+  RowSet rs;
+  rs.row_of_site.resize(N);
+  std::unordered_map<int, int> by_ref;  // Hamiltonian row -> distinct row
+  std::unordered_map<RowKey, int, RowKeyHash> by_content;
+  for (int i = 0; i < N; ++i) {
+    int ref = row_ref[i];
+    auto f = by_ref.find(ref);
+    if (f != by_ref.end()) {
+      rs.row_of_site[i] = f->second;
+      continue;
+    }
+    int L = lenarr[ref];
+    RowKey key;
+    key.bytes.assign(reinterpret_cast<const char *>(&L), sizeof(int));
+    const double *v = vals + (size_t)ref * width * hdim;
+    key.bytes.append(reinterpret_cast<const char *>(v), sizeof(double) * (size_t)L * hdim);
+    auto g = by_content.find(key);
+    int id;
+    if (g == by_content.end()) {
+      id = (int)rs.len.size();
+      by_content.emplace(std::move(key), id);
+      rs.len.push_back(L);
+      rs.vals.emplace_back(v, v + (size_t)L * hdim);
+    } else {
+      id = g->second;
+    }
+    by_ref.emplace(ref, id);
+    rs.row_of_site[i] = id;
+  }
+  return rs;
+}
+
+int check_ctx(uasd_ctx *c) {
+  if (!c) return fail(UASD_ERR_ARG, "null context");
+  return UASD_OK;
+}
+
+hipEvent_t take_event(uasd_ctx *c) {
+  if (!c->ev_pool.empty()) {
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void tic(uasd_ctx *c) {
+  if (!c->timing) return;
+  c->cur = take_event(c);
+  if (c->cur) hipEventRecord(c->cur, c->stream);
+}
+
+int toc(uasd_ctx *c, int kind) {
+  if (!c->timing || !c->cur) return UASD_OK;
+  hipEvent_t b = take_event(c);
+  if (!b) return fail(UASD_ERR_HIP, "hipEventCreate failed");
+  HIP_TRY(hipEventRecord(b, c->stream));
+  c->pending.push_back({c->cur, b, kind});
+  c->cur = nullptr;
+  return UASD_OK;
+}
+
+int resolve_timing(uasd_ctx *c) {
+  for (auto &p : c->pending) {
+    HIP_TRY(hipEventSynchronize(p.b));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, p.a, p.b));
+    c->timers[p.kind].launches += 1;
+    c->timers[p.kind].ms += ms;
+    c->ev_pool.push_back(p.a);
+    c->ev_pool.push_back(p.b);
+  }
+  c->pending.clear();
+  return UASD_OK;
+}
+
+template <typename R>
+uasd::LLGDev<R> llg_view(uasd_ctx *c) {
+  uasd::LLGDev<R> p{};
+  const double *arr = c->llgarr.as<double>();
+  p.lam = c->lam_uniform ? nullptr : arr;
+  p.temp = c->temp_uniform ? nullptr : arr + c->N;
+  p.lande = c->lande_uniform ? nullptr : arr + 2 * (size_t)c->N;
+  p.lam0 = c->lam0;
+  p.temp0 = c->temp0;
+  p.lande0 = c->lande0;
+  p.dt = c->dt;
+  p.temprescale = c->temprescale;
+  p.bn = c->bn;
+  p.noise_mode = c->noise_mode;
+  p.seed = c->seed;
+  p.step = c->step;
+  if (c->noise_mode == UASD_NOISE_INJECTED) {
+    p.nx = plane<R>(c, c->noise, 0);
+    p.ny = plane<R>(c, c->noise, 1);
+    p.nz = plane<R>(c, c->noise, 2);
+  }
+  return p;
+}
+
+int ready_for_llg(uasd_ctx *c) {
+  if (!c->have_ex) return fail(UASD_ERR_STATE, "exchange not set (uasd_set_exchange)");
+  if (!c->have_state) return fail(UASD_ERR_STATE, "moments not set (uasd_set_moments)");
+  if (!c->have_llg) return fail(UASD_ERR_STATE, "LLG parameters not set (uasd_set_llg)");
+  if (c->noise_mode == UASD_NOISE_INJECTED && !c->noise.p)
+    return fail(UASD_ERR_STATE, "injected noise selected but no normals uploaded");
+  return rebuild_classes(c);
+}
+
+// Planes in c->spins: 0-2 e, 3-5 p (emom2), 6-8 b2 (b2eff).
+template <typename R>
+int run_llg_half(uasd_ctx *c, int alg, int half, bool fused, bool write_e) {
+  uasd::HamDev<R> h = ham_view<R>(c);
+  uasd::LLGDev<R> prm = llg_view<R>(c);
+  uasd::Spins<R> e = spins_at<R>(c, 0), p = spins_at<R>(c, 3), b2 = spins_at<R>(c, 6);
+  const R *bx = nullptr, *by = nullptr, *bz = nullptr;
+  if (!fused) {
+    bx = plane<R>(c, c->field, 0);
+    by = plane<R>(c, c->field, 1);
+    bz = plane<R>(c, c->field, 2);
+  }
+  uasd::Spins<R> out = write_e ? e : p;
+  tic(c);
+  HIP_TRY(uasd::launch_llg<R>(c->stream, alg, half, fused, h, prm, e, p, b2, c->mm.as<R>(), bx,
+                              by, bz, out));
+  return toc(c, half);
+}
+
+template <typename R>
+int run_field(uasd_ctx *c, bool from_scratch, bool want_energy) {
+  uasd::HamDev<R> h = ham_view<R>(c);
+  uasd::MomSrc<R> src;
+  if (from_scratch) {
+    src.x = plane<R>(c, c->scratch, 0);
+    src.y = plane<R>(c, c->scratch, 1);
+    src.z = plane<R>(c, c->scratch, 2);
+    src.m = nullptr;
+  } else {
+    src.x = plane<R>(c, c->spins, 0);
+    src.y = plane<R>(c, c->spins, 1);
+    src.z = plane<R>(c, c->spins, 2);
+    src.m = c->mm.as<R>();
+  }
+  HIP_TRY(c->field.ensure(3 * plane_bytes(c)));
+  double *en = nullptr;
+  if (want_energy) {
+    HIP_TRY(c->energy.ensure((size_t)c->N * c->M * sizeof(double)));
+    en = c->energy.as<double>();
+  }
+  tic(c);
+  HIP_TRY(uasd::launch_field<R>(c->stream, h, src, plane<R>(c, c->field, 0),
+                                plane<R>(c, c->field, 1), plane<R>(c, c->field, 2), en));
+  return toc(c, 2);
+}
+
+// Energy sum per ensemble of the per-q terms written by the field kernel.
+template <typename R>
+int reduce_moments(uasd_ctx *c, bool with_energy, bool from_scratch, std::vector<double> &out) {
+  const int nv = with_energy ? 4 : 3;
+  const int nblocks = std::max(1, std::min(1024, (c->N + 4095) / 4096));
+  HIP_TRY(c->partial.ensure((size_t)c->M * nblocks * nv * sizeof(double)));
+  HIP_TRY(c->redout.ensure((size_t)c->M * nv * sizeof(double)));
+  uasd::Spins<R> e;
+  const R *mm;
+  if (from_scratch) {
+    e.x = plane<R>(c, c->scratch, 0);
+    e.y = plane<R>(c, c->scratch, 1);
+    e.z = plane<R>(c, c->scratch, 2);
+    mm = nullptr;
+  } else {
+    e = spins_at<R>(c, 0);
+    mm = c->mm.as<R>();
+  }
+  if (!mm) {
+    // emomM planes given directly: reduce against a unit moment
+    HIP_TRY(c->mcbuf.ensure(plane_bytes(c)));
+    std::vector<R> ones((size_t)c->N * c->M, (R)1);
+    HIP_TRY(hipMemcpyAsync(c->mcbuf.p, ones.data(), ones.size() * sizeof(R),
+                           hipMemcpyHostToDevice, c->stream));
+    mm = c->mcbuf.as<R>();
+  }
+  HIP_TRY(uasd::launch_reduce<R>(c->stream, c->N, c->M, e, mm,
+                                 with_energy ? c->energy.as<double>() : nullptr, nv,
+                                 c->partial.as<double>(), nblocks, c->redout.as<double>()));
+  out.resize((size_t)c->M * nv);
+  HIP_TRY(hipMemcpyAsync(out.data(), c->redout.p, out.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return UASD_OK;
+}
+
+int build_coloring(uasd_ctx *c) {
+  if (!c->coloring_dirty) return UASD_OK;
+  const int N = c->N;
+  // undirected bond graph from exchange and DM lists
+  std::vector<std::vector<int>> adj(N);
+  auto add_list = [&](const std::vector<int> &row_of_site, const std::vector<int> &len,
+                      const std::vector<int> &nb, int W) {
+    if (W == 0 || row_of_site.empty()) return;
+    for (int i = 0; i < N; ++i) {
+      int L = len[row_of_site[i]];
+      for (int j = 0; j < L; ++j) {
+        int n = nb[(size_t)j * N + i];
+        if (n == i) continue;
+        adj[i].push_back(n);
+        adj[n].push_back(i);
+      }
+    }
+  };
+  add_list(c->ex_row_of_site, c->ex_len_host, c->nbr_host, c->W);
+  add_list(c->dm_row_of_site, c->dm_len_host, c->dnbr_host, c->Wd);
+  std::vector<int> col(N, -1);
+  int ncol = 0;
+  std::vector<int> mark;
+  for (int i = 0; i < N; ++i) {
+    for (int n : adj[i])
+      if (col[n] >= 0) {
+        if ((int)mark.size() <= col[n]) mark.resize(col[n] + 1, -1);
+        mark[col[n]] = i;
+      }
+    int k = 0;
+    while (k < (int)mark.size() && mark[k] == i) ++k;
+    col[i] = k;
+    ncol = std::max(ncol, k + 1);
+  }
+  for (int i = 0; i < N; ++i)
+    for (int n : adj[i])
+      if (col[n] == col[i]) return fail(UASD_ERR_ARG, "colouring failed (self-bond?)");
+  c->ncolors = ncol;
+  c->color_of_site = col;
+  c->color_ptr.assign(ncol + 1, 0);
+  for (int i = 0; i < N; ++i) c->color_ptr[col[i] + 1]++;
+  for (int k = 0; k < ncol; ++k) c->color_ptr[k + 1] += c->color_ptr[k];
+  c->color_sites.assign(N, 0);
+  std::vector<int> fillp(c->color_ptr.begin(), c->color_ptr.end() - 1);
+  for (int i = 0; i < N; ++i) c->color_sites[fillp[col[i]]++] = i;
+  c->coloring_dirty = false;
+  return UASD_OK;
+}
+
+}  // namespace
+
+// =================================================================== C-ABI
+extern "C" {
+
+const char *uasd_last_error(void) { return g_err.c_str(); }
+
+int uasd_version(void) { return UASD_API_VERSION; }
+
+int uasd_device_count(int *count) {
+  if (!count) return fail(UASD_ERR_ARG, "count is null");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  *count = n;
+  return UASD_OK;
+}
+
+int uasd_ctx_create(uasd_ctx **out, int natom, int mensemble, int precision) {
+  if (!out) return fail(UASD_ERR_ARG, "out is null");
+  *out = nullptr;
+  if (natom <= 0 || mensemble <= 0) return fail(UASD_ERR_ARG, "natom and mensemble must be > 0");
+  if ((int64_t)natom * mensemble > (int64_t)1 << 31)
+    return fail(UASD_ERR_ARG, "natom*mensemble exceeds 2^31");
+  if (precision != UASD_FP64 && precision != UASD_FP32)
+    return fail(UASD_ERR_ARG, "precision must be UASD_FP64 or UASD_FP32");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(UASD_ERR_NODEVICE, "no HIP device available");
+  auto c = std::make_unique<uasd_ctx>();
+  c->N = natom;
+  c->M = mensemble;
+  c->prec = precision;
+  c->rsize = precision == UASD_FP64 ? 8 : 4;
+  HIP_TRY(hipGetDevice(&c->device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, c->device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(UASD_ERR_NODEVICE, std::string("device is ") + prop.gcnArchName +
+                                       ", the library is built for gfx950");
+  HIP_TRY(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+  c->stream = c->own_stream;
+  HIP_TRY(c->spins.ensure(9 * plane_bytes(c.get())));
+  HIP_TRY(hipMemsetAsync(c->spins.p, 0, 9 * plane_bytes(c.get()), c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  *out = c.release();
+  return UASD_OK;
+}
+
+int uasd_ctx_destroy(uasd_ctx *c) {
+  if (!c) return UASD_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  resolve_timing(c);
+  if (c->cur) hipEventDestroy(c->cur);
+  for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+  if (c->own_stream) hipStreamDestroy(c->own_stream);
+  delete c;
+  return UASD_OK;
+}
+
+int uasd_ctx_set_stream(uasd_ctx *c, void *s) {
+  if (int r = check_ctx(c)) return r;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->stream = s ? static_cast<hipStream_t>(s) : c->own_stream;
+  return UASD_OK;
+}
+
+void *uasd_ctx_stream(uasd_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int uasd_ctx_sync(uasd_ctx *c) {
+  if (int r = check_ctx(c)) return r;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return UASD_OK;
+}
+
+int uasd_set_exchange(uasd_ctx *c, int mnn, int nham, const int *nlist, const int *nlistsize,
+                      const double *ncoup, const int *aham) {
+  if (int r = check_ctx(c)) return r;
+  const int N = c->N;
+  if (mnn < 0 || nham <= 0 || !nlist || !nlistsize || (mnn > 0 && !ncoup))
+    return fail(UASD_ERR_ARG, "uasd_set_exchange: bad arguments");
+  std::vector<int> ref(N);
+  for (int i = 0; i < N; ++i) {
+    int r = aham ? aham[i] - 1 : i;
+    if (r < 0 || r >= nham) return fail(UASD_ERR_ARG, "aham entry out of range");
+    if (nlistsize[r] < 0 || nlistsize[r] > mnn) return fail(UASD_ERR_ARG, "nlistsize > max_no_neigh");
+    ref[i] = r;
+  }
+  RowSet rs = dedupe_rows(N, ref, nlistsize, ncoup, mnn, 1);
+  int W = 0;
+  for (int L : rs.len) W = std::max(W, L);
+  std::vector<int> nb((size_t)std::max(W, 1) * N);
+  for (int i = 0; i < N; ++i) {
+    int L = rs.len[rs.row_of_site[i]];
+    for (int j = 0; j < W; ++j) {
+      int v = i;
+      if (j < L) {
+        v = nlist[(size_t)j + (size_t)mnn * i] - 1;
+        if (v < 0 || v >= N) return fail(UASD_ERR_ARG, "nlist entry out of range");
+      }
+      nb[(size_t)j * N + i] = v;
+    }
+  }
+  const int rows = (int)rs.len.size();
+  std::vector<double> cp((size_t)std::max(W, 1) * rows, 0.0);
+  for (int r = 0; r < rows; ++r)
+    for (int j = 0; j < rs.len[r]; ++j) cp[(size_t)j * rows + r] = rs.vals[r][j];
+  c->W = W;
+  c->ex_rows = rows;
+  c->ex_row_of_site = rs.row_of_site;
+  c->ex_len_host = rs.len;
+  c->nbr_host = nb;
+  if (int r = upload_raw(c, c->nbr, nb)) return r;
+  if (int r = upload_raw(c, c->ex_len, rs.len)) return r;
+  if (int r = upload_real(c, c->cpl, cp)) return r;
+  c->have_ex = true;
+  c->classes_dirty = true;
+  c->coloring_dirty = true;
+  return UASD_OK;
+}
+
+int uasd_set_dm(uasd_ctx *c, int mdm, int nham, const int *dmlist, const int *dmlistsize,
+                const double *dm_vect, const int *aham) {
+  if (int r = check_ctx(c)) return r;
+  const int N = c->N;
+  if (!dmlist || mdm <= 0) {  // clear
+    c->Wd = 0;
+    c->dm_rows = 0;
+    c->dm_row_of_site.clear();
+    c->classes_dirty = c->coloring_dirty = true;
+    return UASD_OK;
+  }
+  if (nham <= 0 || !dmlistsize || !dm_vect) return fail(UASD_ERR_ARG, "uasd_set_dm: bad arguments");
+  std::vector<int> ref(N);
+  for (int i = 0; i < N; ++i) {
+    int r = aham ? aham[i] - 1 : i;
+    if (r < 0 || r >= nham) return fail(UASD_ERR_ARG, "aham entry out of range");
+    if (dmlistsize[r] < 0 || dmlistsize[r] > mdm) return fail(UASD_ERR_ARG, "dmlistsize > max");
+    ref[i] = r;
+  }
+  RowSet rs = dedupe_rows(N, ref, dmlistsize, dm_vect, mdm, 3);
+  int W = 0;
+  for (int L : rs.len) W = std::max(W, L);
+  std::vector<int> nb((size_t)std::max(W, 1) * N);
+  for (int i = 0; i < N; ++i) {
+    int L = rs.len[rs.row_of_site[i]];
+    for (int j = 0; j < W; ++j) {
+      int v = i;
+      if (j < L) {
+        v = dmlist[(size_t)j + (size_t)mdm * i] - 1;
+        if (v < 0 || v >= N) return fail(UASD_ERR_ARG, "dmlist entry out of range");
+      }
+      nb[(size_t)j * N + i] = v;
+    }
+  }
+  const int rows = (int)rs.len.size();
+  const size_t n = (size_t)std::max(W, 1) * rows;
+  std::vector<double> dv(3 * n, 0.0);
+  for (int r = 0; r < rows; ++r)
+    for (int j = 0; j < rs.len[r]; ++j)
+      for (int d = 0; d < 3; ++d) dv[d * n + (size_t)j * rows + r] = rs.vals[r][3 * j + d];
+  c->Wd = W;
+  c->dm_rows = rows;
+  c->dm_row_of_site = rs.row_of_site;
+  c->dm_len_host = rs.len;
+  c->dnbr_host = nb;
+  if (int r = upload_raw(c, c->dnbr, nb)) return r;
+  if (int r = upload_raw(c, c->dm_len, rs.len)) return r;
+  if (int r = upload_real(c, c->dv, dv)) return r;
+  c->classes_dirty = c->coloring_dirty = true;
+  return UASD_OK;
+}
+
+int uasd_set_anisotropy(uasd_ctx *c, const int *taniso, const double *eaniso,
+                        const double *kaniso, const double *sb) {
+  if (int r = check_ctx(c)) return r;
+  const int N = c->N;
+  if (!taniso) {
+    c->an_rows = 0;
+    c->an_row_of_site.clear();
+    c->classes_dirty = true;
+    return UASD_OK;
+  }
+  if (!eaniso || !kaniso || !sb) return fail(UASD_ERR_ARG, "uasd_set_anisotropy: null array");
+  std::map<std::vector<double>, int> rows;
+  std::vector<int> row_of(N, -1);
+  std::vector<std::vector<double>> vals;
+  std::vector<int> types;
+  for (int i = 0; i < N; ++i) {
+    int t = taniso[i];
+    if (t != 0 && t != 1 && t != 2 && t != 7)
+      return fail(UASD_ERR_UNSUPPORTED, "anisotropy type must be 0, 1, 2 or 7");
+    if (t == 0) continue;
+    std::vector<double> key = {(double)t,     eaniso[3 * (size_t)i], eaniso[3 * (size_t)i + 1],
+                               eaniso[3 * (size_t)i + 2], kaniso[2 * (size_t)i],
+                               kaniso[2 * (size_t)i + 1], sb[i]};
+    auto it = rows.find(key);
+    if (it == rows.end()) {
+      it = rows.emplace(key, (int)vals.size()).first;
+      vals.push_back(key);
+      types.push_back(t);
+    }
+    row_of[i] = it->second;
+  }
+  const int nr = (int)vals.size();
+  c->an_rows = nr;
+  c->an_row_of_site = row_of;
+  if (nr > 0) {
+    std::vector<double> v(6 * (size_t)nr);
+    for (int r = 0; r < nr; ++r)
+      for (int k = 0; k < 6; ++k) v[(size_t)k * nr + r] = vals[r][1 + k];
+    if (int r = upload_raw(c, c->an_type, types)) return r;
+    if (int r = upload_real(c, c->an_vals, v)) return r;
+  }
+  c->classes_dirty = true;
+  return UASD_OK;
+}
+
+int uasd_set_external_field(uasd_ctx *c, const double *field) {
+  if (int r = check_ctx(c)) return r;
+  const int N = c->N, M = c->M;
+  if (!field) {
+    c->ext_mode = 0;
+    c->ext_u_host.assign(3 * (size_t)M, 0.0);
+    return upload_real(c, c->ext_u, c->ext_u_host);
+  }
+  bool uniform = true, zero = true;
+  for (int k = 0; k < M && uniform; ++k) {
+    const double *f0 = field + 3 * ((size_t)N * k);
+    for (int i = 0; i < N && uniform; ++i) {
+      const double *f = field + 3 * ((size_t)i + (size_t)N * k);
+      for (int d = 0; d < 3; ++d) {
+        if (std::memcmp(&f[d], &f0[d], sizeof(double)) != 0) uniform = false;
+        if (f[d] != 0.0) zero = false;
+      }
+    }
+  }
+  c->ext_u_host.assign(3 * (size_t)M, 0.0);
+  if (uniform) {
+    for (int k = 0; k < M; ++k)
+      for (int d = 0; d < 3; ++d) c->ext_u_host[3 * k + d] = field[3 * ((size_t)N * k) + d];
+    c->ext_mode = zero ? 0 : 1;
+    return upload_real(c, c->ext_u, c->ext_u_host);
+  }
+  c->ext_mode = 2;
+  if (int r = upload_real(c, c->ext_u, c->ext_u_host)) return r;
+  return upload_real(c, c->ext_full, to_planes(field, N, M));
+}
+
+int uasd_set_llg(uasd_ctx *c, const double *lambda1, const double *temp, const double *landeg,
+                 double delta_t, double temprescale, double bn) {
+  if (int r = check_ctx(c)) return r;
+  const int N = c->N;
+  if (!lambda1 || !temp) return fail(UASD_ERR_ARG, "uasd_set_llg: lambda1 and temp required");
+  if (!(delta_t > 0.0)) return fail(UASD_ERR_ARG, "delta_t must be > 0");
+  auto is_uniform = [N](const double *a) {
+    for (int i = 1; i < N; ++i)
+      if (std::memcmp(&a[i], &a[0], sizeof(double)) != 0) return false;
+    return true;
+  };
+  std::vector<double> arr(3 * (size_t)N);
+  for (int i = 0; i < N; ++i) {
+    arr[i] = lambda1[i];
+    arr[N + i] = temp[i];
+    arr[2 * (size_t)N + i] = landeg ? landeg[i] : 1.0;
+  }
+  c->lam_uniform = is_uniform(lambda1);
+  c->temp_uniform = is_uniform(temp);
+  c->lande_uniform = landeg ? is_uniform(landeg) : true;
+  c->lam0 = lambda1[0];
+  c->temp0 = temp[0];
+  c->lande0 = landeg ? landeg[0] : 1.0;
+  c->dt = delta_t;
+  c->temprescale = temprescale;
+  c->bn = bn;
+  if (int r = upload_raw(c, c->llgarr, arr)) return r;
+  c->have_llg = true;
+  return UASD_OK;
+}
+
+int uasd_set_noise(uasd_ctx *c, int mode, uint64_t seed, const double *normals) {
+  if (int r = check_ctx(c)) return r;
+  if (mode != UASD_NOISE_NONE && mode != UASD_NOISE_PHILOX && mode != UASD_NOISE_INJECTED)
+    return fail(UASD_ERR_ARG, "unknown noise mode");
+  if (mode == UASD_NOISE_INJECTED) {
+    if (!normals) return fail(UASD_ERR_ARG, "injected noise needs the normals array");
+    if (int r = upload_real(c, c->noise, to_planes(normals, c->N, c->M))) return r;
+  }
+  if (mode == UASD_NOISE_PHILOX && (seed != c->seed || c->noise_mode != mode)) c->step = 0;
+  c->noise_mode = mode;
+  c->seed = seed;
+  return UASD_OK;
+}
+
+int uasd_set_moments(uasd_ctx *c, const double *emom, const double *mmom) {
+  if (int r = check_ctx(c)) return r;
+  if (!emom || !mmom) return fail(UASD_ERR_ARG, "uasd_set_moments: null array");
+  const int N = c->N, M = c->M;
+  std::vector<double> pl = to_planes(emom, N, M);
+  std::vector<double> mq((size_t)N * M);
+  for (int k = 0; k < M; ++k)
+    for (int i = 0; i < N; ++i) mq[(size_t)i * M + k] = mmom[(size_t)i + (size_t)N * k];
+  const size_t P = (size_t)N * M;
+  if (c->prec == UASD_FP64) {
+    HIP_TRY(hipMemcpyAsync(c->spins.p, pl.data(), 3 * P * 8, hipMemcpyHostToDevice, c->stream));
+  } else {
+    std::vector<float> f(pl.begin(), pl.end());
+    HIP_TRY(hipMemcpyAsync(c->spins.p, f.data(), 3 * P * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (int r = upload_real(c, c->mm, mq)) return r;
+  c->have_state = true;
+  return UASD_OK;
+}
+
+static int get_mm_host(uasd_ctx *c, std::vector<double> &mq) {
+  const size_t P = (size_t)c->N * c->M;
+  if (c->prec == UASD_FP64) {
+    mq.resize(P);
+    HIP_TRY(hipMemcpyAsync(mq.data(), c->mm.p, P * 8, hipMemcpyDeviceToHost, c->stream));
+  } else {
+    std::vector<float> f(P);
+    HIP_TRY(hipMemcpyAsync(f.data(), c->mm.p, P * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    mq.assign(f.begin(), f.end());
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return UASD_OK;
+}
+
+int uasd_get_moments(uasd_ctx *c, double *emom, double *emomM) {
+  if (int r = check_ctx(c)) return r;
+  if (!c->have_state) return fail(UASD_ERR_STATE, "moments not set");
+  std::vector<double> pl, mq;
+  int r = c->prec == UASD_FP64 ? download_planes<double>(c, c->spins, 0, pl)
+                               : download_planes<float>(c, c->spins, 0, pl);
+  if (r) return r;
+  if (emom) from_planes(pl, c->N, c->M, emom, nullptr);
+  if (emomM) {
+    if ((r = get_mm_host(c, mq))) return r;
+    from_planes(pl, c->N, c->M, emomM, &mq);
+  }
+  return UASD_OK;
+}
+
+// ---------------------------------------------------------- drop-in (host)
+static int upload_planes_to(uasd_ctx *c, DevBuf &b, int first_plane, const double *a) {
+  std::vector<double> pl = to_planes(a, c->N, c->M);
+  const size_t P = (size_t)c->N * c->M;
+  HIP_TRY(b.ensure(std::max(b.bytes, (size_t)(first_plane + 3) * plane_bytes(c))));
+  if (c->prec == UASD_FP64) {
+    HIP_TRY(hipMemcpyAsync(plane<double>(c, b, first_plane), pl.data(), 3 * P * 8,
+                           hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> f(pl.begin(), pl.end());
+    HIP_TRY(hipMemcpyAsync(plane<float>(c, b, first_plane), f.data(), 3 * P * 4,
+                           hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  return UASD_OK;
+}
+
+static int download_planes_to(uasd_ctx *c, const DevBuf &b, int first_plane, double *a,
+                              bool times_mmom) {
+  std::vector<double> pl, mq;
+  int r = c->prec == UASD_FP64 ? download_planes<double>(c, b, first_plane, pl)
+                               : download_planes<float>(c, b, first_plane, pl);
+  if (r) return r;
+  if (times_mmom && (r = get_mm_host(c, mq))) return r;
+  from_planes(pl, c->N, c->M, a, times_mmom ? &mq : nullptr);
+  return UASD_OK;
+}
+
+int uasd_effective_field_host(uasd_ctx *c, const double *emomM, double *beff, double *energy) {
+  if (int r = check_ctx(c)) return r;
+  if (!c->have_ex) return fail(UASD_ERR_STATE, "exchange not set");
+  if (!emomM || !beff) return fail(UASD_ERR_ARG, "null array");
+  if (int r = rebuild_classes(c)) return r;
+  if (int r = upload_planes_to(c, c->scratch, 0, emomM)) return r;
+  int r = c->prec == UASD_FP64 ? run_field<double>(c, true, energy != nullptr)
+                               : run_field<float>(c, true, energy != nullptr);
+  if (r) return r;
+  if ((r = download_planes_to(c, c->field, 0, beff, false))) return r;
+  if (energy) {
+    std::vector<double> red;
+    r = c->prec == UASD_FP64 ? reduce_moments<double>(c, true, true, red)
+                             : reduce_moments<float>(c, true, true, red);
+    if (r) return r;
+    double e = 0.0;
+    for (int k = 0; k < c->M; ++k) e += red[4 * (size_t)k + 3];
+    *energy = e * uasd::kMub / uasd::kMry;
+  }
+  return UASD_OK;
+}
+
+int uasd_evolve_first_host(uasd_ctx *c, int alg, const double *beff, double *emom, double *emom2,
+                           double *emomM, double *b2eff) {
+  if (int r = check_ctx(c)) return r;
+  if (alg != UASD_SDE_MIDPOINT && alg != UASD_SDE_DEPONDT)
+    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint) or 5 (Depondt)");
+  if (!beff || !emom || !emom2 || !emomM || (alg == 5 && !b2eff))
+    return fail(UASD_ERR_ARG, "null array");
+  if (int r = ready_for_llg(c)) return r;
+  if (int r = upload_planes_to(c, c->field, 0, beff)) return r;
+  if (int r = upload_planes_to(c, c->spins, 0, emom)) return r;
+  int r = c->prec == UASD_FP64 ? run_llg_half<double>(c, alg, 0, false, false)
+                               : run_llg_half<float>(c, alg, 0, false, false);
+  if (r) return r;
+  if (alg == UASD_SDE_DEPONDT) {
+    // depondt.f90:182-187: emom2 = m(t), emom = emomM/mmom = rotated, b2eff = bdup
+    std::memcpy(emom2, emom, sizeof(double) * 3 * (size_t)c->N * c->M);
+    if ((r = download_planes_to(c, c->spins, 3, emom, false))) return r;
+    if ((r = download_planes_to(c, c->spins, 3, emomM, true))) return r;
+    if ((r = download_planes_to(c, c->spins, 6, b2eff, false))) return r;
+  } else {
+    // midpoint.f90:167-175: emom2 = et', emomM = et'*mmom, emom unchanged
+    if ((r = download_planes_to(c, c->spins, 3, emom2, false))) return r;
+    if ((r = download_planes_to(c, c->spins, 3, emomM, true))) return r;
+  }
+  return UASD_OK;
+}
+
+int uasd_evolve_second_host(uasd_ctx *c, int alg, const double *beff, const double *b2eff,
+                            double *emom, double *emom2) {
+  if (int r = check_ctx(c)) return r;
+  if (alg != UASD_SDE_MIDPOINT && alg != UASD_SDE_DEPONDT)
+    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint) or 5 (Depondt)");
+  if (!beff || !emom || !emom2 || (alg == 5 && !b2eff)) return fail(UASD_ERR_ARG, "null array");
+  if (int r = ready_for_llg(c)) return r;
+  if (int r = upload_planes_to(c, c->field, 0, beff)) return r;
+  if (alg == UASD_SDE_DEPONDT) {
+    // reference on entry: emom = predicted, emom2 = m(t)
+    if (int r = upload_planes_to(c, c->spins, 3, emom)) return r;
+    if (int r = upload_planes_to(c, c->spins, 0, emom2)) return r;
+    if (int r = upload_planes_to(c, c->spins, 6, b2eff)) return r;
+  } else {
+    if (int r = upload_planes_to(c, c->spins, 0, emom)) return r;
+    if (int r = upload_planes_to(c, c->spins, 3, emom2)) return r;
+  }
+  int r = c->prec == UASD_FP64 ? run_llg_half<double>(c, alg, 1, false, false)
+                               : run_llg_half<float>(c, alg, 1, false, false);
+  if (r) return r;
+  if (alg == UASD_SDE_DEPONDT) std::memcpy(emom, emom2, sizeof(double) * 3 * (size_t)c->N * c->M);
+  if ((r = download_planes_to(c, c->spins, 3, emom2, false))) return r;
+  c->step += 1;
+  return UASD_OK;
+}
+
+// ----------------------------------------------------------- resident path
+int uasd_sd_steps(uasd_ctx *c, int alg, int64_t nsteps) {
+  if (int r = check_ctx(c)) return r;
+  if (alg != UASD_SDE_MIDPOINT && alg != UASD_SDE_DEPONDT)
+    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint) or 5 (Depondt)");
+  if (nsteps < 0) return fail(UASD_ERR_ARG, "nsteps < 0");
+  if (int r = ready_for_llg(c)) return r;
+  for (int64_t s = 0; s < nsteps; ++s) {
+    int r = c->prec == UASD_FP64 ? run_llg_half<double>(c, alg, 0, true, false)
+                                 : run_llg_half<float>(c, alg, 0, true, false);
+    if (r) return r;
+    r = c->prec == UASD_FP64 ? run_llg_half<double>(c, alg, 1, true, true)
+                             : run_llg_half<float>(c, alg, 1, true, true);
+    if (r) return r;
+    c->step += 1;
+  }
+  return UASD_OK;
+}
+
+int uasd_effective_field(uasd_ctx *c, double *beff, double *energy) {
+  if (int r = check_ctx(c)) return r;
+  if (!c->have_ex || !c->have_state) return fail(UASD_ERR_STATE, "exchange or moments not set");
+  if (int r = rebuild_classes(c)) return r;
+  int r = c->prec == UASD_FP64 ? run_field<double>(c, false, energy != nullptr)
+                               : run_field<float>(c, false, energy != nullptr);
+  if (r) return r;
+  if (beff && (r = download_planes_to(c, c->field, 0, beff, false))) return r;
+  if (energy) {
+    std::vector<double> red;
+    r = c->prec == UASD_FP64 ? reduce_moments<double>(c, true, false, red)
+                             : reduce_moments<float>(c, true, false, red);
+    if (r) return r;
+    double e = 0.0;
+    for (int k = 0; k < c->M; ++k) e += red[4 * (size_t)k + 3];
+    *energy = e * uasd::kMub / uasd::kMry;
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return UASD_OK;
+}
+
+int uasd_mc_sweeps(uasd_ctx *c, char mode, const double *temps, int64_t nsweeps, uint64_t seed) {
+  if (int r = check_ctx(c)) return r;
+  if (mode != 'M' && mode != 'H') return fail(UASD_ERR_UNSUPPORTED, "mode must be 'M' or 'H'");
+  if (!temps) return fail(UASD_ERR_ARG, "temps is null");
+  if (!c->have_ex || !c->have_state) return fail(UASD_ERR_STATE, "exchange or moments not set");
+  if (c->ext_mode == 2)
+    return fail(UASD_ERR_UNSUPPORTED, "MC takes a uniform external field (as mc_evolve's extfield)");
+  if (int r = rebuild_classes(c)) return r;
+  if (int r = build_coloring(c)) return r;
+  const int M = c->M;
+  for (int k = 0; k < M; ++k)
+    if (!(temps[k] > 0.0)) return fail(UASD_ERR_ARG, "MC temperatures must be > 0");
+  // per-ensemble constants (montecarlo.f90:141, montecarlo_common.f90:190, 397)
+  std::vector<double> host(6 * (size_t)M);
+  for (int k = 0; k < M; ++k) {
+    host[k] = (double)(2.0f / 25.0f) * std::pow(uasd::kKBolt * temps[k] / uasd::kMub, 0.20);
+    host[M + k] = 1.0 / uasd::kKBolt / (c->temprescale * temps[k] + 1.0e-15);
+    host[2 * (size_t)M + k] = 1.0 / uasd::kKBolt / (c->temprescale * temps[k]);
+  }
+  for (int k = 0; k < M; ++k)
+    for (int d = 0; d < 3; ++d)
+      host[3 * (size_t)M + 3 * k + d] = c->ext_u_host.empty() ? 0.0 : c->ext_u_host[3 * k + d];
+  const size_t cbytes = host.size() * sizeof(double);
+  HIP_TRY(c->mcbuf.ensure(cbytes + c->color_sites.size() * sizeof(int)));
+  HIP_TRY(hipMemcpyAsync(c->mcbuf.p, host.data(), cbytes, hipMemcpyHostToDevice, c->stream));
+  int *sites_dev = reinterpret_cast<int *>(static_cast<char *>(c->mcbuf.p) + cbytes);
+  HIP_TRY(hipMemcpyAsync(sites_dev, c->color_sites.data(), c->color_sites.size() * sizeof(int),
+                         hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  uasd::MCParams mp{};
+  const double *base = c->mcbuf.as<double>();
+  mp.delta = base;
+  mp.beta = base + M;
+  mp.hbeta = base + 2 * (size_t)M;
+  mp.ext = base + 3 * (size_t)M;
+  mp.seed = seed;
+  mp.heat_bath = mode == 'H';
+  for (int64_t s = 0; s < nsweeps; ++s) {
+    mp.sweep = c->mc_sweep;
+    for (int col = 0; col < c->ncolors; ++col) {
+      mp.sites = sites_dev + c->color_ptr[col];
+      mp.nsites = c->color_ptr[col + 1] - c->color_ptr[col];
+      tic(c);
+      if (c->prec == UASD_FP64) {
+        HIP_TRY(uasd::launch_mc<double>(c->stream, ham_view<double>(c), mp, spins_at<double>(c, 0),
+                                        c->mm.as<double>()));
+      } else {
+        HIP_TRY(uasd::launch_mc<float>(c->stream, ham_view<float>(c), mp, spins_at<float>(c, 0),
+                                       c->mm.as<float>()));
+      }
+      if (int r = toc(c, 3)) return r;
+    }
+    c->mc_sweep += 1;
+  }
+  return UASD_OK;
+}
+
+int uasd_measure(uasd_ctx *c, double *msum, double *energy) {
+  if (int r = check_ctx(c)) return r;
+  if (!c->have_state) return fail(UASD_ERR_STATE, "moments not set");
+  if (!msum) return fail(UASD_ERR_ARG, "msum is null");
+  if (energy) {
+    if (!c->have_ex) return fail(UASD_ERR_STATE, "exchange not set");
+    if (int r = rebuild_classes(c)) return r;
+    int r = c->prec == UASD_FP64 ? run_field<double>(c, false, true)
+                                 : run_field<float>(c, false, true);
+    if (r) return r;
+  }
+  std::vector<double> red;
+  int r = c->prec == UASD_FP64 ? reduce_moments<double>(c, energy != nullptr, false, red)
+                               : reduce_moments<float>(c, energy != nullptr, false, red);
+  if (r) return r;
+  const int nv = energy ? 4 : 3;
+  for (int k = 0; k < c->M; ++k) {
+    for (int d = 0; d < 3; ++d) msum[3 * k + d] = red[(size_t)nv * k + d];
+    if (energy) energy[k] = red[(size_t)nv * k + 3] * uasd::kMub / uasd::kMry;
+  }
+  return UASD_OK;
+}
+
+int uasd_get_coloring(uasd_ctx *c, int *ncolors, int *color_of_site) {
+  if (int r = check_ctx(c)) return r;
+  if (!c->have_ex) return fail(UASD_ERR_STATE, "exchange not set");
+  if (int r = build_coloring(c)) return r;
+  if (ncolors) *ncolors = c->ncolors;
+  if (color_of_site) std::copy(c->color_of_site.begin(), c->color_of_site.end(), color_of_site);
+  return UASD_OK;
+}
+
+int uasd_set_kernel_timing(uasd_ctx *c, int enable) {
+  if (int r = check_ctx(c)) return r;
+  if (int r = resolve_timing(c)) return r;
+  c->timing = enable != 0;
+  for (auto &t : c->timers) t = Timing{};
+  return UASD_OK;
+}
+
+int uasd_get_kernel_timing(uasd_ctx *c, int kind, int64_t *launches, double *total_ms) {
+  if (int r = check_ctx(c)) return r;
+  if (kind < 0 || kind > 3) return fail(UASD_ERR_ARG, "kind must be 0..3");
+  if (int r = resolve_timing(c)) return r;
+  if (launches) *launches = c->timers[kind].launches;
+  if (total_ms) *total_ms = c->timers[kind].ms;
+  return UASD_OK;
+}
+
+}  // extern "C"

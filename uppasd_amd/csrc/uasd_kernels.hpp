@@ -1,0 +1,750 @@
+// uasd_kernels.hpp -- device code of the spin-update engine, templated on the
+// arithmetic type R (double for the plain drop-in, float for the reduced
+// precision form).  Included by uasd_kernels_f64.hip (built with
+// -ffp-contract=off so every product and sum rounds as the reference's
+// Fortran does) and uasd_kernels_f32.hip.
+//
+// Data layout in HBM (one context = one lattice, M ensembles):
+//   spins      three planes x,y,z of R, index q = site*M + ens   (SoA)
+//   mmom       R per q
+//   neighbours slot-major ELL int32 nbr[j*N + site], 0-based
+//   couplings  slot-major per Hamiltonian row cpl[j*nrows + row]; rows are
+//              the distinct (length, coupling) rows of ham%ncoup, so a
+//              periodic lattice with NA basis atoms stores NA rows
+//   site class int32 per site -> {exchange row, DM row, anisotropy row}
+// Lanes of a wavefront take consecutive q, so with M > 1 the lanes of one
+// site share its neighbour indices (one broadcast load) and gather
+// neighbouring spins with unit stride.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace uasd {
+
+// ---------------------------------------------------------------- constants
+// source/Parameters/constants.f90
+constexpr double kGama = 1.760859644e11;
+constexpr double kKBolt = 1.38064852e-23;
+constexpr double kMub = 9.274009994e-24;
+constexpr double kMry = 2.179872325e-21;
+constexpr double kPi = 3.141592653589793;
+constexpr double kDblTolerance = (double)1e-14f;  // parameters.f90:12 (default real literal)
+
+constexpr uint32_t kTagLLG = 0x11u;  // Philox tag of the thermal field
+constexpr uint32_t kTagMCU = 0x21u;  // MC uniforms
+constexpr uint32_t kTagMCG = 0x22u;  // MC Gaussian trial step
+
+struct SiteClass {
+  int ex_row, dm_row, an_row, pad;
+};
+
+template <typename R>
+struct HamDev {
+  int N, M;
+  // exchange
+  int W, ex_rows;
+  const int *__restrict__ nbr;
+  const int *__restrict__ ex_len;
+  const R *__restrict__ cpl;
+  // DM
+  int Wd, dm_rows;
+  const int *__restrict__ dnbr;
+  const int *__restrict__ dm_len;
+  const R *__restrict__ dvx, *__restrict__ dvy, *__restrict__ dvz;
+  // anisotropy rows
+  const int *__restrict__ an_type;
+  const R *__restrict__ an_ex, *__restrict__ an_ey, *__restrict__ an_ez;
+  const R *__restrict__ an_k1, *__restrict__ an_k2, *__restrict__ an_sb;
+  // site classes
+  const int *__restrict__ cls;
+  const SiteClass *__restrict__ classes;
+  // external field: 0 none, 1 uniform per ensemble (ext_u[3*ens+c]), 2 full planes
+  int ext_mode;
+  const R *__restrict__ ext_u;
+  const R *__restrict__ extx, *__restrict__ exty, *__restrict__ extz;
+};
+
+// Spin planes.  No __restrict__: the corrector writes the plane it read at
+// the same index, and MC sweeps gather from the planes they update.
+template <typename R>
+struct Spins {
+  R *x;
+  R *y;
+  R *z;
+};
+
+// Where gathered moments come from: unit vectors times mmom (resident
+// state: emomM = emom*mmom bit-for-bit), or an emomM array.
+template <typename R>
+struct MomSrc {
+  const R *x;
+  const R *y;
+  const R *z;
+  const R *m;  // nullptr: x,y,z already hold emomM
+  __device__ __forceinline__ void load(size_t q, R &a, R &b, R &c) const {
+    if (m) {
+      R mm = m[q];
+      a = x[q] * mm;
+      b = y[q] * mm;
+      c = z[q] * mm;
+    } else {
+      a = x[q];
+      b = y[q];
+      c = z[q];
+    }
+  }
+};
+
+// ------------------------------------------------------------------- Philox
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+    uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    uint32_t n0 = hi1 ^ c1 ^ k0;
+    uint32_t n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+__device__ __forceinline__ void counter_words(uint64_t seed, uint32_t tag, uint64_t step,
+                                              uint32_t site, uint32_t ens, uint32_t w[4]) {
+  philox4x32_10(site, ens, (uint32_t)step, (tag & 0xffu) | ((uint32_t)(step >> 32) << 8),
+                (uint32_t)seed, (uint32_t)(seed >> 32), w);
+}
+
+template <typename R>
+struct Noise;
+
+template <>
+struct Noise<double> {
+  // three N(0,1) deviates: Box-Muller on (u0,u1) and (u2,u3)
+  __device__ __forceinline__ static void normal3(uint64_t seed, uint32_t tag, uint64_t step,
+                                                 uint32_t site, uint32_t ens, double &a,
+                                                 double &b, double &c) {
+    uint32_t w[4];
+    counter_words(seed, tag, step, site, ens, w);
+    const double s = 2.3283064365386963e-10, two_pi = 6.283185307179586;
+    double u0 = ((double)w[0] + 0.5) * s, u1 = ((double)w[1] + 0.5) * s;
+    double u2 = ((double)w[2] + 0.5) * s, u3 = ((double)w[3] + 0.5) * s;
+    double r0 = sqrt(-2.0 * log(u0)), r1 = sqrt(-2.0 * log(u2));
+    double t0 = two_pi * u1, t1 = two_pi * u3;
+    a = r0 * cos(t0);
+    b = r0 * sin(t0);
+    c = r1 * cos(t1);
+  }
+};
+
+template <>
+struct Noise<float> {
+  __device__ __forceinline__ static void normal3(uint64_t seed, uint32_t tag, uint64_t step,
+                                                 uint32_t site, uint32_t ens, float &a, float &b,
+                                                 float &c) {
+    uint32_t w[4];
+    counter_words(seed, tag, step, site, ens, w);
+    const float s = 5.9604644775390625e-08f, two_pi = 6.28318530717958647692f;
+    float u0 = ((float)(w[0] >> 8) + 0.5f) * s, u1 = ((float)(w[1] >> 8) + 0.5f) * s;
+    float u2 = ((float)(w[2] >> 8) + 0.5f) * s, u3 = ((float)(w[3] >> 8) + 0.5f) * s;
+    float r0 = sqrtf(-2.0f * logf(u0)), r1 = sqrtf(-2.0f * logf(u2));
+    float t0 = two_pi * u1, t1 = two_pi * u3;
+    a = r0 * cosf(t0);
+    b = r0 * sinf(t0);
+    c = r1 * cosf(t1);
+  }
+};
+
+__device__ __forceinline__ void uniform4(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
+                                         uint32_t ens, double u[4]) {
+  uint32_t w[4];
+  counter_words(seed, tag, step, site, ens, w);
+  const double s = 2.3283064365386963e-10;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) u[i] = ((double)w[i] + 0.5) * s;
+}
+
+// ------------------------------------------------------------- field terms
+// hamiltonianactions.f90:431-466 (exchange) and 546-580 (DM), summed in list
+// order; 842-879 / 883-919 anisotropy; 183-248 assembly.
+template <typename R>
+struct FieldParts {
+  R s0, s1, s2;  // beff_s (pair + uniaxial)
+  R q0, q1, q2;  // beff_q (cubic * sb for type 7)
+  R e0, e1, e2;  // external field
+};
+
+template <typename R>
+__device__ __forceinline__ void uniaxial(const HamDev<R> &h, int row, R m0, R m1, R m2, R &f0,
+                                         R &f1, R &f2) {
+  R ea0 = h.an_ex[row], ea1 = h.an_ey[row], ea2 = h.an_ez[row];
+  R k1 = h.an_k1[row], k2 = h.an_k2[row];
+  R tt1 = m0 * ea0 + m1 * ea1 + m2 * ea2;
+  R tt2 = k1 + (R)2.0 * k2 * ((R)1.0 - tt1 * tt1);
+  R tt3 = (R)2.0 * tt1 * tt2;
+  f0 = f0 - tt3 * ea0;
+  f1 = f1 - tt3 * ea1;
+  f2 = f2 - tt3 * ea2;
+}
+
+template <typename R>
+__device__ __forceinline__ void cubic(const HamDev<R> &h, int row, R m0, R m1, R m2, R &f0, R &f1,
+                                      R &f2) {
+  R k1 = h.an_k1[row], k2 = h.an_k2[row];
+  R s0 = m0 * m0, s1 = m1 * m1, s2 = m2 * m2;
+  f0 = f0 + (R)2.0 * k1 * m0 * (s1 + s2) + (R)2.0 * k2 * m0 * (s1 * s2);
+  f1 = f1 + (R)2.0 * k1 * m1 * (s2 + s0) + (R)2.0 * k2 * m1 * (s2 * s0);
+  f2 = f2 + (R)2.0 * k1 * m2 * (s0 + s1) + (R)2.0 * k2 * m2 * (s0 * s1);
+}
+
+template <typename R>
+__device__ __forceinline__ FieldParts<R> site_field(const HamDev<R> &h, const MomSrc<R> &src,
+                                                    int site, int ens) {
+  FieldParts<R> p;
+  p.s0 = p.s1 = p.s2 = (R)0;
+  p.q0 = p.q1 = p.q2 = (R)0;
+  const SiteClass sc = h.classes[h.cls[site]];
+  const int M = h.M, N = h.N;
+  {
+    const int len = h.ex_len[sc.ex_row];
+    const int *nb = h.nbr + site;
+    const R *cp = h.cpl + sc.ex_row;
+    for (int j = 0; j < len; ++j) {
+      int n = nb[(size_t)j * N];
+      R J = cp[(size_t)j * h.ex_rows];
+      R m0, m1, m2;
+      src.load((size_t)n * M + ens, m0, m1, m2);
+      p.s0 = p.s0 + J * m0;
+      p.s1 = p.s1 + J * m1;
+      p.s2 = p.s2 + J * m2;
+    }
+  }
+  if (h.Wd > 0 && sc.dm_row >= 0) {
+    const int len = h.dm_len[sc.dm_row];
+    const int *nb = h.dnbr + site;
+    for (int j = 0; j < len; ++j) {
+      int n = nb[(size_t)j * N];
+      size_t r = (size_t)j * h.dm_rows + sc.dm_row;
+      R D0 = h.dvx[r], D1 = h.dvy[r], D2 = h.dvz[r];
+      R m0, m1, m2;
+      src.load((size_t)n * M + ens, m0, m1, m2);
+      p.s0 = p.s0 + D2 * m1 - D1 * m2;
+      p.s1 = p.s1 + D0 * m2 - D2 * m0;
+      p.s2 = p.s2 + D1 * m0 - D0 * m1;
+    }
+  }
+  if (sc.an_row >= 0) {
+    const int t = h.an_type[sc.an_row];
+    if (t == 1 || t == 2 || t == 7) {
+      R m0, m1, m2;
+      src.load((size_t)site * M + ens, m0, m1, m2);
+      if (t == 1) {
+        uniaxial(h, sc.an_row, m0, m1, m2, p.s0, p.s1, p.s2);
+      } else if (t == 2) {
+        cubic(h, sc.an_row, m0, m1, m2, p.s0, p.s1, p.s2);
+      } else {
+        R t0 = 0, t1 = 0, t2 = 0;
+        uniaxial(h, sc.an_row, m0, m1, m2, p.s0, p.s1, p.s2);
+        cubic(h, sc.an_row, m0, m1, m2, t0, t1, t2);
+        R sb = h.an_sb[sc.an_row];
+        p.q0 = p.q0 + t0 * sb;
+        p.q1 = p.q1 + t1 * sb;
+        p.q2 = p.q2 + t2 * sb;
+      }
+    }
+  }
+  if (h.ext_mode == 1) {
+    p.e0 = h.ext_u[3 * ens];
+    p.e1 = h.ext_u[3 * ens + 1];
+    p.e2 = h.ext_u[3 * ens + 2];
+  } else if (h.ext_mode == 2) {
+    size_t q = (size_t)site * M + ens;
+    p.e0 = h.extx[q];
+    p.e1 = h.exty[q];
+    p.e2 = h.extz[q];
+  } else {
+    p.e0 = p.e1 = p.e2 = (R)0;
+  }
+  return p;
+}
+
+// beff = 0 + beff1 + beff2, beff2 = beff_q + ext + 0  (hamiltonianactions.f90:240-243)
+template <typename R>
+__device__ __forceinline__ void total_field(const FieldParts<R> &p, R &b0, R &b1, R &b2) {
+  b0 = (R)0 + p.s0 + (p.q0 + p.e0 + (R)0);
+  b1 = (R)0 + p.s1 + (p.q1 + p.e1 + (R)0);
+  b2 = (R)0 + p.s2 + (p.q2 + p.e2 + (R)0);
+}
+
+// ------------------------------------------------------------ LLG params
+template <typename R>
+struct LLGDev {
+  const double *lam, *temp, *lande;  // per site or nullptr
+  double lam0, temp0, lande0;        // uniform values when the arrays are null
+  double dt, temprescale, bn;
+  int noise_mode;
+  uint64_t seed;
+  uint64_t step;
+  const R *nx, *ny, *nz;  // injected N(0,1) planes
+  __device__ __forceinline__ double lambda(int s) const { return lam ? lam[s] : lam0; }
+  __device__ __forceinline__ double temperature(int s) const { return temp ? temp[s] : temp0; }
+  __device__ __forceinline__ double landeg(int s) const { return lande ? lande[s] : lande0; }
+  __device__ __forceinline__ void normals(int site, int ens, size_t q, R &a, R &b, R &c) const {
+    if (noise_mode == 1) {
+      Noise<R>::normal3(seed, kTagLLG, step, (uint32_t)site, (uint32_t)ens, a, b, c);
+    } else if (noise_mode == 2) {
+      a = nx[q];
+      b = ny[q];
+      c = nz[q];
+    } else {
+      a = b = c = (R)0;
+    }
+  }
+};
+
+// rannum, llg = 1 (randomnumbers.f90:674-676, 735-745)
+template <typename R>
+__device__ __forceinline__ R midpoint_sigma(double lam, double bn, double mmom, double T,
+                                            double temprescale) {
+  double Dk = (lam / (1.0 + lam * lam) * kKBolt / kGama / (kMub)) * (kGama / bn);
+  double mmomi = 1.0 / mmom;
+  double D = Dk * mmomi * T * temprescale;
+  return (R)sqrt(2.0 * D);
+}
+
+// depondt.f90:142-145
+template <typename R>
+__device__ __forceinline__ R depondt_sigma(double lam, double dt, double mmom, double T,
+                                           double temprescale) {
+  double Dp = (2.0 * lam * kKBolt) / (dt * kGama * kMub);
+  return (R)sqrt(Dp * temprescale * T / mmom);
+}
+
+template <typename R>
+__device__ __forceinline__ R fsqrt(R x) {
+  return sqrt(x);
+}
+template <typename R>
+__device__ __forceinline__ R fcos(R x) {
+  return cos(x);
+}
+template <typename R>
+__device__ __forceinline__ R fsin(R x) {
+  return sin(x);
+}
+
+// Rodrigues rotation of e about bd (depondt.f90:156-179)
+template <typename R>
+__device__ __forceinline__ void rodrigues(R bd0, R bd1, R bd2, R dt, R lam, R e0, R e1, R e2,
+                                          R &o0, R &o1, R &o2) {
+  R lldamp = (R)1.0 / ((R)1.0 + lam * lam);
+  R Bnorm = bd0 * bd0 + bd1 * bd1 + bd2 * bd2;
+  Bnorm = fsqrt(Bnorm) + (R)1.0e-15;
+  R hx = bd0 / Bnorm, hy = bd1 / Bnorm, hz = bd2 / Bnorm;
+  R v = Bnorm * dt * (R)kGama * lldamp;
+  R cosv = fcos(v), sinv = fsin(v), u = (R)1.0 - cosv;
+  o0 = hx * hx * u * e0 + cosv * e0 + hx * hy * u * e1 - hz * sinv * e1 + hx * hz * u * e2 +
+       hy * sinv * e2;
+  o1 = hy * hx * u * e0 + hz * sinv * e0 + hy * hy * u * e1 + cosv * e1 + hy * hz * u * e2 -
+       hx * sinv * e2;
+  o2 = hx * hz * u * e0 - hy * sinv * e0 + hz * hy * u * e1 + hx * sinv * e1 + hz * hz * u * e2 +
+       cosv * e2;
+}
+
+// bdup = 0 + bloc + lambda e x bloc (depondt.f90:148-154)
+template <typename R>
+__device__ __forceinline__ void depondt_bdup(R lam, R e0, R e1, R e2, R bl0, R bl1, R bl2, R &d0,
+                                             R &d1, R &d2) {
+  d0 = (R)0 + bl0 + lam * e1 * bl2 - lam * e2 * bl1;
+  d1 = (R)0 + bl1 + lam * e2 * bl0 - lam * e0 * bl2;
+  d2 = (R)0 + bl2 + lam * e0 * bl1 - lam * e1 * bl0;
+}
+
+// midpoint A vector (midpoint.f90:127-151 / 270-296)
+template <typename R>
+__device__ __forceinline__ void midpoint_A(R lam, R deltat, R bn, R lande, R b0, R b1, R b2, R r0,
+                                           R r1, R r2, R p0, R p1, R p2, R &A0, R &A1, R &A2) {
+  R lldamp = (R)1.0 / ((R)1.0 + lam * lam);
+  R dt = deltat * bn * (R)kGama * lldamp;
+  R sqrtdt = fsqrt(dt);
+  R dtg = dt * lande;
+  R sqrtdtg = sqrtdt * lande;
+  R a0 = -(R)0.0 - b0 - lam * (p1 * b2 - p2 * b1);
+  R a1 = -(R)0.0 - b1 - lam * (p2 * b0 - p0 * b2);
+  R a2 = -(R)0.0 - b2 - lam * (p0 * b1 - p1 * b0);
+  R s0 = -r0 - lam * (p1 * r2 - p2 * r1);
+  R s1 = -r1 - lam * (p2 * r0 - p0 * r2);
+  R s2 = -r2 - lam * (p0 * r1 - p1 * r0);
+  A0 = (R)0.5 * dtg * a0 + (R)0.5 * sqrtdtg * s0;
+  A1 = (R)0.5 * dtg * a1 + (R)0.5 * sqrtdtg * s1;
+  A2 = (R)0.5 * dtg * a2 + (R)0.5 * sqrtdtg * s2;
+}
+
+// Cramer solve of (I + [A]x) e_t = (I - [A]x) e  (midpoint.f90:153-164)
+template <typename R>
+__device__ __forceinline__ void midpoint_solve(R A0, R A1, R A2, R e0, R e1, R e2, R &t0, R &t1,
+                                               R &t2, R &detAi) {
+  detAi = (R)1.0 / ((R)1.0 + (A0 * A0 + A1 * A1 + A2 * A2));
+  R a0 = e0 + e1 * A2 - e2 * A1;
+  R a1 = e1 + e2 * A0 - e0 * A2;
+  R a2 = e2 + e0 * A1 - e1 * A0;
+  t0 = a0 * ((R)1 + A0 * A0) + a1 * (A0 * A1 + A2) + a2 * (A0 * A2 - A1);
+  t1 = a0 * (A1 * A0 - A2) + a1 * ((R)1 + A1 * A1) + a2 * (A1 * A2 + A0);
+  t2 = a0 * (A2 * A0 + A1) + a1 * (A2 * A1 - A0) + a2 * ((R)1 + A2 * A2);
+}
+
+}  // namespace uasd
+
+// =================================================================== kernels
+namespace uasd {
+
+// Moments of the resident state as a gather source.
+template <typename R>
+__device__ __forceinline__ MomSrc<R> src_of(const Spins<R> &s, const R *mm) {
+  MomSrc<R> m;
+  m.x = s.x;
+  m.y = s.y;
+  m.z = s.z;
+  m.m = mm;
+  return m;
+}
+
+// effective_field_full over all (site, ens): beff planes and, when en != null,
+// the per-site field-routine energy term -m.(0.5*(bs+2bq+2bext)) in FP64.
+template <typename R>
+__global__ void __launch_bounds__(256) k_field(HamDev<R> h, MomSrc<R> src, R *bx, R *by, R *bz,
+                                               double *en) {
+  const int64_t total = (int64_t)h.N * h.M;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= total) return;
+  const int site = (int)(q / h.M), ens = (int)(q - (int64_t)site * h.M);
+  FieldParts<R> p = site_field(h, src, site, ens);
+  R b0, b1, b2;
+  total_field(p, b0, b1, b2);
+  bx[q] = b0;
+  by[q] = b1;
+  bz[q] = b2;
+  if (en) {
+    R m0, m1, m2;
+    src.load((size_t)q, m0, m1, m2);
+    double t0 = 0.5 * ((double)p.s0 + 2.0 * (double)p.q0 + 2.0 * (double)p.e0 + 0.0);
+    double t1 = 0.5 * ((double)p.s1 + 2.0 * (double)p.q1 + 2.0 * (double)p.e1 + 0.0);
+    double t2 = 0.5 * ((double)p.s2 + 2.0 * (double)p.q2 + 2.0 * (double)p.e2 + 0.0);
+    en[q] = -(double)m0 * t0 - (double)m1 * t1 - (double)m2 * t2;
+  }
+}
+
+// One half of an LLG step.
+//   ALG 1 midpoint (midpoint.f90), 5 Depondt (depondt.f90)
+//   HALF 0 predictor (evolve_first), 1 corrector (evolve_second)
+//   FUSED: compute the field of the gathered moments in-kernel; otherwise read
+//   beff planes written by k_field.
+// Buffers: e = emom at step start, p = predicted emom (emom2 after the first
+// half), b2 = Depondt's b2eff.  The corrector writes the new emom into `out`
+// (e when moment_update is fused, the emom2 array otherwise).
+template <typename R, int ALG, int HALF, bool FUSED>
+__global__ void __launch_bounds__(256)
+    k_llg(HamDev<R> h, LLGDev<R> prm, Spins<R> e, Spins<R> p, Spins<R> b2, const R *__restrict__ mm,
+          const R *__restrict__ bx, const R *__restrict__ by, const R *__restrict__ bz,
+          Spins<R> out) {
+  const int64_t total = (int64_t)h.N * h.M;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= total) return;
+  const int site = (int)(q / h.M), ens = (int)(q - (int64_t)site * h.M);
+  R b0, b1, b2v;
+  if (FUSED) {
+    FieldParts<R> fp = site_field(h, HALF == 0 ? src_of(e, mm) : src_of(p, mm), site, ens);
+    total_field(fp, b0, b1, b2v);
+  } else {
+    b0 = bx[q];
+    b1 = by[q];
+    b2v = bz[q];
+  }
+  const double lam = prm.lambda(site), T = prm.temperature(site);
+  const R m = mm[q];
+  R n0, n1, n2;
+  prm.normals(site, ens, (size_t)q, n0, n1, n2);
+  if (ALG == 5) {
+    const R sig = depondt_sigma<R>(lam, prm.dt, (double)m, T, prm.temprescale);
+    const R t0 = n0 * sig, t1 = n1 * sig, t2 = n2 * sig;
+    const R bl0 = b0 + t0, bl1 = b1 + t1, bl2 = b2v + t2;
+    if (HALF == 0) {
+      const R e0 = e.x[q], e1 = e.y[q], e2 = e.z[q];
+      R d0, d1, d2, o0, o1, o2;
+      depondt_bdup((R)lam, e0, e1, e2, bl0, bl1, bl2, d0, d1, d2);
+      rodrigues(d0, d1, d2, (R)prm.dt, (R)lam, e0, e1, e2, o0, o1, o2);
+      p.x[q] = o0;
+      p.y[q] = o1;
+      p.z[q] = o2;
+      b2.x[q] = d0;
+      b2.y[q] = d1;
+      b2.z[q] = d2;
+    } else {
+      const R p0 = p.x[q], p1 = p.y[q], p2 = p.z[q];
+      R d0, d1, d2, o0, o1, o2;
+      depondt_bdup((R)lam, p0, p1, p2, bl0, bl1, bl2, d0, d1, d2);
+      d0 = (R)0.5 * d0 + (R)0.5 * b2.x[q];
+      d1 = (R)0.5 * d1 + (R)0.5 * b2.y[q];
+      d2 = (R)0.5 * d2 + (R)0.5 * b2.z[q];
+      const R e0 = e.x[q], e1 = e.y[q], e2 = e.z[q];
+      rodrigues(d0, d1, d2, (R)prm.dt, (R)lam, e0, e1, e2, o0, o1, o2);
+      out.x[q] = o0;
+      out.y[q] = o1;
+      out.z[q] = o2;
+    }
+  } else {
+    const R sig = midpoint_sigma<R>(lam, prm.bn, (double)m, T, prm.temprescale);
+    const R r0 = n0 * sig, r1 = n1 * sig, r2 = n2 * sig;
+    const R e0 = e.x[q], e1 = e.y[q], e2 = e.z[q];
+    R A0, A1, A2, t0, t1, t2, detAi;
+    if (HALF == 0) {
+      midpoint_A((R)lam, (R)prm.dt, (R)prm.bn, (R)prm.landeg(site), b0, b1, b2v, r0, r1, r2, e0,
+                 e1, e2, A0, A1, A2);
+      midpoint_solve(A0, A1, A2, e0, e1, e2, t0, t1, t2, detAi);
+      p.x[q] = (R)0.5 * (e0 + t0 * detAi);
+      p.y[q] = (R)0.5 * (e1 + t1 * detAi);
+      p.z[q] = (R)0.5 * (e2 + t2 * detAi);
+    } else {
+      const R p0 = p.x[q], p1 = p.y[q], p2 = p.z[q];
+      midpoint_A((R)lam, (R)prm.dt, (R)prm.bn, (R)prm.landeg(site), b0, b1, b2v, r0, r1, r2, p0,
+                 p1, p2, A0, A1, A2);
+      midpoint_solve(A0, A1, A2, e0, e1, e2, t0, t1, t2, detAi);
+      out.x[q] = t0 * detAi;
+      out.y[q] = t1 * detAi;
+      out.z[q] = t2 * detAi;
+    }
+  }
+}
+
+// ----------------------------------------------------------------- MC
+// choose_random_flip (montecarlo_common.f90:25-79)
+template <typename R>
+__device__ __forceinline__ void trial_move(R e0, R e1, R e2, double delta, const double u[4],
+                                          const double g[3], R &n0, R &n1, R &n2) {
+  int ftype = (int)floor(3 * u[0]);
+  if (ftype == 0) {
+    double phi = u[1] * 2 * kPi;
+    double theta = acos(1 - 2 * u[2]);
+    double st = sin(theta);
+    n0 = (R)(st * cos(phi));
+    n1 = (R)(st * sin(phi));
+    n2 = (R)cos(theta);
+  } else if (ftype == 1) {
+    R g0 = (R)(g[0] * delta), g1 = (R)(g[1] * delta), g2 = (R)(g[2] * delta);
+    R a0 = e0 + g0, a1 = e1 + g1, a2 = e2 + g2;
+    R l = fsqrt(a0 * a0 + a1 * a1 + a2 * a2);
+    n0 = (e0 + g0) / l;
+    n1 = (e1 + g1) / l;
+    n2 = (e2 + g2) / l;
+  } else {
+    n0 = -e0;
+    n1 = -e1;
+    n2 = -e2;
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ R sq(R x) {
+  return x * x;
+}
+
+// calculate_energy (montecarlo_common.f90:431-865), exchange + anisotropy +
+// DM + external field; returns mub*(E_trial - E_current).
+template <typename R>
+__device__ __forceinline__ R mc_delta_e(const HamDev<R> &h, const MomSrc<R> &src, const R *ex,
+                                        const R *ey, const R *ez, int site, int ens, R m, R n0,
+                                        R n1, R n2, R x0, R x1, R x2) {
+  const SiteClass sc = h.classes[h.cls[site]];
+  const int M = h.M, N = h.N;
+  const size_t qi = (size_t)site * M + ens;
+  R c0, c1, c2;
+  src.load(qi, c0, c1, c2);
+  const R t0 = n0 * m, t1 = n1 * m, t2 = n2 * m;
+  R e_c = 0, e_t = 0;
+  {
+    const int len = h.ex_len[sc.ex_row];
+    for (int j = 0; j < len; ++j) {
+      int nb = h.nbr[(size_t)j * N + site];
+      R J = h.cpl[(size_t)j * h.ex_rows + sc.ex_row];
+      R a0, a1, a2;
+      src.load((size_t)nb * M + ens, a0, a1, a2);
+      e_c = e_c - J * (c0 * a0 + c1 * a1 + c2 * a2);
+      e_t = e_t - J * (t0 * a0 + t1 * a1 + t2 * a2);
+    }
+  }
+  if (sc.an_row >= 0) {
+    const int ty = h.an_type[sc.an_row];
+    const R k1 = h.an_k1[sc.an_row], k2 = h.an_k2[sc.an_row];
+    const R ea0 = h.an_ex[sc.an_row], ea1 = h.an_ey[sc.an_row], ea2 = h.an_ez[sc.an_row];
+    if (ty == 1 || ty == 7) {
+      R tta = c0 * ea0 + c1 * ea1 + c2 * ea2;
+      R ttb = t0 * ea0 + t1 * ea1 + t2 * ea2;
+      e_c = e_c + k1 * sq(tta) + k2 * sq(sq(tta));
+      e_t = e_t + k1 * sq(ttb) + k2 * sq(sq(ttb));
+    } else if (ty == 2) {
+      e_c = e_c - k1 * (sq(c0) * sq(c1) + sq(c1) * sq(c2) + sq(c2) * sq(c0)) -
+            k2 * (sq(c0) * sq(c1) * sq(c2));
+      e_t = e_t - k1 * (sq(t0) * sq(t1) + sq(t1) * sq(t2) + sq(t2) * sq(t0)) -
+            k2 * (sq(t0) * sq(t1) * sq(t2));
+    }
+    if (ty == 7) {
+      const R sb = h.an_sb[sc.an_row];
+      const R aw1 = k1 * sb, aw2 = k2 * sb;
+      e_c = e_c + aw1 * (sq(c0) * sq(c1) + sq(c1) * sq(c2) + sq(c2) * sq(c0)) +
+            aw2 * (sq(c0) * sq(c1) * sq(c2));
+      e_t = e_t + aw1 * (sq(t0) * sq(t1) + sq(t1) * sq(t2) + sq(t2) * sq(t0)) +
+            aw2 * (sq(t0) * sq(t1) * sq(t2));
+    }
+  }
+  if (h.Wd > 0 && sc.dm_row >= 0) {
+    const R u0 = ex[qi], u2 = ez[qi];  // unit-vector terms as in montecarlo_common.f90:612,615
+    const int len = h.dm_len[sc.dm_row];
+    for (int j = 0; j < len; ++j) {
+      int nb = h.dnbr[(size_t)j * N + site];
+      size_t r = (size_t)j * h.dm_rows + sc.dm_row;
+      R D0 = h.dvx[r], D1 = h.dvy[r], D2 = h.dvz[r];
+      R a0, a1, a2;
+      src.load((size_t)nb * M + ens, a0, a1, a2);
+      e_c = e_c - D0 * (c1 * a2 - u2 * a1) - D1 * (c2 * a0 - c0 * a2) - D2 * (u0 * a1 - c1 * a0);
+      e_t = e_t - D0 * (t1 * a2 - t2 * a1) - D1 * (t2 * a0 - t0 * a2) - D2 * (t0 * a1 - t1 * a0);
+    }
+  }
+  e_c = e_c - x0 * c0 - x1 * c1 - x2 * c2;
+  e_t = e_t - x0 * t0 - x1 * t1 - x2 * t2;
+  R tt = e_t - e_c;
+  return (R)kMub * tt;
+}
+
+// flip_h (montecarlo_common.f90:371-422)
+template <typename R>
+__device__ __forceinline__ void heat_bath(double beta, R m, R f0, R f1, R f2, double flipprob,
+                                          double qr, R &o0, R &o1, R &o2) {
+  double z0 = beta * (double)f0 * kMub * (double)m;
+  double z1 = beta * (double)f1 * kMub * (double)m;
+  double z2 = beta * (double)f2 * kMub * (double)m;
+  double zarg = sqrt(z0 * z0 + z1 * z1 + z2 * z2);
+  double zct = z2 / zarg;
+  double zst = sqrt(1.0 - zct * zct) + kDblTolerance;
+  double zcp = z0 / (zarg * zst);
+  double zsp = z1 / (zarg * zst);
+  double ct = 1.0 + (1.0 / zarg) * log((1.0 - exp(-2.0 * zarg)) * qr + exp(-2.0 * zarg) + kDblTolerance);
+  double stv = sqrt(1.0 - ct * ct);
+  double phi = kPi * (2.0 * flipprob - 1.0);
+  double s0 = stv * cos(phi), s1 = stv * sin(phi), s2 = ct;
+  o0 = (R)(zcp * zct * s0 - zsp * s1 + zcp * zst * s2);
+  o1 = (R)(zsp * zct * s0 + zcp * s1 + zsp * zst * s2);
+  o2 = (R)(-zst * s0 + zct * s2);
+}
+
+struct MCParams {
+  const int *sites;     // sites of the colour being updated
+  int nsites;
+  const double *delta;  // (M) Gaussian step width per ensemble
+  const double *beta;   // (M) Metropolis 1/(k_B (T*temprescale + 1e-15))
+  const double *hbeta;  // (M) heat-bath 1/k_B/(temprescale*T)
+  const double *ext;    // (3*M) uniform field per ensemble
+  uint64_t seed, sweep;
+  int heat_bath;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_mc_colour(HamDev<R> h, MCParams mp, Spins<R> e, const R *__restrict__ mm) {
+  const int64_t total = (int64_t)mp.nsites * h.M;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int pidx = (int)(t / h.M), ens = (int)(t - (int64_t)pidx * h.M);
+  const int site = mp.sites[pidx];
+  const size_t q = (size_t)site * h.M + ens;
+  double u[4];
+  uniform4(mp.seed, kTagMCU, mp.sweep, (uint32_t)site, (uint32_t)ens, u);
+  const R m = mm[q];
+  const MomSrc<R> src = src_of(e, mm);
+  const R x0 = (R)mp.ext[3 * ens], x1 = (R)mp.ext[3 * ens + 1], x2 = (R)mp.ext[3 * ens + 2];
+  R o0, o1, o2;
+  if (mp.heat_bath) {
+    FieldParts<R> fp = site_field(h, src, site, ens);
+    fp.e0 = x0;
+    fp.e1 = x1;
+    fp.e2 = x2;
+    R f0, f1, f2;
+    total_field(fp, f0, f1, f2);
+    heat_bath<R>(mp.hbeta[ens], m, f0, f1, f2, u[1], u[0], o0, o1, o2);
+  } else {
+    const R e0 = e.x[q], e1 = e.y[q], e2 = e.z[q];
+    double g[3] = {0.0, 0.0, 0.0};
+    if ((int)floor(3 * u[0]) == 1)
+      Noise<double>::normal3(mp.seed, kTagMCG, mp.sweep, (uint32_t)site, (uint32_t)ens, g[0], g[1],
+                             g[2]);
+    R n0, n1, n2;
+    trial_move<R>(e0, e1, e2, mp.delta[ens], u, g, n0, n1, n2);
+    R de = mc_delta_e(h, src, e.x, e.y, e.z, site, ens, m, n0, n1, n2, x0, x1, x2);
+    if (!(de <= (R)0 || u[3] < exp(-mp.beta[ens] * (double)de))) return;
+    o0 = n0;
+    o1 = n1;
+    o2 = n2;
+  }
+  e.x[q] = o0;
+  e.y[q] = o1;
+  e.z[q] = o2;
+}
+
+// ---------------------------------------------------------- FP64 reductions
+// Stage 1: block b sums sites [b*chunk, (b+1)*chunk) of ensemble blockIdx.y.
+// vals: 3 components of emomM (and the energy term when en != null).
+template <typename R, int NV>
+__global__ void __launch_bounds__(256)
+    k_reduce_stage1(int N, int M, int chunk, Spins<R> e, const R *__restrict__ mm,
+                    const double *__restrict__ en, double *__restrict__ partial) {
+  const int ens = blockIdx.y;
+  const int b = blockIdx.x;
+  const int s0 = b * chunk, s1 = min(N, s0 + chunk);
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  for (int s = s0 + (int)threadIdx.x; s < s1; s += blockDim.x) {
+    size_t q = (size_t)s * M + ens;
+    double m = (double)mm[q];
+    acc[0] += (double)e.x[q] * m;
+    acc[1] += (double)e.y[q] * m;
+    acc[2] += (double)e.z[q] * m;
+    if (NV > 3) acc[3] += en[q];
+  }
+  __shared__ double red[NV][256 / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    double x = acc[v];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) red[v][wid] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double x = 0.0;
+    for (int w = 0; w < (int)(blockDim.x / 64); ++w) x += red[threadIdx.x][w];
+    partial[((size_t)ens * gridDim.x + b) * NV + threadIdx.x] = x;
+  }
+}
+
+// Stage 2: one thread per (ensemble, value) sums the block partials in order.
+template <int NV>
+__global__ void k_reduce_stage2(int M, int nblocks, const double *__restrict__ partial,
+                                double *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M * NV) return;
+  const int ens = t / NV, v = t - ens * NV;
+  double x = 0.0;
+  for (int b = 0; b < nblocks; ++b) x += partial[((size_t)ens * nblocks + b) * NV + v];
+  out[t] = x;
+}
+
+}  // namespace uasd

@@ -1,0 +1,90 @@
+// uasd_kernels_impl.hpp -- definitions of the launchers declared in
+// uasd_launch.hpp, instantiated for one arithmetic type by the including
+// translation unit.
+#pragma once
+#include "uasd_launch.hpp"
+
+namespace uasd {
+
+static inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+template <typename R>
+hipError_t launch_field(hipStream_t s, const HamDev<R> &h, const MomSrc<R> &src, R *bx, R *by,
+                        R *bz, double *en) {
+  const int64_t n = (int64_t)h.N * h.M;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_field<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, h, src, bx, by, bz, en);
+  return hipGetLastError();
+}
+
+template <typename R, int ALG, int HALF, bool FUSED>
+static hipError_t llg_one(hipStream_t s, const HamDev<R> &h, const LLGDev<R> &prm, Spins<R> e,
+                          Spins<R> p, Spins<R> b2, const R *mm, const R *bx, const R *by,
+                          const R *bz, Spins<R> out) {
+  const int64_t n = (int64_t)h.N * h.M;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_llg<R, ALG, HALF, FUSED>), dim3(grid_for(n)), dim3(kBlock), 0, s, h, prm,
+                     e, p, b2, mm, bx, by, bz, out);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_llg(hipStream_t s, int alg, int half, bool fused, const HamDev<R> &h,
+                      const LLGDev<R> &prm, Spins<R> e, Spins<R> p, Spins<R> b2, const R *mm,
+                      const R *bx, const R *by, const R *bz, Spins<R> out) {
+#define UASD_LLG_CASE(A, H, F)                                                              \
+  if (alg == A && half == H && fused == F)                                                  \
+    return llg_one<R, A, H, F>(s, h, prm, e, p, b2, mm, bx, by, bz, out);
+  UASD_LLG_CASE(1, 0, true)
+  UASD_LLG_CASE(1, 1, true)
+  UASD_LLG_CASE(1, 0, false)
+  UASD_LLG_CASE(1, 1, false)
+  UASD_LLG_CASE(5, 0, true)
+  UASD_LLG_CASE(5, 1, true)
+  UASD_LLG_CASE(5, 0, false)
+  UASD_LLG_CASE(5, 1, false)
+#undef UASD_LLG_CASE
+  return hipErrorInvalidValue;
+}
+
+template <typename R>
+hipError_t launch_mc(hipStream_t s, const HamDev<R> &h, const MCParams &mp, Spins<R> e,
+                     const R *mm) {
+  const int64_t n = (int64_t)mp.nsites * h.M;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mc_colour<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, h, mp, e, mm);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_reduce(hipStream_t s, int N, int M, Spins<R> e, const R *mm, const double *en,
+                         int nv, double *partial, int nblocks, double *out) {
+  if (N == 0 || M == 0) return hipSuccess;
+  const int chunk = (N + nblocks - 1) / nblocks;
+  dim3 g1(nblocks, M);
+  if (nv == 4) {
+    hipLaunchKernelGGL((k_reduce_stage1<R, 4>), g1, dim3(kBlock), 0, s, N, M, chunk, e, mm, en,
+                       partial);
+    hipLaunchKernelGGL(k_reduce_stage2<4>, dim3((M * 4 + 63) / 64), dim3(64), 0, s, M, nblocks,
+                       partial, out);
+  } else {
+    hipLaunchKernelGGL((k_reduce_stage1<R, 3>), g1, dim3(kBlock), 0, s, N, M, chunk, e, mm, en,
+                       partial);
+    hipLaunchKernelGGL(k_reduce_stage2<3>, dim3((M * 3 + 63) / 64), dim3(64), 0, s, M, nblocks,
+                       partial, out);
+  }
+  return hipGetLastError();
+}
+
+#define UASD_INSTANTIATE(R)                                                                     \
+  template hipError_t launch_field<R>(hipStream_t, const HamDev<R> &, const MomSrc<R> &, R *,   \
+                                      R *, R *, double *);                                      \
+  template hipError_t launch_llg<R>(hipStream_t, int, int, bool, const HamDev<R> &,             \
+                                    const LLGDev<R> &, Spins<R>, Spins<R>, Spins<R>, const R *, \
+                                    const R *, const R *, const R *, Spins<R>);                 \
+  template hipError_t launch_mc<R>(hipStream_t, const HamDev<R> &, const MCParams &, Spins<R>,  \
+                                   const R *);                                                  \
+  template hipError_t launch_reduce<R>(hipStream_t, int, int, Spins<R>, const R *,              \
+                                       const double *, int, double *, int, double *);
+
+}  // namespace uasd

@@ -1,0 +1,199 @@
+"""Host-side mirror of the reference's hot-path routines over libuasd_hip.so.
+
+``Engine`` holds one device context.  Its methods keep the names, argument
+meaning and in/out behaviour of the reference routines they replace:
+
+  effective_field   effective_field_full  (Hamiltonian/hamiltonianactions.f90:108-252)
+  evolve_first      evolve_first          (Evolution/evolution.f90:38-273), SDEalgh 1 & 5
+  evolve_second     evolve_second         (Evolution/evolution.f90:280-406)
+  moment_update     moment_update / copym (Evolution/updatemoments.f90:19-95)
+  mc_sweeps         mc_evolve             (MonteCarlo/montecarlo.f90:44-273), coloured
+  sd_steps          sd_mphase loop body   (sd_driver.f90:668-765), device resident
+
+Arrays are numpy, C-order with the Fortran dimensions reversed: a reference
+(3,Natom,Mensemble) array is shape (Mensemble, Natom, 3) here.  Errors raise
+``UasdError`` with the library's message; nothing falls back to the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as B
+from ._lib import (UASD_FP32, UASD_FP64, UASD_NOISE_INJECTED, UASD_NOISE_NONE,
+                   UASD_NOISE_PHILOX, UASD_SDE_DEPONDT, UASD_SDE_MIDPOINT, UasdError)
+
+__all__ = ["Engine", "moment_update", "UasdError", "UASD_NOISE_NONE", "UASD_NOISE_PHILOX",
+           "UASD_NOISE_INJECTED", "UASD_SDE_MIDPOINT", "UASD_SDE_DEPONDT"]
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    B.check(B.lib().uasd_device_count(C.byref(n)), "uasd_device_count")
+    return n.value
+
+
+class Engine:
+    def __init__(self, natom: int, mensemble: int = 1, precision: str = "fp64"):
+        prec = {"fp64": UASD_FP64, "fp32": UASD_FP32}[precision]
+        self.N, self.M, self.precision = int(natom), int(mensemble), precision
+        h = C.c_void_p()
+        B.check(B.lib().uasd_ctx_create(C.byref(h), self.N, self.M, prec), "uasd_ctx_create")
+        self._h = h
+
+    # -------------------------------------------------------------- lifetime
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            B.lib().uasd_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_stream(self, hip_stream: int | None):
+        B.check(B.lib().uasd_ctx_set_stream(self._h, hip_stream), "uasd_ctx_set_stream")
+
+    def stream(self) -> int:
+        return B.lib().uasd_ctx_stream(self._h)
+
+    def sync(self):
+        B.check(B.lib().uasd_ctx_sync(self._h), "uasd_ctx_sync")
+
+    # ----------------------------------------------------------- Hamiltonian
+    def set_exchange(self, nlist, nlistsize, ncoup, aham=None):
+        nl, ns, nc = B.i32(nlist), B.i32(nlistsize), B.f64(ncoup)
+        mnn = nl.shape[1]
+        B.check(B.lib().uasd_set_exchange(self._h, mnn, ns.shape[0], B.ptr(nl), B.ptr(ns),
+                                          B.ptr(nc), B.ptr(B.i32(aham))), "uasd_set_exchange")
+
+    def set_dm(self, dmlist, dmlistsize, dm_vect, aham=None):
+        if dmlist is None:
+            B.check(B.lib().uasd_set_dm(self._h, 0, 0, None, None, None, None), "uasd_set_dm")
+            return
+        dl, ds, dv = B.i32(dmlist), B.i32(dmlistsize), B.f64(dm_vect)
+        B.check(B.lib().uasd_set_dm(self._h, dl.shape[1], ds.shape[0], B.ptr(dl), B.ptr(ds),
+                                    B.ptr(dv), B.ptr(B.i32(aham))), "uasd_set_dm")
+
+    def set_anisotropy(self, taniso, eaniso, kaniso, sb):
+        B.check(B.lib().uasd_set_anisotropy(self._h, B.ptr(B.i32(taniso)), B.ptr(B.f64(eaniso)),
+                                            B.ptr(B.f64(kaniso)), B.ptr(B.f64(sb))),
+                "uasd_set_anisotropy")
+
+    def set_external_field(self, field):
+        """field: (M,N,3) array, a 3-vector applied everywhere, or None."""
+        if field is not None:
+            f = np.asarray(field, dtype=np.float64)
+            if f.shape == (3,):
+                f = np.broadcast_to(f, (self.M, self.N, 3))
+            field = np.ascontiguousarray(f)
+        B.check(B.lib().uasd_set_external_field(self._h, B.ptr(field)), "uasd_set_external_field")
+
+    # ------------------------------------------------------------ parameters
+    def set_llg(self, lambda1, temp, delta_t, landeg=None, temprescale=1.0, bn=1.0):
+        lam = B.f64(np.broadcast_to(np.asarray(lambda1, dtype=np.float64), (self.N,)))
+        tmp = B.f64(np.broadcast_to(np.asarray(temp, dtype=np.float64), (self.N,)))
+        lg = None if landeg is None else B.f64(np.broadcast_to(np.asarray(landeg, dtype=np.float64), (self.N,)))
+        B.check(B.lib().uasd_set_llg(self._h, B.ptr(lam), B.ptr(tmp), B.ptr(lg), float(delta_t),
+                                     float(temprescale), float(bn)), "uasd_set_llg")
+
+    def set_noise(self, mode: int, seed: int = 0, normals=None):
+        B.check(B.lib().uasd_set_noise(self._h, int(mode), int(seed), B.ptr(B.f64(normals))),
+                "uasd_set_noise")
+
+    # ---------------------------------------------------------------- state
+    def set_moments(self, emom, mmom):
+        """emom (M,N,3) unit vectors, mmom (M,N) magnitudes."""
+        e, m = B.f64(emom), B.f64(np.broadcast_to(np.asarray(mmom, dtype=np.float64), (self.M, self.N)))
+        B.check(B.lib().uasd_set_moments(self._h, B.ptr(e), B.ptr(m)), "uasd_set_moments")
+
+    def get_moments(self):
+        e = np.zeros((self.M, self.N, 3))
+        eM = np.zeros((self.M, self.N, 3))
+        B.check(B.lib().uasd_get_moments(self._h, B.ptr(e), B.ptr(eM)), "uasd_get_moments")
+        return e, eM
+
+    # -------------------------------------------------- drop-in (host arrays)
+    def effective_field(self, emomM):
+        """beff and the field-routine energy (mRy) of emomM, as effective_field_full."""
+        eM = B.f64(emomM)
+        beff = np.zeros_like(eM)
+        en = C.c_double(0.0)
+        B.check(B.lib().uasd_effective_field_host(self._h, B.ptr(eM), B.ptr(beff), C.byref(en)),
+                "uasd_effective_field_host")
+        return beff, en.value
+
+    def evolve_first(self, sdealgh, beff, emom, emom2, emomM, b2eff=None):
+        """In place on emom, emom2, emomM (and b2eff), like evolve_first."""
+        for a in (emom, emom2, emomM) + ((b2eff,) if b2eff is not None else ()):
+            assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+        B.check(B.lib().uasd_evolve_first_host(self._h, int(sdealgh), B.ptr(B.f64(beff)),
+                                               B.ptr(emom), B.ptr(emom2), B.ptr(emomM),
+                                               B.ptr(b2eff)), "uasd_evolve_first_host")
+
+    def evolve_second(self, sdealgh, beff, b2eff, emom, emom2):
+        """In place on emom and emom2, like evolve_second."""
+        for a in (emom, emom2):
+            assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+        B.check(B.lib().uasd_evolve_second_host(self._h, int(sdealgh), B.ptr(B.f64(beff)),
+                                                B.ptr(B.f64(b2eff)), B.ptr(emom), B.ptr(emom2)),
+                "uasd_evolve_second_host")
+
+    # -------------------------------------------------------- resident path
+    def sd_steps(self, sdealgh: int, nsteps: int):
+        B.check(B.lib().uasd_sd_steps(self._h, int(sdealgh), int(nsteps)), "uasd_sd_steps")
+
+    def field(self, with_energy: bool = True):
+        beff = np.zeros((self.M, self.N, 3))
+        en = C.c_double(0.0)
+        B.check(B.lib().uasd_effective_field(self._h, B.ptr(beff),
+                                             C.byref(en) if with_energy else None),
+                "uasd_effective_field")
+        return beff, en.value
+
+    def mc_sweeps(self, mode: str, temps, nsweeps: int, seed: int):
+        t = B.f64(np.broadcast_to(np.asarray(temps, dtype=np.float64), (self.M,)))
+        B.check(B.lib().uasd_mc_sweeps(self._h, mode.encode()[:1], B.ptr(t), int(nsweeps),
+                                       int(seed)), "uasd_mc_sweeps")
+
+    def measure(self, with_energy: bool = False):
+        """(msum (M,3) = sum_i emomM, energy (M,) in mRy or None)."""
+        ms = np.zeros((self.M, 3))
+        en = np.zeros(self.M) if with_energy else None
+        B.check(B.lib().uasd_measure(self._h, B.ptr(ms), B.ptr(en)), "uasd_measure")
+        return ms, en
+
+    def coloring(self):
+        nc = C.c_int(0)
+        col = np.zeros(self.N, dtype=np.int32)
+        B.check(B.lib().uasd_get_coloring(self._h, C.byref(nc), B.ptr(col)), "uasd_get_coloring")
+        return nc.value, col
+
+    def set_kernel_timing(self, enable: bool):
+        B.check(B.lib().uasd_set_kernel_timing(self._h, 1 if enable else 0), "uasd_set_kernel_timing")
+
+    def kernel_timing(self, kind: int):
+        n = C.c_int64(0)
+        ms = C.c_double(0.0)
+        B.check(B.lib().uasd_get_kernel_timing(self._h, kind, C.byref(n), C.byref(ms)),
+                "uasd_get_kernel_timing")
+        return n.value, ms.value
+
+
+def moment_update(mmom, emom, emom2, emomM):
+    """copym (updatemoments.f90:48-95): emom <- emom2, emomM <- emom2*mmom (in place)."""
+    emom[...] = emom2
+    emomM[...] = emom2 * np.asarray(mmom)[..., None]

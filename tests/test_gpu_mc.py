@@ -62,6 +62,31 @@ def test_coloured_sweeps_match_oracle(O, mode):
     assert np.max(np.abs(e_gpu - e)) <= 1e-12
 
 
+def test_flip_moves_reverse_the_spin(O):
+    """Regression: the spin-flip trial (newmom = -emom) must reverse the spin.
+    All spins along +z in a strong field along -z: every site that draws the
+    flip move (3*u0 >= 2) lowers its Zeeman energy and ends exactly at -z."""
+    sysd = CF.bcc_shells(4, [1.0])
+    N = sysd["lattice"].natom
+    eng = U.Engine(N, 1)
+    eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
+    eng.set_external_field(np.array([0.0, 0.0, -20000.0]))  # Zeeman gain >> exchange cost
+    e0 = np.zeros((1, N, 3))
+    e0[..., 2] = 1.0
+    eng.set_moments(e0, sysd["mmom"][None])
+    seed = 31337
+    eng.mc_sweeps("M", [10.0], 1, seed)
+    e, _ = eng.get_moments()
+    flips = 0
+    for i in range(N):
+        u = np.zeros(4)
+        O.lib().or_uniform4(seed, 0x21, 0, i, 0, u)
+        if int(np.floor(3 * u[0])) == 2:
+            flips += 1
+            assert np.array_equal(np.abs(e[0, i]), [0.0, 0.0, 1.0]) and e[0, i, 2] == -1.0, i
+    assert flips > N // 5
+
+
 def test_colouring_is_proper():
     sysd = CF.bcc_shells(5, CF.bcc5_couplings())
     eng = U.Engine(sysd["lattice"].natom, 1)

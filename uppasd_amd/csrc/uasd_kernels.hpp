@@ -545,9 +545,14 @@ __device__ __forceinline__ void trial_move(R e0, R e1, R e2, double delta, const
     n1 = (e1 + g1) / l;
     n2 = (e2 + g2) / l;
   } else {
-    n0 = -e0;
-    n1 = -e1;
-    n2 = -e2;
+    // Spin flip, newmom = -emom.  Written as a subtraction: hipcc (clang 22,
+    // -O1 and above) drops a bare negation of a loaded value when it merges
+    // the three branches of this function, turning every flip into a no-op.
+    // 0 - e equals -e except that a +0 component stays +0, which changes no
+    // energy or later arithmetic.
+    n0 = (R)0 - e0;
+    n1 = (R)0 - e1;
+    n2 = (R)0 - e2;
   }
 }
 

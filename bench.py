@@ -76,8 +76,25 @@ def cpu_baseline(sysd, seconds_target: float = 12.0):
                       f"at 300 K), {nsteps} steps in {dt:.1f} s on one host core"}
 
 
+def copy_calibration(mib: int = 512, reps: int = 5):
+    """A plain device-to-device copy of known size (torch), so that rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters can be scaled to bytes."""
+    import torch
+    n = mib * 1024 * 1024 // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read_per_dispatch": n * 4,
+                                           "bytes_written_per_dispatch": n * 4,
+                                           "dispatches": reps}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain copy kernel of known size (for counter scaling)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
@@ -86,6 +103,9 @@ def main():
     ap.add_argument("--solver", type=int, choices=[1, 5], default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
               "(the multi-GPU sharding is described in DESIGN.md but not run).", file=sys.stderr)

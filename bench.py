@@ -32,16 +32,17 @@ METRIC = "spin-updates/s (LLG step, MC sweep); FP32+batched vs FP64 at stated er
 HBM_PEAK_GBS = 8000.0
 
 
-def kernel_bytes(W: int, s: int, mmom_bytes: int) -> dict:
+def kernel_bytes(W: int, s: int) -> dict:
     """Algorithmic HBM bytes per site of the fused midpoint kernels.
 
-    predictor: gather e_j,m_j (3s + m), index stream 4W, class id 4, write p (3s)
-    corrector: gather p_j,m_j (3s + m), own e (3s), index 4W, class id 4, write e (3s)
+    Spins are stored as quads {x, y, z, mmom} of s-byte reals.
+    predictor: gather e_j quads (4s), index stream 4W, class id 4, write p (4s)
+    corrector: gather p_j quads (4s), own e (4s), index 4W, class id 4, write e (4s)
     Each site's own spin is one of its neighbours' gathers, so the gathered
-    planes count once per site (perfect reuse through L2 / MALL).
+    quads count once per site (perfect reuse through L2 / MALL).
     """
-    pred = 3 * s + mmom_bytes + 4 * W + 4 + 3 * s
-    corr = 3 * s + mmom_bytes + 3 * s + 4 * W + 4 + 3 * s
+    pred = 4 * s + 4 * W + 4 + 4 * s
+    corr = 4 * s + 4 * s + 4 * W + 4 + 4 * s
     return {"predictor": pred, "corrector": corr, "step": pred + corr}
 
 
@@ -108,7 +109,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the multi-GPU sharding is described in DESIGN.md but not run).", file=sys.stderr)
+              "(the engine is single-GPU; a multi-GPU run would shard the lattice by slabs).",
+              file=sys.stderr)
         sys.exit(2)
 
     import uppasd_amd as U
@@ -140,7 +142,7 @@ def main():
     eng.set_kernel_timing(False)
 
     s = 8 if args.precision == "fp64" else 4
-    kb = kernel_bytes(W, s, s)
+    kb = kernel_bytes(W, s)
     t_corr = ms_corr / max(n_corr, 1) * 1e-3
     t_pred = ms_pred / max(n_pred, 1) * 1e-3
     achieved = kb["corrector"] * N / t_corr / 1e9

@@ -25,7 +25,7 @@ HIP_COMMON = [f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
 UNITS = [
     ("uasd_kernels_f64.hip", ["-ffp-contract=off"]),
     ("uasd_kernels_f32.hip", ["-ffp-contract=fast"]),
-    ("uasd_ctx.hip", ["-ffp-contract=off"]),
+    ("uasd_engine.hip", ["-ffp-contract=off"]),
 ]
 
 

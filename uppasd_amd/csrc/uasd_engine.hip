@@ -1,11 +1,11 @@
-// uasd_ctx.hip -- context, uploads and the C-ABI of libuasd_hip.so.
+// uasd_engine.hip -- context, uploads and the C-ABI of libuasd_hip.so.
 //
 // A context mirrors the state the reference keeps in module arrays (ham_t,
 // MomentData, FieldData, the LLG inputs) on one device, in the layout
 // described in uasd_kernels.hpp.  Uploads convert the Fortran layout
-// (3,Natom,Mensemble) to three planes indexed site*M+ens and compress the
-// per-site coupling rows of ham%ncoup / ham%dm_vect / the anisotropy arrays to
-// their distinct rows.
+// (3,Natom,Mensemble) to quads {x, y, z, w} indexed site*M+ens (w = mmom for
+// spins) and compress the per-site coupling rows of ham%ncoup / ham%dm_vect /
+// the anisotropy arrays to their distinct rows.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -108,11 +108,10 @@ struct uasd_ctx {
   DevBuf an_type, an_vals;
   DevBuf cls, classes;
   DevBuf ext_u, ext_full;
-  DevBuf spins;   // e(3) p(3) b2(3) planes
-  DevBuf mm;      // mmom per q
-  DevBuf field;   // beff planes for the unfused path
-  DevBuf scratch; // emomM planes for host field calls
-  DevBuf noise;   // injected N(0,1) planes
+  DevBuf spins;   // quad arrays: e, p, b2
+  DevBuf field;   // beff quads for the unfused path
+  DevBuf scratch; // emomM quads (w = 1) for host field calls
+  DevBuf noise;   // injected N(0,1) quads
   DevBuf llgarr;  // lambda, temp, lande (double, N each)
   DevBuf energy;  // per q energy terms
   DevBuf partial, redout;
@@ -150,20 +149,14 @@ struct uasd_ctx {
 
 namespace {
 
-size_t plane_bytes(const uasd_ctx *c) { return (size_t)c->N * c->M * c->rsize; }
+// Quad buffers: one Quad<R> = {x, y, z, w} per q = site*M + ens.
+size_t quad_bytes(const uasd_ctx *c) { return (size_t)c->N * c->M * 4 * c->rsize; }
 
+// spins holds three quad arrays: 0 e (emom, w = mmom), 1 p (emom2, w = mmom),
+// 2 b2 (b2eff, w unused).
 template <typename R>
-R *plane(const uasd_ctx *c, const DevBuf &b, int k) {
-  return reinterpret_cast<R *>(static_cast<char *>(b.p) + (size_t)k * plane_bytes(c));
-}
-
-template <typename R>
-uasd::Spins<R> spins_at(const uasd_ctx *c, int first_plane) {
-  uasd::Spins<R> s;
-  s.x = plane<R>(c, c->spins, first_plane);
-  s.y = plane<R>(c, c->spins, first_plane + 1);
-  s.z = plane<R>(c, c->spins, first_plane + 2);
-  return s;
+uasd::Quad<R> *quads(const uasd_ctx *c, const DevBuf &b, int k = 0) {
+  return reinterpret_cast<uasd::Quad<R> *>(static_cast<char *>(b.p) + (size_t)k * quad_bytes(c));
 }
 
 template <typename R>
@@ -201,11 +194,7 @@ uasd::HamDev<R> ham_view(const uasd_ctx *c) {
   h.classes = c->classes.as<uasd::SiteClass>();
   h.ext_mode = c->ext_mode;
   h.ext_u = c->ext_u.as<R>();
-  if (c->ext_mode == 2) {
-    h.extx = c->ext_full.as<R>();
-    h.exty = h.extx + (size_t)c->N * c->M;
-    h.extz = h.exty + (size_t)c->N * c->M;
-  }
+  if (c->ext_mode == 2) h.ext_q = quads<R>(c, c->ext_full);
   return h;
 }
 
@@ -234,41 +223,62 @@ int upload_raw(uasd_ctx *c, DevBuf &dst, const std::vector<T> &src) {
   return UASD_OK;
 }
 
-// (3,N,M) Fortran array -> three planes indexed site*M+ens
-std::vector<double> to_planes(const double *a, int N, int M) {
-  std::vector<double> out((size_t)3 * N * M);
-  const size_t P = (size_t)N * M;
+// (3,N,M) Fortran array (+ w per (N,M) Fortran array, or a constant) -> quads
+std::vector<double> to_quads(const double *a, int N, int M, const double *w, double wconst) {
+  std::vector<double> out((size_t)4 * N * M);
   for (int k = 0; k < M; ++k)
     for (int i = 0; i < N; ++i) {
       const double *v = a + 3 * ((size_t)i + (size_t)N * k);
-      size_t q = (size_t)i * M + k;
-      out[q] = v[0];
-      out[P + q] = v[1];
-      out[2 * P + q] = v[2];
+      const size_t q = (size_t)i * M + k;
+      out[4 * q] = v[0];
+      out[4 * q + 1] = v[1];
+      out[4 * q + 2] = v[2];
+      out[4 * q + 3] = w ? w[(size_t)i + (size_t)N * k] : wconst;
     }
   return out;
 }
 
-template <typename R>
-int download_planes(uasd_ctx *c, const DevBuf &b, int first_plane, std::vector<double> &out) {
-  const size_t P = (size_t)c->N * c->M;
-  std::vector<R> tmp(3 * P);
-  HIP_TRY(hipMemcpyAsync(tmp.data(), plane<R>(c, b, first_plane), 3 * P * sizeof(R),
-                         hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  out.assign(tmp.begin(), tmp.end());
+// upload host quads (double) into quad array k of buffer b (converted to R)
+int upload_quads(uasd_ctx *c, DevBuf &b, int k, const std::vector<double> &hq) {
+  HIP_TRY(b.ensure(std::max(b.bytes, (size_t)(k + 1) * quad_bytes(c))));
+  if (c->prec == UASD_FP64) {
+    HIP_TRY(hipMemcpyAsync(quads<double>(c, b, k), hq.data(), hq.size() * 8,
+                           hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> f(hq.begin(), hq.end());
+    HIP_TRY(hipMemcpyAsync(quads<float>(c, b, k), f.data(), f.size() * 4, hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
   return UASD_OK;
 }
 
-// planes (3*P) -> Fortran (3,N,M); when m != null multiply by mmom (emomM)
-void from_planes(const std::vector<double> &pl, int N, int M, double *out,
-                 const std::vector<double> *m) {
-  const size_t P = (size_t)N * M;
+int download_quads(uasd_ctx *c, const DevBuf &b, int k, std::vector<double> &hq) {
+  const size_t n = (size_t)4 * c->N * c->M;
+  if (c->prec == UASD_FP64) {
+    hq.resize(n);
+    HIP_TRY(hipMemcpyAsync(hq.data(), quads<double>(c, b, k), n * 8, hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> f(n);
+    HIP_TRY(hipMemcpyAsync(f.data(), quads<float>(c, b, k), n * 4, hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    hq.assign(f.begin(), f.end());
+  }
+  return UASD_OK;
+}
+
+// quads -> Fortran (3,N,M); times_w gives emomM = xyz*w
+void from_quads(const std::vector<double> &hq, int N, int M, double *out, bool times_w) {
   for (int k = 0; k < M; ++k)
     for (int i = 0; i < N; ++i) {
-      size_t q = (size_t)i * M + k;
+      const size_t q = (size_t)i * M + k;
       double *v = out + 3 * ((size_t)i + (size_t)N * k);
-      for (int d = 0; d < 3; ++d) v[d] = m ? pl[d * P + q] * (*m)[q] : pl[d * P + q];
+      const double w = hq[4 * q + 3];
+      for (int d = 0; d < 3; ++d) v[d] = times_w ? hq[4 * q + d] * w : hq[4 * q + d];
     }
 }
 
@@ -401,11 +411,7 @@ uasd::LLGDev<R> llg_view(uasd_ctx *c) {
   p.noise_mode = c->noise_mode;
   p.seed = c->seed;
   p.step = c->step;
-  if (c->noise_mode == UASD_NOISE_INJECTED) {
-    p.nx = plane<R>(c, c->noise, 0);
-    p.ny = plane<R>(c, c->noise, 1);
-    p.nz = plane<R>(c, c->noise, 2);
-  }
+  if (c->noise_mode == UASD_NOISE_INJECTED) p.noise = quads<R>(c, c->noise);
   return p;
 }
 
@@ -418,79 +424,45 @@ int ready_for_llg(uasd_ctx *c) {
   return rebuild_classes(c);
 }
 
-// Planes in c->spins: 0-2 e, 3-5 p (emom2), 6-8 b2 (b2eff).
+// Quad arrays in c->spins: 0 e (emom), 1 p (emom2), 2 b2 (b2eff).
 template <typename R>
 int run_llg_half(uasd_ctx *c, int alg, int half, bool fused, bool write_e) {
   uasd::HamDev<R> h = ham_view<R>(c);
   uasd::LLGDev<R> prm = llg_view<R>(c);
-  uasd::Spins<R> e = spins_at<R>(c, 0), p = spins_at<R>(c, 3), b2 = spins_at<R>(c, 6);
-  const R *bx = nullptr, *by = nullptr, *bz = nullptr;
-  if (!fused) {
-    bx = plane<R>(c, c->field, 0);
-    by = plane<R>(c, c->field, 1);
-    bz = plane<R>(c, c->field, 2);
-  }
-  uasd::Spins<R> out = write_e ? e : p;
+  uasd::Quad<R> *e = quads<R>(c, c->spins, 0), *p = quads<R>(c, c->spins, 1),
+                *b2 = quads<R>(c, c->spins, 2);
+  const uasd::Quad<R> *beff = fused ? nullptr : quads<R>(c, c->field);
+  uasd::Quad<R> *out = write_e ? e : p;
   tic(c);
-  HIP_TRY(uasd::launch_llg<R>(c->stream, alg, half, fused, h, prm, e, p, b2, c->mm.as<R>(), bx,
-                              by, bz, out));
+  HIP_TRY(uasd::launch_llg<R>(c->stream, alg, half, fused, h, prm, e, p, b2, beff, out));
   return toc(c, half);
 }
 
 template <typename R>
 int run_field(uasd_ctx *c, bool from_scratch, bool want_energy) {
   uasd::HamDev<R> h = ham_view<R>(c);
-  uasd::MomSrc<R> src;
-  if (from_scratch) {
-    src.x = plane<R>(c, c->scratch, 0);
-    src.y = plane<R>(c, c->scratch, 1);
-    src.z = plane<R>(c, c->scratch, 2);
-    src.m = nullptr;
-  } else {
-    src.x = plane<R>(c, c->spins, 0);
-    src.y = plane<R>(c, c->spins, 1);
-    src.z = plane<R>(c, c->spins, 2);
-    src.m = c->mm.as<R>();
-  }
-  HIP_TRY(c->field.ensure(3 * plane_bytes(c)));
+  const uasd::Quad<R> *src = from_scratch ? quads<R>(c, c->scratch) : quads<R>(c, c->spins, 0);
+  HIP_TRY(c->field.ensure(quad_bytes(c)));
   double *en = nullptr;
   if (want_energy) {
     HIP_TRY(c->energy.ensure((size_t)c->N * c->M * sizeof(double)));
     en = c->energy.as<double>();
   }
   tic(c);
-  HIP_TRY(uasd::launch_field<R>(c->stream, h, src, plane<R>(c, c->field, 0),
-                                plane<R>(c, c->field, 1), plane<R>(c, c->field, 2), en));
+  HIP_TRY(uasd::launch_field<R>(c->stream, h, src, quads<R>(c, c->field), en));
   return toc(c, 2);
 }
 
-// Energy sum per ensemble of the per-q terms written by the field kernel.
+// FP64 sums per ensemble of emomM (and of the per-q energy terms written by
+// the field kernel when with_energy).
 template <typename R>
 int reduce_moments(uasd_ctx *c, bool with_energy, bool from_scratch, std::vector<double> &out) {
   const int nv = with_energy ? 4 : 3;
   const int nblocks = std::max(1, std::min(1024, (c->N + 4095) / 4096));
   HIP_TRY(c->partial.ensure((size_t)c->M * nblocks * nv * sizeof(double)));
   HIP_TRY(c->redout.ensure((size_t)c->M * nv * sizeof(double)));
-  uasd::Spins<R> e;
-  const R *mm;
-  if (from_scratch) {
-    e.x = plane<R>(c, c->scratch, 0);
-    e.y = plane<R>(c, c->scratch, 1);
-    e.z = plane<R>(c, c->scratch, 2);
-    mm = nullptr;
-  } else {
-    e = spins_at<R>(c, 0);
-    mm = c->mm.as<R>();
-  }
-  if (!mm) {
-    // emomM planes given directly: reduce against a unit moment
-    HIP_TRY(c->mcbuf.ensure(plane_bytes(c)));
-    std::vector<R> ones((size_t)c->N * c->M, (R)1);
-    HIP_TRY(hipMemcpyAsync(c->mcbuf.p, ones.data(), ones.size() * sizeof(R),
-                           hipMemcpyHostToDevice, c->stream));
-    mm = c->mcbuf.as<R>();
-  }
-  HIP_TRY(uasd::launch_reduce<R>(c->stream, c->N, c->M, e, mm,
+  const uasd::Quad<R> *s = from_scratch ? quads<R>(c, c->scratch) : quads<R>(c, c->spins, 0);
+  HIP_TRY(uasd::launch_reduce<R>(c->stream, c->N, c->M, s,
                                  with_energy ? c->energy.as<double>() : nullptr, nv,
                                  c->partial.as<double>(), nblocks, c->redout.as<double>()));
   out.resize((size_t)c->M * nv);
@@ -591,8 +563,8 @@ int uasd_ctx_create(uasd_ctx **out, int natom, int mensemble, int precision) {
                                        ", the library is built for gfx950");
   HIP_TRY(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
   c->stream = c->own_stream;
-  HIP_TRY(c->spins.ensure(9 * plane_bytes(c.get())));
-  HIP_TRY(hipMemsetAsync(c->spins.p, 0, 9 * plane_bytes(c.get()), c->stream));
+  HIP_TRY(c->spins.ensure(3 * quad_bytes(c.get())));
+  HIP_TRY(hipMemsetAsync(c->spins.p, 0, 3 * quad_bytes(c.get()), c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   *out = c.release();
   return UASD_OK;
@@ -795,7 +767,7 @@ int uasd_set_external_field(uasd_ctx *c, const double *field) {
   }
   c->ext_mode = 2;
   if (int r = upload_real(c, c->ext_u, c->ext_u_host)) return r;
-  return upload_real(c, c->ext_full, to_planes(field, N, M));
+  return upload_quads(c, c->ext_full, 0, to_quads(field, N, M, nullptr, 0.0));
 }
 
 int uasd_set_llg(uasd_ctx *c, const double *lambda1, const double *temp, const double *landeg,
@@ -835,7 +807,7 @@ int uasd_set_noise(uasd_ctx *c, int mode, uint64_t seed, const double *normals) 
     return fail(UASD_ERR_ARG, "unknown noise mode");
   if (mode == UASD_NOISE_INJECTED) {
     if (!normals) return fail(UASD_ERR_ARG, "injected noise needs the normals array");
-    if (int r = upload_real(c, c->noise, to_planes(normals, c->N, c->M))) return r;
+    if (int r = upload_quads(c, c->noise, 0, to_quads(normals, c->N, c->M, nullptr, 0.0))) return r;
   }
   if (mode == UASD_NOISE_PHILOX && (seed != c->seed || c->noise_mode != mode)) c->step = 0;
   c->noise_mode = mode;
@@ -846,82 +818,32 @@ int uasd_set_noise(uasd_ctx *c, int mode, uint64_t seed, const double *normals) 
 int uasd_set_moments(uasd_ctx *c, const double *emom, const double *mmom) {
   if (int r = check_ctx(c)) return r;
   if (!emom || !mmom) return fail(UASD_ERR_ARG, "uasd_set_moments: null array");
-  const int N = c->N, M = c->M;
-  std::vector<double> pl = to_planes(emom, N, M);
-  std::vector<double> mq((size_t)N * M);
-  for (int k = 0; k < M; ++k)
-    for (int i = 0; i < N; ++i) mq[(size_t)i * M + k] = mmom[(size_t)i + (size_t)N * k];
-  const size_t P = (size_t)N * M;
-  if (c->prec == UASD_FP64) {
-    HIP_TRY(hipMemcpyAsync(c->spins.p, pl.data(), 3 * P * 8, hipMemcpyHostToDevice, c->stream));
-  } else {
-    std::vector<float> f(pl.begin(), pl.end());
-    HIP_TRY(hipMemcpyAsync(c->spins.p, f.data(), 3 * P * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-  }
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  if (int r = upload_real(c, c->mm, mq)) return r;
+  std::vector<double> hq = to_quads(emom, c->N, c->M, mmom, 0.0);
+  if (int r = upload_quads(c, c->spins, 0, hq)) return r;
+  if (int r = upload_quads(c, c->spins, 1, hq)) return r;  // emom2 starts as emom
   c->have_state = true;
-  return UASD_OK;
-}
-
-static int get_mm_host(uasd_ctx *c, std::vector<double> &mq) {
-  const size_t P = (size_t)c->N * c->M;
-  if (c->prec == UASD_FP64) {
-    mq.resize(P);
-    HIP_TRY(hipMemcpyAsync(mq.data(), c->mm.p, P * 8, hipMemcpyDeviceToHost, c->stream));
-  } else {
-    std::vector<float> f(P);
-    HIP_TRY(hipMemcpyAsync(f.data(), c->mm.p, P * 4, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    mq.assign(f.begin(), f.end());
-  }
-  HIP_TRY(hipStreamSynchronize(c->stream));
   return UASD_OK;
 }
 
 int uasd_get_moments(uasd_ctx *c, double *emom, double *emomM) {
   if (int r = check_ctx(c)) return r;
   if (!c->have_state) return fail(UASD_ERR_STATE, "moments not set");
-  std::vector<double> pl, mq;
-  int r = c->prec == UASD_FP64 ? download_planes<double>(c, c->spins, 0, pl)
-                               : download_planes<float>(c, c->spins, 0, pl);
-  if (r) return r;
-  if (emom) from_planes(pl, c->N, c->M, emom, nullptr);
-  if (emomM) {
-    if ((r = get_mm_host(c, mq))) return r;
-    from_planes(pl, c->N, c->M, emomM, &mq);
-  }
+  std::vector<double> hq;
+  if (int r = download_quads(c, c->spins, 0, hq)) return r;
+  if (emom) from_quads(hq, c->N, c->M, emom, false);
+  if (emomM) from_quads(hq, c->N, c->M, emomM, true);
   return UASD_OK;
 }
 
 // ---------------------------------------------------------- drop-in (host)
-static int upload_planes_to(uasd_ctx *c, DevBuf &b, int first_plane, const double *a) {
-  std::vector<double> pl = to_planes(a, c->N, c->M);
-  const size_t P = (size_t)c->N * c->M;
-  HIP_TRY(b.ensure(std::max(b.bytes, (size_t)(first_plane + 3) * plane_bytes(c))));
-  if (c->prec == UASD_FP64) {
-    HIP_TRY(hipMemcpyAsync(plane<double>(c, b, first_plane), pl.data(), 3 * P * 8,
-                           hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-  } else {
-    std::vector<float> f(pl.begin(), pl.end());
-    HIP_TRY(hipMemcpyAsync(plane<float>(c, b, first_plane), f.data(), 3 * P * 4,
-                           hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-  }
-  return UASD_OK;
-}
-
-static int download_planes_to(uasd_ctx *c, const DevBuf &b, int first_plane, double *a,
-                              bool times_mmom) {
-  std::vector<double> pl, mq;
-  int r = c->prec == UASD_FP64 ? download_planes<double>(c, b, first_plane, pl)
-                               : download_planes<float>(c, b, first_plane, pl);
-  if (r) return r;
-  if (times_mmom && (r = get_mm_host(c, mq))) return r;
-  from_planes(pl, c->N, c->M, a, times_mmom ? &mq : nullptr);
-  return UASD_OK;
+// Host vec3 arrays are uploaded as quads; spin quads keep the resident mmom
+// in w, the others carry a constant.
+static int upload_vec_keep_w(uasd_ctx *c, int k, const double *a) {
+  std::vector<double> cur;
+  if (int r = download_quads(c, c->spins, 0, cur)) return r;  // w = mmom of the state
+  std::vector<double> hq = to_quads(a, c->N, c->M, nullptr, 0.0);
+  for (size_t q = 0; q < (size_t)c->N * c->M; ++q) hq[4 * q + 3] = cur[4 * q + 3];
+  return upload_quads(c, c->spins, k, hq);
 }
 
 int uasd_effective_field_host(uasd_ctx *c, const double *emomM, double *beff, double *energy) {
@@ -929,11 +851,13 @@ int uasd_effective_field_host(uasd_ctx *c, const double *emomM, double *beff, do
   if (!c->have_ex) return fail(UASD_ERR_STATE, "exchange not set");
   if (!emomM || !beff) return fail(UASD_ERR_ARG, "null array");
   if (int r = rebuild_classes(c)) return r;
-  if (int r = upload_planes_to(c, c->scratch, 0, emomM)) return r;
+  if (int r = upload_quads(c, c->scratch, 0, to_quads(emomM, c->N, c->M, nullptr, 1.0))) return r;
   int r = c->prec == UASD_FP64 ? run_field<double>(c, true, energy != nullptr)
                                : run_field<float>(c, true, energy != nullptr);
   if (r) return r;
-  if ((r = download_planes_to(c, c->field, 0, beff, false))) return r;
+  std::vector<double> hq;
+  if ((r = download_quads(c, c->field, 0, hq))) return r;
+  from_quads(hq, c->N, c->M, beff, false);
   if (energy) {
     std::vector<double> red;
     r = c->prec == UASD_FP64 ? reduce_moments<double>(c, true, true, red)
@@ -954,21 +878,25 @@ int uasd_evolve_first_host(uasd_ctx *c, int alg, const double *beff, double *emo
   if (!beff || !emom || !emom2 || !emomM || (alg == 5 && !b2eff))
     return fail(UASD_ERR_ARG, "null array");
   if (int r = ready_for_llg(c)) return r;
-  if (int r = upload_planes_to(c, c->field, 0, beff)) return r;
-  if (int r = upload_planes_to(c, c->spins, 0, emom)) return r;
+  if (int r = upload_quads(c, c->field, 0, to_quads(beff, c->N, c->M, nullptr, 0.0))) return r;
+  if (int r = upload_vec_keep_w(c, 0, emom)) return r;
   int r = c->prec == UASD_FP64 ? run_llg_half<double>(c, alg, 0, false, false)
                                : run_llg_half<float>(c, alg, 0, false, false);
   if (r) return r;
+  std::vector<double> pq;
+  if ((r = download_quads(c, c->spins, 1, pq))) return r;
   if (alg == UASD_SDE_DEPONDT) {
-    // depondt.f90:182-187: emom2 = m(t), emom = emomM/mmom = rotated, b2eff = bdup
+    // depondt.f90:182-187: emom2 = m(t), emom = rotated, emomM = emom*mmom, b2eff = bdup
     std::memcpy(emom2, emom, sizeof(double) * 3 * (size_t)c->N * c->M);
-    if ((r = download_planes_to(c, c->spins, 3, emom, false))) return r;
-    if ((r = download_planes_to(c, c->spins, 3, emomM, true))) return r;
-    if ((r = download_planes_to(c, c->spins, 6, b2eff, false))) return r;
+    from_quads(pq, c->N, c->M, emom, false);
+    from_quads(pq, c->N, c->M, emomM, true);
+    std::vector<double> bq;
+    if ((r = download_quads(c, c->spins, 2, bq))) return r;
+    from_quads(bq, c->N, c->M, b2eff, false);
   } else {
     // midpoint.f90:167-175: emom2 = et', emomM = et'*mmom, emom unchanged
-    if ((r = download_planes_to(c, c->spins, 3, emom2, false))) return r;
-    if ((r = download_planes_to(c, c->spins, 3, emomM, true))) return r;
+    from_quads(pq, c->N, c->M, emom2, false);
+    from_quads(pq, c->N, c->M, emomM, true);
   }
   return UASD_OK;
 }
@@ -980,21 +908,23 @@ int uasd_evolve_second_host(uasd_ctx *c, int alg, const double *beff, const doub
     return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint) or 5 (Depondt)");
   if (!beff || !emom || !emom2 || (alg == 5 && !b2eff)) return fail(UASD_ERR_ARG, "null array");
   if (int r = ready_for_llg(c)) return r;
-  if (int r = upload_planes_to(c, c->field, 0, beff)) return r;
+  if (int r = upload_quads(c, c->field, 0, to_quads(beff, c->N, c->M, nullptr, 0.0))) return r;
   if (alg == UASD_SDE_DEPONDT) {
     // reference on entry: emom = predicted, emom2 = m(t)
-    if (int r = upload_planes_to(c, c->spins, 3, emom)) return r;
-    if (int r = upload_planes_to(c, c->spins, 0, emom2)) return r;
-    if (int r = upload_planes_to(c, c->spins, 6, b2eff)) return r;
+    if (int r = upload_vec_keep_w(c, 1, emom)) return r;
+    if (int r = upload_vec_keep_w(c, 0, emom2)) return r;
+    if (int r = upload_quads(c, c->spins, 2, to_quads(b2eff, c->N, c->M, nullptr, 0.0))) return r;
   } else {
-    if (int r = upload_planes_to(c, c->spins, 0, emom)) return r;
-    if (int r = upload_planes_to(c, c->spins, 3, emom2)) return r;
+    if (int r = upload_vec_keep_w(c, 0, emom)) return r;
+    if (int r = upload_vec_keep_w(c, 1, emom2)) return r;
   }
   int r = c->prec == UASD_FP64 ? run_llg_half<double>(c, alg, 1, false, false)
                                : run_llg_half<float>(c, alg, 1, false, false);
   if (r) return r;
   if (alg == UASD_SDE_DEPONDT) std::memcpy(emom, emom2, sizeof(double) * 3 * (size_t)c->N * c->M);
-  if ((r = download_planes_to(c, c->spins, 3, emom2, false))) return r;
+  std::vector<double> pq;
+  if ((r = download_quads(c, c->spins, 1, pq))) return r;
+  from_quads(pq, c->N, c->M, emom2, false);
   c->step += 1;
   return UASD_OK;
 }
@@ -1025,7 +955,11 @@ int uasd_effective_field(uasd_ctx *c, double *beff, double *energy) {
   int r = c->prec == UASD_FP64 ? run_field<double>(c, false, energy != nullptr)
                                : run_field<float>(c, false, energy != nullptr);
   if (r) return r;
-  if (beff && (r = download_planes_to(c, c->field, 0, beff, false))) return r;
+  if (beff) {
+    std::vector<double> hq;
+    if ((r = download_quads(c, c->field, 0, hq))) return r;
+    from_quads(hq, c->N, c->M, beff, false);
+  }
   if (energy) {
     std::vector<double> red;
     r = c->prec == UASD_FP64 ? reduce_moments<double>(c, true, false, red)
@@ -1083,11 +1017,11 @@ int uasd_mc_sweeps(uasd_ctx *c, char mode, const double *temps, int64_t nsweeps,
       mp.nsites = c->color_ptr[col + 1] - c->color_ptr[col];
       tic(c);
       if (c->prec == UASD_FP64) {
-        HIP_TRY(uasd::launch_mc<double>(c->stream, ham_view<double>(c), mp, spins_at<double>(c, 0),
-                                        c->mm.as<double>()));
+        HIP_TRY(uasd::launch_mc<double>(c->stream, ham_view<double>(c), mp,
+                                        quads<double>(c, c->spins, 0)));
       } else {
-        HIP_TRY(uasd::launch_mc<float>(c->stream, ham_view<float>(c), mp, spins_at<float>(c, 0),
-                                       c->mm.as<float>()));
+        HIP_TRY(uasd::launch_mc<float>(c->stream, ham_view<float>(c), mp,
+                                       quads<float>(c, c->spins, 0)));
       }
       if (int r = toc(c, 3)) return r;
     }

@@ -5,8 +5,10 @@
 // Fortran does) and uasd_kernels_f32.hip.
 //
 // Data layout in HBM (one context = one lattice, M ensembles):
-//   spins      three planes x,y,z of R, index q = site*M + ens   (SoA)
-//   mmom       R per q
+//   spins      quads {x, y, z, m} of R per q = site*M + ens: the unit vector
+//              and the moment magnitude, so one 16/32-byte load gathers a
+//              neighbour's emomM = e*m (bit-identical to the reference's
+//              stored emomM = emom2*mmom2)
 //   neighbours slot-major ELL int32 nbr[j*N + site], 0-based
 //   couplings  slot-major per Hamiltonian row cpl[j*nrows + row]; rows are
 //              the distinct (length, coupling) rows of ham%ncoup, so a
@@ -14,7 +16,9 @@
 //   site class int32 per site -> {exchange row, DM row, anisotropy row}
 // Lanes of a wavefront take consecutive q, so with M > 1 the lanes of one
 // site share its neighbour indices (one broadcast load) and gather
-// neighbouring spins with unit stride.
+// neighbouring spins with unit stride.  Workgroups are mapped to the eight
+// XCDs in contiguous ranges of sites, so each XCD's L2 holds the spins its
+// own neighbour gathers reuse.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -33,6 +37,13 @@ constexpr double kDblTolerance = (double)1e-14f;  // parameters.f90:12 (default 
 constexpr uint32_t kTagLLG = 0x11u;  // Philox tag of the thermal field
 constexpr uint32_t kTagMCU = 0x21u;  // MC uniforms
 constexpr uint32_t kTagMCG = 0x22u;  // MC Gaussian trial step
+
+constexpr int kXCD = 8;  // MI355X: 8 accelerator complex dies, each with its own L2
+
+template <typename R>
+struct alignas(4 * sizeof(R)) Quad {
+  R x, y, z, w;
+};
 
 struct SiteClass {
   int ex_row, dm_row, an_row, pad;
@@ -58,46 +69,25 @@ struct HamDev {
   // site classes
   const int *__restrict__ cls;
   const SiteClass *__restrict__ classes;
-  // external field: 0 none, 1 uniform per ensemble (ext_u[3*ens+c]), 2 full planes
+  // external field: 0 none, 1 uniform per ensemble (ext_u[3*ens+c]), 2 per q
   int ext_mode;
   const R *__restrict__ ext_u;
-  const R *__restrict__ extx, *__restrict__ exty, *__restrict__ extz;
+  const Quad<R> *__restrict__ ext_q;
 };
 
-// Spin planes.  No __restrict__: the corrector writes the plane it read at
-// the same index, and MC sweeps gather from the planes they update.
-template <typename R>
-struct Spins {
-  R *x;
-  R *y;
-  R *z;
-};
-
-// Where gathered moments come from: unit vectors times mmom (resident
-// state: emomM = emom*mmom bit-for-bit), or an emomM array.
-template <typename R>
-struct MomSrc {
-  const R *x;
-  const R *y;
-  const R *z;
-  const R *m;  // nullptr: x,y,z already hold emomM
-  __device__ __forceinline__ void load(size_t q, R &a, R &b, R &c) const {
-    if (m) {
-      R mm = m[q];
-      a = x[q] * mm;
-      b = y[q] * mm;
-      c = z[q] * mm;
-    } else {
-      a = x[q];
-      b = y[q];
-      c = z[q];
-    }
-  }
-};
+// Block index of this workgroup in a grid laid out so that XCD k (which
+// receives workgroups k, k+8, k+16, ...) works on the k-th contiguous eighth
+// of the logical blocks.  The launch pads the grid to a multiple of 8.
+__device__ __forceinline__ int64_t xcd_block() {
+  const uint32_t per = gridDim.x / kXCD;
+  const uint32_t b = blockIdx.x;
+  return (int64_t)(b % kXCD) * per + (b / kXCD);
+}
 
 // ------------------------------------------------------------------- Philox
 __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                                              uint32_t k0, uint32_t k1, uint32_t out[4]) {
+                                              uint32_t k0, uint32_t k1, uint32_t &o0,
+                                              uint32_t &o1, uint32_t &o2, uint32_t &o3) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
     uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
@@ -111,16 +101,19 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;
   }
-  out[0] = c0;
-  out[1] = c1;
-  out[2] = c2;
-  out[3] = c3;
+  o0 = c0;
+  o1 = c1;
+  o2 = c2;
+  o3 = c3;
 }
 
+// Counter layout shared with the oracle: (site, ens, step_lo, tag | step_hi << 8),
+// key = seed.
 __device__ __forceinline__ void counter_words(uint64_t seed, uint32_t tag, uint64_t step,
-                                              uint32_t site, uint32_t ens, uint32_t w[4]) {
+                                              uint32_t site, uint32_t ens, uint32_t &w0,
+                                              uint32_t &w1, uint32_t &w2, uint32_t &w3) {
   philox4x32_10(site, ens, (uint32_t)step, (tag & 0xffu) | ((uint32_t)(step >> 32) << 8),
-                (uint32_t)seed, (uint32_t)(seed >> 32), w);
+                (uint32_t)seed, (uint32_t)(seed >> 32), w0, w1, w2, w3);
 }
 
 template <typename R>
@@ -132,11 +125,11 @@ struct Noise<double> {
   __device__ __forceinline__ static void normal3(uint64_t seed, uint32_t tag, uint64_t step,
                                                  uint32_t site, uint32_t ens, double &a,
                                                  double &b, double &c) {
-    uint32_t w[4];
-    counter_words(seed, tag, step, site, ens, w);
+    uint32_t w0, w1, w2, w3;
+    counter_words(seed, tag, step, site, ens, w0, w1, w2, w3);
     const double s = 2.3283064365386963e-10, two_pi = 6.283185307179586;
-    double u0 = ((double)w[0] + 0.5) * s, u1 = ((double)w[1] + 0.5) * s;
-    double u2 = ((double)w[2] + 0.5) * s, u3 = ((double)w[3] + 0.5) * s;
+    double u0 = ((double)w0 + 0.5) * s, u1 = ((double)w1 + 0.5) * s;
+    double u2 = ((double)w2 + 0.5) * s, u3 = ((double)w3 + 0.5) * s;
     double r0 = sqrt(-2.0 * log(u0)), r1 = sqrt(-2.0 * log(u2));
     double t0 = two_pi * u1, t1 = two_pi * u3;
     a = r0 * cos(t0);
@@ -150,11 +143,11 @@ struct Noise<float> {
   __device__ __forceinline__ static void normal3(uint64_t seed, uint32_t tag, uint64_t step,
                                                  uint32_t site, uint32_t ens, float &a, float &b,
                                                  float &c) {
-    uint32_t w[4];
-    counter_words(seed, tag, step, site, ens, w);
+    uint32_t w0, w1, w2, w3;
+    counter_words(seed, tag, step, site, ens, w0, w1, w2, w3);
     const float s = 5.9604644775390625e-08f, two_pi = 6.28318530717958647692f;
-    float u0 = ((float)(w[0] >> 8) + 0.5f) * s, u1 = ((float)(w[1] >> 8) + 0.5f) * s;
-    float u2 = ((float)(w[2] >> 8) + 0.5f) * s, u3 = ((float)(w[3] >> 8) + 0.5f) * s;
+    float u0 = ((float)(w0 >> 8) + 0.5f) * s, u1 = ((float)(w1 >> 8) + 0.5f) * s;
+    float u2 = ((float)(w2 >> 8) + 0.5f) * s, u3 = ((float)(w3 >> 8) + 0.5f) * s;
     float r0 = sqrtf(-2.0f * logf(u0)), r1 = sqrtf(-2.0f * logf(u2));
     float t0 = two_pi * u1, t1 = two_pi * u3;
     a = r0 * cosf(t0);
@@ -164,23 +157,29 @@ struct Noise<float> {
 };
 
 __device__ __forceinline__ void uniform4(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
-                                         uint32_t ens, double u[4]) {
-  uint32_t w[4];
-  counter_words(seed, tag, step, site, ens, w);
+                                         uint32_t ens, double &u0, double &u1, double &u2,
+                                         double &u3) {
+  uint32_t w0, w1, w2, w3;
+  counter_words(seed, tag, step, site, ens, w0, w1, w2, w3);
   const double s = 2.3283064365386963e-10;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) u[i] = ((double)w[i] + 0.5) * s;
+  u0 = ((double)w0 + 0.5) * s;
+  u1 = ((double)w1 + 0.5) * s;
+  u2 = ((double)w2 + 0.5) * s;
+  u3 = ((double)w3 + 0.5) * s;
 }
 
 // ------------------------------------------------------------- field terms
 // hamiltonianactions.f90:431-466 (exchange) and 546-580 (DM), summed in list
-// order; 842-879 / 883-919 anisotropy; 183-248 assembly.
+// order; 842-879 / 883-919 anisotropy; 183-248 assembly.  The moments are
+// gathered as q.xyz * q.w (emomM).
+
 template <typename R>
-struct FieldParts {
-  R s0, s1, s2;  // beff_s (pair + uniaxial)
-  R q0, q1, q2;  // beff_q (cubic * sb for type 7)
-  R e0, e1, e2;  // external field
-};
+__device__ __forceinline__ void moment(const Quad<R> *__restrict__ s, size_t q, R &a, R &b, R &c) {
+  const Quad<R> v = s[q];
+  a = v.x * v.w;
+  b = v.y * v.w;
+  c = v.z * v.w;
+}
 
 template <typename R>
 __device__ __forceinline__ void uniaxial(const HamDev<R> &h, int row, R m0, R m1, R m2, R &f0,
@@ -205,12 +204,14 @@ __device__ __forceinline__ void cubic(const HamDev<R> &h, int row, R m0, R m1, R
   f2 = f2 + (R)2.0 * k1 * m2 * (s0 + s1) + (R)2.0 * k2 * m2 * (s0 * s1);
 }
 
+// Field parts of one site: s = beff_s (pair terms + uniaxial), qf = beff_q
+// (type-7 cubic part times sb), x = external field.
 template <typename R>
-__device__ __forceinline__ FieldParts<R> site_field(const HamDev<R> &h, const MomSrc<R> &src,
-                                                    int site, int ens) {
-  FieldParts<R> p;
-  p.s0 = p.s1 = p.s2 = (R)0;
-  p.q0 = p.q1 = p.q2 = (R)0;
+__device__ __forceinline__ void site_field(const HamDev<R> &h, const Quad<R> *__restrict__ src,
+                                           int site, int ens, R &s0, R &s1, R &s2, R &qf0,
+                                           R &qf1, R &qf2, R &x0, R &x1, R &x2) {
+  s0 = s1 = s2 = (R)0;
+  qf0 = qf1 = qf2 = (R)0;
   const SiteClass sc = h.classes[h.cls[site]];
   const int M = h.M, N = h.N;
   {
@@ -218,70 +219,72 @@ __device__ __forceinline__ FieldParts<R> site_field(const HamDev<R> &h, const Mo
     const int *nb = h.nbr + site;
     const R *cp = h.cpl + sc.ex_row;
     for (int j = 0; j < len; ++j) {
-      int n = nb[(size_t)j * N];
-      R J = cp[(size_t)j * h.ex_rows];
+      const int n = nb[(size_t)j * N];
+      const R J = cp[(size_t)j * h.ex_rows];
       R m0, m1, m2;
-      src.load((size_t)n * M + ens, m0, m1, m2);
-      p.s0 = p.s0 + J * m0;
-      p.s1 = p.s1 + J * m1;
-      p.s2 = p.s2 + J * m2;
+      moment(src, (size_t)n * M + ens, m0, m1, m2);
+      s0 = s0 + J * m0;
+      s1 = s1 + J * m1;
+      s2 = s2 + J * m2;
     }
   }
   if (h.Wd > 0 && sc.dm_row >= 0) {
     const int len = h.dm_len[sc.dm_row];
     const int *nb = h.dnbr + site;
     for (int j = 0; j < len; ++j) {
-      int n = nb[(size_t)j * N];
-      size_t r = (size_t)j * h.dm_rows + sc.dm_row;
-      R D0 = h.dvx[r], D1 = h.dvy[r], D2 = h.dvz[r];
+      const int n = nb[(size_t)j * N];
+      const size_t r = (size_t)j * h.dm_rows + sc.dm_row;
+      const R D0 = h.dvx[r], D1 = h.dvy[r], D2 = h.dvz[r];
       R m0, m1, m2;
-      src.load((size_t)n * M + ens, m0, m1, m2);
-      p.s0 = p.s0 + D2 * m1 - D1 * m2;
-      p.s1 = p.s1 + D0 * m2 - D2 * m0;
-      p.s2 = p.s2 + D1 * m0 - D0 * m1;
+      moment(src, (size_t)n * M + ens, m0, m1, m2);
+      s0 = s0 + D2 * m1 - D1 * m2;
+      s1 = s1 + D0 * m2 - D2 * m0;
+      s2 = s2 + D1 * m0 - D0 * m1;
     }
   }
   if (sc.an_row >= 0) {
     const int t = h.an_type[sc.an_row];
     if (t == 1 || t == 2 || t == 7) {
       R m0, m1, m2;
-      src.load((size_t)site * M + ens, m0, m1, m2);
+      moment(src, (size_t)site * M + ens, m0, m1, m2);
       if (t == 1) {
-        uniaxial(h, sc.an_row, m0, m1, m2, p.s0, p.s1, p.s2);
+        uniaxial(h, sc.an_row, m0, m1, m2, s0, s1, s2);
       } else if (t == 2) {
-        cubic(h, sc.an_row, m0, m1, m2, p.s0, p.s1, p.s2);
+        cubic(h, sc.an_row, m0, m1, m2, s0, s1, s2);
       } else {
         R t0 = 0, t1 = 0, t2 = 0;
-        uniaxial(h, sc.an_row, m0, m1, m2, p.s0, p.s1, p.s2);
+        uniaxial(h, sc.an_row, m0, m1, m2, s0, s1, s2);
         cubic(h, sc.an_row, m0, m1, m2, t0, t1, t2);
-        R sb = h.an_sb[sc.an_row];
-        p.q0 = p.q0 + t0 * sb;
-        p.q1 = p.q1 + t1 * sb;
-        p.q2 = p.q2 + t2 * sb;
+        const R sb = h.an_sb[sc.an_row];
+        qf0 = qf0 + t0 * sb;
+        qf1 = qf1 + t1 * sb;
+        qf2 = qf2 + t2 * sb;
       }
     }
   }
   if (h.ext_mode == 1) {
-    p.e0 = h.ext_u[3 * ens];
-    p.e1 = h.ext_u[3 * ens + 1];
-    p.e2 = h.ext_u[3 * ens + 2];
+    x0 = h.ext_u[3 * ens];
+    x1 = h.ext_u[3 * ens + 1];
+    x2 = h.ext_u[3 * ens + 2];
   } else if (h.ext_mode == 2) {
-    size_t q = (size_t)site * M + ens;
-    p.e0 = h.extx[q];
-    p.e1 = h.exty[q];
-    p.e2 = h.extz[q];
+    const Quad<R> v = h.ext_q[(size_t)site * M + ens];
+    x0 = v.x;
+    x1 = v.y;
+    x2 = v.z;
   } else {
-    p.e0 = p.e1 = p.e2 = (R)0;
+    x0 = x1 = x2 = (R)0;
   }
-  return p;
 }
 
 // beff = 0 + beff1 + beff2, beff2 = beff_q + ext + 0  (hamiltonianactions.f90:240-243)
 template <typename R>
-__device__ __forceinline__ void total_field(const FieldParts<R> &p, R &b0, R &b1, R &b2) {
-  b0 = (R)0 + p.s0 + (p.q0 + p.e0 + (R)0);
-  b1 = (R)0 + p.s1 + (p.q1 + p.e1 + (R)0);
-  b2 = (R)0 + p.s2 + (p.q2 + p.e2 + (R)0);
+__device__ __forceinline__ void total_field(const HamDev<R> &h, const Quad<R> *__restrict__ src,
+                                            int site, int ens, R &b0, R &b1, R &b2) {
+  R s0, s1, s2, q0, q1, q2, x0, x1, x2;
+  site_field(h, src, site, ens, s0, s1, s2, q0, q1, q2, x0, x1, x2);
+  b0 = (R)0 + s0 + (q0 + x0 + (R)0);
+  b1 = (R)0 + s1 + (q1 + x1 + (R)0);
+  b2 = (R)0 + s2 + (q2 + x2 + (R)0);
 }
 
 // ------------------------------------------------------------ LLG params
@@ -293,7 +296,7 @@ struct LLGDev {
   int noise_mode;
   uint64_t seed;
   uint64_t step;
-  const R *nx, *ny, *nz;  // injected N(0,1) planes
+  const Quad<R> *noise;  // injected N(0,1) per q
   __device__ __forceinline__ double lambda(int s) const { return lam ? lam[s] : lam0; }
   __device__ __forceinline__ double temperature(int s) const { return temp ? temp[s] : temp0; }
   __device__ __forceinline__ double landeg(int s) const { return lande ? lande[s] : lande0; }
@@ -301,9 +304,10 @@ struct LLGDev {
     if (noise_mode == 1) {
       Noise<R>::normal3(seed, kTagLLG, step, (uint32_t)site, (uint32_t)ens, a, b, c);
     } else if (noise_mode == 2) {
-      a = nx[q];
-      b = ny[q];
-      c = nz[q];
+      const Quad<R> v = noise[q];
+      a = v.x;
+      b = v.y;
+      c = v.z;
     } else {
       a = b = c = (R)0;
     }
@@ -401,43 +405,31 @@ __device__ __forceinline__ void midpoint_solve(R A0, R A1, R A2, R e0, R e1, R e
   t2 = a0 * (A2 * A0 + A1) + a1 * (A2 * A1 - A0) + a2 * ((R)1 + A2 * A2);
 }
 
-}  // namespace uasd
-
 // =================================================================== kernels
-namespace uasd {
 
-// Moments of the resident state as a gather source.
+// effective_field_full over all (site, ens): beff (quads, w unused) and, when
+// en != null, the per-site field-routine energy -m.(0.5*(bs+2bq+2bext)) in FP64.
 template <typename R>
-__device__ __forceinline__ MomSrc<R> src_of(const Spins<R> &s, const R *mm) {
-  MomSrc<R> m;
-  m.x = s.x;
-  m.y = s.y;
-  m.z = s.z;
-  m.m = mm;
-  return m;
-}
-
-// effective_field_full over all (site, ens): beff planes and, when en != null,
-// the per-site field-routine energy term -m.(0.5*(bs+2bq+2bext)) in FP64.
-template <typename R>
-__global__ void __launch_bounds__(256) k_field(HamDev<R> h, MomSrc<R> src, R *bx, R *by, R *bz,
-                                               double *en) {
+__global__ void __launch_bounds__(256) k_field(HamDev<R> h, const Quad<R> *__restrict__ src,
+                                               Quad<R> *__restrict__ beff, double *en) {
   const int64_t total = (int64_t)h.N * h.M;
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t q = xcd_block() * blockDim.x + threadIdx.x;
   if (q >= total) return;
   const int site = (int)(q / h.M), ens = (int)(q - (int64_t)site * h.M);
-  FieldParts<R> p = site_field(h, src, site, ens);
-  R b0, b1, b2;
-  total_field(p, b0, b1, b2);
-  bx[q] = b0;
-  by[q] = b1;
-  bz[q] = b2;
+  R s0, s1, s2, q0, q1, q2, x0, x1, x2;
+  site_field(h, src, site, ens, s0, s1, s2, q0, q1, q2, x0, x1, x2);
+  Quad<R> b;
+  b.x = (R)0 + s0 + (q0 + x0 + (R)0);
+  b.y = (R)0 + s1 + (q1 + x1 + (R)0);
+  b.z = (R)0 + s2 + (q2 + x2 + (R)0);
+  b.w = (R)0;
+  beff[q] = b;
   if (en) {
     R m0, m1, m2;
-    src.load((size_t)q, m0, m1, m2);
-    double t0 = 0.5 * ((double)p.s0 + 2.0 * (double)p.q0 + 2.0 * (double)p.e0 + 0.0);
-    double t1 = 0.5 * ((double)p.s1 + 2.0 * (double)p.q1 + 2.0 * (double)p.e1 + 0.0);
-    double t2 = 0.5 * ((double)p.s2 + 2.0 * (double)p.q2 + 2.0 * (double)p.e2 + 0.0);
+    moment(src, (size_t)q, m0, m1, m2);
+    double t0 = 0.5 * ((double)s0 + 2.0 * (double)q0 + 2.0 * (double)x0 + 0.0);
+    double t1 = 0.5 * ((double)s1 + 2.0 * (double)q1 + 2.0 * (double)x1 + 0.0);
+    double t2 = 0.5 * ((double)s2 + 2.0 * (double)q2 + 2.0 * (double)x2 + 0.0);
     en[q] = -(double)m0 * t0 - (double)m1 * t1 - (double)m2 * t2;
   }
 }
@@ -446,30 +438,30 @@ __global__ void __launch_bounds__(256) k_field(HamDev<R> h, MomSrc<R> src, R *bx
 //   ALG 1 midpoint (midpoint.f90), 5 Depondt (depondt.f90)
 //   HALF 0 predictor (evolve_first), 1 corrector (evolve_second)
 //   FUSED: compute the field of the gathered moments in-kernel; otherwise read
-//   beff planes written by k_field.
+//   beff quads written by k_field.
 // Buffers: e = emom at step start, p = predicted emom (emom2 after the first
-// half), b2 = Depondt's b2eff.  The corrector writes the new emom into `out`
-// (e when moment_update is fused, the emom2 array otherwise).
+// half), b2 = Depondt's b2eff.  Quads carry mmom in w.  The corrector writes
+// the new emom into `out` (e when moment_update is fused, p otherwise).
 template <typename R, int ALG, int HALF, bool FUSED>
 __global__ void __launch_bounds__(256)
-    k_llg(HamDev<R> h, LLGDev<R> prm, Spins<R> e, Spins<R> p, Spins<R> b2, const R *__restrict__ mm,
-          const R *__restrict__ bx, const R *__restrict__ by, const R *__restrict__ bz,
-          Spins<R> out) {
+    k_llg(HamDev<R> h, LLGDev<R> prm, const Quad<R> *e, Quad<R> *p, Quad<R> *b2,
+          const Quad<R> *__restrict__ beff, Quad<R> *out) {
   const int64_t total = (int64_t)h.N * h.M;
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t q = xcd_block() * blockDim.x + threadIdx.x;
   if (q >= total) return;
   const int site = (int)(q / h.M), ens = (int)(q - (int64_t)site * h.M);
   R b0, b1, b2v;
   if (FUSED) {
-    FieldParts<R> fp = site_field(h, HALF == 0 ? src_of(e, mm) : src_of(p, mm), site, ens);
-    total_field(fp, b0, b1, b2v);
+    total_field(h, HALF == 0 ? e : (const Quad<R> *)p, site, ens, b0, b1, b2v);
   } else {
-    b0 = bx[q];
-    b1 = by[q];
-    b2v = bz[q];
+    const Quad<R> b = beff[q];
+    b0 = b.x;
+    b1 = b.y;
+    b2v = b.z;
   }
   const double lam = prm.lambda(site), T = prm.temperature(site);
-  const R m = mm[q];
+  const Quad<R> eq = e[q];
+  const R m = eq.w;
   R n0, n1, n2;
   prm.normals(site, ens, (size_t)q, n0, n1, n2);
   if (ALG == 5) {
@@ -477,49 +469,38 @@ __global__ void __launch_bounds__(256)
     const R t0 = n0 * sig, t1 = n1 * sig, t2 = n2 * sig;
     const R bl0 = b0 + t0, bl1 = b1 + t1, bl2 = b2v + t2;
     if (HALF == 0) {
-      const R e0 = e.x[q], e1 = e.y[q], e2 = e.z[q];
       R d0, d1, d2, o0, o1, o2;
-      depondt_bdup((R)lam, e0, e1, e2, bl0, bl1, bl2, d0, d1, d2);
-      rodrigues(d0, d1, d2, (R)prm.dt, (R)lam, e0, e1, e2, o0, o1, o2);
-      p.x[q] = o0;
-      p.y[q] = o1;
-      p.z[q] = o2;
-      b2.x[q] = d0;
-      b2.y[q] = d1;
-      b2.z[q] = d2;
+      depondt_bdup((R)lam, eq.x, eq.y, eq.z, bl0, bl1, bl2, d0, d1, d2);
+      rodrigues(d0, d1, d2, (R)prm.dt, (R)lam, eq.x, eq.y, eq.z, o0, o1, o2);
+      p[q] = Quad<R>{o0, o1, o2, m};
+      b2[q] = Quad<R>{d0, d1, d2, (R)0};
     } else {
-      const R p0 = p.x[q], p1 = p.y[q], p2 = p.z[q];
+      const Quad<R> pq = p[q];
+      const Quad<R> bq = b2[q];
       R d0, d1, d2, o0, o1, o2;
-      depondt_bdup((R)lam, p0, p1, p2, bl0, bl1, bl2, d0, d1, d2);
-      d0 = (R)0.5 * d0 + (R)0.5 * b2.x[q];
-      d1 = (R)0.5 * d1 + (R)0.5 * b2.y[q];
-      d2 = (R)0.5 * d2 + (R)0.5 * b2.z[q];
-      const R e0 = e.x[q], e1 = e.y[q], e2 = e.z[q];
-      rodrigues(d0, d1, d2, (R)prm.dt, (R)lam, e0, e1, e2, o0, o1, o2);
-      out.x[q] = o0;
-      out.y[q] = o1;
-      out.z[q] = o2;
+      depondt_bdup((R)lam, pq.x, pq.y, pq.z, bl0, bl1, bl2, d0, d1, d2);
+      d0 = (R)0.5 * d0 + (R)0.5 * bq.x;
+      d1 = (R)0.5 * d1 + (R)0.5 * bq.y;
+      d2 = (R)0.5 * d2 + (R)0.5 * bq.z;
+      rodrigues(d0, d1, d2, (R)prm.dt, (R)lam, eq.x, eq.y, eq.z, o0, o1, o2);
+      out[q] = Quad<R>{o0, o1, o2, m};
     }
   } else {
     const R sig = midpoint_sigma<R>(lam, prm.bn, (double)m, T, prm.temprescale);
     const R r0 = n0 * sig, r1 = n1 * sig, r2 = n2 * sig;
-    const R e0 = e.x[q], e1 = e.y[q], e2 = e.z[q];
     R A0, A1, A2, t0, t1, t2, detAi;
     if (HALF == 0) {
-      midpoint_A((R)lam, (R)prm.dt, (R)prm.bn, (R)prm.landeg(site), b0, b1, b2v, r0, r1, r2, e0,
-                 e1, e2, A0, A1, A2);
-      midpoint_solve(A0, A1, A2, e0, e1, e2, t0, t1, t2, detAi);
-      p.x[q] = (R)0.5 * (e0 + t0 * detAi);
-      p.y[q] = (R)0.5 * (e1 + t1 * detAi);
-      p.z[q] = (R)0.5 * (e2 + t2 * detAi);
+      midpoint_A((R)lam, (R)prm.dt, (R)prm.bn, (R)prm.landeg(site), b0, b1, b2v, r0, r1, r2, eq.x,
+                 eq.y, eq.z, A0, A1, A2);
+      midpoint_solve(A0, A1, A2, eq.x, eq.y, eq.z, t0, t1, t2, detAi);
+      p[q] = Quad<R>{(R)0.5 * (eq.x + t0 * detAi), (R)0.5 * (eq.y + t1 * detAi),
+                     (R)0.5 * (eq.z + t2 * detAi), m};
     } else {
-      const R p0 = p.x[q], p1 = p.y[q], p2 = p.z[q];
-      midpoint_A((R)lam, (R)prm.dt, (R)prm.bn, (R)prm.landeg(site), b0, b1, b2v, r0, r1, r2, p0,
-                 p1, p2, A0, A1, A2);
-      midpoint_solve(A0, A1, A2, e0, e1, e2, t0, t1, t2, detAi);
-      out.x[q] = t0 * detAi;
-      out.y[q] = t1 * detAi;
-      out.z[q] = t2 * detAi;
+      const Quad<R> pq = p[q];
+      midpoint_A((R)lam, (R)prm.dt, (R)prm.bn, (R)prm.landeg(site), b0, b1, b2v, r0, r1, r2, pq.x,
+                 pq.y, pq.z, A0, A1, A2);
+      midpoint_solve(A0, A1, A2, eq.x, eq.y, eq.z, t0, t1, t2, detAi);
+      out[q] = Quad<R>{t0 * detAi, t1 * detAi, t2 * detAi, m};
     }
   }
 }
@@ -527,23 +508,23 @@ __global__ void __launch_bounds__(256)
 // ----------------------------------------------------------------- MC
 // choose_random_flip (montecarlo_common.f90:25-79)
 template <typename R>
-__device__ __forceinline__ void trial_move(R e0, R e1, R e2, double delta, const double u[4],
-                                          const double g[3], R &n0, R &n1, R &n2) {
-  int ftype = (int)floor(3 * u[0]);
+__device__ __forceinline__ void trial_move(R e0, R e1, R e2, double delta, double u0, double u1,
+                                          double u2, double g0, double g1, double g2, R &n0,
+                                          R &n1, R &n2) {
+  const int ftype = (int)floor(3 * u0);
   if (ftype == 0) {
-    double phi = u[1] * 2 * kPi;
-    double theta = acos(1 - 2 * u[2]);
+    double phi = u1 * 2 * kPi;
+    double theta = acos(1 - 2 * u2);
     double st = sin(theta);
     n0 = (R)(st * cos(phi));
     n1 = (R)(st * sin(phi));
     n2 = (R)cos(theta);
   } else if (ftype == 1) {
-    R g0 = (R)(g[0] * delta), g1 = (R)(g[1] * delta), g2 = (R)(g[2] * delta);
-    R a0 = e0 + g0, a1 = e1 + g1, a2 = e2 + g2;
+    R a0 = e0 + (R)(g0 * delta), a1 = e1 + (R)(g1 * delta), a2 = e2 + (R)(g2 * delta);
     R l = fsqrt(a0 * a0 + a1 * a1 + a2 * a2);
-    n0 = (e0 + g0) / l;
-    n1 = (e1 + g1) / l;
-    n2 = (e2 + g2) / l;
+    n0 = a0 / l;
+    n1 = a1 / l;
+    n2 = a2 / l;
   } else {
     // Spin flip, newmom = -emom.  Written as a subtraction: hipcc (clang 22,
     // -O1 and above) drops a bare negation of a loaded value when it merges
@@ -564,23 +545,23 @@ __device__ __forceinline__ R sq(R x) {
 // calculate_energy (montecarlo_common.f90:431-865), exchange + anisotropy +
 // DM + external field; returns mub*(E_trial - E_current).
 template <typename R>
-__device__ __forceinline__ R mc_delta_e(const HamDev<R> &h, const MomSrc<R> &src, const R *ex,
-                                        const R *ey, const R *ez, int site, int ens, R m, R n0,
-                                        R n1, R n2, R x0, R x1, R x2) {
+__device__ __forceinline__ R mc_delta_e(const HamDev<R> &h, const Quad<R> *__restrict__ s,
+                                        int site, int ens, R n0, R n1, R n2, R x0, R x1, R x2) {
   const SiteClass sc = h.classes[h.cls[site]];
   const int M = h.M, N = h.N;
   const size_t qi = (size_t)site * M + ens;
-  R c0, c1, c2;
-  src.load(qi, c0, c1, c2);
+  const Quad<R> me = s[qi];
+  const R m = me.w;
+  const R c0 = me.x * m, c1 = me.y * m, c2 = me.z * m;
   const R t0 = n0 * m, t1 = n1 * m, t2 = n2 * m;
   R e_c = 0, e_t = 0;
   {
     const int len = h.ex_len[sc.ex_row];
     for (int j = 0; j < len; ++j) {
-      int nb = h.nbr[(size_t)j * N + site];
-      R J = h.cpl[(size_t)j * h.ex_rows + sc.ex_row];
+      const int nb = h.nbr[(size_t)j * N + site];
+      const R J = h.cpl[(size_t)j * h.ex_rows + sc.ex_row];
       R a0, a1, a2;
-      src.load((size_t)nb * M + ens, a0, a1, a2);
+      moment(s, (size_t)nb * M + ens, a0, a1, a2);
       e_c = e_c - J * (c0 * a0 + c1 * a1 + c2 * a2);
       e_t = e_t - J * (t0 * a0 + t1 * a1 + t2 * a2);
     }
@@ -610,21 +591,21 @@ __device__ __forceinline__ R mc_delta_e(const HamDev<R> &h, const MomSrc<R> &src
     }
   }
   if (h.Wd > 0 && sc.dm_row >= 0) {
-    const R u0 = ex[qi], u2 = ez[qi];  // unit-vector terms as in montecarlo_common.f90:612,615
+    const R u0 = me.x, u2 = me.z;  // unit-vector terms as in montecarlo_common.f90:612,615
     const int len = h.dm_len[sc.dm_row];
     for (int j = 0; j < len; ++j) {
-      int nb = h.dnbr[(size_t)j * N + site];
-      size_t r = (size_t)j * h.dm_rows + sc.dm_row;
-      R D0 = h.dvx[r], D1 = h.dvy[r], D2 = h.dvz[r];
+      const int nb = h.dnbr[(size_t)j * N + site];
+      const size_t r = (size_t)j * h.dm_rows + sc.dm_row;
+      const R D0 = h.dvx[r], D1 = h.dvy[r], D2 = h.dvz[r];
       R a0, a1, a2;
-      src.load((size_t)nb * M + ens, a0, a1, a2);
+      moment(s, (size_t)nb * M + ens, a0, a1, a2);
       e_c = e_c - D0 * (c1 * a2 - u2 * a1) - D1 * (c2 * a0 - c0 * a2) - D2 * (u0 * a1 - c1 * a0);
       e_t = e_t - D0 * (t1 * a2 - t2 * a1) - D1 * (t2 * a0 - t0 * a2) - D2 * (t0 * a1 - t1 * a0);
     }
   }
   e_c = e_c - x0 * c0 - x1 * c1 - x2 * c2;
   e_t = e_t - x0 * t0 - x1 * t1 - x2 * t2;
-  R tt = e_t - e_c;
+  const R tt = e_t - e_c;
   return (R)kMub * tt;
 }
 
@@ -661,76 +642,75 @@ struct MCParams {
 };
 
 template <typename R>
-__global__ void __launch_bounds__(256)
-    k_mc_colour(HamDev<R> h, MCParams mp, Spins<R> e, const R *__restrict__ mm) {
+__global__ void __launch_bounds__(256) k_mc_colour(HamDev<R> h, MCParams mp, Quad<R> *s) {
   const int64_t total = (int64_t)mp.nsites * h.M;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= total) return;
   const int pidx = (int)(t / h.M), ens = (int)(t - (int64_t)pidx * h.M);
   const int site = mp.sites[pidx];
   const size_t q = (size_t)site * h.M + ens;
-  double u[4];
-  uniform4(mp.seed, kTagMCU, mp.sweep, (uint32_t)site, (uint32_t)ens, u);
-  const R m = mm[q];
-  const MomSrc<R> src = src_of(e, mm);
+  double u0, u1, u2, u3;
+  uniform4(mp.seed, kTagMCU, mp.sweep, (uint32_t)site, (uint32_t)ens, u0, u1, u2, u3);
+  const Quad<R> me = s[q];
   const R x0 = (R)mp.ext[3 * ens], x1 = (R)mp.ext[3 * ens + 1], x2 = (R)mp.ext[3 * ens + 2];
   R o0, o1, o2;
   if (mp.heat_bath) {
-    FieldParts<R> fp = site_field(h, src, site, ens);
-    fp.e0 = x0;
-    fp.e1 = x1;
-    fp.e2 = x2;
-    R f0, f1, f2;
-    total_field(fp, f0, f1, f2);
-    heat_bath<R>(mp.hbeta[ens], m, f0, f1, f2, u[1], u[0], o0, o1, o2);
+    R s0, s1, s2, q0, q1, q2, y0, y1, y2;
+    site_field(h, (const Quad<R> *)s, site, ens, s0, s1, s2, q0, q1, q2, y0, y1, y2);
+    const R f0 = (R)0 + s0 + (q0 + x0 + (R)0);
+    const R f1 = (R)0 + s1 + (q1 + x1 + (R)0);
+    const R f2 = (R)0 + s2 + (q2 + x2 + (R)0);
+    heat_bath<R>(mp.hbeta[ens], me.w, f0, f1, f2, u1, u0, o0, o1, o2);
   } else {
-    const R e0 = e.x[q], e1 = e.y[q], e2 = e.z[q];
-    double g[3] = {0.0, 0.0, 0.0};
-    if ((int)floor(3 * u[0]) == 1)
-      Noise<double>::normal3(mp.seed, kTagMCG, mp.sweep, (uint32_t)site, (uint32_t)ens, g[0], g[1],
-                             g[2]);
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    if ((int)floor(3 * u0) == 1)
+      Noise<double>::normal3(mp.seed, kTagMCG, mp.sweep, (uint32_t)site, (uint32_t)ens, g0, g1,
+                             g2);
     R n0, n1, n2;
-    trial_move<R>(e0, e1, e2, mp.delta[ens], u, g, n0, n1, n2);
-    R de = mc_delta_e(h, src, e.x, e.y, e.z, site, ens, m, n0, n1, n2, x0, x1, x2);
-    if (!(de <= (R)0 || u[3] < exp(-mp.beta[ens] * (double)de))) return;
+    trial_move<R>(me.x, me.y, me.z, mp.delta[ens], u0, u1, u2, g0, g1, g2, n0, n1, n2);
+    const R de = mc_delta_e(h, (const Quad<R> *)s, site, ens, n0, n1, n2, x0, x1, x2);
+    if (!(de <= (R)0 || u3 < exp(-mp.beta[ens] * (double)de))) return;
     o0 = n0;
     o1 = n1;
     o2 = n2;
   }
-  e.x[q] = o0;
-  e.y[q] = o1;
-  e.z[q] = o2;
+  s[q] = Quad<R>{o0, o1, o2, me.w};
 }
 
 // ---------------------------------------------------------- FP64 reductions
-// Stage 1: block b sums sites [b*chunk, (b+1)*chunk) of ensemble blockIdx.y.
-// vals: 3 components of emomM (and the energy term when en != null).
+// Stage 1: block b sums sites [b*chunk, (b+1)*chunk) of ensemble blockIdx.y:
+// the three components of emomM = xyz*w (and the energy term when en != null).
 template <typename R, int NV>
 __global__ void __launch_bounds__(256)
-    k_reduce_stage1(int N, int M, int chunk, Spins<R> e, const R *__restrict__ mm,
+    k_reduce_stage1(int N, int M, int chunk, const Quad<R> *__restrict__ s,
                     const double *__restrict__ en, double *__restrict__ partial) {
   const int ens = blockIdx.y;
   const int b = blockIdx.x;
   const int s0 = b * chunk, s1 = min(N, s0 + chunk);
-  double acc[NV];
-#pragma unroll
-  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-  for (int s = s0 + (int)threadIdx.x; s < s1; s += blockDim.x) {
-    size_t q = (size_t)s * M + ens;
-    double m = (double)mm[q];
-    acc[0] += (double)e.x[q] * m;
-    acc[1] += (double)e.y[q] * m;
-    acc[2] += (double)e.z[q] * m;
-    if (NV > 3) acc[3] += en[q];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  for (int i = s0 + (int)threadIdx.x; i < s1; i += blockDim.x) {
+    const size_t q = (size_t)i * M + ens;
+    const Quad<R> v = s[q];
+    const double m = (double)v.w;
+    a0 += (double)v.x * m;
+    a1 += (double)v.y * m;
+    a2 += (double)v.z * m;
+    if (NV > 3) a3 += en[q];
   }
-  __shared__ double red[NV][256 / 64];
+  __shared__ double red[4][256 / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-  for (int v = 0; v < NV; ++v) {
-    double x = acc[v];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-    if (lane == 0) red[v][wid] = x;
+  for (int off = 32; off > 0; off >>= 1) {
+    a0 += __shfl_down(a0, off, 64);
+    a1 += __shfl_down(a1, off, 64);
+    a2 += __shfl_down(a2, off, 64);
+    if (NV > 3) a3 += __shfl_down(a3, off, 64);
+  }
+  if (lane == 0) {
+    red[0][wid] = a0;
+    red[1][wid] = a1;
+    red[2][wid] = a2;
+    red[3][wid] = a3;
   }
   __syncthreads();
   if (threadIdx.x < NV) {

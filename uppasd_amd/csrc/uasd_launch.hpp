@@ -10,23 +10,22 @@ namespace uasd {
 constexpr int kBlock = 256;
 
 template <typename R>
-hipError_t launch_field(hipStream_t s, const HamDev<R> &h, const MomSrc<R> &src, R *bx, R *by,
-                        R *bz, double *en);
+hipError_t launch_field(hipStream_t s, const HamDev<R> &h, const Quad<R> *src, Quad<R> *beff,
+                        double *en);
 
 // alg: 1 midpoint / 5 Depondt; half: 0 predictor, 1 corrector
 template <typename R>
 hipError_t launch_llg(hipStream_t s, int alg, int half, bool fused, const HamDev<R> &h,
-                      const LLGDev<R> &prm, Spins<R> e, Spins<R> p, Spins<R> b2, const R *mm,
-                      const R *bx, const R *by, const R *bz, Spins<R> out);
+                      const LLGDev<R> &prm, const Quad<R> *e, Quad<R> *p, Quad<R> *b2,
+                      const Quad<R> *beff, Quad<R> *out);
 
 template <typename R>
-hipError_t launch_mc(hipStream_t s, const HamDev<R> &h, const MCParams &mp, Spins<R> e,
-                     const R *mm);
+hipError_t launch_mc(hipStream_t s, const HamDev<R> &h, const MCParams &mp, Quad<R> *spins);
 
 // FP64 sums per ensemble of emomM (nv = 3) or of emomM and en (nv = 4);
 // partial must hold M*nblocks*nv doubles, out M*nv doubles.
 template <typename R>
-hipError_t launch_reduce(hipStream_t s, int N, int M, Spins<R> e, const R *mm, const double *en,
+hipError_t launch_reduce(hipStream_t s, int N, int M, const Quad<R> *spins, const double *en,
                          int nv, double *partial, int nblocks, double *out);
 
 }  // namespace uasd

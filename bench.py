@@ -103,6 +103,9 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--solver", type=int, choices=[1, 5], default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--gather", type=int, default=0,
+                    help="fixed gather batch (2/4/8/16) instead of autotuning")
+    ap.add_argument("--block", type=int, default=256)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -128,6 +131,14 @@ def main():
     eng.set_llg(0.1, 300.0, 1e-16)
     eng.set_noise(U.UASD_NOISE_PHILOX, 0xC2C2C2C2)
     setup_s = time.perf_counter() - t_setup
+
+    t_tune = time.perf_counter()
+    if args.gather:
+        eng.set_tuning(eng.TUNE_LLG, args.gather, args.block)
+    else:
+        eng.autotune_llg(args.solver, steps=3)
+    tune_s = time.perf_counter() - t_tune
+    gather, block = eng.tuning(eng.TUNE_LLG)
 
     eng.sd_steps(args.solver, args.warmup)
     eng.sync()
@@ -170,6 +181,8 @@ def main():
                          "device-resident"),
             "spins": N, "neighbours": W, "solver": args.solver, "precision": args.precision,
             "setup_s": round(setup_s, 2),
+            "launch": {"gather": gather, "block": block,
+                       "autotuned": not args.gather, "tune_s": round(tune_s, 3)},
             "kernel_ms": {"predictor": round(t_pred * 1e3, 4), "corrector": round(t_corr * 1e3, 4)},
             "bytes_per_site": kb,
             "max_norm_drift": norm_drift,

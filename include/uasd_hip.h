@@ -83,6 +83,9 @@ int uasd_set_external_field(uasd_ctx *ctx, const double *field);
 /* ---- LLG parameters: lambda1_array, Temp_array, Landeg (Natom each) ---- */
 int uasd_set_llg(uasd_ctx *ctx, const double *lambda1, const double *temp,
                  const double *landeg, double delta_t, double temprescale, double bn);
+/* Temperature rescaling of the quantum heat bath (temprescale), used by the
+ * LLG noise and the MC acceptance; uasd_set_llg also sets it. */
+int uasd_set_temprescale(uasd_ctx *ctx, double temprescale);
 /* Noise source; normals is (3,Natom,Mensemble) N(0,1) for UASD_NOISE_INJECTED
  * and is used by the next evolve call (both halves of the step). */
 int uasd_set_noise(uasd_ctx *ctx, int mode, uint64_t seed, const double *normals);
@@ -93,10 +96,11 @@ int uasd_set_moments(uasd_ctx *ctx, const double *emom, const double *mmom);
 int uasd_get_moments(uasd_ctx *ctx, double *emom, double *emomM);
 
 /* ---- drop-in entry points, host arrays in and out on every call ---- */
-/* effective_field_full (hamiltonianactions.f90:108-252): beff from emomM;
- * energy in mRy as the reference's field routine reports it. */
-int uasd_effective_field_host(uasd_ctx *ctx, const double *emomM, double *beff,
-                              double *energy);
+/* effective_field_full (hamiltonianactions.f90:108-252): beff from emomM,
+ * and (each may be NULL) beff1 = pair + anisotropy field, beff2 = cubic*sb +
+ * external field, energy in mRy as the reference's field routine reports it. */
+int uasd_effective_field_host(uasd_ctx *ctx, const double *emomM, double *beff, double *beff1,
+                              double *beff2, double *energy);
 /* evolve_first (evolution.f90:38-273) for SDEalgh 1 and 5: reads beff, emom,
  * mmom; writes emom2, emomM (and, for Depondt, b2eff and the updated emom). */
 int uasd_evolve_first_host(uasd_ctx *ctx, int sdealgh, const double *beff, double *emom,
@@ -109,6 +113,12 @@ int uasd_evolve_second_host(uasd_ctx *ctx, int sdealgh, const double *beff,
 /* nsteps of the sd_mphase body (sd_driver.f90:668-765): field -> evolve_first
  * -> field -> evolve_second -> moment_update, fused into two kernels/step. */
 int uasd_sd_steps(uasd_ctx *ctx, int sdealgh, int64_t nsteps);
+/* Graph-captured stepping (no reference counterpart: the reference has no
+ * launches to capture): with steps_per_graph = K > 0, uasd_sd_steps and
+ * uasd_mc_sweeps replay a hipGraph of K steps (K sweeps) for every full run of
+ * K, and launch the rest one by one; the numbers are the same either way.
+ * The Philox counters then advance on the device.  0 (the default) disables. */
+int uasd_set_graph_steps(uasd_ctx *ctx, int steps_per_graph);
 /* Field of the resident moments into a host array (NULL skips the copy). */
 int uasd_effective_field(uasd_ctx *ctx, double *beff, double *energy);
 /* nsweeps of Metropolis ('M') or heat-bath ('H') updates (mc_evolve,
@@ -119,13 +129,49 @@ int uasd_mc_sweeps(uasd_ctx *ctx, char mode, const double *temps, int64_t nsweep
 /* FP64 reductions: msum (3,M) = sum_i emomM(:,i,k); energy (M) or NULL. */
 int uasd_measure(uasd_ctx *ctx, double *msum, double *energy);
 
+/* ---- batches: replicas of uneven size as segments of one lattice ----
+ * A batch concatenates R systems (their neighbour lists shifted by the site
+ * offset of each) into one context; one launch then advances every replica.
+ * offsets (nseg+1): segment s = sites [offsets[s], offsets[s+1]).  Without a
+ * call the whole lattice is one segment.  Resets the accumulators. */
+int uasd_set_segments(uasd_ctx *ctx, int nseg, const int *offsets);
+/* FP64 per (segment, ensemble) sums: msum (3,M,nseg) = sum emomM over the
+ * segment, energy (M,nseg) the field-routine energy in mRy, or NULL. */
+int uasd_measure_segments(uasd_ctx *ctx, double *msum, double *energy);
+/* One cumulant sample per (segment, ensemble), kept on the device in FP64:
+ * m = |sum emomM|/n and e = energy/n (mRy per site), as running means with
+ * the linearly increasing weights of calc_and_print_cumulant
+ * (prn_averages.f90:956-1000), each (segment, ensemble) its own sequence.
+ * Asynchronous: no host copy, no synchronisation. */
+int uasd_accumulate(uasd_ctx *ctx, int with_energy);
+/* out (8,M,nseg): W, n, <m>, <m^2>, <m^4>, <e>, <e^2>, 0 per (segment,
+ * ensemble); U4 = 1 - <m^4>/(3<m^2>^2).  nsamples may be NULL. */
+int uasd_get_accumulators(uasd_ctx *ctx, double *out, int64_t *nsamples);
+int uasd_reset_accumulators(uasd_ctx *ctx);
+
 /* ---- colouring used by the MC sweeps ---- */
 int uasd_get_coloring(uasd_ctx *ctx, int *ncolors, int *color_of_site);
 
 /* ---- per-kernel timing with HIP events on the context stream ---- */
 int uasd_set_kernel_timing(uasd_ctx *ctx, int enable);
-/* kind: 0 predictor, 1 corrector, 2 field, 3 MC colour pass */
+/* kind: 0 predictor, 1 corrector, 2 field, 3 MC colour pass, 4 LLG graph
+ * replay (K steps), 5 MC graph replay (K sweeps) */
 int uasd_get_kernel_timing(uasd_ctx *ctx, int kind, int64_t *launches, double *total_ms);
+
+/* ---- launch shapes of the gathering kernels (no reference counterpart:
+ * the reference's OpenMP loops have no launch shape) ----
+ * kind: 0 fused LLG step, 1 field, 2 MC colour pass.  gather = neighbours
+ * whose loads are issued together (2, 4, 8, 16); block = workgroup size
+ * (64, 128, 256).  Results do not depend on either. */
+int uasd_set_tuning(uasd_ctx *ctx, int kind, int gather, int block);
+int uasd_get_tuning(uasd_ctx *ctx, int kind, int *gather, int *block);
+/* Autotuner: times every shape on the context's own lattice and state
+ * (steps LLG steps / sweeps MC sweeps per candidate), keeps the fastest and
+ * restores the spins and step counters, so a run is unchanged by tuning.
+ * best_ms (may be NULL) receives the fastest candidate's time. */
+int uasd_autotune_llg(uasd_ctx *ctx, int sdealgh, int steps, double *best_ms);
+int uasd_autotune_mc(uasd_ctx *ctx, char mode, const double *temps, int sweeps,
+                     double *best_ms);
 
 /* ---- host set-up: the reference neighbour map, bit-exact ----
  * setup_nm + setup_neighbour_hamiltonian (neighbourmap.f90:32-359,

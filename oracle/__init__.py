@@ -307,7 +307,13 @@ class OracleHam:
         lib().or_effective_field(C.byref(self.h), self.N, M, np.ascontiguousarray(emomM),
                                  ext_arr.ctypes.data if ext_arr is not None else None,
                                  beff, b1.ctypes.data, b2.ctypes.data, C.byref(en))
+        self.last_parts = (b1, b2)
         return beff, en.value
+
+    def field_parts(self, emomM, ext=None):
+        """(beff, energy, beff1, beff2) as effective_field_full returns them."""
+        beff, en = self.field(emomM, ext)
+        return beff, en, self.last_parts[0], self.last_parts[1]
 
     def field_single(self, emomM, i, k, ext_uniform=None):
         out = np.zeros(3)

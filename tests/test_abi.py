@@ -47,3 +47,19 @@ def test_builder_rejects_bad_symmetry():
         assert e.code == -4
     else:
         raise AssertionError("sym 7 accepted")
+
+
+def test_fortran_binding_names_match_header():
+    """Every bind(C) name in the Fortran module is declared in the header,
+    and the module and the test driver compile and link with amdflang."""
+    src = open(os.path.join(ROOT, "uppasd_amd", "fortran", "uasd_hip.f90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(uasd_[a-z0-9_]+)"\)', src))
+    assert bound and bound <= set(declared())
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests", "fortran"))
+    import build_driver
+    binary = build_driver.build()
+    if binary is None:
+        import pytest
+        pytest.skip("amdflang not installed")
+    assert os.path.exists(binary)

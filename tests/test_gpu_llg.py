@@ -68,10 +68,12 @@ def test_effective_field_matches_oracle(O, kind):
     emom = CF.random_unit(N, rng, M)
     emomM = emom * sysd["mmom"][None, :, None]
     eng = engine_for(sysd, M)
-    beff, en = eng.effective_field(np.ascontiguousarray(emomM))
+    beff, en, b1, b2 = eng.effective_field(np.ascontiguousarray(emomM), parts=True)
     ham = oracle_ham(O, sysd)
-    bref, eref = ham.field(np.ascontiguousarray(emomM), ext_array(sysd, M))
+    bref, eref, b1ref, b2ref = ham.field_parts(np.ascontiguousarray(emomM), ext_array(sysd, M))
     assert rel(beff, bref) <= 1e-13
+    assert rel(b1, b1ref) <= 1e-13
+    assert np.max(np.abs(b2 - b2ref)) <= 1e-13 * max(np.max(np.abs(bref)), 1.0)
     assert abs(en - eref) <= 1e-12 * abs(eref)
 
 

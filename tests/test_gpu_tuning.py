@@ -1,0 +1,126 @@
+"""Launch shapes and the autotuner.
+
+The gather batch only changes when loads are issued, not the order in which
+terms are added, and the workgroup size only changes which lanes share a
+workgroup, so every shape must give bit-identical results.  The autotuner
+must leave the state and the step counters as it found them.
+"""
+import numpy as np
+import pytest
+
+import uppasd_amd as U
+from uppasd_amd import configs as CF
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(2, 64), (4, 128), (8, 256), (16, 64), (16, 256)]
+
+
+def _llg_run(sysd, emom, gather, block, precision, alg, steps=20):
+    N = sysd["lattice"].natom
+    M = emom.shape[0]
+    eng = U.Engine(N, M, precision)
+    eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
+    if "dmlist" in sysd:
+        eng.set_dm(sysd["dmlist"], sysd["dmlistsize"], sysd["dm_vect"])
+    if "taniso" in sysd:
+        eng.set_anisotropy(sysd["taniso"], sysd["eaniso"], sysd["kaniso"], sysd["sb"])
+    if "ext" in sysd:
+        eng.set_external_field(sysd["ext"])
+    eng.set_moments(emom, np.broadcast_to(sysd["mmom"], (M, N)))
+    eng.set_llg(0.1, 300.0, 1e-16)
+    eng.set_noise(U.UASD_NOISE_PHILOX, 1234)
+    eng.set_tuning(eng.TUNE_LLG, gather, block)
+    eng.set_tuning(eng.TUNE_FIELD, gather, block)
+    eng.sd_steps(alg, steps)
+    e, _ = eng.get_moments()
+    beff, en = eng.field(with_energy=True)
+    return e, beff, en
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("alg", [1, 5])
+def test_llg_results_independent_of_launch_shape(precision, alg):
+    rng = np.random.default_rng(5)
+    sysd = CF.triangular_dmi(24)  # exchange + DM + anisotropy + field
+    N = sysd["lattice"].natom
+    emom = np.ascontiguousarray(CF.random_unit(N, rng, 2))
+    ref = _llg_run(sysd, emom, 4, 256, precision, alg)
+    for g, b in SHAPES:
+        got = _llg_run(sysd, emom, g, b, precision, alg)
+        for x, y in zip(got, ref):
+            assert np.array_equal(x, y), (g, b)
+
+
+def test_mc_results_independent_of_launch_shape():
+    rng = np.random.default_rng(6)
+    sysd = CF.bcc_shells(6, CF.bcc5_couplings())
+    N = sysd["lattice"].natom
+    emom = np.ascontiguousarray(CF.random_unit(N, rng, 2))
+    out = []
+    for g, b in [(4, 256)] + SHAPES:
+        eng = U.Engine(N, 2, "fp64")
+        eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
+        eng.set_moments(emom, np.broadcast_to(sysd["mmom"], (2, N)))
+        eng.set_tuning(eng.TUNE_MC, g, b)
+        eng.mc_sweeps("M", [300.0, 900.0], 5, 77)
+        eng.mc_sweeps("H", [300.0, 900.0], 5, 78)
+        out.append(eng.get_moments()[0])
+    for x in out[1:]:
+        assert np.array_equal(x, out[0])
+
+
+def test_bad_shape_rejected():
+    sysd = CF.bcc_shells(3, CF.bcc5_couplings())
+    eng = U.Engine(sysd["lattice"].natom, 1, "fp64")
+    with pytest.raises(U.UasdError):
+        eng.set_tuning(eng.TUNE_LLG, 3, 256)
+    with pytest.raises(U.UasdError):
+        eng.set_tuning(eng.TUNE_LLG, 4, 512)
+    with pytest.raises(U.UasdError):
+        eng.set_tuning(7, 4, 256)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_autotune_llg_leaves_trajectory_unchanged(precision):
+    rng = np.random.default_rng(8)
+    sysd = CF.bcc_shells(8, CF.bcc5_couplings())
+    N = sysd["lattice"].natom
+    emom = np.ascontiguousarray(CF.random_unit(N, rng, 1))
+
+    def run(tune):
+        eng = U.Engine(N, 1, precision)
+        eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
+        eng.set_moments(emom, sysd["mmom"][None])
+        eng.set_llg(0.1, 300.0, 1e-16)
+        eng.set_noise(U.UASD_NOISE_PHILOX, 99)
+        eng.sd_steps(1, 3)
+        if tune:
+            ms = eng.autotune_llg(1, steps=2)
+            assert ms > 0.0
+            g, b = eng.tuning(eng.TUNE_LLG)
+            assert g in (2, 4, 8, 16) and b in (64, 128, 256)
+            assert eng.tuning(eng.TUNE_FIELD) == (g, b)
+        eng.sd_steps(1, 7)
+        return eng.get_moments()[0]
+
+    assert np.array_equal(run(True), run(False))
+
+
+def test_autotune_mc_leaves_chain_unchanged():
+    rng = np.random.default_rng(10)
+    sysd = CF.bcc_shells(6, CF.bcc5_couplings())
+    N = sysd["lattice"].natom
+    emom = np.ascontiguousarray(CF.random_unit(N, rng, 2))
+
+    def run(tune):
+        eng = U.Engine(N, 2, "fp64")
+        eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
+        eng.set_moments(emom, np.broadcast_to(sysd["mmom"], (2, N)))
+        eng.mc_sweeps("M", [500.0, 700.0], 2, 5)
+        if tune:
+            assert eng.autotune_mc("M", [500.0, 700.0], sweeps=1) > 0.0
+        eng.mc_sweeps("M", [500.0, 700.0], 4, 5)
+        return eng.get_moments()[0]
+
+    assert np.array_equal(run(True), run(False))

@@ -30,6 +30,9 @@ EXPORTS = (
     "uasd_effective_field_host", "uasd_evolve_first_host", "uasd_evolve_second_host",
     "uasd_sd_steps", "uasd_effective_field", "uasd_mc_sweeps", "uasd_measure",
     "uasd_get_coloring", "uasd_set_kernel_timing", "uasd_get_kernel_timing",
+    "uasd_set_tuning", "uasd_get_tuning", "uasd_autotune_llg", "uasd_autotune_mc",
+    "uasd_set_temprescale", "uasd_set_segments", "uasd_measure_segments", "uasd_accumulate",
+    "uasd_get_accumulators", "uasd_reset_accumulators", "uasd_set_graph_steps",
     "uasd_build_pair_list",
 )
 
@@ -68,7 +71,7 @@ def lib() -> C.CDLL:
         L.uasd_set_noise.argtypes = [vp, C.c_int, C.c_uint64, vp]
         L.uasd_set_moments.argtypes = [vp, vp, vp]
         L.uasd_get_moments.argtypes = [vp, vp, vp]
-        L.uasd_effective_field_host.argtypes = [vp, vp, vp, dp]
+        L.uasd_effective_field_host.argtypes = [vp, vp, vp, vp, vp, dp]
         L.uasd_evolve_first_host.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
         L.uasd_evolve_second_host.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         L.uasd_sd_steps.argtypes = [vp, C.c_int, C.c_int64]
@@ -78,6 +81,17 @@ def lib() -> C.CDLL:
         L.uasd_get_coloring.argtypes = [vp, ip, vp]
         L.uasd_set_kernel_timing.argtypes = [vp, C.c_int]
         L.uasd_get_kernel_timing.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), dp]
+        L.uasd_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.uasd_set_temprescale.argtypes = [vp, C.c_double]
+        L.uasd_set_segments.argtypes = [vp, C.c_int, vp]
+        L.uasd_measure_segments.argtypes = [vp, vp, vp]
+        L.uasd_accumulate.argtypes = [vp, C.c_int]
+        L.uasd_get_accumulators.argtypes = [vp, vp, C.POINTER(C.c_int64)]
+        L.uasd_reset_accumulators.argtypes = [vp]
+        L.uasd_set_graph_steps.argtypes = [vp, C.c_int]
+        L.uasd_get_tuning.argtypes = [vp, C.c_int, ip, ip]
+        L.uasd_autotune_llg.argtypes = [vp, C.c_int, C.c_int, dp]
+        L.uasd_autotune_mc.argtypes = [vp, C.c_char, vp, C.c_int, dp]
         L.uasd_build_pair_list.argtypes = [C.c_int] * 5 + [vp] * 6 + [C.c_int, C.c_int] + \
             [vp, vp, vp] + [C.c_int, C.c_int] + [vp, vp, vp] + [ip, vp, vp, vp]
         _lib = L

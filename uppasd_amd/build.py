@@ -57,5 +57,23 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FLANG = os.path.join(ROCM, "bin", "amdflang")
+
+
+def build_fortran(verbose: bool = False):
+    """Compiles the Fortran binding (fortran/uasd_hip.f90) with amdflang into
+    _build/fortran (object + .mod).  Returns (object, module dir), or None when
+    no amdflang is installed."""
+    if not os.path.exists(FLANG):
+        return None
+    outdir = os.path.join(HERE, "_build", "fortran")
+    os.makedirs(outdir, exist_ok=True)
+    obj = os.path.join(outdir, "uasd_hip.o")
+    _run([FLANG, "-O2", "-fPIC", "-module-dir", outdir, "-c",
+          os.path.join(HERE, "fortran", "uasd_hip.f90"), "-o", obj], verbose)
+    return obj, outdir
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_fortran(verbose="-v" in sys.argv))

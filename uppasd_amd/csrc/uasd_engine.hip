@@ -129,6 +129,32 @@ struct uasd_ctx {
   uint64_t step = 0;      // LLG step counter (Philox counter)
   uint64_t mc_sweep = 0;  // MC sweep counter
 
+  // launch shapes of the gathering kernels: 0 fused LLG, 1 field, 2 MC
+  uasd::Tune tune[3];
+
+  // segments (replicas of a batch): site ranges, their chunks, accumulators
+  int nseg = 0;
+  std::vector<int> seg_off;  // (nseg+1)
+  int nchunks = 0;
+  DevBuf seg_dev;            // int2 chunks (nchunks), int seg_chunk (nseg+1), int seg_n (nseg)
+  DevBuf seg_partial, seg_out, acc;
+  int64_t acc_samples = 0;
+
+  // captured step graphs: replayed for runs of graph_steps LLG steps / MC
+  // sweeps; the Philox counters then live in ctr (0 LLG step, 1 MC sweep).
+  int graph_steps = 0;
+  uint64_t gen = 0;           // bumped whenever a captured graph may be stale
+  uint64_t graphs_gen = 0;
+  DevBuf ctr;
+  bool ctr_valid[2] = {false, false};
+  uint64_t ctr_val[2] = {0, 0};
+  bool capturing = false;
+  struct Graph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::map<std::tuple<int, int, int, uint64_t>, Graph> graphs;  // kind, alg/mode, K, seed
+
   // colouring
   int ncolors = 0;
   std::vector<int> color_of_site, color_ptr, color_sites;
@@ -137,7 +163,7 @@ struct uasd_ctx {
   // Per-kernel timing: an event pair is recorded around each launch without
   // a host wait; pairs are resolved when the totals are read.
   bool timing = false;
-  Timing timers[4];
+  Timing timers[6];  // 0 predictor, 1 corrector, 2 field, 3 MC, 4 LLG graph, 5 MC graph
   std::vector<hipEvent_t> ev_pool;  // free events
   struct Pending {
     hipEvent_t a, b;
@@ -148,6 +174,8 @@ struct uasd_ctx {
 };
 
 namespace {
+
+void destroy_graphs(uasd_ctx *c);
 
 // Quad buffers: one Quad<R> = {x, y, z, w} per q = site*M + ens.
 size_t quad_bytes(const uasd_ctx *c) { return (size_t)c->N * c->M * 4 * c->rsize; }
@@ -304,6 +332,7 @@ int rebuild_classes(uasd_ctx *c) {
   r = upload_raw(c, c->classes, table);
   if (r) return r;
   c->classes_dirty = false;
+  c->gen += 1;
   return UASD_OK;
 }
 
@@ -366,13 +395,13 @@ hipEvent_t take_event(uasd_ctx *c) {
 }
 
 void tic(uasd_ctx *c) {
-  if (!c->timing) return;
+  if (!c->timing || c->capturing) return;
   c->cur = take_event(c);
   if (c->cur) hipEventRecord(c->cur, c->stream);
 }
 
 int toc(uasd_ctx *c, int kind) {
-  if (!c->timing || !c->cur) return UASD_OK;
+  if (!c->timing || c->capturing || !c->cur) return UASD_OK;
   hipEvent_t b = take_event(c);
   if (!b) return fail(UASD_ERR_HIP, "hipEventCreate failed");
   HIP_TRY(hipEventRecord(b, c->stream));
@@ -425,31 +454,40 @@ int ready_for_llg(uasd_ctx *c) {
 }
 
 // Quad arrays in c->spins: 0 e (emom), 1 p (emom2), 2 b2 (b2eff).
+// step_off >= 0: inside a graph capture, the step is *ctr[0] + step_off.
 template <typename R>
-int run_llg_half(uasd_ctx *c, int alg, int half, bool fused, bool write_e) {
+int run_llg_half(uasd_ctx *c, int alg, int half, bool fused, bool write_e,
+                 int64_t step_off = -1) {
   uasd::HamDev<R> h = ham_view<R>(c);
   uasd::LLGDev<R> prm = llg_view<R>(c);
+  if (step_off >= 0) {
+    prm.step = (uint64_t)step_off;
+    prm.step_dev = c->ctr.as<uint64_t>();
+  }
   uasd::Quad<R> *e = quads<R>(c, c->spins, 0), *p = quads<R>(c, c->spins, 1),
                 *b2 = quads<R>(c, c->spins, 2);
   const uasd::Quad<R> *beff = fused ? nullptr : quads<R>(c, c->field);
   uasd::Quad<R> *out = write_e ? e : p;
   tic(c);
-  HIP_TRY(uasd::launch_llg<R>(c->stream, alg, half, fused, h, prm, e, p, b2, beff, out));
+  HIP_TRY(uasd::launch_llg<R>(c->stream, c->tune[0], alg, half, fused, h, prm, e, p, b2, beff,
+                              out));
   return toc(c, half);
 }
 
+// parts: also write beff1 and beff2 into quad arrays 1 and 2 of c->field.
 template <typename R>
-int run_field(uasd_ctx *c, bool from_scratch, bool want_energy) {
+int run_field(uasd_ctx *c, bool from_scratch, bool want_energy, bool parts = false) {
   uasd::HamDev<R> h = ham_view<R>(c);
   const uasd::Quad<R> *src = from_scratch ? quads<R>(c, c->scratch) : quads<R>(c, c->spins, 0);
-  HIP_TRY(c->field.ensure(quad_bytes(c)));
+  HIP_TRY(c->field.ensure(std::max(c->field.bytes, (parts ? 3 : 1) * quad_bytes(c))));
   double *en = nullptr;
   if (want_energy) {
     HIP_TRY(c->energy.ensure((size_t)c->N * c->M * sizeof(double)));
     en = c->energy.as<double>();
   }
   tic(c);
-  HIP_TRY(uasd::launch_field<R>(c->stream, h, src, quads<R>(c, c->field), en));
+  HIP_TRY(uasd::launch_field<R>(c->stream, c->tune[1], h, src, quads<R>(c, c->field), en,
+                                parts ? quads<R>(c, c->field, 1) : nullptr));
   return toc(c, 2);
 }
 
@@ -518,10 +556,61 @@ int build_coloring(uasd_ctx *c) {
   std::vector<int> fillp(c->color_ptr.begin(), c->color_ptr.end() - 1);
   for (int i = 0; i < N; ++i) c->color_sites[fillp[col[i]]++] = i;
   c->coloring_dirty = false;
+  c->gen += 1;
   return UASD_OK;
 }
 
 }  // namespace
+
+// ------------------------------------------------------------ segments
+constexpr int kSegChunk = 4096;  // sites per reduction chunk
+
+int set_segments(uasd_ctx *c, int nseg, const int *off) {
+  std::vector<int> chunks, seg_chunk(1, 0), seg_n;
+  for (int s = 0; s < nseg; ++s) {
+    for (int a = off[s]; a < off[s + 1]; a += kSegChunk) {
+      chunks.push_back(a);
+      chunks.push_back(std::min(off[s + 1], a + kSegChunk));
+    }
+    seg_chunk.push_back((int)chunks.size() / 2);
+    seg_n.push_back(off[s + 1] - off[s]);
+  }
+  std::vector<int> all(chunks);
+  all.insert(all.end(), seg_chunk.begin(), seg_chunk.end());
+  all.insert(all.end(), seg_n.begin(), seg_n.end());
+  if (int r = upload_raw(c, c->seg_dev, all)) return r;
+  c->nseg = nseg;
+  c->seg_off.assign(off, off + nseg + 1);
+  c->nchunks = (int)chunks.size() / 2;
+  const size_t nacc = (size_t)nseg * c->M * uasd::kAccWidth;
+  HIP_TRY(c->acc.ensure(nacc * sizeof(double)));
+  std::vector<double> zero(nacc, 0.0);  // uploaded, not memset: the accumulators
+  if (int r = upload_raw(c, c->acc, zero)) return r;  // may later sit under a graph
+  c->acc_samples = 0;
+  return UASD_OK;
+}
+
+int ensure_segments(uasd_ctx *c) {
+  if (c->nseg > 0) return UASD_OK;
+  const int off[2] = {0, c->N};
+  return set_segments(c, 1, off);
+}
+
+// Per-(segment, ensemble) FP64 sums into c->seg_out; energy terms from the
+// last field launch when with_energy.  Asynchronous.
+template <typename R>
+int segment_sums(uasd_ctx *c, bool with_energy) {
+  const int nv = with_energy ? 4 : 3;
+  HIP_TRY(c->seg_partial.ensure((size_t)c->nchunks * c->M * nv * sizeof(double)));
+  HIP_TRY(c->seg_out.ensure((size_t)c->nseg * c->M * 4 * sizeof(double)));
+  const int2 *chunks = c->seg_dev.as<int2>();
+  const int *seg_chunk = reinterpret_cast<const int *>(chunks + c->nchunks);
+  HIP_TRY(uasd::launch_segsum<R>(c->stream, c->M, c->nseg, c->nchunks, chunks, seg_chunk,
+                                 quads<R>(c, c->spins, 0),
+                                 with_energy ? c->energy.as<double>() : nullptr, nv,
+                                 c->seg_partial.as<double>(), c->seg_out.as<double>()));
+  return UASD_OK;
+}
 
 // =================================================================== C-ABI
 extern "C" {
@@ -574,6 +663,7 @@ int uasd_ctx_destroy(uasd_ctx *c) {
   if (!c) return UASD_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
   resolve_timing(c);
+  destroy_graphs(c);
   if (c->cur) hipEventDestroy(c->cur);
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
   if (c->own_stream) hipStreamDestroy(c->own_stream);
@@ -599,6 +689,7 @@ int uasd_ctx_sync(uasd_ctx *c) {
 int uasd_set_exchange(uasd_ctx *c, int mnn, int nham, const int *nlist, const int *nlistsize,
                       const double *ncoup, const int *aham) {
   if (int r = check_ctx(c)) return r;
+  c->gen += 1;
   const int N = c->N;
   if (mnn < 0 || nham <= 0 || !nlist || !nlistsize || (mnn > 0 && !ncoup))
     return fail(UASD_ERR_ARG, "uasd_set_exchange: bad arguments");
@@ -645,6 +736,7 @@ int uasd_set_exchange(uasd_ctx *c, int mnn, int nham, const int *nlist, const in
 int uasd_set_dm(uasd_ctx *c, int mdm, int nham, const int *dmlist, const int *dmlistsize,
                 const double *dm_vect, const int *aham) {
   if (int r = check_ctx(c)) return r;
+  c->gen += 1;
   const int N = c->N;
   if (!dmlist || mdm <= 0) {  // clear
     c->Wd = 0;
@@ -697,6 +789,7 @@ int uasd_set_dm(uasd_ctx *c, int mdm, int nham, const int *dmlist, const int *dm
 int uasd_set_anisotropy(uasd_ctx *c, const int *taniso, const double *eaniso,
                         const double *kaniso, const double *sb) {
   if (int r = check_ctx(c)) return r;
+  c->gen += 1;
   const int N = c->N;
   if (!taniso) {
     c->an_rows = 0;
@@ -741,6 +834,7 @@ int uasd_set_anisotropy(uasd_ctx *c, const int *taniso, const double *eaniso,
 
 int uasd_set_external_field(uasd_ctx *c, const double *field) {
   if (int r = check_ctx(c)) return r;
+  c->gen += 1;
   const int N = c->N, M = c->M;
   if (!field) {
     c->ext_mode = 0;
@@ -773,6 +867,7 @@ int uasd_set_external_field(uasd_ctx *c, const double *field) {
 int uasd_set_llg(uasd_ctx *c, const double *lambda1, const double *temp, const double *landeg,
                  double delta_t, double temprescale, double bn) {
   if (int r = check_ctx(c)) return r;
+  c->gen += 1;
   const int N = c->N;
   if (!lambda1 || !temp) return fail(UASD_ERR_ARG, "uasd_set_llg: lambda1 and temp required");
   if (!(delta_t > 0.0)) return fail(UASD_ERR_ARG, "delta_t must be > 0");
@@ -801,8 +896,17 @@ int uasd_set_llg(uasd_ctx *c, const double *lambda1, const double *temp, const d
   return UASD_OK;
 }
 
+int uasd_set_temprescale(uasd_ctx *c, double temprescale) {
+  if (int r = check_ctx(c)) return r;
+  c->gen += 1;
+  if (!(temprescale > 0.0)) return fail(UASD_ERR_ARG, "temprescale must be > 0");
+  c->temprescale = temprescale;
+  return UASD_OK;
+}
+
 int uasd_set_noise(uasd_ctx *c, int mode, uint64_t seed, const double *normals) {
   if (int r = check_ctx(c)) return r;
+  c->gen += 1;
   if (mode != UASD_NOISE_NONE && mode != UASD_NOISE_PHILOX && mode != UASD_NOISE_INJECTED)
     return fail(UASD_ERR_ARG, "unknown noise mode");
   if (mode == UASD_NOISE_INJECTED) {
@@ -846,18 +950,28 @@ static int upload_vec_keep_w(uasd_ctx *c, int k, const double *a) {
   return upload_quads(c, c->spins, k, hq);
 }
 
-int uasd_effective_field_host(uasd_ctx *c, const double *emomM, double *beff, double *energy) {
+int uasd_effective_field_host(uasd_ctx *c, const double *emomM, double *beff, double *beff1,
+                              double *beff2, double *energy) {
   if (int r = check_ctx(c)) return r;
   if (!c->have_ex) return fail(UASD_ERR_STATE, "exchange not set");
   if (!emomM || !beff) return fail(UASD_ERR_ARG, "null array");
   if (int r = rebuild_classes(c)) return r;
   if (int r = upload_quads(c, c->scratch, 0, to_quads(emomM, c->N, c->M, nullptr, 1.0))) return r;
-  int r = c->prec == UASD_FP64 ? run_field<double>(c, true, energy != nullptr)
-                               : run_field<float>(c, true, energy != nullptr);
+  const bool parts = beff1 || beff2;
+  int r = c->prec == UASD_FP64 ? run_field<double>(c, true, energy != nullptr, parts)
+                               : run_field<float>(c, true, energy != nullptr, parts);
   if (r) return r;
   std::vector<double> hq;
   if ((r = download_quads(c, c->field, 0, hq))) return r;
   from_quads(hq, c->N, c->M, beff, false);
+  if (beff1) {
+    if ((r = download_quads(c, c->field, 1, hq))) return r;
+    from_quads(hq, c->N, c->M, beff1, false);
+  }
+  if (beff2) {
+    if ((r = download_quads(c, c->field, 2, hq))) return r;
+    from_quads(hq, c->N, c->M, beff2, false);
+  }
   if (energy) {
     std::vector<double> red;
     r = c->prec == UASD_FP64 ? reduce_moments<double>(c, true, true, red)
@@ -930,13 +1044,115 @@ int uasd_evolve_second_host(uasd_ctx *c, int alg, const double *beff, const doub
 }
 
 // ----------------------------------------------------------- resident path
+}  // extern "C"
+
+namespace {
+
+void destroy_graphs(uasd_ctx *c) {
+  for (auto &kv : c->graphs) {
+    if (kv.second.exec) hipGraphExecDestroy(kv.second.exec);
+    if (kv.second.g) hipGraphDestroy(kv.second.g);
+  }
+  c->graphs.clear();
+}
+
+// Records what body() enqueues on the context stream and instantiates it.
+template <typename F>
+int capture_graph(uasd_ctx *c, F &&body, uasd_ctx::Graph &out) {
+  HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  const int r = body();
+  c->capturing = false;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (r) {
+    if (g) hipGraphDestroy(g);
+    return r;
+  }
+  if (e != hipSuccess)
+    return fail(UASD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  hipGraphExec_t x = nullptr;
+  const hipError_t e2 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e2 != hipSuccess) {
+    hipGraphDestroy(g);
+    return fail(UASD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+  }
+  out.g = g;
+  out.exec = x;
+  return UASD_OK;
+}
+
+// Replays graph `key` (capturing it on first use), which advances device
+// counter idx by K.  The device counter is first set to host_ctr when they
+// differ (after eager steps), with a kernel, so no memset or copy node is
+// ever recorded.
+template <typename F>
+int replay_graph(uasd_ctx *c, const std::tuple<int, int, int, uint64_t> &key, int idx,
+                 uint64_t &host_ctr, int K, int timer, F &&body) {
+  if (c->graphs_gen != c->gen || c->graphs.size() > 16) {
+    destroy_graphs(c);
+    c->graphs_gen = c->gen;
+  }
+  if (!c->ctr.p) {
+    HIP_TRY(c->ctr.ensure(2 * sizeof(uint64_t)));
+    c->ctr_valid[0] = c->ctr_valid[1] = false;
+  }
+  auto it = c->graphs.find(key);
+  if (it == c->graphs.end()) {
+    uasd_ctx::Graph gr;
+    if (int r = capture_graph(c, body, gr)) return r;
+    it = c->graphs.emplace(key, gr).first;
+  }
+  if (!c->ctr_valid[idx] || c->ctr_val[idx] != host_ctr) {
+    HIP_TRY(uasd::launch_counter(c->stream, idx, c->ctr.as<uint64_t>(), host_ctr, 0));
+    c->ctr_valid[idx] = true;
+  }
+  tic(c);
+  HIP_TRY(hipGraphLaunch(it->second.exec, c->stream));
+  if (int r = toc(c, timer)) return r;
+  host_ctr += (uint64_t)K;
+  c->ctr_val[idx] = host_ctr;
+  return UASD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int uasd_set_graph_steps(uasd_ctx *c, int steps_per_graph) {
+  if (int r = check_ctx(c)) return r;
+  if (steps_per_graph < 0 || steps_per_graph > 4096)
+    return fail(UASD_ERR_ARG, "steps_per_graph must be 0 (off) .. 4096");
+  c->graph_steps = steps_per_graph;
+  return UASD_OK;
+}
+
 int uasd_sd_steps(uasd_ctx *c, int alg, int64_t nsteps) {
   if (int r = check_ctx(c)) return r;
   if (alg != UASD_SDE_MIDPOINT && alg != UASD_SDE_DEPONDT)
     return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint) or 5 (Depondt)");
   if (nsteps < 0) return fail(UASD_ERR_ARG, "nsteps < 0");
   if (int r = ready_for_llg(c)) return r;
-  for (int64_t s = 0; s < nsteps; ++s) {
+  int64_t s = 0;
+  const int K = c->graph_steps;
+  if (K > 0) {
+    auto body = [&]() -> int {
+      for (int j = 0; j < K; ++j) {
+        int r = c->prec == UASD_FP64 ? run_llg_half<double>(c, alg, 0, true, false, j)
+                                     : run_llg_half<float>(c, alg, 0, true, false, j);
+        if (r) return r;
+        r = c->prec == UASD_FP64 ? run_llg_half<double>(c, alg, 1, true, true, j)
+                                 : run_llg_half<float>(c, alg, 1, true, true, j);
+        if (r) return r;
+      }
+      HIP_TRY(uasd::launch_counter(c->stream, 0, c->ctr.as<uint64_t>(), (uint64_t)K, 1));
+      return UASD_OK;
+    };
+    for (; nsteps - s >= K; s += K)
+      if (int r = replay_graph(c, std::make_tuple(0, alg, K, (uint64_t)0), 0, c->step, K, 4, body))
+        return r;
+  }
+  for (; s < nsteps; ++s) {
     int r = c->prec == UASD_FP64 ? run_llg_half<double>(c, alg, 0, true, false)
                                  : run_llg_half<float>(c, alg, 0, true, false);
     if (r) return r;
@@ -973,8 +1189,9 @@ int uasd_effective_field(uasd_ctx *c, double *beff, double *energy) {
   return UASD_OK;
 }
 
-int uasd_mc_sweeps(uasd_ctx *c, char mode, const double *temps, int64_t nsweeps, uint64_t seed) {
-  if (int r = check_ctx(c)) return r;
+// Uploads the per-ensemble MC constants and the colour lists and fills mp.
+static int mc_prepare(uasd_ctx *c, char mode, const double *temps, uint64_t seed,
+                      uasd::MCParams &mp) {
   if (mode != 'M' && mode != 'H') return fail(UASD_ERR_UNSUPPORTED, "mode must be 'M' or 'H'");
   if (!temps) return fail(UASD_ERR_ARG, "temps is null");
   if (!c->have_ex || !c->have_state) return fail(UASD_ERR_STATE, "exchange or moments not set");
@@ -1002,14 +1219,21 @@ int uasd_mc_sweeps(uasd_ctx *c, char mode, const double *temps, int64_t nsweeps,
   HIP_TRY(hipMemcpyAsync(sites_dev, c->color_sites.data(), c->color_sites.size() * sizeof(int),
                          hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  uasd::MCParams mp{};
+  mp = uasd::MCParams{};
   const double *base = c->mcbuf.as<double>();
+  mp.sites = sites_dev;
   mp.delta = base;
   mp.beta = base + M;
   mp.hbeta = base + 2 * (size_t)M;
   mp.ext = base + 3 * (size_t)M;
   mp.seed = seed;
   mp.heat_bath = mode == 'H';
+  return UASD_OK;
+}
+
+// nsweeps colour-ordered sweeps; every launch is asynchronous on the stream.
+static int mc_run(uasd_ctx *c, uasd::MCParams mp, int64_t nsweeps) {
+  const int *sites_dev = mp.sites;
   for (int64_t s = 0; s < nsweeps; ++s) {
     mp.sweep = c->mc_sweep;
     for (int col = 0; col < c->ncolors; ++col) {
@@ -1017,10 +1241,10 @@ int uasd_mc_sweeps(uasd_ctx *c, char mode, const double *temps, int64_t nsweeps,
       mp.nsites = c->color_ptr[col + 1] - c->color_ptr[col];
       tic(c);
       if (c->prec == UASD_FP64) {
-        HIP_TRY(uasd::launch_mc<double>(c->stream, ham_view<double>(c), mp,
+        HIP_TRY(uasd::launch_mc<double>(c->stream, c->tune[2], ham_view<double>(c), mp,
                                         quads<double>(c, c->spins, 0)));
       } else {
-        HIP_TRY(uasd::launch_mc<float>(c->stream, ham_view<float>(c), mp,
+        HIP_TRY(uasd::launch_mc<float>(c->stream, c->tune[2], ham_view<float>(c), mp,
                                        quads<float>(c, c->spins, 0)));
       }
       if (int r = toc(c, 3)) return r;
@@ -1028,6 +1252,174 @@ int uasd_mc_sweeps(uasd_ctx *c, char mode, const double *temps, int64_t nsweeps,
     c->mc_sweep += 1;
   }
   return UASD_OK;
+}
+
+int uasd_mc_sweeps(uasd_ctx *c, char mode, const double *temps, int64_t nsweeps, uint64_t seed) {
+  if (int r = check_ctx(c)) return r;
+  if (nsweeps < 0) return fail(UASD_ERR_ARG, "nsweeps < 0");
+  uasd::MCParams mp;
+  if (int r = mc_prepare(c, mode, temps, seed, mp)) return r;
+  int64_t s = 0;
+  const int K = c->graph_steps;
+  if (K > 0) {
+    auto body = [&]() -> int {
+      uasd::MCParams g = mp;
+      g.sweep_dev = c->ctr.as<uint64_t>() + 1;
+      for (int j = 0; j < K; ++j) {
+        g.sweep = (uint64_t)j;
+        for (int col = 0; col < c->ncolors; ++col) {
+          g.sites = mp.sites + c->color_ptr[col];
+          g.nsites = c->color_ptr[col + 1] - c->color_ptr[col];
+          if (c->prec == UASD_FP64) {
+            HIP_TRY(uasd::launch_mc<double>(c->stream, c->tune[2], ham_view<double>(c), g,
+                                            quads<double>(c, c->spins, 0)));
+          } else {
+            HIP_TRY(uasd::launch_mc<float>(c->stream, c->tune[2], ham_view<float>(c), g,
+                                           quads<float>(c, c->spins, 0)));
+          }
+        }
+      }
+      HIP_TRY(uasd::launch_counter(c->stream, 1, c->ctr.as<uint64_t>(), (uint64_t)K, 1));
+      return UASD_OK;
+    };
+    for (; nsweeps - s >= K; s += K)
+      if (int r = replay_graph(c, std::make_tuple(1, (int)mode, K, seed), 1, c->mc_sweep, K, 5,
+                               body))
+        return r;
+  }
+  return mc_run(c, mp, nsweeps - s);
+}
+
+// ---------------------------------------------------------------- tuning
+int uasd_set_tuning(uasd_ctx *c, int kind, int gather, int block) {
+  if (int r = check_ctx(c)) return r;
+  c->gen += 1;
+  if (kind < 0 || kind > 2) return fail(UASD_ERR_ARG, "kind must be 0 (LLG), 1 (field) or 2 (MC)");
+  uasd::Tune t{gather, block};
+  if (!uasd::tune_valid(t))
+    return fail(UASD_ERR_ARG, "gather must be 2, 4, 8 or 16 and block 64, 128 or 256");
+  c->tune[kind] = t;
+  return UASD_OK;
+}
+
+int uasd_get_tuning(uasd_ctx *c, int kind, int *gather, int *block) {
+  if (int r = check_ctx(c)) return r;
+  if (kind < 0 || kind > 2) return fail(UASD_ERR_ARG, "kind must be 0 (LLG), 1 (field) or 2 (MC)");
+  if (gather) *gather = c->tune[kind].gather;
+  if (block) *block = c->tune[kind].block;
+  return UASD_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Copies of the spin quads and the counters, restored after a tuning run so
+// that tuning leaves the trajectory untouched.
+struct StateGuard {
+  uasd_ctx *c;
+  DevBuf save;
+  uint64_t step, sweep;
+  bool timing;
+  int err = UASD_OK;
+  explicit StateGuard(uasd_ctx *cc) : c(cc), step(cc->step), sweep(cc->mc_sweep), timing(cc->timing) {
+    const size_t n = 3 * quad_bytes(c);
+    hipError_t e = save.ensure(n);
+    if (e == hipSuccess) e = hipMemcpyAsync(save.p, c->spins.p, n, hipMemcpyDeviceToDevice, c->stream);
+    if (e != hipSuccess) err = fail(UASD_ERR_HIP, std::string("tuning state copy: ") + hipGetErrorString(e));
+    c->timing = false;
+  }
+  int restore() {
+    const size_t n = 3 * quad_bytes(c);
+    HIP_TRY(hipMemcpyAsync(c->spins.p, save.p, n, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->step = step;
+    c->mc_sweep = sweep;
+    c->timing = timing;
+    return UASD_OK;
+  }
+};
+
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  ~EventPair() {
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+  }
+};
+
+// Times `run` (asynchronous launches on c->stream) for every candidate shape,
+// once untimed and once timed, and leaves the fastest in *slot.
+template <typename Run>
+int tune_search(uasd_ctx *c, uasd::Tune *slot, Run &&run, double *best_ms) {
+  EventPair ev;
+  HIP_TRY(hipEventCreate(&ev.a));
+  HIP_TRY(hipEventCreate(&ev.b));
+  const int gathers[] = {2, 4, 8, 16};
+  const int blocks[] = {64, 128, 256};
+  uasd::Tune best = *slot;
+  float best_t = 1e30f;
+  for (int g : gathers)
+    for (int bl : blocks) {
+      *slot = uasd::Tune{g, bl};
+      if (int r = run()) return r;
+      HIP_TRY(hipEventRecord(ev.a, c->stream));
+      if (int r = run()) return r;
+      HIP_TRY(hipEventRecord(ev.b, c->stream));
+      HIP_TRY(hipEventSynchronize(ev.b));
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+      if (ms < best_t) {
+        best_t = ms;
+        best = *slot;
+      }
+    }
+  *slot = best;
+  if (best_ms) *best_ms = best_t;
+  return UASD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int uasd_autotune_llg(uasd_ctx *c, int alg, int steps, double *best_ms) {
+  if (int r = check_ctx(c)) return r;
+  if (alg != UASD_SDE_MIDPOINT && alg != UASD_SDE_DEPONDT)
+    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint) or 5 (Depondt)");
+  if (steps <= 0) return fail(UASD_ERR_ARG, "steps must be > 0");
+  if (int r = ready_for_llg(c)) return r;
+  StateGuard g(c);
+  if (g.err) return g.err;
+  int r = tune_search(c, &c->tune[0], [&]() -> int {
+    for (int s = 0; s < steps; ++s) {
+      int e = c->prec == UASD_FP64 ? run_llg_half<double>(c, alg, 0, true, false)
+                                   : run_llg_half<float>(c, alg, 0, true, false);
+      if (e) return e;
+      e = c->prec == UASD_FP64 ? run_llg_half<double>(c, alg, 1, true, true)
+                               : run_llg_half<float>(c, alg, 1, true, true);
+      if (e) return e;
+      c->step += 1;
+    }
+    return UASD_OK;
+  }, best_ms);
+  if (r) return r;
+  c->tune[1] = c->tune[0];  // the field kernel gathers the same way
+  c->gen += 1;
+  return g.restore();
+}
+
+int uasd_autotune_mc(uasd_ctx *c, char mode, const double *temps, int sweeps, double *best_ms) {
+  if (int r = check_ctx(c)) return r;
+  if (sweeps <= 0) return fail(UASD_ERR_ARG, "sweeps must be > 0");
+  uasd::MCParams mp;
+  if (int r = mc_prepare(c, mode, temps, 0x7u, mp)) return r;
+  StateGuard g(c);
+  if (g.err) return g.err;
+  int r = tune_search(c, &c->tune[2], [&]() -> int { return mc_run(c, mp, sweeps); }, best_ms);
+  if (r) return r;
+  c->gen += 1;
+  return g.restore();
 }
 
 int uasd_measure(uasd_ctx *c, double *msum, double *energy) {
@@ -1053,6 +1445,83 @@ int uasd_measure(uasd_ctx *c, double *msum, double *energy) {
   return UASD_OK;
 }
 
+int uasd_set_segments(uasd_ctx *c, int nseg, const int *offsets) {
+  if (int r = check_ctx(c)) return r;
+  if (nseg <= 0 || !offsets) return fail(UASD_ERR_ARG, "nseg must be > 0 with offsets (nseg+1)");
+  if (offsets[0] != 0 || offsets[nseg] != c->N)
+    return fail(UASD_ERR_ARG, "segment offsets must start at 0 and end at natom");
+  for (int s = 0; s < nseg; ++s)
+    if (offsets[s + 1] <= offsets[s]) return fail(UASD_ERR_ARG, "segments must be non-empty and increasing");
+  return set_segments(c, nseg, offsets);
+}
+
+int uasd_measure_segments(uasd_ctx *c, double *msum, double *energy) {
+  if (int r = check_ctx(c)) return r;
+  if (!c->have_state) return fail(UASD_ERR_STATE, "moments not set");
+  if (!msum) return fail(UASD_ERR_ARG, "msum is null");
+  if (int r = ensure_segments(c)) return r;
+  if (energy) {
+    if (!c->have_ex) return fail(UASD_ERR_STATE, "exchange not set");
+    if (int r = rebuild_classes(c)) return r;
+    int r = c->prec == UASD_FP64 ? run_field<double>(c, false, true)
+                                 : run_field<float>(c, false, true);
+    if (r) return r;
+  }
+  int r = c->prec == UASD_FP64 ? segment_sums<double>(c, energy != nullptr)
+                               : segment_sums<float>(c, energy != nullptr);
+  if (r) return r;
+  const int nv = energy ? 4 : 3;
+  std::vector<double> out((size_t)c->nseg * c->M * nv);
+  HIP_TRY(hipMemcpyAsync(out.data(), c->seg_out.p, out.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (size_t t = 0; t < (size_t)c->nseg * c->M; ++t) {
+    for (int d = 0; d < 3; ++d) msum[3 * t + d] = out[nv * t + d];
+    if (energy) energy[t] = out[nv * t + 3] * uasd::kMub / uasd::kMry;
+  }
+  return UASD_OK;
+}
+
+int uasd_accumulate(uasd_ctx *c, int with_energy) {
+  if (int r = check_ctx(c)) return r;
+  if (!c->have_state) return fail(UASD_ERR_STATE, "moments not set");
+  if (int r = ensure_segments(c)) return r;
+  if (with_energy) {
+    if (!c->have_ex) return fail(UASD_ERR_STATE, "exchange not set");
+    if (int r = rebuild_classes(c)) return r;
+    int r = c->prec == UASD_FP64 ? run_field<double>(c, false, true)
+                                 : run_field<float>(c, false, true);
+    if (r) return r;
+  }
+  int r = c->prec == UASD_FP64 ? segment_sums<double>(c, with_energy != 0)
+                               : segment_sums<float>(c, with_energy != 0);
+  if (r) return r;
+  const int *seg_n = reinterpret_cast<const int *>(c->seg_dev.as<int2>() + c->nchunks) + c->nseg + 1;
+  HIP_TRY(uasd::launch_accumulate(c->stream, c->nseg, c->M, seg_n, c->seg_out.as<double>(),
+                                  with_energy ? 4 : 3, c->acc.as<double>()));
+  c->acc_samples += 1;
+  return UASD_OK;
+}
+
+int uasd_get_accumulators(uasd_ctx *c, double *out, int64_t *nsamples) {
+  if (int r = check_ctx(c)) return r;
+  if (int r = ensure_segments(c)) return r;
+  if (out) {
+    HIP_TRY(hipMemcpyAsync(out, c->acc.p, (size_t)c->nseg * c->M * uasd::kAccWidth * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  if (nsamples) *nsamples = c->acc_samples;
+  return UASD_OK;
+}
+
+int uasd_reset_accumulators(uasd_ctx *c) {
+  if (int r = check_ctx(c)) return r;
+  if (c->nseg == 0) return ensure_segments(c);
+  std::vector<int> off(c->seg_off);
+  return set_segments(c, c->nseg, off.data());
+}
+
 int uasd_get_coloring(uasd_ctx *c, int *ncolors, int *color_of_site) {
   if (int r = check_ctx(c)) return r;
   if (!c->have_ex) return fail(UASD_ERR_STATE, "exchange not set");
@@ -1072,7 +1541,7 @@ int uasd_set_kernel_timing(uasd_ctx *c, int enable) {
 
 int uasd_get_kernel_timing(uasd_ctx *c, int kind, int64_t *launches, double *total_ms) {
   if (int r = check_ctx(c)) return r;
-  if (kind < 0 || kind > 3) return fail(UASD_ERR_ARG, "kind must be 0..3");
+  if (kind < 0 || kind > 5) return fail(UASD_ERR_ARG, "kind must be 0..5");
   if (int r = resolve_timing(c)) return r;
   if (launches) *launches = c->timers[kind].launches;
   if (total_ms) *total_ms = c->timers[kind].ms;

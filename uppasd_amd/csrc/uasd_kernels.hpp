@@ -206,7 +206,8 @@ __device__ __forceinline__ void cubic(const HamDev<R> &h, int row, R m0, R m1, R
 
 // Field parts of one site: s = beff_s (pair terms + uniaxial), qf = beff_q
 // (type-7 cubic part times sb), x = external field.
-template <typename R>
+// G = neighbours whose loads are issued together (a tuning parameter).
+template <typename R, int G>
 __device__ __forceinline__ void site_field(const HamDev<R> &h, const Quad<R> *__restrict__ src,
                                            int site, int ens, R &s0, R &s1, R &s2, R &qf0,
                                            R &qf1, R &qf2, R &x0, R &x1, R &x2) {
@@ -215,31 +216,79 @@ __device__ __forceinline__ void site_field(const HamDev<R> &h, const Quad<R> *__
   const SiteClass sc = h.classes[h.cls[site]];
   const int M = h.M, N = h.N;
   {
+    // Neighbours are taken G at a time: all index loads of a batch are
+    // issued, then all spin and coupling loads, then the terms are added in
+    // neighbour order (the same order, hence the same rounding, as one at a
+    // time).  A last partial batch masks the slots past len.
     const int len = h.ex_len[sc.ex_row];
     const int *nb = h.nbr + site;
     const R *cp = h.cpl + sc.ex_row;
-    for (int j = 0; j < len; ++j) {
-      const int n = nb[(size_t)j * N];
-      const R J = cp[(size_t)j * h.ex_rows];
-      R m0, m1, m2;
-      moment(src, (size_t)n * M + ens, m0, m1, m2);
-      s0 = s0 + J * m0;
-      s1 = s1 + J * m1;
-      s2 = s2 + J * m2;
+    const size_t rs = (size_t)h.ex_rows;
+    int j = 0;
+    for (; j + G <= len; j += G) {
+      int n[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) n[u] = nb[(size_t)(j + u) * N];
+      Quad<R> v[G];
+      R J[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        v[u] = src[(size_t)n[u] * M + ens];
+        J[u] = cp[(size_t)(j + u) * rs];
+      }
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        s0 = s0 + J[u] * (v[u].x * v[u].w);
+        s1 = s1 + J[u] * (v[u].y * v[u].w);
+        s2 = s2 + J[u] * (v[u].z * v[u].w);
+      }
+    }
+    if (j < len) {
+      int n[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) n[u] = j + u < len ? nb[(size_t)(j + u) * N] : site;
+      Quad<R> v[G];
+      R J[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        if (j + u < len) {
+          v[u] = src[(size_t)n[u] * M + ens];
+          J[u] = cp[(size_t)(j + u) * rs];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        if (j + u < len) {
+          s0 = s0 + J[u] * (v[u].x * v[u].w);
+          s1 = s1 + J[u] * (v[u].y * v[u].w);
+          s2 = s2 + J[u] * (v[u].z * v[u].w);
+        }
+      }
     }
   }
   if (h.Wd > 0 && sc.dm_row >= 0) {
     const int len = h.dm_len[sc.dm_row];
     const int *nb = h.dnbr + site;
-    for (int j = 0; j < len; ++j) {
-      const int n = nb[(size_t)j * N];
-      const size_t r = (size_t)j * h.dm_rows + sc.dm_row;
-      const R D0 = h.dvx[r], D1 = h.dvy[r], D2 = h.dvz[r];
-      R m0, m1, m2;
-      moment(src, (size_t)n * M + ens, m0, m1, m2);
-      s0 = s0 + D2 * m1 - D1 * m2;
-      s1 = s1 + D0 * m2 - D2 * m0;
-      s2 = s2 + D1 * m0 - D0 * m1;
+    const size_t rs = (size_t)h.dm_rows;
+    for (int j = 0; j < len; j += G) {
+      int n[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) n[u] = j + u < len ? nb[(size_t)(j + u) * N] : site;
+      Quad<R> v[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u)
+        if (j + u < len) v[u] = src[(size_t)n[u] * M + ens];
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        if (j + u < len) {
+          const size_t r = (size_t)(j + u) * rs + sc.dm_row;
+          const R D0 = h.dvx[r], D1 = h.dvy[r], D2 = h.dvz[r];
+          const R m0 = v[u].x * v[u].w, m1 = v[u].y * v[u].w, m2 = v[u].z * v[u].w;
+          s0 = s0 + D2 * m1 - D1 * m2;
+          s1 = s1 + D0 * m2 - D2 * m0;
+          s2 = s2 + D1 * m0 - D0 * m1;
+        }
+      }
     }
   }
   if (sc.an_row >= 0) {
@@ -277,11 +326,11 @@ __device__ __forceinline__ void site_field(const HamDev<R> &h, const Quad<R> *__
 }
 
 // beff = 0 + beff1 + beff2, beff2 = beff_q + ext + 0  (hamiltonianactions.f90:240-243)
-template <typename R>
+template <typename R, int G>
 __device__ __forceinline__ void total_field(const HamDev<R> &h, const Quad<R> *__restrict__ src,
                                             int site, int ens, R &b0, R &b1, R &b2) {
   R s0, s1, s2, q0, q1, q2, x0, x1, x2;
-  site_field(h, src, site, ens, s0, s1, s2, q0, q1, q2, x0, x1, x2);
+  site_field<R, G>(h, src, site, ens, s0, s1, s2, q0, q1, q2, x0, x1, x2);
   b0 = (R)0 + s0 + (q0 + x0 + (R)0);
   b1 = (R)0 + s1 + (q1 + x1 + (R)0);
   b2 = (R)0 + s2 + (q2 + x2 + (R)0);
@@ -295,14 +344,16 @@ struct LLGDev {
   double dt, temprescale, bn;
   int noise_mode;
   uint64_t seed;
-  uint64_t step;
+  uint64_t step;              // step counter, or offset from *step_dev
+  const uint64_t *step_dev;   // device-held step base (graph replays) or nullptr
   const Quad<R> *noise;  // injected N(0,1) per q
   __device__ __forceinline__ double lambda(int s) const { return lam ? lam[s] : lam0; }
   __device__ __forceinline__ double temperature(int s) const { return temp ? temp[s] : temp0; }
   __device__ __forceinline__ double landeg(int s) const { return lande ? lande[s] : lande0; }
   __device__ __forceinline__ void normals(int site, int ens, size_t q, R &a, R &b, R &c) const {
     if (noise_mode == 1) {
-      Noise<R>::normal3(seed, kTagLLG, step, (uint32_t)site, (uint32_t)ens, a, b, c);
+      const uint64_t st = step_dev ? *step_dev + step : step;
+      Noise<R>::normal3(seed, kTagLLG, st, (uint32_t)site, (uint32_t)ens, a, b, c);
     } else if (noise_mode == 2) {
       const Quad<R> v = noise[q];
       a = v.x;
@@ -408,22 +459,29 @@ __device__ __forceinline__ void midpoint_solve(R A0, R A1, R A2, R e0, R e1, R e
 // =================================================================== kernels
 
 // effective_field_full over all (site, ens): beff (quads, w unused) and, when
-// en != null, the per-site field-routine energy -m.(0.5*(bs+2bq+2bext)) in FP64.
-template <typename R>
+// en != null, the per-site field-routine energy -m.(0.5*(bs+2bq+2bext)) in FP64;
+// when parts != null also beff1 = bs into parts[q] and beff2 = bq + bext into
+// parts[N*M + q] (hamiltonianactions.f90:240-241).
+template <typename R, int G>
 __global__ void __launch_bounds__(256) k_field(HamDev<R> h, const Quad<R> *__restrict__ src,
-                                               Quad<R> *__restrict__ beff, double *en) {
+                                               Quad<R> *__restrict__ beff, double *en,
+                                               Quad<R> *__restrict__ parts) {
   const int64_t total = (int64_t)h.N * h.M;
   const int64_t q = xcd_block() * blockDim.x + threadIdx.x;
   if (q >= total) return;
   const int site = (int)(q / h.M), ens = (int)(q - (int64_t)site * h.M);
   R s0, s1, s2, q0, q1, q2, x0, x1, x2;
-  site_field(h, src, site, ens, s0, s1, s2, q0, q1, q2, x0, x1, x2);
+  site_field<R, G>(h, src, site, ens, s0, s1, s2, q0, q1, q2, x0, x1, x2);
   Quad<R> b;
   b.x = (R)0 + s0 + (q0 + x0 + (R)0);
   b.y = (R)0 + s1 + (q1 + x1 + (R)0);
   b.z = (R)0 + s2 + (q2 + x2 + (R)0);
   b.w = (R)0;
   beff[q] = b;
+  if (parts) {
+    parts[q] = Quad<R>{s0, s1, s2, (R)0};
+    parts[total + q] = Quad<R>{q0 + x0 + (R)0, q1 + x1 + (R)0, q2 + x2 + (R)0, (R)0};
+  }
   if (en) {
     R m0, m1, m2;
     moment(src, (size_t)q, m0, m1, m2);
@@ -442,7 +500,7 @@ __global__ void __launch_bounds__(256) k_field(HamDev<R> h, const Quad<R> *__res
 // Buffers: e = emom at step start, p = predicted emom (emom2 after the first
 // half), b2 = Depondt's b2eff.  Quads carry mmom in w.  The corrector writes
 // the new emom into `out` (e when moment_update is fused, p otherwise).
-template <typename R, int ALG, int HALF, bool FUSED>
+template <typename R, int ALG, int HALF, bool FUSED, int G>
 __global__ void __launch_bounds__(256)
     k_llg(HamDev<R> h, LLGDev<R> prm, const Quad<R> *e, Quad<R> *p, Quad<R> *b2,
           const Quad<R> *__restrict__ beff, Quad<R> *out) {
@@ -452,7 +510,7 @@ __global__ void __launch_bounds__(256)
   const int site = (int)(q / h.M), ens = (int)(q - (int64_t)site * h.M);
   R b0, b1, b2v;
   if (FUSED) {
-    total_field(h, HALF == 0 ? e : (const Quad<R> *)p, site, ens, b0, b1, b2v);
+    total_field<R, G>(h, HALF == 0 ? e : (const Quad<R> *)p, site, ens, b0, b1, b2v);
   } else {
     const Quad<R> b = beff[q];
     b0 = b.x;
@@ -544,7 +602,7 @@ __device__ __forceinline__ R sq(R x) {
 
 // calculate_energy (montecarlo_common.f90:431-865), exchange + anisotropy +
 // DM + external field; returns mub*(E_trial - E_current).
-template <typename R>
+template <typename R, int G>
 __device__ __forceinline__ R mc_delta_e(const HamDev<R> &h, const Quad<R> *__restrict__ s,
                                         int site, int ens, R n0, R n1, R n2, R x0, R x1, R x2) {
   const SiteClass sc = h.classes[h.cls[site]];
@@ -556,14 +614,29 @@ __device__ __forceinline__ R mc_delta_e(const HamDev<R> &h, const Quad<R> *__res
   const R t0 = n0 * m, t1 = n1 * m, t2 = n2 * m;
   R e_c = 0, e_t = 0;
   {
+    // batched loads, terms added in neighbour order (as in site_field)
     const int len = h.ex_len[sc.ex_row];
-    for (int j = 0; j < len; ++j) {
-      const int nb = h.nbr[(size_t)j * N + site];
-      const R J = h.cpl[(size_t)j * h.ex_rows + sc.ex_row];
-      R a0, a1, a2;
-      moment(s, (size_t)nb * M + ens, a0, a1, a2);
-      e_c = e_c - J * (c0 * a0 + c1 * a1 + c2 * a2);
-      e_t = e_t - J * (t0 * a0 + t1 * a1 + t2 * a2);
+    for (int j = 0; j < len; j += G) {
+      int nb[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) nb[u] = j + u < len ? h.nbr[(size_t)(j + u) * N + site] : site;
+      Quad<R> v[G];
+      R J[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        if (j + u < len) {
+          v[u] = s[(size_t)nb[u] * M + ens];
+          J[u] = h.cpl[(size_t)(j + u) * h.ex_rows + sc.ex_row];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        if (j + u < len) {
+          const R a0 = v[u].x * v[u].w, a1 = v[u].y * v[u].w, a2 = v[u].z * v[u].w;
+          e_c = e_c - J[u] * (c0 * a0 + c1 * a1 + c2 * a2);
+          e_t = e_t - J[u] * (t0 * a0 + t1 * a1 + t2 * a2);
+        }
+      }
     }
   }
   if (sc.an_row >= 0) {
@@ -637,11 +710,12 @@ struct MCParams {
   const double *beta;   // (M) Metropolis 1/(k_B (T*temprescale + 1e-15))
   const double *hbeta;  // (M) heat-bath 1/k_B/(temprescale*T)
   const double *ext;    // (3*M) uniform field per ensemble
-  uint64_t seed, sweep;
+  uint64_t seed, sweep;      // sweep counter, or offset from *sweep_dev
+  const uint64_t *sweep_dev;  // device-held sweep base (graph replays) or nullptr
   int heat_bath;
 };
 
-template <typename R>
+template <typename R, int G>
 __global__ void __launch_bounds__(256) k_mc_colour(HamDev<R> h, MCParams mp, Quad<R> *s) {
   const int64_t total = (int64_t)mp.nsites * h.M;
   const int64_t t = xcd_block() * blockDim.x + threadIdx.x;
@@ -650,13 +724,14 @@ __global__ void __launch_bounds__(256) k_mc_colour(HamDev<R> h, MCParams mp, Qua
   const int site = mp.sites[pidx];
   const size_t q = (size_t)site * h.M + ens;
   double u0, u1, u2, u3;
-  uniform4(mp.seed, kTagMCU, mp.sweep, (uint32_t)site, (uint32_t)ens, u0, u1, u2, u3);
+  const uint64_t sweep = mp.sweep_dev ? *mp.sweep_dev + mp.sweep : mp.sweep;
+  uniform4(mp.seed, kTagMCU, sweep, (uint32_t)site, (uint32_t)ens, u0, u1, u2, u3);
   const Quad<R> me = s[q];
   const R x0 = (R)mp.ext[3 * ens], x1 = (R)mp.ext[3 * ens + 1], x2 = (R)mp.ext[3 * ens + 2];
   R o0, o1, o2;
   if (mp.heat_bath) {
     R s0, s1, s2, q0, q1, q2, y0, y1, y2;
-    site_field(h, (const Quad<R> *)s, site, ens, s0, s1, s2, q0, q1, q2, y0, y1, y2);
+    site_field<R, G>(h, (const Quad<R> *)s, site, ens, s0, s1, s2, q0, q1, q2, y0, y1, y2);
     const R f0 = (R)0 + s0 + (q0 + x0 + (R)0);
     const R f1 = (R)0 + s1 + (q1 + x1 + (R)0);
     const R f2 = (R)0 + s2 + (q2 + x2 + (R)0);
@@ -664,11 +739,11 @@ __global__ void __launch_bounds__(256) k_mc_colour(HamDev<R> h, MCParams mp, Qua
   } else {
     double g0 = 0.0, g1 = 0.0, g2 = 0.0;
     if ((int)floor(3 * u0) == 1)
-      Noise<double>::normal3(mp.seed, kTagMCG, mp.sweep, (uint32_t)site, (uint32_t)ens, g0, g1,
+      Noise<double>::normal3(mp.seed, kTagMCG, sweep, (uint32_t)site, (uint32_t)ens, g0, g1,
                              g2);
     R n0, n1, n2;
     trial_move<R>(me.x, me.y, me.z, mp.delta[ens], u0, u1, u2, g0, g1, g2, n0, n1, n2);
-    const R de = mc_delta_e(h, (const Quad<R> *)s, site, ens, n0, n1, n2, x0, x1, x2);
+    const R de = mc_delta_e<R, G>(h, (const Quad<R> *)s, site, ens, n0, n1, n2, x0, x1, x2);
     if (!(de <= (R)0 || u3 < exp(-mp.beta[ens] * (double)de))) return;
     o0 = n0;
     o1 = n1;
@@ -730,6 +805,124 @@ __global__ void k_reduce_stage2(int M, int nblocks, const double *__restrict__ p
   double x = 0.0;
   for (int b = 0; b < nblocks; ++b) x += partial[((size_t)ens * nblocks + b) * NV + v];
   out[t] = x;
+}
+
+// ------------------------------------------------- per-replica reductions
+// The sites of a batch are split into segments (replicas); each segment is
+// cut into chunks of consecutive sites, chunk c = sites [chunks[c].x,
+// chunks[c].y).  partial[(c*M + ens)*NV + v] holds the FP64 sum of value v
+// (emomM x, y, z and, for NV = 4, the per-q energy) over the chunk.
+//
+// Wide form, for M dividing the workgroup size: a workgroup reads the chunk's
+// q range contiguously and thread t always meets ensemble t % M.
+template <typename R, int NV>
+__global__ void __launch_bounds__(256)
+    k_segsum_wide(int M, const int2 *__restrict__ chunks, const Quad<R> *__restrict__ s,
+                  const double *__restrict__ en, double *__restrict__ partial) {
+  const int2 ch = chunks[blockIdx.x];
+  const size_t q1 = (size_t)ch.y * M;
+  double a[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) a[v] = 0.0;
+  for (size_t q = (size_t)ch.x * M + threadIdx.x; q < q1; q += blockDim.x) {
+    const Quad<R> x = s[q];
+    const double m = (double)x.w;
+    a[0] += (double)x.x * m;
+    a[1] += (double)x.y * m;
+    a[2] += (double)x.z * m;
+    if (NV > 3) a[3] += en[q];
+  }
+  __shared__ double red[256][NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) red[threadIdx.x][v] = a[v];
+  __syncthreads();
+  if ((int)threadIdx.x < M * NV) {
+    const int ens = threadIdx.x / NV, v = threadIdx.x - ens * NV;
+    double x = 0.0;
+    for (int j = ens; j < (int)blockDim.x; j += M) x += red[j][v];
+    partial[((size_t)blockIdx.x * M + ens) * NV + v] = x;
+  }
+}
+
+// General form: grid (chunks, M), one ensemble per workgroup.
+template <typename R, int NV>
+__global__ void __launch_bounds__(256)
+    k_segsum_strided(int M, const int2 *__restrict__ chunks, const Quad<R> *__restrict__ s,
+                     const double *__restrict__ en, double *__restrict__ partial) {
+  const int2 ch = chunks[blockIdx.x];
+  const int ens = blockIdx.y;
+  double a[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) a[v] = 0.0;
+  for (int i = ch.x + (int)threadIdx.x; i < ch.y; i += blockDim.x) {
+    const size_t q = (size_t)i * M + ens;
+    const Quad<R> x = s[q];
+    const double m = (double)x.w;
+    a[0] += (double)x.x * m;
+    a[1] += (double)x.y * m;
+    a[2] += (double)x.z * m;
+    if (NV > 3) a[3] += en[q];
+  }
+  __shared__ double red[256][NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) red[threadIdx.x][v] = a[v];
+  __syncthreads();
+  if ((int)threadIdx.x < NV) {
+    double x = 0.0;
+    for (int j = 0; j < (int)blockDim.x; ++j) x += red[j][threadIdx.x];
+    partial[((size_t)blockIdx.x * M + ens) * NV + threadIdx.x] = x;
+  }
+}
+
+// One thread per (segment, ensemble, value): the segment's chunk partials
+// summed in chunk order into out[(seg*M + ens)*NV + v].
+template <int NV>
+__global__ void k_segsum_final(int nseg, int M, const int *__restrict__ seg_chunk,
+                               const double *__restrict__ partial, double *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nseg * M * NV) return;
+  const int v = t % NV, se = t / NV, ens = se % M, seg = se / M;
+  double x = 0.0;
+  for (int c = seg_chunk[seg]; c < seg_chunk[seg + 1]; ++c)
+    x += partial[((size_t)c * M + ens) * NV + v];
+  out[t] = x;
+}
+
+// Cumulant accumulators per (segment, ensemble), the running weighted means
+// of calc_and_print_cumulant (prn_averages.f90:956-1000): sample n has weight
+// n, mean_n = (mean_{n-1} * W_{n-1} + x_n * n) / (W_{n-1} + n).
+// acc[8*t + ...] = {W, n, <m>, <m^2>, <m^4>, <e>, <e^2>, 0} with
+// m = |sum emomM| / nsites and e the energy per site in mRy.
+constexpr int kAccWidth = 8;
+template <int W>
+__global__ void k_accumulate(int nseg, int M, const int *__restrict__ seg_n,
+                             const double *__restrict__ sums, int nv, double *__restrict__ acc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nseg * M) return;
+  const int seg = t / M;
+  const double n = (double)seg_n[seg];
+  const double *x = sums + (size_t)t * nv;
+  double *a = acc + (size_t)t * W;
+  const double me = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]) / n;
+  const double m2 = me * me, m4 = m2 * m2;
+  const double tot = a[0], w = a[1] + 1.0;
+  a[2] = (a[2] * tot + me * w) / (tot + w);
+  a[3] = (a[3] * tot + m2 * w) / (tot + w);
+  a[4] = (a[4] * tot + m4 * w) / (tot + w);
+  if (nv > 3) {
+    const double e = x[3] * (kMub / kMry) / n;
+    a[5] = (a[5] * tot + e * w) / (tot + w);
+    a[6] = (a[6] * tot + e * e * w) / (tot + w);
+  }
+  a[0] = tot + w;
+  a[1] = w;
+}
+
+// Graph support: the step / sweep counters live in device memory while a
+// captured graph runs; one thread adds or sets them (a vector global store).
+template <int I>
+__global__ void k_counter(uint64_t *ctr, uint64_t value, int add) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctr[I] = add ? ctr[I] + value : value;
 }
 
 }  // namespace uasd

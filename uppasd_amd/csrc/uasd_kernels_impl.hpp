@@ -2,44 +2,71 @@
 // uasd_launch.hpp, instantiated for one arithmetic type by the including
 // translation unit.
 #pragma once
+#include <type_traits>
+
 #include "uasd_launch.hpp"
 
 namespace uasd {
 
 // Grid for n work items, padded to a multiple of the XCD count (the kernels
 // remap blockIdx so each XCD takes a contiguous range; padding blocks exit).
-static inline unsigned grid_for(int64_t n) {
-  const int64_t b = (n + kBlock - 1) / kBlock;
+static inline unsigned grid_for(int64_t n, int block) {
+  const int64_t b = (n + block - 1) / block;
   return (unsigned)((b + kXCD - 1) / kXCD * kXCD);
 }
 
+// Calls f(std::integral_constant<int, G>) for the gather width of t.
+template <typename F>
+static hipError_t with_gather(const Tune &t, F &&f) {
+  switch (t.gather) {
+    case 2: return f(std::integral_constant<int, 2>{});
+    case 4: return f(std::integral_constant<int, 4>{});
+    case 8: return f(std::integral_constant<int, 8>{});
+    case 16: return f(std::integral_constant<int, 16>{});
+    default: return hipErrorInvalidValue;
+  }
+}
+
 template <typename R>
-hipError_t launch_field(hipStream_t s, const HamDev<R> &h, const Quad<R> *src, Quad<R> *beff,
-                        double *en) {
+hipError_t launch_field(hipStream_t s, const Tune &t, const HamDev<R> &h, const Quad<R> *src,
+                        Quad<R> *beff, double *en, Quad<R> *parts) {
   const int64_t n = (int64_t)h.N * h.M;
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_field<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, h, src, beff, en);
-  return hipGetLastError();
+  if (!tune_valid(t)) return hipErrorInvalidValue;
+  return with_gather(t, [&](auto g) {
+    hipLaunchKernelGGL((k_field<R, decltype(g)::value>), dim3(grid_for(n, t.block)),
+                       dim3(t.block), 0, s, h, src, beff, en, parts);
+    return hipGetLastError();
+  });
 }
 
 template <typename R, int ALG, int HALF, bool FUSED>
-static hipError_t llg_one(hipStream_t s, const HamDev<R> &h, const LLGDev<R> &prm,
-                          const Quad<R> *e, Quad<R> *p, Quad<R> *b2, const Quad<R> *beff,
-                          Quad<R> *out) {
+static hipError_t llg_one(hipStream_t s, const Tune &t, const HamDev<R> &h,
+                          const LLGDev<R> &prm, const Quad<R> *e, Quad<R> *p, Quad<R> *b2,
+                          const Quad<R> *beff, Quad<R> *out) {
   const int64_t n = (int64_t)h.N * h.M;
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_llg<R, ALG, HALF, FUSED>), dim3(grid_for(n)), dim3(kBlock), 0, s, h, prm,
-                     e, p, b2, beff, out);
-  return hipGetLastError();
+  const dim3 grid(grid_for(n, t.block)), block(t.block);
+  if (!FUSED) {  // no gather: one instance
+    hipLaunchKernelGGL((k_llg<R, ALG, HALF, false, 4>), grid, block, 0, s, h, prm, e, p, b2,
+                       beff, out);
+    return hipGetLastError();
+  }
+  return with_gather(t, [&](auto g) {
+    hipLaunchKernelGGL((k_llg<R, ALG, HALF, true, decltype(g)::value>), grid, block, 0, s, h,
+                       prm, e, p, b2, beff, out);
+    return hipGetLastError();
+  });
 }
 
 template <typename R>
-hipError_t launch_llg(hipStream_t s, int alg, int half, bool fused, const HamDev<R> &h,
-                      const LLGDev<R> &prm, const Quad<R> *e, Quad<R> *p, Quad<R> *b2,
-                      const Quad<R> *beff, Quad<R> *out) {
-#define UASD_LLG_CASE(A, H, F)                              \
-  if (alg == A && half == H && fused == F)                  \
-    return llg_one<R, A, H, F>(s, h, prm, e, p, b2, beff, out);
+hipError_t launch_llg(hipStream_t s, const Tune &t, int alg, int half, bool fused,
+                      const HamDev<R> &h, const LLGDev<R> &prm, const Quad<R> *e, Quad<R> *p,
+                      Quad<R> *b2, const Quad<R> *beff, Quad<R> *out) {
+  if (!tune_valid(t)) return hipErrorInvalidValue;
+#define UASD_LLG_CASE(A, H, F)             \
+  if (alg == A && half == H && fused == F) \
+    return llg_one<R, A, H, F>(s, t, h, prm, e, p, b2, beff, out);
   UASD_LLG_CASE(1, 0, true)
   UASD_LLG_CASE(1, 1, true)
   UASD_LLG_CASE(1, 0, false)
@@ -53,11 +80,16 @@ hipError_t launch_llg(hipStream_t s, int alg, int half, bool fused, const HamDev
 }
 
 template <typename R>
-hipError_t launch_mc(hipStream_t s, const HamDev<R> &h, const MCParams &mp, Quad<R> *spins) {
+hipError_t launch_mc(hipStream_t s, const Tune &t, const HamDev<R> &h, const MCParams &mp,
+                     Quad<R> *spins) {
   const int64_t n = (int64_t)mp.nsites * h.M;
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_mc_colour<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, h, mp, spins);
-  return hipGetLastError();
+  if (!tune_valid(t)) return hipErrorInvalidValue;
+  return with_gather(t, [&](auto g) {
+    hipLaunchKernelGGL((k_mc_colour<R, decltype(g)::value>), dim3(grid_for(n, t.block)),
+                       dim3(t.block), 0, s, h, mp, spins);
+    return hipGetLastError();
+  });
 }
 
 template <typename R>
@@ -80,15 +112,47 @@ hipError_t launch_reduce(hipStream_t s, int N, int M, const Quad<R> *spins, cons
   return hipGetLastError();
 }
 
-#define UASD_INSTANTIATE(R)                                                                    \
-  template hipError_t launch_field<R>(hipStream_t, const HamDev<R> &, const Quad<R> *,         \
-                                      Quad<R> *, double *);                                    \
-  template hipError_t launch_llg<R>(hipStream_t, int, int, bool, const HamDev<R> &,            \
-                                    const LLGDev<R> &, const Quad<R> *, Quad<R> *, Quad<R> *,  \
-                                    const Quad<R> *, Quad<R> *);                               \
-  template hipError_t launch_mc<R>(hipStream_t, const HamDev<R> &, const MCParams &,           \
-                                   Quad<R> *);                                                 \
+template <typename R>
+hipError_t launch_segsum(hipStream_t s, int M, int nseg, int nchunks, const int2 *chunks,
+                         const int *seg_chunk, const Quad<R> *spins, const double *en, int nv,
+                         double *partial, double *out) {
+  if (nseg == 0 || M == 0 || nchunks == 0) return hipSuccess;
+  const bool wide = kBlock % M == 0;
+  const dim3 g = wide ? dim3(nchunks) : dim3(nchunks, M);
+  if (nv == 4) {
+    if (wide)
+      hipLaunchKernelGGL((k_segsum_wide<R, 4>), g, dim3(kBlock), 0, s, M, chunks, spins, en,
+                         partial);
+    else
+      hipLaunchKernelGGL((k_segsum_strided<R, 4>), g, dim3(kBlock), 0, s, M, chunks, spins, en,
+                         partial);
+    hipLaunchKernelGGL(k_segsum_final<4>, dim3((nseg * M * 4 + 255) / 256), dim3(256), 0, s, nseg,
+                       M, seg_chunk, partial, out);
+  } else {
+    if (wide)
+      hipLaunchKernelGGL((k_segsum_wide<R, 3>), g, dim3(kBlock), 0, s, M, chunks, spins, en,
+                         partial);
+    else
+      hipLaunchKernelGGL((k_segsum_strided<R, 3>), g, dim3(kBlock), 0, s, M, chunks, spins, en,
+                         partial);
+    hipLaunchKernelGGL(k_segsum_final<3>, dim3((nseg * M * 3 + 255) / 256), dim3(256), 0, s, nseg,
+                       M, seg_chunk, partial, out);
+  }
+  return hipGetLastError();
+}
+
+#define UASD_INSTANTIATE(R)                                                                   \
+  template hipError_t launch_field<R>(hipStream_t, const Tune &, const HamDev<R> &,           \
+                                      const Quad<R> *, Quad<R> *, double *, Quad<R> *);       \
+  template hipError_t launch_llg<R>(hipStream_t, const Tune &, int, int, bool,                \
+                                    const HamDev<R> &, const LLGDev<R> &, const Quad<R> *,    \
+                                    Quad<R> *, Quad<R> *, const Quad<R> *, Quad<R> *);        \
+  template hipError_t launch_mc<R>(hipStream_t, const Tune &, const HamDev<R> &,              \
+                                   const MCParams &, Quad<R> *);                              \
   template hipError_t launch_reduce<R>(hipStream_t, int, int, const Quad<R> *, const double *, \
-                                       int, double *, int, double *);
+                                       int, double *, int, double *);                         \
+  template hipError_t launch_segsum<R>(hipStream_t, int, int, int, const int2 *, const int *,  \
+                                       const Quad<R> *, const double *, int, double *,        \
+                                       double *);
 
 }  // namespace uasd

@@ -7,25 +7,55 @@
 
 namespace uasd {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;  // largest workgroup the kernels are built for
 
+// Launch shape of the gathering kernels (chosen by the autotuner):
+//   gather  neighbours whose loads are issued together: 2, 4, 8 or 16
+//   block   workgroup size: 64, 128 or 256
+struct Tune {
+  int gather = 4;
+  int block = 256;
+};
+
+inline bool tune_valid(const Tune &t) {
+  return (t.gather == 2 || t.gather == 4 || t.gather == 8 || t.gather == 16) &&
+         (t.block == 64 || t.block == 128 || t.block == 256);
+}
+
+// parts (nullable) receives beff1 then beff2, N*M quads each.
 template <typename R>
-hipError_t launch_field(hipStream_t s, const HamDev<R> &h, const Quad<R> *src, Quad<R> *beff,
-                        double *en);
+hipError_t launch_field(hipStream_t s, const Tune &t, const HamDev<R> &h, const Quad<R> *src,
+                        Quad<R> *beff, double *en, Quad<R> *parts);
 
 // alg: 1 midpoint / 5 Depondt; half: 0 predictor, 1 corrector
 template <typename R>
-hipError_t launch_llg(hipStream_t s, int alg, int half, bool fused, const HamDev<R> &h,
-                      const LLGDev<R> &prm, const Quad<R> *e, Quad<R> *p, Quad<R> *b2,
-                      const Quad<R> *beff, Quad<R> *out);
+hipError_t launch_llg(hipStream_t s, const Tune &t, int alg, int half, bool fused,
+                      const HamDev<R> &h, const LLGDev<R> &prm, const Quad<R> *e, Quad<R> *p,
+                      Quad<R> *b2, const Quad<R> *beff, Quad<R> *out);
 
 template <typename R>
-hipError_t launch_mc(hipStream_t s, const HamDev<R> &h, const MCParams &mp, Quad<R> *spins);
+hipError_t launch_mc(hipStream_t s, const Tune &t, const HamDev<R> &h, const MCParams &mp,
+                     Quad<R> *spins);
 
 // FP64 sums per ensemble of emomM (nv = 3) or of emomM and en (nv = 4);
 // partial must hold M*nblocks*nv doubles, out M*nv doubles.
 template <typename R>
 hipError_t launch_reduce(hipStream_t s, int N, int M, const Quad<R> *spins, const double *en,
                          int nv, double *partial, int nblocks, double *out);
+
+// Per-segment FP64 sums (see k_segsum_*): chunks (nchunks), seg_chunk
+// (nseg+1, chunk ranges of each segment), partial (nchunks*M*nv doubles),
+// out (nseg*M*nv doubles).
+template <typename R>
+hipError_t launch_segsum(hipStream_t s, int M, int nseg, int nchunks, const int2 *chunks,
+                         const int *seg_chunk, const Quad<R> *spins, const double *en, int nv,
+                         double *partial, double *out);
+
+// ctr[idx] = value (add = 0) or ctr[idx] += value (add = 1), on the stream.
+hipError_t launch_counter(hipStream_t s, int idx, uint64_t *ctr, uint64_t value, int add);
+
+// Appends one cumulant sample per (segment, ensemble) from out of launch_segsum.
+hipError_t launch_accumulate(hipStream_t s, int nseg, int M, const int *seg_n,
+                             const double *sums, int nv, double *acc);
 
 }  // namespace uasd

@@ -127,13 +127,19 @@ class Engine:
         return e, eM
 
     # -------------------------------------------------- drop-in (host arrays)
-    def effective_field(self, emomM):
-        """beff and the field-routine energy (mRy) of emomM, as effective_field_full."""
+    def effective_field(self, emomM, parts: bool = False):
+        """beff and the field-routine energy (mRy) of emomM, as effective_field_full;
+        with parts=True also (beff1, beff2) as that routine splits them."""
         eM = B.f64(emomM)
         beff = np.zeros_like(eM)
+        b1 = np.zeros_like(eM) if parts else None
+        b2 = np.zeros_like(eM) if parts else None
         en = C.c_double(0.0)
-        B.check(B.lib().uasd_effective_field_host(self._h, B.ptr(eM), B.ptr(beff), C.byref(en)),
+        B.check(B.lib().uasd_effective_field_host(self._h, B.ptr(eM), B.ptr(beff), B.ptr(b1),
+                                                  B.ptr(b2), C.byref(en)),
                 "uasd_effective_field_host")
+        if parts:
+            return beff, en.value, b1, b2
         return beff, en.value
 
     def evolve_first(self, sdealgh, beff, emom, emom2, emomM, b2eff=None):
@@ -191,6 +197,74 @@ class Engine:
         B.check(B.lib().uasd_get_kernel_timing(self._h, kind, C.byref(n), C.byref(ms)),
                 "uasd_get_kernel_timing")
         return n.value, ms.value
+
+    def set_graph_steps(self, k: int):
+        """Replay hipGraphs of k LLG steps / MC sweeps (0 = launch one by one)."""
+        B.check(B.lib().uasd_set_graph_steps(self._h, int(k)), "uasd_set_graph_steps")
+
+    # --------------------------------------------------------------- batches
+    def set_segments(self, offsets):
+        """Replica site ranges: segment s = sites [offsets[s], offsets[s+1])."""
+        off = B.i32(offsets)
+        B.check(B.lib().uasd_set_segments(self._h, off.shape[0] - 1, B.ptr(off)),
+                "uasd_set_segments")
+        self.nseg = off.shape[0] - 1
+
+    def measure_segments(self, with_energy: bool = False):
+        """(msum (nseg, M, 3), energy (nseg, M) in mRy or None), FP64."""
+        nseg = getattr(self, "nseg", 1)
+        ms = np.zeros((nseg, self.M, 3))
+        en = np.zeros((nseg, self.M)) if with_energy else None
+        B.check(B.lib().uasd_measure_segments(self._h, B.ptr(ms), B.ptr(en)),
+                "uasd_measure_segments")
+        return ms, en
+
+    def accumulate(self, with_energy: bool = True):
+        """One device-side cumulant sample per (segment, ensemble); no host sync."""
+        B.check(B.lib().uasd_accumulate(self._h, 1 if with_energy else 0), "uasd_accumulate")
+
+    def accumulators(self):
+        """dict of (nseg, M) arrays: W, n, m, m2, m4, e, e2 and U4; plus samples."""
+        nseg = getattr(self, "nseg", 1)
+        a = np.zeros((nseg, self.M, 8))
+        n = C.c_int64(0)
+        B.check(B.lib().uasd_get_accumulators(self._h, B.ptr(a), C.byref(n)),
+                "uasd_get_accumulators")
+        out = {k: a[..., i] for i, k in enumerate(("W", "n", "m", "m2", "m4", "e", "e2"))}
+        with np.errstate(divide="ignore", invalid="ignore"):
+            out["U4"] = 1.0 - out["m4"] / (3.0 * out["m2"] ** 2)
+        out["samples"] = n.value
+        return out
+
+    def reset_accumulators(self):
+        B.check(B.lib().uasd_reset_accumulators(self._h), "uasd_reset_accumulators")
+
+    # kinds of launch shape
+    TUNE_LLG, TUNE_FIELD, TUNE_MC = 0, 1, 2
+
+    def set_tuning(self, kind: int, gather: int, block: int):
+        """Launch shape of a gathering kernel (results do not depend on it)."""
+        B.check(B.lib().uasd_set_tuning(self._h, kind, gather, block), "uasd_set_tuning")
+
+    def tuning(self, kind: int):
+        g, b = C.c_int(0), C.c_int(0)
+        B.check(B.lib().uasd_get_tuning(self._h, kind, C.byref(g), C.byref(b)), "uasd_get_tuning")
+        return g.value, b.value
+
+    def autotune_llg(self, sdealgh: int, steps: int = 3) -> float:
+        """Times every launch shape on this lattice, keeps the fastest; the
+        state and the step counter are restored.  Returns the best time (ms)."""
+        ms = C.c_double(0.0)
+        B.check(B.lib().uasd_autotune_llg(self._h, sdealgh, steps, C.byref(ms)),
+                "uasd_autotune_llg")
+        return ms.value
+
+    def autotune_mc(self, mode: str, temps, sweeps: int = 2) -> float:
+        t = B.f64(np.broadcast_to(np.asarray(temps, dtype=np.float64), (self.M,)))
+        ms = C.c_double(0.0)
+        B.check(B.lib().uasd_autotune_mc(self._h, mode.encode()[:1], B.ptr(t), sweeps, C.byref(ms)),
+                "uasd_autotune_mc")
+        return ms.value
 
 
 def moment_update(mmom, emom, emom2, emomM):

@@ -92,6 +92,208 @@ def copy_calibration(mib: int = 512, reps: int = 5):
                                            "dispatches": reps}}), flush=True)
 
 
+# ---------------------------------------------------------------- forms
+# Each form runs a bounded sample of one configuration and reports the
+# throughput of the accelerated form beside the plain FP64 path on the same
+# workload, with the error of the reduced-precision arithmetic.
+
+def _sync_time(fn):
+    t0 = time.perf_counter()
+    fn()
+    return time.perf_counter() - t0
+
+
+def form_c3(U, CF, steps=5000):
+    """Precision: triangular 512x512 monolayer, J 1 mRy (6 NN), interfacial
+    DMI 0.2 mRy, Kz 0.05 mRy, Bz 0.3 mRy/muB, Depondt, alpha 0.05, T = 0 from
+    a random start.  FP32 field gather + rotation with FP64 reductions against
+    the FP64 path (itself within 1e-12 of the FP64 oracle); errors stated per
+    run: energy per spin at checkpoints, max field error on one state, max
+    |m_i| norm drift."""
+    sysd = CF.triangular_dmi(512)
+    N = sysd["lattice"].natom
+    rng = np.random.default_rng(3)
+    e0 = np.ascontiguousarray(CF.random_unit(N, rng, 1))
+    engs = {}
+    for prec in ("fp64", "fp32"):
+        eng = U.Engine(N, 1, prec)
+        eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
+        eng.set_dm(sysd["dmlist"], sysd["dmlistsize"], sysd["dm_vect"])
+        eng.set_anisotropy(sysd["taniso"], sysd["eaniso"], sysd["kaniso"], sysd["sb"])
+        eng.set_external_field(sysd["ext"])
+        eng.set_moments(e0, sysd["mmom"][None])
+        eng.set_llg(0.05, 0.0, 1e-16)
+        eng.set_noise(U.UASD_NOISE_NONE)
+        eng.autotune_llg(5, steps=2)
+        engs[prec] = eng
+    checkpoints = [s for s in (10, 100, 1000, steps) if s <= steps]
+    rec = {p: {} for p in engs}
+    times = {p: 0.0 for p in engs}
+    done = 0
+    for c in checkpoints:
+        for p, eng in engs.items():
+            def run(eng=eng, n=c - done):
+                eng.sd_steps(5, n)
+                eng.sync()
+            times[p] += _sync_time(run)
+            ms, en = eng.measure(with_energy=True)
+            rec[p][c] = (float(en[0]) / N, float(np.linalg.norm(ms[0])) / N)
+        done = c
+    e64, _ = engs["fp64"].get_moments()
+    e32, eM32 = engs["fp32"].get_moments()
+    drift = float(np.max(np.abs(np.linalg.norm(eM32[0], axis=1) / sysd["mmom"] - 1.0)))
+    # field error of the FP32 gather on the FP64 final state
+    emomM = np.ascontiguousarray(e64 * sysd["mmom"][None, :, None])
+    b64, _ = engs["fp64"].effective_field(emomM)
+    b32, _ = engs["fp32"].effective_field(emomM)
+    ferr = float(np.max(np.abs(b32 - b64)) / np.max(np.abs(b64)))
+    rel = {c: abs(rec["fp32"][c][0] - rec["fp64"][c][0]) / abs(rec["fp64"][c][0]) for c in checkpoints}
+    return {
+        "workload": f"triangular 512^2 ({N} spins), J+DMI+Kz+Bz, Depondt alpha 0.05, T 0, {steps} steps",
+        "fp64_value": N * steps / times["fp64"],
+        "fp32_value": N * steps / times["fp32"],
+        "fp32_over_fp64": times["fp64"] / times["fp32"],
+        "energy_per_spin_mRy": {str(c): rec["fp64"][c][0] for c in checkpoints},
+        "energy_rel_err": {str(c): rel[c] for c in checkpoints},
+        "field_max_rel_err": ferr,
+        "norm_drift_fp32": drift,
+        "unit": "spin-updates/s",
+    }
+
+
+def _c4_replicas(CF, n_rep=64):
+    Ls = np.round(np.linspace(8, 48, n_rep)).astype(int)
+    reps = []
+    for r, L in enumerate(Ls):
+        rng = np.random.default_rng(4000 + r)
+        occ = np.where(rng.random(2 * L ** 3) < 0.5, 1, 2)
+        reps.append(CF.bcc_alloy(int(L), occ))
+    return reps
+
+
+def form_c4(U, CF, sweeps=40):
+    """Batched MC: random binary bcc alloy, 64 replicas of 8^3..48^3 cells
+    joined into one lattice, 32 temperatures 200..1600 K as ensembles,
+    checkerboard Metropolis; FP32 spins with FP64 observables (one cumulant
+    sample per sweep).  The plain path is the FP64 engine run one replica at
+    a time (all 32 temperatures per launch), on a bounded sample."""
+    from uppasd_amd import batch as Bt
+    reps = _c4_replicas(CF)
+    j = Bt.join(reps)
+    M = 32
+    temps = np.linspace(200.0, 1600.0, M)
+    Ntot = j.natom
+    rng = np.random.default_rng(44)
+    out = {"workload": f"bcc alloy 64 replicas (8^3..48^3 cells, {Ntot} spins) x 32 T, Metropolis",
+           "unit": "spin-updates/s"}
+    for prec in ("fp32", "fp64"):
+        eng = Bt.engine_for(j, M, prec)
+        e0 = np.ascontiguousarray(CF.random_unit(Ntot, rng, M))
+        eng.set_moments(e0, np.ascontiguousarray(np.broadcast_to(j.mmom, (M, Ntot))))
+        eng.autotune_mc("M", temps, sweeps=1)
+        eng.set_graph_steps(10)
+        eng.mc_sweeps("M", temps, 10, 7)
+        eng.sync()
+
+        def run(eng=eng):
+            for _ in range(sweeps // 10):
+                eng.mc_sweeps("M", temps, 10, 7)
+                eng.accumulate(with_energy=True)
+            eng.sync()
+        t = _sync_time(run)
+        out[f"batched_{prec}_value"] = Ntot * M * sweeps / t
+        if prec == "fp32":
+            acc = eng.accumulators()
+            out["U4_first_replica_lowT_highT"] = [float(acc["U4"][0, 0]), float(acc["U4"][0, -1])]
+        del eng
+    # plain FP64: replica by replica, eager launches, bounded to 8 replicas
+    t_plain, n_plain = 0.0, 0
+    for r in range(0, len(reps), 8):
+        rep = reps[r]
+        N = rep["lattice"].natom
+        eng = U.Engine(N, M, "fp64")
+        eng.set_exchange(rep["nlist"], rep["nlistsize"], rep["ncoup"])
+        eng.set_moments(np.ascontiguousarray(CF.random_unit(N, rng, M)),
+                        np.ascontiguousarray(np.broadcast_to(rep["mmom"], (M, N))))
+        eng.mc_sweeps("M", temps, 2, 7)
+        eng.sync()
+
+        def run(eng=eng):
+            eng.mc_sweeps("M", temps, 10, 7)
+            eng.sync()
+        t_plain += _sync_time(run)
+        n_plain += N * M * 10
+    out["plain_fp64_value"] = n_plain / t_plain
+    out["batched_fp32_over_plain_fp64"] = out["batched_fp32_value"] / out["plain_fp64_value"]
+    return out
+
+
+def form_c5(U, CF, steps=2000):
+    """Batched LLG: kagome antiferromagnet (J1 -1 mRy, Dz 0.08 mRy), 256
+    ensembles of 24^2..96^2 cells at log-spaced 1..300 K joined into one
+    lattice, FP32 Depondt with hipGraph-captured steps, alpha 0.2, dt 5e-17.
+    Plain path: the FP64 engine one ensemble at a time with eager launches
+    (bounded sample)."""
+    from uppasd_amd import batch as Bt
+    Ls = np.round(np.linspace(24, 96, 256)).astype(int)
+    Ts = np.logspace(0.0, np.log10(300.0), 256)
+    cache = {}
+    reps = []
+    for L, T in zip(Ls, Ts):
+        if L not in cache:
+            cache[L] = CF.kagome_afm(int(L), J=-1.0, Dz=0.08)
+        d = dict(cache[L])
+        d["temp"] = T
+        reps.append(d)
+    j = Bt.join(reps, per_site=("temp",))
+    Ntot = j.natom
+    rng = np.random.default_rng(55)
+    e0 = np.ascontiguousarray(CF.random_unit(Ntot, rng, 1))
+    out = {"workload": f"kagome AFM+DMI, 256 ensembles (24^2..96^2 cells, {Ntot} spins), "
+                       f"T 1..300 K, Depondt alpha 0.2, dt 5e-17, {steps} steps",
+           "unit": "spin-updates/s"}
+    lam = np.full(Ntot, 0.2)
+    for prec, graph in (("fp32", 50), ("fp64", 50)):
+        eng = Bt.engine_for(j, 1, prec)
+        eng.set_moments(e0, j.mmom[None])
+        eng.set_llg(lam, j.extra["temp"], 5e-17)
+        eng.set_noise(U.UASD_NOISE_PHILOX, 5)
+        eng.autotune_llg(5, steps=2)
+        eng.set_graph_steps(graph)
+        eng.sd_steps(5, graph)
+        eng.sync()
+
+        def run(eng=eng):
+            for _ in range(steps // graph):
+                eng.sd_steps(5, graph)
+                eng.accumulate(with_energy=False)
+            eng.sync()
+        t = _sync_time(run)
+        out[f"batched_{prec}_value"] = Ntot * steps / t
+        del eng
+    t_plain, n_plain = 0.0, 0
+    for r in range(0, 256, 32):
+        rep = reps[r]
+        N = rep["lattice"].natom
+        eng = U.Engine(N, 1, "fp64")
+        eng.set_exchange(rep["nlist"], rep["nlistsize"], rep["ncoup"])
+        eng.set_dm(rep["dmlist"], rep["dmlistsize"], rep["dm_vect"])
+        eng.set_moments(np.ascontiguousarray(CF.random_unit(N, rng, 1)), rep["mmom"][None])
+        eng.set_llg(0.2, rep["temp"], 5e-17)
+        eng.set_noise(U.UASD_NOISE_PHILOX, 5)
+        eng.sd_steps(5, 5)
+        eng.sync()
+
+        def run(eng=eng):
+            eng.sd_steps(5, 100)
+            eng.sync()
+        t_plain += _sync_time(run)
+        n_plain += N * 100
+    out["plain_fp64_value"] = n_plain / t_plain
+    out["batched_fp32_over_plain_fp64"] = out["batched_fp32_value"] / out["plain_fp64_value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-calibration", action="store_true",
@@ -106,6 +308,8 @@ def main():
     ap.add_argument("--gather", type=int, default=0,
                     help="fixed gather batch (2/4/8/16) instead of autotuning")
     ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--forms", default="c3,c4,c5",
+                    help="comma list of extra forms to run (c3,c4,c5) or 'none'")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -196,6 +400,17 @@ def main():
             "traffic": None,
         },
     }
+    del eng
+    forms = [f for f in args.forms.split(",") if f and f != "none"]
+    if forms:
+        from uppasd_amd import configs as CF
+        out["forms"] = {}
+        for f in forms:
+            t0 = time.perf_counter()
+            res = {"c3": form_c3, "c4": form_c4, "c5": form_c5}[f](U, CF)
+            res["wall_s"] = round(time.perf_counter() - t0, 1)
+            out["forms"][f] = res
+            print(f"form {f} done in {res['wall_s']} s", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sysd)
     print(json.dumps(out), flush=True)

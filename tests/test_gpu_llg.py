@@ -273,3 +273,31 @@ def test_measure_reductions(O):
     for k in range(M):
         _, ek = ham.field(np.ascontiguousarray(eM[k:k + 1]), None if ext is None else ext[k:k + 1])
         assert abs(en[k] - ek) <= 1e-12 * abs(ek)
+
+
+def test_kagome_dmi_golden_on_gpu(O):
+    """The reference's kagome case (DMI on every bond, T = 0 midpoint) on the
+    device: <m> at iteration 13000 and atom 2 at iteration 2400."""
+    c = RC.read_case("kagome")
+    lat = U.Lattice(c["cell"], c["bas"], c["atype"], c["n"], c["bc"])
+    latO = O.LatticeSpec(c["cell"], c["bas"], c["atype"], c["n"], c["bc"])
+    pairs = O.build_pairs(latO, c["jlines"], c["sym"], ammom=c["ammom"].reshape(-1, 1))
+    dm = O.build_pairs(latO, c["dmlines"], 0, hdim=3, use_nntype=False,
+                       ammom=c["ammom"].reshape(-1, 1))
+    N = lat.natom
+    eng = U.Engine(N, 1)
+    eng.set_exchange(pairs["nlist"], pairs["nlistsize"], pairs["ncoup"])
+    eng.set_dm(dm["nlist"], dm["nlistsize"], dm["ncoup"])
+    e0 = np.ascontiguousarray(np.tile(c["edir"], (N // lat.NA, 1))[None])
+    eng.set_moments(e0, np.tile(c["ammom"], N // lat.NA)[None])
+    eng.set_llg(0.5, 0.0, 1.0e-16)
+    eng.set_noise(U.UASD_NOISE_NONE)
+    gm = [g for g in RC.golden("kagome") if g["type"] == "Magnetization"][0]
+    gt = [g for g in RC.golden("kagome") if g["type"] == "Trajectory"][0]
+    eng.sd_steps(1, gt["select"]["#_Iter."])
+    e, _ = eng.get_moments()
+    assert np.allclose([2.0, *e[0, 1], 2.0], gt["expected"], atol=1e-8)
+    eng.sd_steps(1, gm["select"]["#_Iter."] - gt["select"]["#_Iter."])
+    _, eM = eng.get_moments()
+    m = eM[0].sum(axis=0) / N
+    assert np.allclose([*m, np.linalg.norm(m)], gm["expected"], atol=1e-8)

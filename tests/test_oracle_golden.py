@@ -119,3 +119,46 @@ def test_heisstripe_anisotropy_golden(O):
         sim.step(1, 1.0e-16, 0.10, 0.0, normals=noise)
     m = sim.mavg()[0]
     assert np.allclose([*m, np.linalg.norm(m)], g["expected"], atol=1e-8), (m, g["expected"])
+
+
+def test_kagome_dmi_golden(O):
+    """Kagome lattice 12x12 (PBC in-plane), J = 1 mRy and a DM vector on
+    every bond (dmfile, sym 0), moments from the momfile directions, T = 0
+    midpoint with damping 0.5: <m> at iteration 13000 and atom 2's moment at
+    iteration 2400 (trajectory file) pin the DMI field term and the DM
+    neighbour map."""
+    c = RC.read_case("kagome")
+    lat = _lattice(O, c)
+    pairs = _pairs(O, c, lat)
+    dm = O.build_pairs(lat, c["dmlines"], 0, hdim=3, use_nntype=False,
+                       ammom=c["ammom"].reshape(-1, 1))
+    sim = RC.OracleSD(O, lat, pairs, _sites(lat, c["ammom"]), dm=dm)
+    sim.set_state(np.tile(c["edir"], (lat.natom // lat.NA, 1))[None])
+    gm = _expected("kagome", "Magnetization")[0]
+    gt = _expected("kagome", "Trajectory")[0]
+    traj = None
+    for it in range(gm["select"]["#_Iter."]):
+        if it == gt["select"]["#_Iter."]:
+            traj = sim.e[0, 1].copy()
+        sim.step(1, 1.0e-16, 0.5, 0.0)
+    m = sim.mavg()[0]
+    assert np.allclose([*m, np.linalg.norm(m)], gm["expected"], atol=1e-8), (m, gm["expected"])
+    assert np.allclose([2.0, *traj, 2.0], gt["expected"], atol=1e-8), (traj, gt["expected"])
+
+
+def test_kagome_golden_rejects_flipped_dm(O):
+    """The kagome golden values are sensitive to the DM term: the same run
+    with every DM vector negated (a plausible sign mistake) misses them."""
+    c = RC.read_case("kagome")
+    lat = _lattice(O, c)
+    pairs = _pairs(O, c, lat)
+    dm = O.build_pairs(lat, c["dmlines"], 0, hdim=3, use_nntype=False,
+                       ammom=c["ammom"].reshape(-1, 1))
+    dm["ncoup"] = np.ascontiguousarray(-dm["ncoup"])
+    sim = RC.OracleSD(O, lat, pairs, _sites(lat, c["ammom"]), dm=dm)
+    sim.set_state(np.tile(c["edir"], (lat.natom // lat.NA, 1))[None])
+    gm = _expected("kagome", "Magnetization")[0]
+    for _ in range(gm["select"]["#_Iter."]):
+        sim.step(1, 1.0e-16, 0.5, 0.0)
+    m = sim.mavg()[0]
+    assert not np.allclose([*m, np.linalg.norm(m)], gm["expected"], atol=1e-4)

@@ -32,17 +32,18 @@ METRIC = "spin-updates/s (LLG step, MC sweep); FP32+batched vs FP64 at stated er
 HBM_PEAK_GBS = 8000.0
 
 
-def kernel_bytes(W: int, s: int) -> dict:
+def kernel_bytes(W: int, s: int, ib: int = 4) -> dict:
     """Algorithmic HBM bytes per site of the fused midpoint kernels.
 
-    Spins are stored as quads {x, y, z, mmom} of s-byte reals.
-    predictor: gather e_j quads (4s), index stream 4W, class id 4, write p (4s)
-    corrector: gather p_j quads (4s), own e (4s), index 4W, class id 4, write e (4s)
+    Spins are stored as quads {x, y, z, mmom} of s-byte reals; neighbour
+    entries take ib bytes (4 for int32 indices, 2 for int16 differences).
+    predictor: gather e_j quads (4s), index stream ib*W, class id 4, write p (4s)
+    corrector: gather p_j quads (4s), own e (4s), index ib*W, class id 4, write e (4s)
     Each site's own spin is one of its neighbours' gathers, so the gathered
     quads count once per site (perfect reuse through L2 / MALL).
     """
-    pred = 4 * s + 4 * W + 4 + 4 * s
-    corr = 4 * s + 4 * s + 4 * W + 4 + 4 * s
+    pred = 4 * s + ib * W + 4 + 4 * s
+    corr = 4 * s + 4 * s + ib * W + 4 + 4 * s
     return {"predictor": pred, "corrector": corr, "step": pred + corr}
 
 
@@ -357,7 +358,8 @@ def main():
     eng.set_kernel_timing(False)
 
     s = 8 if args.precision == "fp64" else 4
-    kb = kernel_bytes(W, s)
+    ib = eng.index_bytes()[0]
+    kb = kernel_bytes(W, s, ib)
     t_corr = ms_corr / max(n_corr, 1) * 1e-3
     t_pred = ms_pred / max(n_pred, 1) * 1e-3
     achieved = kb["corrector"] * N / t_corr / 1e9
@@ -389,6 +391,7 @@ def main():
                        "autotuned": not args.gather, "tune_s": round(tune_s, 3)},
             "kernel_ms": {"predictor": round(t_pred * 1e3, 4), "corrector": round(t_corr * 1e3, 4)},
             "bytes_per_site": kb,
+            "index_bytes": ib,
             "max_norm_drift": norm_drift,
         },
         "roofline": {

@@ -158,6 +158,11 @@ int uasd_set_kernel_timing(uasd_ctx *ctx, int enable);
  * replay (K steps), 5 MC graph replay (K sweeps) */
 int uasd_get_kernel_timing(uasd_ctx *ctx, int kind, int64_t *launches, double *total_ms);
 
+/* Bytes per neighbour-list entry the kernels read (no reference
+ * counterpart): 2 when every (neighbour - site) mod Natom difference of the
+ * list fits in int16 (periodic lattices numbered cell by cell), else 4. */
+int uasd_index_bytes(uasd_ctx *ctx, int *exchange, int *dm);
+
 /* ---- launch shapes of the gathering kernels (no reference counterpart:
  * the reference's OpenMP loops have no launch shape) ----
  * kind: 0 fused LLG step, 1 field, 2 MC colour pass.  gather = neighbours

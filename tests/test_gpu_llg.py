@@ -301,3 +301,28 @@ def test_kagome_dmi_golden_on_gpu(O):
     _, eM = eng.get_moments()
     m = eM[0].sum(axis=0) / N
     assert np.allclose([*m, np.linalg.norm(m)], gm["expected"], atol=1e-8)
+
+
+def test_index_width_selection_and_int32_path(O):
+    """Periodic lattices use int16 index differences; a list whose differences
+    do not fit in int16 (random partners over 70,000 sites) keeps int32
+    indices, and both give the oracle's field."""
+    sysd = CF.bcc_shells(6, CF.bcc5_couplings())
+    eng = engine_for(sysd)
+    assert eng.index_bytes() == (2, 4)
+    eng = engine_for(CF.triangular_dmi(16))
+    assert eng.index_bytes() == (2, 2)
+    rng = np.random.default_rng(12)
+    N, W = 70000, 8
+    nlist = rng.integers(1, N + 1, size=(N, W)).astype(np.int32)
+    nsize = rng.integers(3, W + 1, size=N).astype(np.int32)
+    ncoup = rng.normal(size=(N, W)) * 1e-3
+    eng = U.Engine(N, 1)
+    eng.set_exchange(nlist, nsize, ncoup)
+    assert eng.index_bytes()[0] == 4
+    emom = CF.random_unit(N, rng, 1)
+    emomM = np.ascontiguousarray(emom * 1.5)
+    beff, en = eng.effective_field(emomM)
+    bref, eref = O.OracleHam(N, nlist, nsize, ncoup).field(emomM)
+    assert rel(beff, bref) <= 1e-13
+    assert abs(en - eref) <= 1e-12 * abs(eref)

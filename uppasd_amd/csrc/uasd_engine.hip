@@ -105,6 +105,8 @@ struct uasd_ctx {
   // device buffers
   DevBuf nbr, ex_len, cpl;
   DevBuf dnbr, dm_len, dv;
+  DevBuf nbr16, dnbr16;  // int16 index differences when they all fit (else empty)
+  bool use16 = false, use16_dm = false;
   DevBuf an_type, an_vals;
   DevBuf cls, classes;
   DevBuf ext_u, ext_full;
@@ -195,11 +197,13 @@ uasd::HamDev<R> ham_view(const uasd_ctx *c) {
   h.W = c->W;
   h.ex_rows = c->ex_rows;
   h.nbr = c->nbr.as<int>();
+  h.nbr16 = c->use16 ? c->nbr16.as<int16_t>() : nullptr;
   h.ex_len = c->ex_len.as<int>();
   h.cpl = c->cpl.as<R>();
   h.Wd = c->Wd;
   h.dm_rows = c->dm_rows;
   h.dnbr = c->dnbr.as<int>();
+  h.dnbr16 = c->use16_dm ? c->dnbr16.as<int16_t>() : nullptr;
   h.dm_len = c->dm_len.as<int>();
   if (c->Wd > 0) {
     size_t n = (size_t)c->Wd * c->dm_rows;
@@ -562,6 +566,21 @@ int build_coloring(uasd_ctx *c) {
 
 }  // namespace
 
+// Slot-major int16 differences (n - site) mod N in [-N/2, N/2) of a
+// slot-major 0-based list; false when any does not fit.
+bool diff16(const std::vector<int> &nb, int N, int W, std::vector<int16_t> &out) {
+  out.resize((size_t)W * N);
+  for (int j = 0; j < W; ++j)
+    for (int i = 0; i < N; ++i) {
+      long d = (long)nb[(size_t)j * N + i] - i;
+      if (2 * d >= N) d -= N;
+      if (2 * d < -(long)N) d += N;
+      if (d < -32768 || d > 32767) return false;
+      out[(size_t)j * N + i] = (int16_t)d;
+    }
+  return true;
+}
+
 // ------------------------------------------------------------ segments
 constexpr int kSegChunk = 4096;  // sites per reduction chunk
 
@@ -725,6 +744,13 @@ int uasd_set_exchange(uasd_ctx *c, int mnn, int nham, const int *nlist, const in
   c->ex_len_host = rs.len;
   c->nbr_host = nb;
   if (int r = upload_raw(c, c->nbr, nb)) return r;
+  std::vector<int16_t> d16;
+  c->use16 = W > 0 && diff16(nb, N, W, d16);
+  if (c->use16) {
+    if (int r = upload_raw(c, c->nbr16, d16)) return r;
+  } else {
+    c->nbr16.release();
+  }
   if (int r = upload_raw(c, c->ex_len, rs.len)) return r;
   if (int r = upload_real(c, c->cpl, cp)) return r;
   c->have_ex = true;
@@ -780,6 +806,13 @@ int uasd_set_dm(uasd_ctx *c, int mdm, int nham, const int *dmlist, const int *dm
   c->dm_len_host = rs.len;
   c->dnbr_host = nb;
   if (int r = upload_raw(c, c->dnbr, nb)) return r;
+  std::vector<int16_t> d16;
+  c->use16_dm = W > 0 && diff16(nb, N, W, d16);
+  if (c->use16_dm) {
+    if (int r = upload_raw(c, c->dnbr16, d16)) return r;
+  } else {
+    c->dnbr16.release();
+  }
   if (int r = upload_raw(c, c->dm_len, rs.len)) return r;
   if (int r = upload_real(c, c->dv, dv)) return r;
   c->classes_dirty = c->coloring_dirty = true;
@@ -1291,6 +1324,13 @@ int uasd_mc_sweeps(uasd_ctx *c, char mode, const double *temps, int64_t nsweeps,
 }
 
 // ---------------------------------------------------------------- tuning
+int uasd_index_bytes(uasd_ctx *c, int *exchange, int *dm) {
+  if (int r = check_ctx(c)) return r;
+  if (exchange) *exchange = c->use16 ? 2 : 4;
+  if (dm) *dm = c->use16_dm ? 2 : 4;
+  return UASD_OK;
+}
+
 int uasd_set_tuning(uasd_ctx *c, int kind, int gather, int block) {
   if (int r = check_ctx(c)) return r;
   c->gen += 1;

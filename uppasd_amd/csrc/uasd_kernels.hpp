@@ -55,11 +55,13 @@ struct HamDev {
   // exchange
   int W, ex_rows;
   const int *__restrict__ nbr;
+  const int16_t *__restrict__ nbr16;  // (n - site) mod N differences, or nullptr
   const int *__restrict__ ex_len;
   const R *__restrict__ cpl;
   // DM
   int Wd, dm_rows;
   const int *__restrict__ dnbr;
+  const int16_t *__restrict__ dnbr16;
   const int *__restrict__ dm_len;
   const R *__restrict__ dvx, *__restrict__ dvy, *__restrict__ dvz;
   // anisotropy rows
@@ -206,6 +208,105 @@ __device__ __forceinline__ void cubic(const HamDev<R> &h, int row, R m0, R m1, R
 
 // Field parts of one site: s = beff_s (pair terms + uniaxial), qf = beff_q
 // (type-7 cubic part times sb), x = external field.
+// Neighbour index from a slot-major list entry: int32 indices as stored, or
+// int16 differences d = (n - site) mod N in [-N/2, N/2), used when every
+// difference fits (a periodic lattice numbered cell by cell keeps them within
+// a few planes of cells), which halves the index stream.
+__device__ __forceinline__ int nb_index(int v, int, int) { return v; }
+__device__ __forceinline__ int nb_index(int16_t v, int site, int N) {
+  int n = site + (int)v;
+  n += n < 0 ? N : 0;
+  n -= n >= N ? N : 0;
+  return n;
+}
+
+// Exchange sum of site_field.  Neighbours are taken G at a time: all index
+// loads of a batch are issued, then all spin and coupling loads, then the
+// terms are added in neighbour order (the same order, hence the same
+// rounding, as one at a time).  A last partial batch masks the slots past len.
+template <typename R, int G, typename I>
+__device__ __forceinline__ void exchange_sum(const HamDev<R> &h, const I *__restrict__ nb,
+                                             const Quad<R> *__restrict__ src, int site, int ens,
+                                             int row, R &s0, R &s1, R &s2) {
+  const int M = h.M, N = h.N;
+  const int len = h.ex_len[row];
+  const R *cp = h.cpl + row;
+  const size_t rs = (size_t)h.ex_rows;
+  int j = 0;
+  for (; j + G <= len; j += G) {
+    int n[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) n[u] = nb_index(nb[(size_t)(j + u) * N], site, N);
+    Quad<R> v[G];
+    R J[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      v[u] = src[(size_t)n[u] * M + ens];
+      J[u] = cp[(size_t)(j + u) * rs];
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      s0 = s0 + J[u] * (v[u].x * v[u].w);
+      s1 = s1 + J[u] * (v[u].y * v[u].w);
+      s2 = s2 + J[u] * (v[u].z * v[u].w);
+    }
+  }
+  if (j < len) {
+    int n[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u)
+      n[u] = j + u < len ? nb_index(nb[(size_t)(j + u) * N], site, N) : site;
+    Quad<R> v[G];
+    R J[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      if (j + u < len) {
+        v[u] = src[(size_t)n[u] * M + ens];
+        J[u] = cp[(size_t)(j + u) * rs];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      if (j + u < len) {
+        s0 = s0 + J[u] * (v[u].x * v[u].w);
+        s1 = s1 + J[u] * (v[u].y * v[u].w);
+        s2 = s2 + J[u] * (v[u].z * v[u].w);
+      }
+    }
+  }
+}
+
+// DM sum of site_field: field += m_j x D_ij (hamiltonianactions.f90:565-578).
+template <typename R, int G, typename I>
+__device__ __forceinline__ void dm_sum(const HamDev<R> &h, const I *__restrict__ nb,
+                                       const Quad<R> *__restrict__ src, int site, int ens,
+                                       int row, R &s0, R &s1, R &s2) {
+  const int M = h.M, N = h.N;
+  const int len = h.dm_len[row];
+  const size_t rs = (size_t)h.dm_rows;
+  for (int j = 0; j < len; j += G) {
+    int n[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u)
+      n[u] = j + u < len ? nb_index(nb[(size_t)(j + u) * N], site, N) : site;
+    Quad<R> v[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u)
+      if (j + u < len) v[u] = src[(size_t)n[u] * M + ens];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      if (j + u < len) {
+        const size_t r = (size_t)(j + u) * rs + row;
+        const R D0 = h.dvx[r], D1 = h.dvy[r], D2 = h.dvz[r];
+        const R m0 = v[u].x * v[u].w, m1 = v[u].y * v[u].w, m2 = v[u].z * v[u].w;
+        s0 = s0 + D2 * m1 - D1 * m2;
+        s1 = s1 + D0 * m2 - D2 * m0;
+        s2 = s2 + D1 * m0 - D0 * m1;
+      }
+    }
+  }
+}
+
 // G = neighbours whose loads are issued together (a tuning parameter).
 template <typename R, int G>
 __device__ __forceinline__ void site_field(const HamDev<R> &h, const Quad<R> *__restrict__ src,
@@ -214,82 +315,16 @@ __device__ __forceinline__ void site_field(const HamDev<R> &h, const Quad<R> *__
   s0 = s1 = s2 = (R)0;
   qf0 = qf1 = qf2 = (R)0;
   const SiteClass sc = h.classes[h.cls[site]];
-  const int M = h.M, N = h.N;
-  {
-    // Neighbours are taken G at a time: all index loads of a batch are
-    // issued, then all spin and coupling loads, then the terms are added in
-    // neighbour order (the same order, hence the same rounding, as one at a
-    // time).  A last partial batch masks the slots past len.
-    const int len = h.ex_len[sc.ex_row];
-    const int *nb = h.nbr + site;
-    const R *cp = h.cpl + sc.ex_row;
-    const size_t rs = (size_t)h.ex_rows;
-    int j = 0;
-    for (; j + G <= len; j += G) {
-      int n[G];
-#pragma unroll
-      for (int u = 0; u < G; ++u) n[u] = nb[(size_t)(j + u) * N];
-      Quad<R> v[G];
-      R J[G];
-#pragma unroll
-      for (int u = 0; u < G; ++u) {
-        v[u] = src[(size_t)n[u] * M + ens];
-        J[u] = cp[(size_t)(j + u) * rs];
-      }
-#pragma unroll
-      for (int u = 0; u < G; ++u) {
-        s0 = s0 + J[u] * (v[u].x * v[u].w);
-        s1 = s1 + J[u] * (v[u].y * v[u].w);
-        s2 = s2 + J[u] * (v[u].z * v[u].w);
-      }
-    }
-    if (j < len) {
-      int n[G];
-#pragma unroll
-      for (int u = 0; u < G; ++u) n[u] = j + u < len ? nb[(size_t)(j + u) * N] : site;
-      Quad<R> v[G];
-      R J[G];
-#pragma unroll
-      for (int u = 0; u < G; ++u) {
-        if (j + u < len) {
-          v[u] = src[(size_t)n[u] * M + ens];
-          J[u] = cp[(size_t)(j + u) * rs];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < G; ++u) {
-        if (j + u < len) {
-          s0 = s0 + J[u] * (v[u].x * v[u].w);
-          s1 = s1 + J[u] * (v[u].y * v[u].w);
-          s2 = s2 + J[u] * (v[u].z * v[u].w);
-        }
-      }
-    }
-  }
+  const int M = h.M;
+  if (h.nbr16)
+    exchange_sum<R, G>(h, h.nbr16 + site, src, site, ens, sc.ex_row, s0, s1, s2);
+  else
+    exchange_sum<R, G>(h, h.nbr + site, src, site, ens, sc.ex_row, s0, s1, s2);
   if (h.Wd > 0 && sc.dm_row >= 0) {
-    const int len = h.dm_len[sc.dm_row];
-    const int *nb = h.dnbr + site;
-    const size_t rs = (size_t)h.dm_rows;
-    for (int j = 0; j < len; j += G) {
-      int n[G];
-#pragma unroll
-      for (int u = 0; u < G; ++u) n[u] = j + u < len ? nb[(size_t)(j + u) * N] : site;
-      Quad<R> v[G];
-#pragma unroll
-      for (int u = 0; u < G; ++u)
-        if (j + u < len) v[u] = src[(size_t)n[u] * M + ens];
-#pragma unroll
-      for (int u = 0; u < G; ++u) {
-        if (j + u < len) {
-          const size_t r = (size_t)(j + u) * rs + sc.dm_row;
-          const R D0 = h.dvx[r], D1 = h.dvy[r], D2 = h.dvz[r];
-          const R m0 = v[u].x * v[u].w, m1 = v[u].y * v[u].w, m2 = v[u].z * v[u].w;
-          s0 = s0 + D2 * m1 - D1 * m2;
-          s1 = s1 + D0 * m2 - D2 * m0;
-          s2 = s2 + D1 * m0 - D0 * m1;
-        }
-      }
-    }
+    if (h.dnbr16)
+      dm_sum<R, G>(h, h.dnbr16 + site, src, site, ens, sc.dm_row, s0, s1, s2);
+    else
+      dm_sum<R, G>(h, h.dnbr + site, src, site, ens, sc.dm_row, s0, s1, s2);
   }
   if (sc.an_row >= 0) {
     const int t = h.an_type[sc.an_row];
@@ -492,6 +527,20 @@ __global__ void __launch_bounds__(256) k_field(HamDev<R> h, const Quad<R> *__res
   }
 }
 
+// Corrector output.  The FP32 form renormalises the moment direction: the
+// rotations conserve |e| only to rounding, and FP32 rounding accumulated over
+// thousands of steps is the largest error of that form.  FP64 keeps the
+// reference's arithmetic (no renormalisation anywhere).
+template <typename R>
+__device__ __forceinline__ Quad<R> finish(R o0, R o1, R o2, R m) {
+  return Quad<R>{o0, o1, o2, m};
+}
+template <>
+__device__ __forceinline__ Quad<float> finish<float>(float o0, float o1, float o2, float m) {
+  const float inv = 1.0f / sqrtf(o0 * o0 + o1 * o1 + o2 * o2);
+  return Quad<float>{o0 * inv, o1 * inv, o2 * inv, m};
+}
+
 // One half of an LLG step.
 //   ALG 1 midpoint (midpoint.f90), 5 Depondt (depondt.f90)
 //   HALF 0 predictor (evolve_first), 1 corrector (evolve_second)
@@ -541,7 +590,7 @@ __global__ void __launch_bounds__(256)
       d1 = (R)0.5 * d1 + (R)0.5 * bq.y;
       d2 = (R)0.5 * d2 + (R)0.5 * bq.z;
       rodrigues(d0, d1, d2, (R)prm.dt, (R)lam, eq.x, eq.y, eq.z, o0, o1, o2);
-      out[q] = Quad<R>{o0, o1, o2, m};
+      out[q] = finish<R>(o0, o1, o2, m);
     }
   } else {
     const R sig = midpoint_sigma<R>(lam, prm.bn, (double)m, T, prm.temprescale);
@@ -558,7 +607,7 @@ __global__ void __launch_bounds__(256)
       midpoint_A((R)lam, (R)prm.dt, (R)prm.bn, (R)prm.landeg(site), b0, b1, b2v, r0, r1, r2, pq.x,
                  pq.y, pq.z, A0, A1, A2);
       midpoint_solve(A0, A1, A2, eq.x, eq.y, eq.z, t0, t1, t2, detAi);
-      out[q] = Quad<R>{t0 * detAi, t1 * detAi, t2 * detAi, m};
+      out[q] = finish<R>(t0 * detAi, t1 * detAi, t2 * detAi, m);
     }
   }
 }
@@ -619,7 +668,10 @@ __device__ __forceinline__ R mc_delta_e(const HamDev<R> &h, const Quad<R> *__res
     for (int j = 0; j < len; j += G) {
       int nb[G];
 #pragma unroll
-      for (int u = 0; u < G; ++u) nb[u] = j + u < len ? h.nbr[(size_t)(j + u) * N + site] : site;
+      for (int u = 0; u < G; ++u)
+        nb[u] = j + u >= len ? site
+                : h.nbr16 ? nb_index(h.nbr16[(size_t)(j + u) * N + site], site, N)
+                          : h.nbr[(size_t)(j + u) * N + site];
       Quad<R> v[G];
       R J[G];
 #pragma unroll
@@ -667,7 +719,8 @@ __device__ __forceinline__ R mc_delta_e(const HamDev<R> &h, const Quad<R> *__res
     const R u0 = me.x, u2 = me.z;  // unit-vector terms as in montecarlo_common.f90:612,615
     const int len = h.dm_len[sc.dm_row];
     for (int j = 0; j < len; ++j) {
-      const int nb = h.dnbr[(size_t)j * N + site];
+      const int nb = h.dnbr16 ? nb_index(h.dnbr16[(size_t)j * N + site], site, N)
+                              : h.dnbr[(size_t)j * N + site];
       const size_t r = (size_t)j * h.dm_rows + sc.dm_row;
       const R D0 = h.dvx[r], D1 = h.dvy[r], D2 = h.dvz[r];
       R a0, a1, a2;

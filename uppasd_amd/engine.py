@@ -198,6 +198,12 @@ class Engine:
                 "uasd_get_kernel_timing")
         return n.value, ms.value
 
+    def index_bytes(self):
+        """(exchange, dm) bytes per neighbour entry read by the kernels (2 or 4)."""
+        a, b = C.c_int(0), C.c_int(0)
+        B.check(B.lib().uasd_index_bytes(self._h, C.byref(a), C.byref(b)), "uasd_index_bytes")
+        return a.value, b.value
+
     def set_graph_steps(self, k: int):
         """Replay hipGraphs of k LLG steps / MC sweeps (0 = launch one by one)."""
         B.check(B.lib().uasd_set_graph_steps(self._h, int(k)), "uasd_set_graph_steps")

@@ -47,6 +47,20 @@ def kernel_bytes(W: int, s: int, ib: int = 4) -> dict:
     return {"predictor": pred, "corrector": corr, "step": pred + corr}
 
 
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary of
+    the default workload (FETCH_SIZE + WRITE_SIZE scaled by a plain copy), or
+    None when the summary does not list that kernel."""
+    path = os.path.join(ROOT, "profiles", "k_llg_fp64_midpoint_bcc64_58nn_ell.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.split("|")]
+        if len(cells) > 6 and cells[1] == f"`{kernel}`":
+            return (float(cells[4]) + float(cells[5])) * 1e6
+    return None
+
+
 def build_c2(L: int):
     from uppasd_amd import configs as CF
     return CF.bcc_shells(L, CF.bcc5_couplings(), moment=2.2)
@@ -308,7 +322,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--gather", type=int, default=0,
                     help="fixed gather batch (2/4/8/16) instead of autotuning")
-    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--block", type=int, default=256,
+                    help="workgroup size with --gather; add 65536 for non-temporal index loads")
     ap.add_argument("--forms", default="c3,c4,c5",
                     help="comma list of extra forms to run (c3,c4,c5) or 'none'")
     args = ap.parse_args()
@@ -344,6 +359,8 @@ def main():
         eng.autotune_llg(args.solver, steps=3)
     tune_s = time.perf_counter() - t_tune
     gather, block = eng.tuning(eng.TUNE_LLG)
+    nt = block >> 16
+    block &= 0xFFFF
 
     eng.sd_steps(args.solver, args.warmup)
     eng.sync()
@@ -364,6 +381,9 @@ def main():
     t_pred = ms_pred / max(n_pred, 1) * 1e-3
     achieved = kb["corrector"] * N / t_corr / 1e9
     value = N * args.steps / elapsed
+    traffic = None
+    if args.precision == "fp64" and args.L == 64 and args.solver == 1 and not nt:
+        traffic = profiled_traffic(f"uasd::k_llg<double, 1, 1, true, {gather}>")
     _, eM = eng.get_moments()
     norm_drift = float(np.max(np.abs(np.linalg.norm(eM[0], axis=1) / sysd["mmom"] - 1.0)))
 
@@ -387,7 +407,7 @@ def main():
                          "device-resident"),
             "spins": N, "neighbours": W, "solver": args.solver, "precision": args.precision,
             "setup_s": round(setup_s, 2),
-            "launch": {"gather": gather, "block": block,
+            "launch": {"gather": gather, "block": block, "nt_index_loads": bool(nt),
                        "autotuned": not args.gather, "tune_s": round(tune_s, 3)},
             "kernel_ms": {"predictor": round(t_pred * 1e3, 4), "corrector": round(t_corr * 1e3, 4)},
             "bytes_per_site": kb,
@@ -400,7 +420,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
         },
     }
     del eng

@@ -167,7 +167,8 @@ int uasd_index_bytes(uasd_ctx *ctx, int *exchange, int *dm);
  * the reference's OpenMP loops have no launch shape) ----
  * kind: 0 fused LLG step, 1 field, 2 MC colour pass.  gather = neighbours
  * whose loads are issued together (2, 4, 8, 16); block = workgroup size
- * (64, 128, 256).  Results do not depend on either. */
+ * (64, 128, 256), plus 0x10000 to read the neighbour lists with
+ * non-temporal loads.  Results do not depend on any of them. */
 int uasd_set_tuning(uasd_ctx *ctx, int kind, int gather, int block);
 int uasd_get_tuning(uasd_ctx *ctx, int kind, int *gather, int *block);
 /* Autotuner: times every shape on the context's own lattice and state

@@ -1335,8 +1335,9 @@ int uasd_set_tuning(uasd_ctx *c, int kind, int gather, int block) {
   if (int r = check_ctx(c)) return r;
   c->gen += 1;
   if (kind < 0 || kind > 2) return fail(UASD_ERR_ARG, "kind must be 0 (LLG), 1 (field) or 2 (MC)");
-  uasd::Tune t{gather, block};
-  if (!uasd::tune_valid(t))
+  // block may carry the non-temporal flag in bit 16 (block | 0x10000)
+  uasd::Tune t{gather, block & 0xffff, (block >> 16) & 1};
+  if (!uasd::tune_valid(t) || (block >> 17) != 0)
     return fail(UASD_ERR_ARG, "gather must be 2, 4, 8 or 16 and block 64, 128 or 256");
   c->tune[kind] = t;
   return UASD_OK;
@@ -1346,7 +1347,7 @@ int uasd_get_tuning(uasd_ctx *c, int kind, int *gather, int *block) {
   if (int r = check_ctx(c)) return r;
   if (kind < 0 || kind > 2) return fail(UASD_ERR_ARG, "kind must be 0 (LLG), 1 (field) or 2 (MC)");
   if (gather) *gather = c->tune[kind].gather;
-  if (block) *block = c->tune[kind].block;
+  if (block) *block = c->tune[kind].block | (c->tune[kind].nt << 16);
   return UASD_OK;
 }
 
@@ -1399,9 +1400,10 @@ int tune_search(uasd_ctx *c, uasd::Tune *slot, Run &&run, double *best_ms) {
   const int blocks[] = {64, 128, 256};
   uasd::Tune best = *slot;
   float best_t = 1e30f;
+  for (int nt = 0; nt < 2; ++nt)
   for (int g : gathers)
     for (int bl : blocks) {
-      *slot = uasd::Tune{g, bl};
+      *slot = uasd::Tune{g, bl, nt};
       if (int r = run()) return r;
       HIP_TRY(hipEventRecord(ev.a, c->stream));
       if (int r = run()) return r;

@@ -73,6 +73,9 @@ struct HamDev {
   const SiteClass *__restrict__ classes;
   // external field: 0 none, 1 uniform per ensemble (ext_u[3*ens+c]), 2 per q
   int ext_mode;
+  // 1: neighbour lists are read with non-temporal loads (streamed past L2,
+  // which then keeps the gathered spins); a tuning choice, results identical
+  int nt;
   const R *__restrict__ ext_u;
   const Quad<R> *__restrict__ ext_q;
 };
@@ -213,6 +216,12 @@ __device__ __forceinline__ void cubic(const HamDev<R> &h, int row, R m0, R m1, R
 // difference fits (a periodic lattice numbered cell by cell keeps them within
 // a few planes of cells), which halves the index stream.
 __device__ __forceinline__ int nb_index(int v, int, int) { return v; }
+
+template <bool NT, typename I>
+__device__ __forceinline__ I nb_load(const I *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  return *p;
+}
 __device__ __forceinline__ int nb_index(int16_t v, int site, int N) {
   int n = site + (int)v;
   n += n < 0 ? N : 0;
@@ -224,7 +233,7 @@ __device__ __forceinline__ int nb_index(int16_t v, int site, int N) {
 // loads of a batch are issued, then all spin and coupling loads, then the
 // terms are added in neighbour order (the same order, hence the same
 // rounding, as one at a time).  A last partial batch masks the slots past len.
-template <typename R, int G, typename I>
+template <typename R, int G, bool NT, typename I>
 __device__ __forceinline__ void exchange_sum(const HamDev<R> &h, const I *__restrict__ nb,
                                              const Quad<R> *__restrict__ src, int site, int ens,
                                              int row, R &s0, R &s1, R &s2) {
@@ -236,7 +245,7 @@ __device__ __forceinline__ void exchange_sum(const HamDev<R> &h, const I *__rest
   for (; j + G <= len; j += G) {
     int n[G];
 #pragma unroll
-    for (int u = 0; u < G; ++u) n[u] = nb_index(nb[(size_t)(j + u) * N], site, N);
+    for (int u = 0; u < G; ++u) n[u] = nb_index(nb_load<NT>(nb + (size_t)(j + u) * N), site, N);
     Quad<R> v[G];
     R J[G];
 #pragma unroll
@@ -255,7 +264,7 @@ __device__ __forceinline__ void exchange_sum(const HamDev<R> &h, const I *__rest
     int n[G];
 #pragma unroll
     for (int u = 0; u < G; ++u)
-      n[u] = j + u < len ? nb_index(nb[(size_t)(j + u) * N], site, N) : site;
+      n[u] = j + u < len ? nb_index(nb_load<NT>(nb + (size_t)(j + u) * N), site, N) : site;
     Quad<R> v[G];
     R J[G];
 #pragma unroll
@@ -316,10 +325,17 @@ __device__ __forceinline__ void site_field(const HamDev<R> &h, const Quad<R> *__
   qf0 = qf1 = qf2 = (R)0;
   const SiteClass sc = h.classes[h.cls[site]];
   const int M = h.M;
-  if (h.nbr16)
-    exchange_sum<R, G>(h, h.nbr16 + site, src, site, ens, sc.ex_row, s0, s1, s2);
-  else
-    exchange_sum<R, G>(h, h.nbr + site, src, site, ens, sc.ex_row, s0, s1, s2);
+  if (h.nbr16) {
+    if (h.nt)
+      exchange_sum<R, G, true>(h, h.nbr16 + site, src, site, ens, sc.ex_row, s0, s1, s2);
+    else
+      exchange_sum<R, G, false>(h, h.nbr16 + site, src, site, ens, sc.ex_row, s0, s1, s2);
+  } else {
+    if (h.nt)
+      exchange_sum<R, G, true>(h, h.nbr + site, src, site, ens, sc.ex_row, s0, s1, s2);
+    else
+      exchange_sum<R, G, false>(h, h.nbr + site, src, site, ens, sc.ex_row, s0, s1, s2);
+  }
   if (h.Wd > 0 && sc.dm_row >= 0) {
     if (h.dnbr16)
       dm_sum<R, G>(h, h.dnbr16 + site, src, site, ens, sc.dm_row, s0, s1, s2);

@@ -33,9 +33,11 @@ hipError_t launch_field(hipStream_t s, const Tune &t, const HamDev<R> &h, const 
   const int64_t n = (int64_t)h.N * h.M;
   if (n == 0) return hipSuccess;
   if (!tune_valid(t)) return hipErrorInvalidValue;
+  HamDev<R> hh = h;
+  hh.nt = t.nt;
   return with_gather(t, [&](auto g) {
     hipLaunchKernelGGL((k_field<R, decltype(g)::value>), dim3(grid_for(n, t.block)),
-                       dim3(t.block), 0, s, h, src, beff, en, parts);
+                       dim3(t.block), 0, s, hh, src, beff, en, parts);
     return hipGetLastError();
   });
 }
@@ -47,13 +49,15 @@ static hipError_t llg_one(hipStream_t s, const Tune &t, const HamDev<R> &h,
   const int64_t n = (int64_t)h.N * h.M;
   if (n == 0) return hipSuccess;
   const dim3 grid(grid_for(n, t.block)), block(t.block);
+  HamDev<R> hh = h;
+  hh.nt = t.nt;
   if (!FUSED) {  // no gather: one instance
-    hipLaunchKernelGGL((k_llg<R, ALG, HALF, false, 4>), grid, block, 0, s, h, prm, e, p, b2,
+    hipLaunchKernelGGL((k_llg<R, ALG, HALF, false, 4>), grid, block, 0, s, hh, prm, e, p, b2,
                        beff, out);
     return hipGetLastError();
   }
   return with_gather(t, [&](auto g) {
-    hipLaunchKernelGGL((k_llg<R, ALG, HALF, true, decltype(g)::value>), grid, block, 0, s, h,
+    hipLaunchKernelGGL((k_llg<R, ALG, HALF, true, decltype(g)::value>), grid, block, 0, s, hh,
                        prm, e, p, b2, beff, out);
     return hipGetLastError();
   });
@@ -85,9 +89,11 @@ hipError_t launch_mc(hipStream_t s, const Tune &t, const HamDev<R> &h, const MCP
   const int64_t n = (int64_t)mp.nsites * h.M;
   if (n == 0) return hipSuccess;
   if (!tune_valid(t)) return hipErrorInvalidValue;
+  HamDev<R> hh = h;
+  hh.nt = t.nt;
   return with_gather(t, [&](auto g) {
     hipLaunchKernelGGL((k_mc_colour<R, decltype(g)::value>), dim3(grid_for(n, t.block)),
-                       dim3(t.block), 0, s, h, mp, spins);
+                       dim3(t.block), 0, s, hh, mp, spins);
     return hipGetLastError();
   });
 }

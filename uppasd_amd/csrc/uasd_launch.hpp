@@ -12,14 +12,16 @@ constexpr int kBlock = 256;  // largest workgroup the kernels are built for
 // Launch shape of the gathering kernels (chosen by the autotuner):
 //   gather  neighbours whose loads are issued together: 2, 4, 8 or 16
 //   block   workgroup size: 64, 128 or 256
+//   nt      1: neighbour lists read with non-temporal loads
 struct Tune {
   int gather = 4;
   int block = 256;
+  int nt = 0;
 };
 
 inline bool tune_valid(const Tune &t) {
   return (t.gather == 2 || t.gather == 4 || t.gather == 8 || t.gather == 16) &&
-         (t.block == 64 || t.block == 128 || t.block == 256);
+         (t.block == 64 || t.block == 128 || t.block == 256) && (t.nt == 0 || t.nt == 1);
 }
 
 // parts (nullable) receives beff1 then beff2, N*M quads each.

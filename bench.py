@@ -324,6 +324,8 @@ def main():
                     help="fixed gather batch (2/4/8/16) instead of autotuning")
     ap.add_argument("--block", type=int, default=256,
                     help="workgroup size with --gather; add 65536 for non-temporal index loads")
+    ap.add_argument("--site-order", type=int, default=0,
+                    help="device site order: 0 = basis-major (NA of the lattice), 1 = reference")
     ap.add_argument("--forms", default="c3,c4,c5",
                     help="comma list of extra forms to run (c3,c4,c5) or 'none'")
     args = ap.parse_args()
@@ -342,7 +344,8 @@ def main():
     sysd = build_c2(args.L)
     N = sysd["lattice"].natom
     W = int(sysd["nlistsize"].max())
-    eng = U.Engine(N, 1, args.precision)
+    order = sysd["lattice"].NA if args.site_order == 0 else args.site_order
+    eng = U.Engine(N, 1, args.precision, site_order=order)
     eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
     rng = np.random.default_rng(2)
     e0 = rng.normal(size=(1, N, 3))
@@ -408,6 +411,7 @@ def main():
             "spins": N, "neighbours": W, "solver": args.solver, "precision": args.precision,
             "setup_s": round(setup_s, 2),
             "launch": {"gather": gather, "block": block, "nt_index_loads": bool(nt),
+                       "site_order": order,
                        "autotuned": not args.gather, "tune_s": round(tune_s, 3)},
             "kernel_ms": {"predictor": round(t_pred * 1e3, 4), "corrector": round(t_corr * 1e3, 4)},
             "bytes_per_site": kb,

@@ -66,6 +66,14 @@ int uasd_ctx_set_stream(uasd_ctx *ctx, void *hip_stream);
 void *uasd_ctx_stream(uasd_ctx *ctx);
 int uasd_ctx_sync(uasd_ctx *ctx);
 
+/* Device site order (no reference counterpart; results do not depend on
+ * it): na > 1 stores the sites of each of the na basis atoms contiguously
+ * (reference site i = cell*na + basis goes to slot basis*(natom/na) + cell),
+ * so the lanes of a wavefront gather one sublattice from contiguous memory.
+ * Host arrays keep the reference order in every call.  Must precede the
+ * uploads; not combined with several segments.  Default 1 (reference order). */
+int uasd_set_site_order(uasd_ctx *ctx, int na);
+
 /* ---- Hamiltonian: the ham_t arrays (hamiltoniandatatype.f90:14-100) ---- */
 /* aham (Natom) maps site -> Hamiltonian row (ham%aHam); NULL means row = site.
  * Rows with identical couplings are stored once on the device. */

@@ -87,6 +87,9 @@ struct Timing {
 // ----------------------------------------------------------------- context
 struct uasd_ctx {
   int N = 0, M = 0, prec = UASD_FP64, device = 0;
+  // device site order: slot d holds reference site site_of(d) (see HamDev)
+  int order_na = 1;
+  int order_nc = 0;  // sites per basis atom (N when order_na = 1)
   hipStream_t own_stream = nullptr, stream = nullptr;
   size_t rsize = 8;  // sizeof(R)
 
@@ -179,6 +182,14 @@ namespace {
 
 void destroy_graphs(uasd_ctx *c);
 
+// Reference site held by device slot d, and the slot of reference site i.
+inline int site_of(const uasd_ctx *c, int d) {
+  return c->order_na == 1 ? d : (d % c->order_nc) * c->order_na + d / c->order_nc;
+}
+inline int slot_of(const uasd_ctx *c, int i) {
+  return c->order_na == 1 ? i : (i % c->order_na) * c->order_nc + i / c->order_na;
+}
+
 // Quad buffers: one Quad<R> = {x, y, z, w} per q = site*M + ens.
 size_t quad_bytes(const uasd_ctx *c) { return (size_t)c->N * c->M * 4 * c->rsize; }
 
@@ -225,6 +236,8 @@ uasd::HamDev<R> ham_view(const uasd_ctx *c) {
   h.cls = c->cls.as<int>();
   h.classes = c->classes.as<uasd::SiteClass>();
   h.ext_mode = c->ext_mode;
+  h.order_na = c->order_na;
+  h.order_nc = c->order_na == 1 ? c->N : c->order_nc;
   h.ext_u = c->ext_u.as<R>();
   if (c->ext_mode == 2) h.ext_q = quads<R>(c, c->ext_full);
   return h;
@@ -256,12 +269,14 @@ int upload_raw(uasd_ctx *c, DevBuf &dst, const std::vector<T> &src) {
 }
 
 // (3,N,M) Fortran array (+ w per (N,M) Fortran array, or a constant) -> quads
-std::vector<double> to_quads(const double *a, int N, int M, const double *w, double wconst) {
+// in device site order
+std::vector<double> to_quads(const uasd_ctx *c, const double *a, const double *w, double wconst) {
+  const int N = c->N, M = c->M;
   std::vector<double> out((size_t)4 * N * M);
   for (int k = 0; k < M; ++k)
     for (int i = 0; i < N; ++i) {
       const double *v = a + 3 * ((size_t)i + (size_t)N * k);
-      const size_t q = (size_t)i * M + k;
+      const size_t q = (size_t)slot_of(c, i) * M + k;
       out[4 * q] = v[0];
       out[4 * q + 1] = v[1];
       out[4 * q + 2] = v[2];
@@ -303,11 +318,12 @@ int download_quads(uasd_ctx *c, const DevBuf &b, int k, std::vector<double> &hq)
   return UASD_OK;
 }
 
-// quads -> Fortran (3,N,M); times_w gives emomM = xyz*w
-void from_quads(const std::vector<double> &hq, int N, int M, double *out, bool times_w) {
+// quads (device site order) -> Fortran (3,N,M); times_w gives emomM = xyz*w
+void from_quads(const uasd_ctx *c, const std::vector<double> &hq, double *out, bool times_w) {
+  const int N = c->N, M = c->M;
   for (int k = 0; k < M; ++k)
     for (int i = 0; i < N; ++i) {
-      const size_t q = (size_t)i * M + k;
+      const size_t q = (size_t)slot_of(c, i) * M + k;
       double *v = out + 3 * ((size_t)i + (size_t)N * k);
       const double w = hq[4 * q + 3];
       for (int d = 0; d < 3; ++d) v[d] = times_w ? hq[4 * q + d] * w : hq[4 * q + d];
@@ -537,14 +553,17 @@ int build_coloring(uasd_ctx *c) {
   std::vector<int> col(N, -1);
   int ncol = 0;
   std::vector<int> mark;
-  for (int i = 0; i < N; ++i) {
+  // greedy colouring visiting the sites in reference order, so the colour
+  // classes (and hence MC chains) do not depend on the device site order
+  for (int h = 0; h < N; ++h) {
+    const int i = slot_of(c, h);
     for (int n : adj[i])
       if (col[n] >= 0) {
         if ((int)mark.size() <= col[n]) mark.resize(col[n] + 1, -1);
-        mark[col[n]] = i;
+        mark[col[n]] = h;
       }
     int k = 0;
-    while (k < (int)mark.size() && mark[k] == i) ++k;
+    while (k < (int)mark.size() && mark[k] == h) ++k;
     col[i] = k;
     ncol = std::max(ncol, k + 1);
   }
@@ -722,16 +741,20 @@ int uasd_set_exchange(uasd_ctx *c, int mnn, int nham, const int *nlist, const in
   RowSet rs = dedupe_rows(N, ref, nlistsize, ncoup, mnn, 1);
   int W = 0;
   for (int L : rs.len) W = std::max(W, L);
-  std::vector<int> nb((size_t)std::max(W, 1) * N);
+  // device order: slot d = slot_of(i) holds site i; list values are slots
+  std::vector<int> nb((size_t)std::max(W, 1) * N), row_dev(N);
   for (int i = 0; i < N; ++i) {
+    const int d = slot_of(c, i);
     int L = rs.len[rs.row_of_site[i]];
+    row_dev[d] = rs.row_of_site[i];
     for (int j = 0; j < W; ++j) {
-      int v = i;
+      int v = d;
       if (j < L) {
         v = nlist[(size_t)j + (size_t)mnn * i] - 1;
         if (v < 0 || v >= N) return fail(UASD_ERR_ARG, "nlist entry out of range");
+        v = slot_of(c, v);
       }
-      nb[(size_t)j * N + i] = v;
+      nb[(size_t)j * N + d] = v;
     }
   }
   const int rows = (int)rs.len.size();
@@ -740,7 +763,7 @@ int uasd_set_exchange(uasd_ctx *c, int mnn, int nham, const int *nlist, const in
     for (int j = 0; j < rs.len[r]; ++j) cp[(size_t)j * rows + r] = rs.vals[r][j];
   c->W = W;
   c->ex_rows = rows;
-  c->ex_row_of_site = rs.row_of_site;
+  c->ex_row_of_site = row_dev;
   c->ex_len_host = rs.len;
   c->nbr_host = nb;
   if (int r = upload_raw(c, c->nbr, nb)) return r;
@@ -782,16 +805,19 @@ int uasd_set_dm(uasd_ctx *c, int mdm, int nham, const int *dmlist, const int *dm
   RowSet rs = dedupe_rows(N, ref, dmlistsize, dm_vect, mdm, 3);
   int W = 0;
   for (int L : rs.len) W = std::max(W, L);
-  std::vector<int> nb((size_t)std::max(W, 1) * N);
+  std::vector<int> nb((size_t)std::max(W, 1) * N), row_dev(N);
   for (int i = 0; i < N; ++i) {
+    const int d = slot_of(c, i);
     int L = rs.len[rs.row_of_site[i]];
+    row_dev[d] = rs.row_of_site[i];
     for (int j = 0; j < W; ++j) {
-      int v = i;
+      int v = d;
       if (j < L) {
         v = dmlist[(size_t)j + (size_t)mdm * i] - 1;
         if (v < 0 || v >= N) return fail(UASD_ERR_ARG, "dmlist entry out of range");
+        v = slot_of(c, v);
       }
-      nb[(size_t)j * N + i] = v;
+      nb[(size_t)j * N + d] = v;
     }
   }
   const int rows = (int)rs.len.size();
@@ -802,7 +828,7 @@ int uasd_set_dm(uasd_ctx *c, int mdm, int nham, const int *dmlist, const int *dm
       for (int d = 0; d < 3; ++d) dv[d * n + (size_t)j * rows + r] = rs.vals[r][3 * j + d];
   c->Wd = W;
   c->dm_rows = rows;
-  c->dm_row_of_site = rs.row_of_site;
+  c->dm_row_of_site = row_dev;
   c->dm_len_host = rs.len;
   c->dnbr_host = nb;
   if (int r = upload_raw(c, c->dnbr, nb)) return r;
@@ -853,7 +879,8 @@ int uasd_set_anisotropy(uasd_ctx *c, const int *taniso, const double *eaniso,
   }
   const int nr = (int)vals.size();
   c->an_rows = nr;
-  c->an_row_of_site = row_of;
+  c->an_row_of_site.assign(N, -1);
+  for (int i = 0; i < N; ++i) c->an_row_of_site[slot_of(c, i)] = row_of[i];
   if (nr > 0) {
     std::vector<double> v(6 * (size_t)nr);
     for (int r = 0; r < nr; ++r)
@@ -894,7 +921,7 @@ int uasd_set_external_field(uasd_ctx *c, const double *field) {
   }
   c->ext_mode = 2;
   if (int r = upload_real(c, c->ext_u, c->ext_u_host)) return r;
-  return upload_quads(c, c->ext_full, 0, to_quads(field, N, M, nullptr, 0.0));
+  return upload_quads(c, c->ext_full, 0, to_quads(c, field, nullptr, 0.0));
 }
 
 int uasd_set_llg(uasd_ctx *c, const double *lambda1, const double *temp, const double *landeg,
@@ -911,9 +938,10 @@ int uasd_set_llg(uasd_ctx *c, const double *lambda1, const double *temp, const d
   };
   std::vector<double> arr(3 * (size_t)N);
   for (int i = 0; i < N; ++i) {
-    arr[i] = lambda1[i];
-    arr[N + i] = temp[i];
-    arr[2 * (size_t)N + i] = landeg ? landeg[i] : 1.0;
+    const int d = slot_of(c, i);
+    arr[d] = lambda1[i];
+    arr[N + d] = temp[i];
+    arr[2 * (size_t)N + d] = landeg ? landeg[i] : 1.0;
   }
   c->lam_uniform = is_uniform(lambda1);
   c->temp_uniform = is_uniform(temp);
@@ -944,7 +972,7 @@ int uasd_set_noise(uasd_ctx *c, int mode, uint64_t seed, const double *normals) 
     return fail(UASD_ERR_ARG, "unknown noise mode");
   if (mode == UASD_NOISE_INJECTED) {
     if (!normals) return fail(UASD_ERR_ARG, "injected noise needs the normals array");
-    if (int r = upload_quads(c, c->noise, 0, to_quads(normals, c->N, c->M, nullptr, 0.0))) return r;
+    if (int r = upload_quads(c, c->noise, 0, to_quads(c, normals, nullptr, 0.0))) return r;
   }
   if (mode == UASD_NOISE_PHILOX && (seed != c->seed || c->noise_mode != mode)) c->step = 0;
   c->noise_mode = mode;
@@ -955,7 +983,7 @@ int uasd_set_noise(uasd_ctx *c, int mode, uint64_t seed, const double *normals) 
 int uasd_set_moments(uasd_ctx *c, const double *emom, const double *mmom) {
   if (int r = check_ctx(c)) return r;
   if (!emom || !mmom) return fail(UASD_ERR_ARG, "uasd_set_moments: null array");
-  std::vector<double> hq = to_quads(emom, c->N, c->M, mmom, 0.0);
+  std::vector<double> hq = to_quads(c, emom, mmom, 0.0);
   if (int r = upload_quads(c, c->spins, 0, hq)) return r;
   if (int r = upload_quads(c, c->spins, 1, hq)) return r;  // emom2 starts as emom
   c->have_state = true;
@@ -967,8 +995,8 @@ int uasd_get_moments(uasd_ctx *c, double *emom, double *emomM) {
   if (!c->have_state) return fail(UASD_ERR_STATE, "moments not set");
   std::vector<double> hq;
   if (int r = download_quads(c, c->spins, 0, hq)) return r;
-  if (emom) from_quads(hq, c->N, c->M, emom, false);
-  if (emomM) from_quads(hq, c->N, c->M, emomM, true);
+  if (emom) from_quads(c, hq, emom, false);
+  if (emomM) from_quads(c, hq, emomM, true);
   return UASD_OK;
 }
 
@@ -978,7 +1006,7 @@ int uasd_get_moments(uasd_ctx *c, double *emom, double *emomM) {
 static int upload_vec_keep_w(uasd_ctx *c, int k, const double *a) {
   std::vector<double> cur;
   if (int r = download_quads(c, c->spins, 0, cur)) return r;  // w = mmom of the state
-  std::vector<double> hq = to_quads(a, c->N, c->M, nullptr, 0.0);
+  std::vector<double> hq = to_quads(c, a, nullptr, 0.0);
   for (size_t q = 0; q < (size_t)c->N * c->M; ++q) hq[4 * q + 3] = cur[4 * q + 3];
   return upload_quads(c, c->spins, k, hq);
 }
@@ -989,21 +1017,21 @@ int uasd_effective_field_host(uasd_ctx *c, const double *emomM, double *beff, do
   if (!c->have_ex) return fail(UASD_ERR_STATE, "exchange not set");
   if (!emomM || !beff) return fail(UASD_ERR_ARG, "null array");
   if (int r = rebuild_classes(c)) return r;
-  if (int r = upload_quads(c, c->scratch, 0, to_quads(emomM, c->N, c->M, nullptr, 1.0))) return r;
+  if (int r = upload_quads(c, c->scratch, 0, to_quads(c, emomM, nullptr, 1.0))) return r;
   const bool parts = beff1 || beff2;
   int r = c->prec == UASD_FP64 ? run_field<double>(c, true, energy != nullptr, parts)
                                : run_field<float>(c, true, energy != nullptr, parts);
   if (r) return r;
   std::vector<double> hq;
   if ((r = download_quads(c, c->field, 0, hq))) return r;
-  from_quads(hq, c->N, c->M, beff, false);
+  from_quads(c, hq, beff, false);
   if (beff1) {
     if ((r = download_quads(c, c->field, 1, hq))) return r;
-    from_quads(hq, c->N, c->M, beff1, false);
+    from_quads(c, hq, beff1, false);
   }
   if (beff2) {
     if ((r = download_quads(c, c->field, 2, hq))) return r;
-    from_quads(hq, c->N, c->M, beff2, false);
+    from_quads(c, hq, beff2, false);
   }
   if (energy) {
     std::vector<double> red;
@@ -1025,7 +1053,7 @@ int uasd_evolve_first_host(uasd_ctx *c, int alg, const double *beff, double *emo
   if (!beff || !emom || !emom2 || !emomM || (alg == 5 && !b2eff))
     return fail(UASD_ERR_ARG, "null array");
   if (int r = ready_for_llg(c)) return r;
-  if (int r = upload_quads(c, c->field, 0, to_quads(beff, c->N, c->M, nullptr, 0.0))) return r;
+  if (int r = upload_quads(c, c->field, 0, to_quads(c, beff, nullptr, 0.0))) return r;
   if (int r = upload_vec_keep_w(c, 0, emom)) return r;
   int r = c->prec == UASD_FP64 ? run_llg_half<double>(c, alg, 0, false, false)
                                : run_llg_half<float>(c, alg, 0, false, false);
@@ -1035,15 +1063,15 @@ int uasd_evolve_first_host(uasd_ctx *c, int alg, const double *beff, double *emo
   if (alg == UASD_SDE_DEPONDT) {
     // depondt.f90:182-187: emom2 = m(t), emom = rotated, emomM = emom*mmom, b2eff = bdup
     std::memcpy(emom2, emom, sizeof(double) * 3 * (size_t)c->N * c->M);
-    from_quads(pq, c->N, c->M, emom, false);
-    from_quads(pq, c->N, c->M, emomM, true);
+    from_quads(c, pq, emom, false);
+    from_quads(c, pq, emomM, true);
     std::vector<double> bq;
     if ((r = download_quads(c, c->spins, 2, bq))) return r;
-    from_quads(bq, c->N, c->M, b2eff, false);
+    from_quads(c, bq, b2eff, false);
   } else {
     // midpoint.f90:167-175: emom2 = et', emomM = et'*mmom, emom unchanged
-    from_quads(pq, c->N, c->M, emom2, false);
-    from_quads(pq, c->N, c->M, emomM, true);
+    from_quads(c, pq, emom2, false);
+    from_quads(c, pq, emomM, true);
   }
   return UASD_OK;
 }
@@ -1055,12 +1083,12 @@ int uasd_evolve_second_host(uasd_ctx *c, int alg, const double *beff, const doub
     return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint) or 5 (Depondt)");
   if (!beff || !emom || !emom2 || (alg == 5 && !b2eff)) return fail(UASD_ERR_ARG, "null array");
   if (int r = ready_for_llg(c)) return r;
-  if (int r = upload_quads(c, c->field, 0, to_quads(beff, c->N, c->M, nullptr, 0.0))) return r;
+  if (int r = upload_quads(c, c->field, 0, to_quads(c, beff, nullptr, 0.0))) return r;
   if (alg == UASD_SDE_DEPONDT) {
     // reference on entry: emom = predicted, emom2 = m(t)
     if (int r = upload_vec_keep_w(c, 1, emom)) return r;
     if (int r = upload_vec_keep_w(c, 0, emom2)) return r;
-    if (int r = upload_quads(c, c->spins, 2, to_quads(b2eff, c->N, c->M, nullptr, 0.0))) return r;
+    if (int r = upload_quads(c, c->spins, 2, to_quads(c, b2eff, nullptr, 0.0))) return r;
   } else {
     if (int r = upload_vec_keep_w(c, 0, emom)) return r;
     if (int r = upload_vec_keep_w(c, 1, emom2)) return r;
@@ -1071,7 +1099,7 @@ int uasd_evolve_second_host(uasd_ctx *c, int alg, const double *beff, const doub
   if (alg == UASD_SDE_DEPONDT) std::memcpy(emom, emom2, sizeof(double) * 3 * (size_t)c->N * c->M);
   std::vector<double> pq;
   if ((r = download_quads(c, c->spins, 1, pq))) return r;
-  from_quads(pq, c->N, c->M, emom2, false);
+  from_quads(c, pq, emom2, false);
   c->step += 1;
   return UASD_OK;
 }
@@ -1207,7 +1235,7 @@ int uasd_effective_field(uasd_ctx *c, double *beff, double *energy) {
   if (beff) {
     std::vector<double> hq;
     if ((r = download_quads(c, c->field, 0, hq))) return r;
-    from_quads(hq, c->N, c->M, beff, false);
+    from_quads(c, hq, beff, false);
   }
   if (energy) {
     std::vector<double> red;
@@ -1487,8 +1515,22 @@ int uasd_measure(uasd_ctx *c, double *msum, double *energy) {
   return UASD_OK;
 }
 
+int uasd_set_site_order(uasd_ctx *c, int na) {
+  if (int r = check_ctx(c)) return r;
+  if (na < 1 || c->N % na != 0) return fail(UASD_ERR_ARG, "na must be >= 1 and divide natom");
+  if (c->have_ex || c->have_state || c->have_llg || c->Wd > 0 || c->an_rows > 0 || c->ext_mode == 2)
+    return fail(UASD_ERR_STATE, "set the site order before any Hamiltonian, moment or LLG upload");
+  if (c->nseg > 1 && na > 1) return fail(UASD_ERR_UNSUPPORTED, "site order with several segments");
+  c->order_na = na;
+  c->order_nc = c->N / na;
+  c->gen += 1;
+  return UASD_OK;
+}
+
 int uasd_set_segments(uasd_ctx *c, int nseg, const int *offsets) {
   if (int r = check_ctx(c)) return r;
+  if (nseg > 1 && c->order_na > 1)
+    return fail(UASD_ERR_UNSUPPORTED, "segments need the reference site order");
   if (nseg <= 0 || !offsets) return fail(UASD_ERR_ARG, "nseg must be > 0 with offsets (nseg+1)");
   if (offsets[0] != 0 || offsets[nseg] != c->N)
     return fail(UASD_ERR_ARG, "segment offsets must start at 0 and end at natom");
@@ -1569,7 +1611,8 @@ int uasd_get_coloring(uasd_ctx *c, int *ncolors, int *color_of_site) {
   if (!c->have_ex) return fail(UASD_ERR_STATE, "exchange not set");
   if (int r = build_coloring(c)) return r;
   if (ncolors) *ncolors = c->ncolors;
-  if (color_of_site) std::copy(c->color_of_site.begin(), c->color_of_site.end(), color_of_site);
+  if (color_of_site)
+    for (int i = 0; i < c->N; ++i) color_of_site[i] = c->color_of_site[slot_of(c, i)];
   return UASD_OK;
 }
 

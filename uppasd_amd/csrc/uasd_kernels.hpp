@@ -78,7 +78,18 @@ struct HamDev {
   int nt;
   const R *__restrict__ ext_u;
   const Quad<R> *__restrict__ ext_q;
+  // Site order on the device: slot d holds reference site
+  // (d % order_nc) * order_na + d / order_nc, i.e. the sites of one basis atom
+  // are contiguous (basis-major) when order_na = NA; order_na = 1 keeps the
+  // reference order.  Counter-based random numbers are keyed by the
+  // reference site, so they do not depend on the order.
+  int order_na, order_nc;
 };
+
+template <typename R>
+__device__ __forceinline__ uint32_t host_site(const HamDev<R> &h, int d) {
+  return (uint32_t)((d % h.order_nc) * h.order_na + d / h.order_nc);
+}
 
 // Block index of this workgroup in a grid laid out so that XCD k (which
 // receives workgroups k, k+8, k+16, ...) works on the k-th contiguous eighth
@@ -586,7 +597,7 @@ __global__ void __launch_bounds__(256)
   const Quad<R> eq = e[q];
   const R m = eq.w;
   R n0, n1, n2;
-  prm.normals(site, ens, (size_t)q, n0, n1, n2);
+  prm.normals((int)host_site(h, site), ens, (size_t)q, n0, n1, n2);
   if (ALG == 5) {
     const R sig = depondt_sigma<R>(lam, prm.dt, (double)m, T, prm.temprescale);
     const R t0 = n0 * sig, t1 = n1 * sig, t2 = n2 * sig;
@@ -794,7 +805,8 @@ __global__ void __launch_bounds__(256) k_mc_colour(HamDev<R> h, MCParams mp, Qua
   const size_t q = (size_t)site * h.M + ens;
   double u0, u1, u2, u3;
   const uint64_t sweep = mp.sweep_dev ? *mp.sweep_dev + mp.sweep : mp.sweep;
-  uniform4(mp.seed, kTagMCU, sweep, (uint32_t)site, (uint32_t)ens, u0, u1, u2, u3);
+  const uint32_t hs = host_site(h, site);
+  uniform4(mp.seed, kTagMCU, sweep, hs, (uint32_t)ens, u0, u1, u2, u3);
   const Quad<R> me = s[q];
   const R x0 = (R)mp.ext[3 * ens], x1 = (R)mp.ext[3 * ens + 1], x2 = (R)mp.ext[3 * ens + 2];
   R o0, o1, o2;
@@ -808,7 +820,7 @@ __global__ void __launch_bounds__(256) k_mc_colour(HamDev<R> h, MCParams mp, Qua
   } else {
     double g0 = 0.0, g1 = 0.0, g2 = 0.0;
     if ((int)floor(3 * u0) == 1)
-      Noise<double>::normal3(mp.seed, kTagMCG, sweep, (uint32_t)site, (uint32_t)ens, g0, g1,
+      Noise<double>::normal3(mp.seed, kTagMCG, sweep, hs, (uint32_t)ens, g0, g1,
                              g2);
     R n0, n1, n2;
     trial_move<R>(me.x, me.y, me.z, mp.delta[ens], u0, u1, u2, g0, g1, g2, n0, n1, n2);

@@ -35,12 +35,17 @@ def device_count() -> int:
 
 
 class Engine:
-    def __init__(self, natom: int, mensemble: int = 1, precision: str = "fp64"):
+    def __init__(self, natom: int, mensemble: int = 1, precision: str = "fp64",
+                 site_order: int = 1):
+        """site_order = NA stores each basis atom's sites contiguously on the
+        device (host arrays keep the reference order; results are the same)."""
         prec = {"fp64": UASD_FP64, "fp32": UASD_FP32}[precision]
         self.N, self.M, self.precision = int(natom), int(mensemble), precision
         h = C.c_void_p()
         B.check(B.lib().uasd_ctx_create(C.byref(h), self.N, self.M, prec), "uasd_ctx_create")
         self._h = h
+        if site_order != 1:
+            B.check(B.lib().uasd_set_site_order(self._h, int(site_order)), "uasd_set_site_order")
 
     # -------------------------------------------------------------- lifetime
     def close(self):

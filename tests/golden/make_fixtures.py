@@ -43,6 +43,7 @@ def main(ref):
         for comp in c["comparisons"]:
             for t in comp["testlist"]:
                 entries.append({"name": c["name"], "outfile": os.path.basename(comp["outfile"]),
+                                "headers": comp.get("headers", []),
                                 "select": t["select"], "extract": t["extract"],
                                 "expected": t["expected"],
                                 "comparison": t.get("comparison_func", "similar"),

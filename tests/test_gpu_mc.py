@@ -129,3 +129,32 @@ def test_bccfe_mc_statistics_vs_reference(mode, label):
     want = g["expected"][:4]
     got = [mean[0], mean[1], mean[2], U4]
     assert np.allclose(got, want, rtol=0.02), (got, want)
+
+
+def test_bccfe_mc_files_through_writer(tmp_path):
+    """The device MC run written by output.MeasurementPhase: cumulant row 41
+    of cumulants.<simid>.out against the reference's bccFe (MC) golden
+    values, 2 % (a statistical comparison: the sweep order and random
+    stream differ from the reference's)."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_output import read_selected
+    from uppasd_amd import output as OUT
+    c = RC.read_case("bccFe", "inpsd.dat.base")
+    lat = U.Lattice(c["cell"], c["bas"], c["atype"], c["n"], c["bc"])
+    sh = U.Shells.from_lines(lat, c["jlines"])
+    nl, ns, nc = U.build_pair_list(lat, sh, c["sym"], c["ammom"].reshape(-1, 1))
+    N, M = lat.natom, 16
+    eng = U.Engine(N, M)
+    eng.set_exchange(nl, ns, nc)
+    e0 = np.ascontiguousarray(np.broadcast_to(np.tile(c["edir"], (N // lat.NA, 1)), (M, N, 3)))
+    eng.set_moments(e0, np.full((M, N), 2.23))
+    temps = np.full(M, 500.0)
+    eng.mc_sweeps("M", temps, 2000, 4242)
+    ph = OUT.MeasurementPhase(eng, "bccFeM", 500.0, avrg_step=50, cumu_step=50,
+                              outdir=str(tmp_path), cumu_buff=10)
+    ph.run(2100, lambda n: eng.mc_sweeps("M", temps, n, 4242))
+    g = [x for x in RC.golden("bccFe") if x["name"] == "bcc Fe (MC)" and x["type"] == "Cumulants"][0]
+    got = read_selected(tmp_path / "cumulants.bccFeM.out", g["headers"], g["select"],
+                        g["extract"][:4])
+    assert np.allclose(got, g["expected"][:4], rtol=0.02), (got, g["expected"])

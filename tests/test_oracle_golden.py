@@ -13,7 +13,7 @@ import pytest
 
 import refcase as RC
 
-MUB, KB = 9.274009994e-24, 1.38064852e-23
+MUB, KB, MRY = 9.274009994e-24, 1.38064852e-23, 2.179872325e-21
 
 
 def _lattice(O, c):
@@ -75,9 +75,10 @@ def test_bccfe_sd_golden(O):
     m = sim.mavg()[0]
     assert np.allclose([*m, np.linalg.norm(m)], exp["Initial phase"]["expected"], atol=1e-8)
     dt = float(np.float32(1.0e-16))
-    w = tot = mm = mm2 = mm4 = 0.0
+    w = tot = mm = mm2 = mm4 = ee = ee2 = 0.0
     ncum, cumrow, m2700 = 0, None, None
     N = lat.natom
+    en = None
     for mstep in range(1, 3001):
         if mstep - 1 == 2700:
             m2700 = sim.mavg()[0].copy()
@@ -87,15 +88,23 @@ def test_bccfe_sd_golden(O):
             mm = (mm * tot + me * w) / (tot + w)
             mm2 = (mm2 * tot + me ** 2 * w) / (tot + w)
             mm4 = (mm4 * tot + (me ** 2) ** 2 * w) / (tot + w)
+            ee = (ee * tot + en * w) / (tot + w)
+            ee2 = (ee2 * tot + en ** 2 * w) / (tot + w)
             tot += w
             ncum += 1
             if ncum == 41:
                 U = 1 - (mm4 / 3 / mm2 ** 2)
                 chi = (mm2 - mm ** 2) * MUB ** 2 * N / KB ** 2 / 500.0
-                cumrow = [mm, mm2, mm4, U, chi]
+                cv = (ee2 - ee ** 2) * MRY ** 2 * N / KB ** 2 / 500.0 ** 2
+                cumrow = [mm, mm2, mm4, U, chi, cv]
+        # calc_energy runs after measure when mod(mstep-1, avrg_step) == 0
+        # (sd_driver.f90:554-575), so a cumulant sample uses the energy of 49
+        # steps before it; energy per atom in mRy (energy.f90:346-372)
+        if (mstep - 1) % 50 == 0:
+            en = sim.ham.field(sim.eM)[1] / N
         step(500.0, dt, 0.5)
     assert np.allclose([*m2700, np.linalg.norm(m2700)], exp["Magnetization"]["expected"], atol=1e-8)
-    want = exp["Cumulants"]["expected"][:5]
+    want = exp["Cumulants"]["expected"][:6]
     assert np.allclose(cumrow, want, rtol=1e-8, atol=1e-12), (cumrow, want)
 
 

@@ -47,11 +47,18 @@ def kernel_bytes(W: int, s: int, ib: int = 4) -> dict:
     return {"predictor": pred, "corrector": corr, "step": pred + corr}
 
 
-def profiled_traffic(kernel: str):
+def profile_path(order: int, ib: int) -> str:
+    """Committed rocprofv3 summary of the FP64 midpoint workload for a device
+    site order and neighbour-entry width."""
+    return os.path.join(ROOT, "profiles",
+                        f"k_llg_fp64_midpoint_bcc64_58nn_order{order}_idx{ib}.md")
+
+
+def profiled_traffic(kernel: str, order: int, ib: int):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary of
-    the default workload (FETCH_SIZE + WRITE_SIZE scaled by a plain copy), or
-    None when the summary does not list that kernel."""
-    path = os.path.join(ROOT, "profiles", "k_llg_fp64_midpoint_bcc64_58nn_ell.md")
+    the same configuration (FETCH_SIZE + WRITE_SIZE scaled by a plain copy),
+    or None when there is none for it."""
+    path = profile_path(order, ib)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -129,14 +136,18 @@ def form_c3(U, CF, steps=5000):
     N = sysd["lattice"].natom
     rng = np.random.default_rng(3)
     e0 = np.ascontiguousarray(CF.random_unit(N, rng, 1))
+    # control: FP64 from a start perturbed at the 1e-7 level, which measures
+    # how far any rounding-level difference carries in this relaxation
+    pert = e0 + 1e-7 * rng.normal(size=e0.shape)
+    pert = np.ascontiguousarray(pert / np.linalg.norm(pert, axis=2, keepdims=True))
     engs = {}
-    for prec in ("fp64", "fp32"):
-        eng = U.Engine(N, 1, prec)
+    for prec in ("fp64", "fp32", "fp64_perturbed"):
+        eng = U.Engine(N, 1, prec[:4])
         eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
         eng.set_dm(sysd["dmlist"], sysd["dmlistsize"], sysd["dm_vect"])
         eng.set_anisotropy(sysd["taniso"], sysd["eaniso"], sysd["kaniso"], sysd["sb"])
         eng.set_external_field(sysd["ext"])
-        eng.set_moments(e0, sysd["mmom"][None])
+        eng.set_moments(pert if prec == "fp64_perturbed" else e0, sysd["mmom"][None])
         eng.set_llg(0.05, 0.0, 1e-16)
         eng.set_noise(U.UASD_NOISE_NONE)
         eng.autotune_llg(5, steps=2)
@@ -163,6 +174,8 @@ def form_c3(U, CF, steps=5000):
     b32, _ = engs["fp32"].effective_field(emomM)
     ferr = float(np.max(np.abs(b32 - b64)) / np.max(np.abs(b64)))
     rel = {c: abs(rec["fp32"][c][0] - rec["fp64"][c][0]) / abs(rec["fp64"][c][0]) for c in checkpoints}
+    relp = {c: abs(rec["fp64_perturbed"][c][0] - rec["fp64"][c][0]) / abs(rec["fp64"][c][0])
+            for c in checkpoints}
     return {
         "workload": f"triangular 512^2 ({N} spins), J+DMI+Kz+Bz, Depondt alpha 0.05, T 0, {steps} steps",
         "fp64_value": N * steps / times["fp64"],
@@ -170,6 +183,7 @@ def form_c3(U, CF, steps=5000):
         "fp32_over_fp64": times["fp64"] / times["fp32"],
         "energy_per_spin_mRy": {str(c): rec["fp64"][c][0] for c in checkpoints},
         "energy_rel_err": {str(c): rel[c] for c in checkpoints},
+        "control_fp64_perturbed_1e-7_energy_rel_err": {str(c): relp[c] for c in checkpoints},
         "field_max_rel_err": ferr,
         "norm_drift_fp32": drift,
         "unit": "spin-updates/s",
@@ -386,7 +400,7 @@ def main():
     value = N * args.steps / elapsed
     traffic = None
     if args.precision == "fp64" and args.L == 64 and args.solver == 1 and not nt:
-        traffic = profiled_traffic(f"uasd::k_llg<double, 1, 1, true, {gather}>")
+        traffic = profiled_traffic(f"uasd::k_llg<double, 1, 1, true, {gather}>", order, ib)
     _, eM = eng.get_moments()
     norm_drift = float(np.max(np.abs(np.linalg.norm(eM[0], axis=1) / sysd["mmom"] - 1.0)))
 

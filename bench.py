@@ -376,7 +376,7 @@ def main():
         eng.autotune_llg(args.solver, steps=3)
     tune_s = time.perf_counter() - t_tune
     gather, block = eng.tuning(eng.TUNE_LLG)
-    nt = block >> 16
+    nt, idx32 = (block >> 16) & 1, (block >> 17) & 1
     block &= 0xFFFF
 
     eng.sd_steps(args.solver, args.warmup)
@@ -425,6 +425,7 @@ def main():
             "spins": N, "neighbours": W, "solver": args.solver, "precision": args.precision,
             "setup_s": round(setup_s, 2),
             "launch": {"gather": gather, "block": block, "nt_index_loads": bool(nt),
+                       "int32_index": bool(idx32),
                        "site_order": order,
                        "autotuned": not args.gather, "tune_s": round(tune_s, 3)},
             "kernel_ms": {"predictor": round(t_pred * 1e3, 4), "corrector": round(t_corr * 1e3, 4)},

@@ -166,9 +166,10 @@ int uasd_set_kernel_timing(uasd_ctx *ctx, int enable);
  * replay (K steps), 5 MC graph replay (K sweeps) */
 int uasd_get_kernel_timing(uasd_ctx *ctx, int kind, int64_t *launches, double *total_ms);
 
-/* Bytes per neighbour-list entry the kernels read (no reference
- * counterpart): 2 when every (neighbour - site) mod Natom difference of the
- * list fits in int16 (periodic lattices numbered cell by cell), else 4. */
+/* Bytes per neighbour-list entry the LLG kernels read under the current
+ * tuning (no reference counterpart): 2 when the list has packed int16
+ * entries (every cell difference fits; periodic lattices numbered cell by
+ * cell) and the tuning uses them, else 4. */
 int uasd_index_bytes(uasd_ctx *ctx, int *exchange, int *dm);
 
 /* ---- launch shapes of the gathering kernels (no reference counterpart:
@@ -176,7 +177,8 @@ int uasd_index_bytes(uasd_ctx *ctx, int *exchange, int *dm);
  * kind: 0 fused LLG step, 1 field, 2 MC colour pass.  gather = neighbours
  * whose loads are issued together (2, 4, 8, 16); block = workgroup size
  * (64, 128, 256), plus 0x10000 to read the neighbour lists with
- * non-temporal loads.  Results do not depend on any of them. */
+ * non-temporal loads and 0x20000 to read int32 entries where packed int16
+ * entries exist.  Results do not depend on any of them. */
 int uasd_set_tuning(uasd_ctx *ctx, int kind, int gather, int block);
 int uasd_get_tuning(uasd_ctx *ctx, int kind, int *gather, int *block);
 /* Autotuner: times every shape on the context's own lattice and state

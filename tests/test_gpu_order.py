@@ -32,6 +32,7 @@ def test_llg_same_in_both_orders(precision, alg):
     out = []
     for order in (1, 2):
         eng = _eng(sysd, M, order, precision)
+        assert eng.index_bytes()[0] == 2      # packed int16 entries in both orders
         eng.set_moments(e0, sysd["mmom"][None])
         eng.set_llg(lam, 300.0, 1e-16)
         eng.set_noise(U.UASD_NOISE_PHILOX, 8)
@@ -52,6 +53,7 @@ def test_dropin_and_kagome_same_in_both_orders():
     res = []
     for order in (1, 3):
         eng = _eng(sysd, M, order)
+        assert eng.index_bytes() == (2, 2)
         b, e, b1, b2 = eng.effective_field(emomM, parts=True)
         eng.set_moments(emom, np.broadcast_to(sysd["mmom"], (M, N)))
         eng.set_llg(0.2, 50.0, 5e-17)

@@ -13,8 +13,9 @@ from uppasd_amd import configs as CF
 
 pytestmark = pytest.mark.gpu
 
-NT = 0x10000  # non-temporal neighbour-list loads
-SHAPES = [(2, 64), (4, 128 | NT), (8, 256), (8, 256 | NT), (16, 64), (16, 256 | NT)]
+NT = 0x10000   # non-temporal neighbour-list loads
+I32 = 0x20000  # int32 entries although packed int16 entries exist
+SHAPES = [(2, 64), (4, 128 | NT), (8, 256), (8, 256 | NT | I32), (16, 64 | I32), (16, 256 | NT)]
 
 
 def _llg_run(sysd, emom, gather, block, precision, alg, steps=20):
@@ -100,7 +101,7 @@ def test_autotune_llg_leaves_trajectory_unchanged(precision):
             ms = eng.autotune_llg(1, steps=2)
             assert ms > 0.0
             g, b = eng.tuning(eng.TUNE_LLG)
-            assert g in (2, 4, 8, 16) and b & 0xFFFF in (64, 128, 256) and b >> 16 in (0, 1)
+            assert g in (2, 4, 8, 16) and b & 0xFFFF in (64, 128, 256) and b >> 16 in (0, 1, 2, 3)
             assert eng.tuning(eng.TUNE_FIELD) == (g, b)
         eng.sd_steps(1, 7)
         return eng.get_moments()[0]

@@ -190,6 +190,14 @@ inline int slot_of(const uasd_ctx *c, int i) {
   return c->order_na == 1 ? i : (i % c->order_na) * c->order_nc + i / c->order_na;
 }
 
+// Bits that hold the basis shift in an int16 list entry.
+int idx_kb(const uasd_ctx *c) {
+  int kb = 0;
+  while ((1 << kb) < c->order_na) ++kb;
+  return kb;
+}
+
+
 // Quad buffers: one Quad<R> = {x, y, z, w} per q = site*M + ens.
 size_t quad_bytes(const uasd_ctx *c) { return (size_t)c->N * c->M * 4 * c->rsize; }
 
@@ -237,6 +245,7 @@ uasd::HamDev<R> ham_view(const uasd_ctx *c) {
   h.classes = c->classes.as<uasd::SiteClass>();
   h.ext_mode = c->ext_mode;
   h.order_na = c->order_na;
+  h.idx_kb = idx_kb(c);
   h.order_nc = c->order_na == 1 ? c->N : c->order_nc;
   h.ext_u = c->ext_u.as<R>();
   if (c->ext_mode == 2) h.ext_q = quads<R>(c, c->ext_full);
@@ -585,17 +594,24 @@ int build_coloring(uasd_ctx *c) {
 
 }  // namespace
 
-// Slot-major int16 differences (n - site) mod N in [-N/2, N/2) of a
-// slot-major 0-based list; false when any does not fit.
-bool diff16(const std::vector<int> &nb, int N, int W, std::vector<int16_t> &out) {
+// Slot-major int16 entries of a slot-major list of device slots (see NbDec):
+// v = delta * 2^kb + beta with beta = (b' - b) mod na and delta = (c' - c)
+// mod nc taken in [-nc/2, nc/2).  False when any delta does not fit.
+bool diff16(const uasd_ctx *c, const std::vector<int> &nb, int N, int W,
+            std::vector<int16_t> &out) {
+  const int na = c->order_na, nc = na == 1 ? N : c->order_nc, kb = idx_kb(c);
+  const long lim = 1L << (15 - kb);
   out.resize((size_t)W * N);
   for (int j = 0; j < W; ++j)
-    for (int i = 0; i < N; ++i) {
-      long d = (long)nb[(size_t)j * N + i] - i;
-      if (2 * d >= N) d -= N;
-      if (2 * d < -(long)N) d += N;
-      if (d < -32768 || d > 32767) return false;
-      out[(size_t)j * N + i] = (int16_t)d;
+    for (int d = 0; d < N; ++d) {
+      const int n = nb[(size_t)j * N + d];
+      const int b = d / nc, cc = d - b * nc, b2 = n / nc, c2 = n - b2 * nc;
+      const int beta = ((b2 - b) % na + na) % na;
+      long delta = (long)c2 - cc;
+      if (2 * delta >= nc) delta -= nc;
+      if (2 * delta < -(long)nc) delta += nc;
+      if (delta < -lim || delta >= lim) return false;
+      out[(size_t)j * N + d] = (int16_t)(delta * (1L << kb) + beta);
     }
   return true;
 }
@@ -768,7 +784,7 @@ int uasd_set_exchange(uasd_ctx *c, int mnn, int nham, const int *nlist, const in
   c->nbr_host = nb;
   if (int r = upload_raw(c, c->nbr, nb)) return r;
   std::vector<int16_t> d16;
-  c->use16 = W > 0 && diff16(nb, N, W, d16);
+  c->use16 = W > 0 && diff16(c, nb, N, W, d16);
   if (c->use16) {
     if (int r = upload_raw(c, c->nbr16, d16)) return r;
   } else {
@@ -833,7 +849,7 @@ int uasd_set_dm(uasd_ctx *c, int mdm, int nham, const int *dmlist, const int *dm
   c->dnbr_host = nb;
   if (int r = upload_raw(c, c->dnbr, nb)) return r;
   std::vector<int16_t> d16;
-  c->use16_dm = W > 0 && diff16(nb, N, W, d16);
+  c->use16_dm = W > 0 && diff16(c, nb, N, W, d16);
   if (c->use16_dm) {
     if (int r = upload_raw(c, c->dnbr16, d16)) return r;
   } else {
@@ -1354,8 +1370,9 @@ int uasd_mc_sweeps(uasd_ctx *c, char mode, const double *temps, int64_t nsweeps,
 // ---------------------------------------------------------------- tuning
 int uasd_index_bytes(uasd_ctx *c, int *exchange, int *dm) {
   if (int r = check_ctx(c)) return r;
-  if (exchange) *exchange = c->use16 ? 2 : 4;
-  if (dm) *dm = c->use16_dm ? 2 : 4;
+  const bool i32 = c->tune[0].idx32 != 0;  // as the LLG kernels are tuned
+  if (exchange) *exchange = c->use16 && !i32 ? 2 : 4;
+  if (dm) *dm = c->use16_dm && !i32 ? 2 : 4;
   return UASD_OK;
 }
 
@@ -1363,9 +1380,9 @@ int uasd_set_tuning(uasd_ctx *c, int kind, int gather, int block) {
   if (int r = check_ctx(c)) return r;
   c->gen += 1;
   if (kind < 0 || kind > 2) return fail(UASD_ERR_ARG, "kind must be 0 (LLG), 1 (field) or 2 (MC)");
-  // block may carry the non-temporal flag in bit 16 (block | 0x10000)
-  uasd::Tune t{gather, block & 0xffff, (block >> 16) & 1};
-  if (!uasd::tune_valid(t) || (block >> 17) != 0)
+  // block may carry flags: 0x10000 non-temporal list loads, 0x20000 int32 entries
+  uasd::Tune t{gather, block & 0xffff, (block >> 16) & 1, (block >> 17) & 1};
+  if (!uasd::tune_valid(t) || (block >> 18) != 0)
     return fail(UASD_ERR_ARG, "gather must be 2, 4, 8 or 16 and block 64, 128 or 256");
   c->tune[kind] = t;
   return UASD_OK;
@@ -1375,7 +1392,8 @@ int uasd_get_tuning(uasd_ctx *c, int kind, int *gather, int *block) {
   if (int r = check_ctx(c)) return r;
   if (kind < 0 || kind > 2) return fail(UASD_ERR_ARG, "kind must be 0 (LLG), 1 (field) or 2 (MC)");
   if (gather) *gather = c->tune[kind].gather;
-  if (block) *block = c->tune[kind].block | (c->tune[kind].nt << 16);
+  if (block)
+    *block = c->tune[kind].block | (c->tune[kind].nt << 16) | (c->tune[kind].idx32 << 17);
   return UASD_OK;
 }
 
@@ -1428,10 +1446,12 @@ int tune_search(uasd_ctx *c, uasd::Tune *slot, Run &&run, double *best_ms) {
   const int blocks[] = {64, 128, 256};
   uasd::Tune best = *slot;
   float best_t = 1e30f;
+  const int nidx = c->use16 || c->use16_dm ? 2 : 1;
+  for (int ix = 0; ix < nidx; ++ix)
   for (int nt = 0; nt < 2; ++nt)
   for (int g : gathers)
     for (int bl : blocks) {
-      *slot = uasd::Tune{g, bl, nt};
+      *slot = uasd::Tune{g, bl, nt, ix};
       if (int r = run()) return r;
       HIP_TRY(hipEventRecord(ev.a, c->stream));
       if (int r = run()) return r;

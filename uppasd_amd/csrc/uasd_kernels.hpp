@@ -84,6 +84,7 @@ struct HamDev {
   // reference order.  Counter-based random numbers are keyed by the
   // reference site, so they do not depend on the order.
   int order_na, order_nc;
+  int idx_kb;  // low bits of an int16 list entry holding the basis shift
 };
 
 template <typename R>
@@ -222,22 +223,39 @@ __device__ __forceinline__ void cubic(const HamDev<R> &h, int row, R m0, R m1, R
 
 // Field parts of one site: s = beff_s (pair terms + uniaxial), qf = beff_q
 // (type-7 cubic part times sb), x = external field.
-// Neighbour index from a slot-major list entry: int32 indices as stored, or
-// int16 differences d = (n - site) mod N in [-N/2, N/2), used when every
-// difference fits (a periodic lattice numbered cell by cell keeps them within
-// a few planes of cells), which halves the index stream.
-__device__ __forceinline__ int nb_index(int v, int, int) { return v; }
+// Decodes the slot-major list entries of one site (device slot d = b*nc + c,
+// basis b, cell c).  int32 entries are slots.  int16 entries, used when every
+// entry of the list fits, pack the cell difference and the basis shift of
+// the neighbour, v = delta * 2^kb + beta: the neighbour is slot
+// ((b + beta) mod na)*nc + ((c + delta) mod nc).  With the reference order
+// (na = 1, nc = N, kb = 0) that is the difference (n - site) mod N.  A
+// periodic lattice numbered cell by cell keeps delta within a few planes of
+// cells, so the index stream halves.
+struct NbDec {
+  int b, c, nc, na, kb;
+  template <typename R>
+  __device__ __forceinline__ NbDec(const HamDev<R> &h, int d)
+      : nc(h.order_nc), na(h.order_na), kb(h.idx_kb) {
+    b = d / nc;
+    c = d - b * nc;
+  }
+  __device__ __forceinline__ int operator()(int v) const { return v; }
+  __device__ __forceinline__ int operator()(int16_t v) const {
+    const int beta = (int)v & ((1 << kb) - 1);
+    const int delta = (int)v >> kb;
+    int b2 = b + beta;
+    b2 -= b2 >= na ? na : 0;
+    int c2 = c + delta;
+    c2 += c2 < 0 ? nc : 0;
+    c2 -= c2 >= nc ? nc : 0;
+    return b2 * nc + c2;
+  }
+};
 
 template <bool NT, typename I>
 __device__ __forceinline__ I nb_load(const I *p) {
   if constexpr (NT) return __builtin_nontemporal_load(p);
   return *p;
-}
-__device__ __forceinline__ int nb_index(int16_t v, int site, int N) {
-  int n = site + (int)v;
-  n += n < 0 ? N : 0;
-  n -= n >= N ? N : 0;
-  return n;
 }
 
 // Exchange sum of site_field.  Neighbours are taken G at a time: all index
@@ -249,6 +267,7 @@ __device__ __forceinline__ void exchange_sum(const HamDev<R> &h, const I *__rest
                                              const Quad<R> *__restrict__ src, int site, int ens,
                                              int row, R &s0, R &s1, R &s2) {
   const int M = h.M, N = h.N;
+  const NbDec dec(h, site);
   const int len = h.ex_len[row];
   const R *cp = h.cpl + row;
   const size_t rs = (size_t)h.ex_rows;
@@ -256,7 +275,7 @@ __device__ __forceinline__ void exchange_sum(const HamDev<R> &h, const I *__rest
   for (; j + G <= len; j += G) {
     int n[G];
 #pragma unroll
-    for (int u = 0; u < G; ++u) n[u] = nb_index(nb_load<NT>(nb + (size_t)(j + u) * N), site, N);
+    for (int u = 0; u < G; ++u) n[u] = dec(nb_load<NT>(nb + (size_t)(j + u) * N));
     Quad<R> v[G];
     R J[G];
 #pragma unroll
@@ -275,7 +294,7 @@ __device__ __forceinline__ void exchange_sum(const HamDev<R> &h, const I *__rest
     int n[G];
 #pragma unroll
     for (int u = 0; u < G; ++u)
-      n[u] = j + u < len ? nb_index(nb_load<NT>(nb + (size_t)(j + u) * N), site, N) : site;
+      n[u] = j + u < len ? dec(nb_load<NT>(nb + (size_t)(j + u) * N)) : site;
     Quad<R> v[G];
     R J[G];
 #pragma unroll
@@ -302,13 +321,14 @@ __device__ __forceinline__ void dm_sum(const HamDev<R> &h, const I *__restrict__
                                        const Quad<R> *__restrict__ src, int site, int ens,
                                        int row, R &s0, R &s1, R &s2) {
   const int M = h.M, N = h.N;
+  const NbDec dec(h, site);
   const int len = h.dm_len[row];
   const size_t rs = (size_t)h.dm_rows;
   for (int j = 0; j < len; j += G) {
     int n[G];
 #pragma unroll
     for (int u = 0; u < G; ++u)
-      n[u] = j + u < len ? nb_index(nb[(size_t)(j + u) * N], site, N) : site;
+      n[u] = j + u < len ? dec(nb[(size_t)(j + u) * N]) : site;
     Quad<R> v[G];
 #pragma unroll
     for (int u = 0; u < G; ++u)
@@ -697,7 +717,7 @@ __device__ __forceinline__ R mc_delta_e(const HamDev<R> &h, const Quad<R> *__res
 #pragma unroll
       for (int u = 0; u < G; ++u)
         nb[u] = j + u >= len ? site
-                : h.nbr16 ? nb_index(h.nbr16[(size_t)(j + u) * N + site], site, N)
+                : h.nbr16 ? NbDec(h, site)(h.nbr16[(size_t)(j + u) * N + site])
                           : h.nbr[(size_t)(j + u) * N + site];
       Quad<R> v[G];
       R J[G];
@@ -746,7 +766,7 @@ __device__ __forceinline__ R mc_delta_e(const HamDev<R> &h, const Quad<R> *__res
     const R u0 = me.x, u2 = me.z;  // unit-vector terms as in montecarlo_common.f90:612,615
     const int len = h.dm_len[sc.dm_row];
     for (int j = 0; j < len; ++j) {
-      const int nb = h.dnbr16 ? nb_index(h.dnbr16[(size_t)j * N + site], site, N)
+      const int nb = h.dnbr16 ? NbDec(h, site)(h.dnbr16[(size_t)j * N + site])
                               : h.dnbr[(size_t)j * N + site];
       const size_t r = (size_t)j * h.dm_rows + sc.dm_row;
       const R D0 = h.dvx[r], D1 = h.dvy[r], D2 = h.dvz[r];

@@ -33,8 +33,7 @@ hipError_t launch_field(hipStream_t s, const Tune &t, const HamDev<R> &h, const 
   const int64_t n = (int64_t)h.N * h.M;
   if (n == 0) return hipSuccess;
   if (!tune_valid(t)) return hipErrorInvalidValue;
-  HamDev<R> hh = h;
-  hh.nt = t.nt;
+  const HamDev<R> hh = tuned(h, t);
   return with_gather(t, [&](auto g) {
     hipLaunchKernelGGL((k_field<R, decltype(g)::value>), dim3(grid_for(n, t.block)),
                        dim3(t.block), 0, s, hh, src, beff, en, parts);
@@ -49,8 +48,7 @@ static hipError_t llg_one(hipStream_t s, const Tune &t, const HamDev<R> &h,
   const int64_t n = (int64_t)h.N * h.M;
   if (n == 0) return hipSuccess;
   const dim3 grid(grid_for(n, t.block)), block(t.block);
-  HamDev<R> hh = h;
-  hh.nt = t.nt;
+  const HamDev<R> hh = tuned(h, t);
   if (!FUSED) {  // no gather: one instance
     hipLaunchKernelGGL((k_llg<R, ALG, HALF, false, 4>), grid, block, 0, s, hh, prm, e, p, b2,
                        beff, out);
@@ -89,8 +87,7 @@ hipError_t launch_mc(hipStream_t s, const Tune &t, const HamDev<R> &h, const MCP
   const int64_t n = (int64_t)mp.nsites * h.M;
   if (n == 0) return hipSuccess;
   if (!tune_valid(t)) return hipErrorInvalidValue;
-  HamDev<R> hh = h;
-  hh.nt = t.nt;
+  const HamDev<R> hh = tuned(h, t);
   return with_gather(t, [&](auto g) {
     hipLaunchKernelGGL((k_mc_colour<R, decltype(g)::value>), dim3(grid_for(n, t.block)),
                        dim3(t.block), 0, s, hh, mp, spins);

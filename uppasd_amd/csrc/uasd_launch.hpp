@@ -13,15 +13,30 @@ constexpr int kBlock = 256;  // largest workgroup the kernels are built for
 //   gather  neighbours whose loads are issued together: 2, 4, 8 or 16
 //   block   workgroup size: 64, 128 or 256
 //   nt      1: neighbour lists read with non-temporal loads
+//   idx32   1: int32 list entries even where packed int16 entries exist
 struct Tune {
   int gather = 4;
   int block = 256;
   int nt = 0;
+  int idx32 = 0;
 };
 
 inline bool tune_valid(const Tune &t) {
   return (t.gather == 2 || t.gather == 4 || t.gather == 8 || t.gather == 16) &&
-         (t.block == 64 || t.block == 128 || t.block == 256) && (t.nt == 0 || t.nt == 1);
+         (t.block == 64 || t.block == 128 || t.block == 256) && (t.nt == 0 || t.nt == 1) &&
+         (t.idx32 == 0 || t.idx32 == 1);
+}
+
+// The Hamiltonian view a launch uses under a tuning choice.
+template <typename R>
+inline HamDev<R> tuned(const HamDev<R> &h, const Tune &t) {
+  HamDev<R> hh = h;
+  hh.nt = t.nt;
+  if (t.idx32) {
+    hh.nbr16 = nullptr;
+    hh.dnbr16 = nullptr;
+  }
+  return hh;
 }
 
 // parts (nullable) receives beff1 then beff2, N*M quads each.

@@ -36,7 +36,8 @@ def kernel_bytes(W: int, s: int, ib: int = 4) -> dict:
     """Algorithmic HBM bytes per site of the fused midpoint kernels.
 
     Spins are stored as quads {x, y, z, mmom} of s-byte reals; neighbour
-    entries take ib bytes (4 for int32 indices, 2 for int16 differences).
+    entries take ib bytes (4 for int32 indices, 2 for int16 differences, 0 for
+    the LDS-tiled stencil kernels, which read no lists).
     predictor: gather e_j quads (4s), index stream ib*W, class id 4, write p (4s)
     corrector: gather p_j quads (4s), own e (4s), index ib*W, class id 4, write e (4s)
     Each site's own spin is one of its neighbours' gathers, so the gathered
@@ -47,18 +48,18 @@ def kernel_bytes(W: int, s: int, ib: int = 4) -> dict:
     return {"predictor": pred, "corrector": corr, "step": pred + corr}
 
 
-def profile_path(order: int, ib: int) -> str:
+def profile_path(order: int, variant: str) -> str:
     """Committed rocprofv3 summary of the FP64 midpoint workload for a device
-    site order and neighbour-entry width."""
+    site order and kernel variant (idx4 / idx2 list gathers, tileK stencil)."""
     return os.path.join(ROOT, "profiles",
-                        f"k_llg_fp64_midpoint_bcc64_58nn_order{order}_idx{ib}.md")
+                        f"k_llg_fp64_midpoint_bcc64_58nn_order{order}_{variant}.md")
 
 
-def profiled_traffic(kernel: str, order: int, ib: int):
+def profiled_traffic(kernel: str, order: int, variant: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary of
     the same configuration (FETCH_SIZE + WRITE_SIZE scaled by a plain copy),
     or None when there is none for it."""
-    path = profile_path(order, ib)
+    path = profile_path(order, variant)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -338,6 +339,8 @@ def main():
                     help="fixed gather batch (2/4/8/16) instead of autotuning")
     ap.add_argument("--block", type=int, default=256,
                     help="workgroup size with --gather; add 65536 for non-temporal index loads")
+    ap.add_argument("--no-stencil", action="store_true",
+                    help="do not declare the lattice (list-gather kernels only)")
     ap.add_argument("--site-order", type=int, default=0,
                     help="device site order: 0 = basis-major (NA of the lattice), 1 = reference")
     ap.add_argument("--forms", default="c3,c4,c5",
@@ -360,6 +363,9 @@ def main():
     W = int(sysd["nlistsize"].max())
     order = sysd["lattice"].NA if args.site_order == 0 else args.site_order
     eng = U.Engine(N, 1, args.precision, site_order=order)
+    lat = sysd["lattice"]
+    if not args.no_stencil:
+        eng.set_lattice_dims(lat.NA, *lat.n)     # enables the LDS-tiled stencil kernels
     eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
     rng = np.random.default_rng(2)
     e0 = rng.normal(size=(1, N, 3))
@@ -376,7 +382,7 @@ def main():
         eng.autotune_llg(args.solver, steps=3)
     tune_s = time.perf_counter() - t_tune
     gather, block = eng.tuning(eng.TUNE_LLG)
-    nt, idx32 = (block >> 16) & 1, (block >> 17) & 1
+    nt, idx32, tile = (block >> 16) & 1, (block >> 17) & 1, (block >> 18) & 0xF
     block &= 0xFFFF
 
     eng.sd_steps(args.solver, args.warmup)
@@ -392,7 +398,7 @@ def main():
     eng.set_kernel_timing(False)
 
     s = 8 if args.precision == "fp64" else 4
-    ib = eng.index_bytes()[0]
+    ib = 0 if tile else eng.index_bytes()[0]     # the tile kernel reads no lists
     kb = kernel_bytes(W, s, ib)
     t_corr = ms_corr / max(n_corr, 1) * 1e-3
     t_pred = ms_pred / max(n_pred, 1) * 1e-3
@@ -400,7 +406,11 @@ def main():
     value = N * args.steps / elapsed
     traffic = None
     if args.precision == "fp64" and args.L == 64 and args.solver == 1 and not nt:
-        traffic = profiled_traffic(f"uasd::k_llg<double, 1, 1, true, {gather}>", order, ib)
+        if tile:
+            traffic = profiled_traffic("uasd::k_llg_tile<double, 1, 1>", order, f"tile{tile}")
+        else:
+            traffic = profiled_traffic(f"uasd::k_llg<double, 1, 1, true, {gather}>", order,
+                                       f"idx{ib}")
     _, eM = eng.get_moments()
     norm_drift = float(np.max(np.abs(np.linalg.norm(eM[0], axis=1) / sysd["mmom"] - 1.0)))
 
@@ -426,6 +436,7 @@ def main():
             "setup_s": round(setup_s, 2),
             "launch": {"gather": gather, "block": block, "nt_index_loads": bool(nt),
                        "int32_index": bool(idx32),
+                       "tile_shape": U.engine.TILE_SHAPES[tile - 1] if tile else None,
                        "site_order": order,
                        "autotuned": not args.gather, "tune_s": round(tune_s, 3)},
             "kernel_ms": {"predictor": round(t_pred * 1e3, 4), "corrector": round(t_corr * 1e3, 4)},

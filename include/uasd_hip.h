@@ -74,6 +74,16 @@ int uasd_ctx_sync(uasd_ctx *ctx);
  * uploads; not combined with several segments.  Default 1 (reference order). */
 int uasd_set_site_order(uasd_ctx *ctx, int na);
 
+/* Lattice dimensions (the NA, N1, N2, N3 arguments of effective_field_full,
+ * hamiltonianactions.f90:108-112): reference site i = b + NA*(x + N1*(y +
+ * N2*z)).  When the exchange (and DM) lists are one translation-invariant
+ * stencil per basis atom over the periodic lattice, the fused LLG kernels can
+ * gather from an LDS-staged tile of cells (a tuning choice, same results).
+ * All zero forgets the lattice.  has_stencil reports whether the lists are
+ * such a stencil and its reach in cells (x, y, z). */
+int uasd_set_lattice_dims(uasd_ctx *ctx, int na, int n1, int n2, int n3);
+int uasd_has_stencil(uasd_ctx *ctx, int *has, int *radius3);
+
 /* ---- Hamiltonian: the ham_t arrays (hamiltoniandatatype.f90:14-100) ---- */
 /* aham (Natom) maps site -> Hamiltonian row (ham%aHam); NULL means row = site.
  * Rows with identical couplings are stored once on the device. */
@@ -177,8 +187,10 @@ int uasd_index_bytes(uasd_ctx *ctx, int *exchange, int *dm);
  * kind: 0 fused LLG step, 1 field, 2 MC colour pass.  gather = neighbours
  * whose loads are issued together (2, 4, 8, 16); block = workgroup size
  * (64, 128, 256), plus 0x10000 to read the neighbour lists with
- * non-temporal loads and 0x20000 to read int32 entries where packed int16
- * entries exist.  Results do not depend on any of them. */
+ * non-temporal loads, 0x20000 to read int32 entries where packed int16
+ * entries exist, and (k << 18) for the LDS-tiled stencil kernel with tile
+ * shape k = 1..12 (kind 0 only; needs uasd_set_lattice_dims and a stencil).
+ * Results do not depend on any of them. */
 int uasd_set_tuning(uasd_ctx *ctx, int kind, int gather, int block);
 int uasd_get_tuning(uasd_ctx *ctx, int kind, int *gather, int *block);
 /* Autotuner: times every shape on the context's own lattice and state

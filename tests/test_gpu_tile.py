@@ -1,0 +1,97 @@
+"""LDS-tiled stencil kernels: identical results to the list-gather kernels for
+every tile shape that fits, in both device site orders, for 3D and 2D
+lattices with exchange, DMI, anisotropy and Zeeman terms, with halos that
+wrap around small lattices and tiles that overhang the lattice edge; lists
+that are not a stencil (vacuum boundaries) are detected and keep the list
+kernels."""
+import numpy as np
+import pytest
+
+import uppasd_amd as U
+from uppasd_amd import configs as CF
+
+pytestmark = pytest.mark.gpu
+
+TILE = lambda k: 256 | (k << 18)
+
+
+def _eng(sysd, order, precision="fp64"):
+    lat = sysd["lattice"]
+    eng = U.Engine(lat.natom, 1, precision, site_order=order)
+    eng.set_lattice_dims(lat.NA, *lat.n)
+    eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
+    if "dmlist" in sysd:
+        eng.set_dm(sysd["dmlist"], sysd["dmlistsize"], sysd["dm_vect"])
+    if "taniso" in sysd:
+        eng.set_anisotropy(sysd["taniso"], sysd["eaniso"], sysd["kaniso"], sysd["sb"])
+    if "ext" in sysd:
+        eng.set_external_field(sysd["ext"])
+    return eng
+
+
+def _run(sysd, e0, order, tile, alg, precision="fp64", steps=12):
+    eng = _eng(sysd, order, precision)
+    eng.set_moments(e0, sysd["mmom"][None])
+    eng.set_llg(0.1, 200.0, 1e-16)
+    eng.set_noise(U.UASD_NOISE_PHILOX, 17)
+    eng.set_tuning(eng.TUNE_LLG, 4, TILE(tile) if tile else 256)
+    eng.sd_steps(alg, steps)
+    return eng.get_moments()[0], eng
+
+
+CASES = {
+    "bcc5_L5": lambda: CF.bcc_shells(5, CF.bcc5_couplings()),      # halo wraps (R=1, L=5)
+    "bcc5_L10": lambda: CF.bcc_shells(10, CF.bcc5_couplings()),    # partial tiles
+    "tri_dmi": lambda: CF.triangular_dmi(20),                      # 2D, DM + K + B
+    "kagome": lambda: CF.kagome_afm(9, J=-1.0, Dz=0.08),           # NA = 3, DM
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("alg", [1, 5])
+def test_tile_equals_list_kernels(case, alg):
+    sysd = CASES[case]()
+    lat = sysd["lattice"]
+    rng = np.random.default_rng(90)
+    e0 = np.ascontiguousarray(CF.random_unit(lat.natom, rng, 1))
+    ref, eng = _run(sysd, e0, 1, 0, alg)
+    has, radius = eng.stencil()
+    assert has, case
+    shapes = [k for k, sh in enumerate(U.engine.TILE_SHAPES, 1) if (lat.n[2] > 1) == (sh[2] > 1)]
+    for order in (1, lat.NA):
+        for k in shapes:
+            got, _ = _run(sysd, e0, order, k, alg)
+            assert np.array_equal(got, ref), (case, order, k)
+
+
+def test_tile_fp32_equals_list_fp32():
+    sysd = CF.triangular_dmi(24)
+    rng = np.random.default_rng(91)
+    e0 = np.ascontiguousarray(CF.random_unit(sysd["lattice"].natom, rng, 1))
+    ref, _ = _run(sysd, e0, 1, 0, 5, "fp32")
+    got, _ = _run(sysd, e0, 1, 5, 5, "fp32")
+    assert np.array_equal(got, ref)
+
+
+def test_no_stencil_with_vacuum_boundary():
+    sysd = CF.bcc_shells(6, CF.bcc5_couplings(), bc=("P", "P", "0"))
+    rng = np.random.default_rng(92)
+    e0 = np.ascontiguousarray(CF.random_unit(sysd["lattice"].natom, rng, 1))
+    ref, eng = _run(sysd, e0, 1, 0, 1)
+    assert not eng.stencil()[0]
+    got, _ = _run(sysd, e0, 1, 1, 1)   # tile requested: falls back to the list kernels
+    assert np.array_equal(got, ref)
+
+
+def test_autotune_may_pick_tile_and_keeps_results():
+    sysd = CF.bcc_shells(16, CF.bcc5_couplings())
+    rng = np.random.default_rng(93)
+    e0 = np.ascontiguousarray(CF.random_unit(sysd["lattice"].natom, rng, 1))
+    ref, _ = _run(sysd, e0, 2, 0, 1, steps=6)
+    eng = _eng(sysd, 2)
+    eng.set_moments(e0, sysd["mmom"][None])
+    eng.set_llg(0.1, 200.0, 1e-16)
+    eng.set_noise(U.UASD_NOISE_PHILOX, 17)
+    eng.autotune_llg(1, steps=2)
+    eng.sd_steps(1, 6)
+    assert np.array_equal(eng.get_moments()[0], ref)

@@ -33,7 +33,7 @@ EXPORTS = (
     "uasd_set_tuning", "uasd_get_tuning", "uasd_autotune_llg", "uasd_autotune_mc",
     "uasd_set_temprescale", "uasd_set_segments", "uasd_measure_segments", "uasd_accumulate",
     "uasd_get_accumulators", "uasd_reset_accumulators", "uasd_set_graph_steps",
-    "uasd_index_bytes", "uasd_set_site_order",
+    "uasd_index_bytes", "uasd_set_site_order", "uasd_set_lattice_dims", "uasd_has_stencil",
     "uasd_build_pair_list",
 )
 
@@ -92,6 +92,8 @@ def lib() -> C.CDLL:
         L.uasd_set_graph_steps.argtypes = [vp, C.c_int]
         L.uasd_index_bytes.argtypes = [vp, ip, ip]
         L.uasd_set_site_order.argtypes = [vp, C.c_int]
+        L.uasd_set_lattice_dims.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.uasd_has_stencil.argtypes = [vp, ip, vp]
         L.uasd_get_tuning.argtypes = [vp, C.c_int, ip, ip]
         L.uasd_autotune_llg.argtypes = [vp, C.c_int, C.c_int, dp]
         L.uasd_autotune_mc.argtypes = [vp, C.c_char, vp, C.c_int, dp]

@@ -137,6 +137,22 @@ struct uasd_ctx {
   // launch shapes of the gathering kernels: 0 fused LLG, 1 field, 2 MC
   uasd::Tune tune[3];
 
+  // lattice dimensions (reference site i = b + NA*(x + N1*(y + N2*z))) and
+  // the translation-invariant stencil of the lists, when there is one
+  int lat_na = 0, lat_n1 = 0, lat_n2 = 0, lat_n3 = 0;
+  struct Stencil {
+    bool ok = false;
+    int W = 0, rx = 0, ry = 0, rz = 0;
+    std::vector<int> bp, dx, dy, dz;  // (NA*W)
+  } st_ex, st_dm;
+  bool stencil_ok = false, stencil_dirty = true;
+  DevBuf tile_off;
+  int tile_off_shape = 0;
+  std::vector<int> cls_host;       // site class per device slot
+  int64_t cls_version = 0;         // bumped by every class rebuild
+  int64_t ucls_version = -1;       // cls_version the per-basis check below is for
+  int ucls = 0, cls_b[8] = {0};
+
   // segments (replicas of a batch): site ranges, their chunks, accumulators
   int nseg = 0;
   std::vector<int> seg_off;  // (nseg+1)
@@ -177,6 +193,10 @@ struct uasd_ctx {
   std::vector<Pending> pending;
   hipEvent_t cur = nullptr;
 };
+
+void build_stencil(uasd_ctx *c);
+bool tile_fits(const uasd_ctx *c, int k);
+int make_tile(uasd_ctx *c, int k, uasd::TileDev &t);
 
 namespace {
 
@@ -360,6 +380,8 @@ int rebuild_classes(uasd_ctx *c) {
   if (r) return r;
   r = upload_raw(c, c->classes, table);
   if (r) return r;
+  c->cls_host = cls;
+  c->cls_version += 1;
   c->classes_dirty = false;
   c->gen += 1;
   return UASD_OK;
@@ -497,6 +519,16 @@ int run_llg_half(uasd_ctx *c, int alg, int half, bool fused, bool write_e,
                 *b2 = quads<R>(c, c->spins, 2);
   const uasd::Quad<R> *beff = fused ? nullptr : quads<R>(c, c->field);
   uasd::Quad<R> *out = write_e ? e : p;
+  if (fused && c->tune[0].tile > 0) {
+    build_stencil(c);
+    if (tile_fits(c, c->tune[0].tile)) {
+      uasd::TileDev t;
+      if (int r = make_tile(c, c->tune[0].tile, t)) return r;
+      tic(c);
+      HIP_TRY(uasd::launch_llg_tile<R>(c->stream, t, alg, half, h, prm, e, p, b2, out));
+      return toc(c, half);
+    }
+  }
   tic(c);
   HIP_TRY(uasd::launch_llg<R>(c->stream, c->tune[0], alg, half, fused, h, prm, e, p, b2, beff,
                               out));
@@ -614,6 +646,133 @@ bool diff16(const uasd_ctx *c, const std::vector<int> &nb, int N, int W,
       out[(size_t)j * N + d] = (int16_t)(delta * (1L << kb) + beta);
     }
   return true;
+}
+
+// ------------------------------------------------------------ stencil
+// True when the list (device order) is one translation of a per-basis
+// stencil over the periodic lattice; fills st.
+bool detect_stencil(const uasd_ctx *c, const std::vector<int> &nb, const std::vector<int> &row,
+                    const std::vector<int> &len, int W, uasd_ctx::Stencil &st) {
+  const int NA = c->lat_na, N1 = c->lat_n1, N2 = c->lat_n2, N3 = c->lat_n3, N = c->N;
+  st = uasd_ctx::Stencil{};
+  if (NA <= 0 || W <= 0 || row.empty()) return false;
+  st.W = W;
+  st.bp.assign((size_t)NA * W, 0);
+  st.dx = st.dy = st.dz = st.bp;
+  auto wrap = [](int d, int n) { return 2 * d > n ? d - n : d; };
+  std::vector<int> L(NA);
+  for (int b = 0; b < NA; ++b) {
+    const int d0 = slot_of(c, b);
+    L[b] = len[row[d0]];
+    for (int j = 0; j < L[b]; ++j) {
+      const int n = site_of(c, nb[(size_t)j * N + d0]);
+      const int bb = n % NA, cell = n / NA;
+      const int x = cell % N1, y = (cell / N1) % N2, z = cell / (N1 * N2);
+      st.bp[b * W + j] = bb;
+      st.dx[b * W + j] = wrap(x, N1);
+      st.dy[b * W + j] = wrap(y, N2);
+      st.dz[b * W + j] = wrap(z, N3);
+    }
+  }
+  for (int i = 0; i < N; ++i) {
+    const int b = i % NA, cell = i / NA;
+    const int x = cell % N1, y = (cell / N1) % N2, z = cell / (N1 * N2);
+    const int d = slot_of(c, i);
+    if (len[row[d]] != L[b]) return false;
+    for (int j = 0; j < L[b]; ++j) {
+      const int k = b * W + j;
+      const int xx = ((x + st.dx[k]) % N1 + N1) % N1, yy = ((y + st.dy[k]) % N2 + N2) % N2,
+                zz = ((z + st.dz[k]) % N3 + N3) % N3;
+      const int want = st.bp[k] + NA * (xx + N1 * (yy + N2 * zz));
+      if (site_of(c, nb[(size_t)j * N + d]) != want) return false;
+    }
+  }
+  for (size_t k = 0; k < st.bp.size(); ++k) {
+    st.rx = std::max(st.rx, std::abs(st.dx[k]));
+    st.ry = std::max(st.ry, std::abs(st.dy[k]));
+    st.rz = std::max(st.rz, std::abs(st.dz[k]));
+  }
+  st.ok = true;
+  return true;
+}
+
+void build_stencil(uasd_ctx *c) {
+  if (!c->stencil_dirty) return;
+  c->stencil_dirty = false;
+  c->stencil_ok = false;
+  if (c->lat_na <= 0 || !c->have_ex) return;
+  if (!detect_stencil(c, c->nbr_host, c->ex_row_of_site, c->ex_len_host, c->W, c->st_ex)) return;
+  if (c->Wd > 0 && !detect_stencil(c, c->dnbr_host, c->dm_row_of_site, c->dm_len_host, c->Wd,
+                                   c->st_dm))
+    return;
+  c->stencil_ok = true;
+}
+
+// Tile shape k (1-based) usable for this lattice and precision?
+bool tile_fits(const uasd_ctx *c, int k) {
+  if (!c->stencil_ok || c->M != 1 || k < 1 || k > uasd::kNumTileShapes) return false;
+  if (c->order_na != 1 && c->order_na != c->lat_na) return false;  // tile kernel's slot formula
+  const int *sh = uasd::kTileShapes[k - 1];
+  if ((c->lat_n3 > 1) != (sh[2] > 1)) return false;
+  const int rx = std::max(c->st_ex.rx, c->st_dm.rx), ry = std::max(c->st_ex.ry, c->st_dm.ry),
+            rz = std::max(c->st_ex.rz, c->st_dm.rz);
+  const int hv = (sh[0] + 2 * rx) * (sh[1] + 2 * ry) * (sh[2] + 2 * rz);
+  return uasd::tile_lds_bytes(hv, c->lat_na, c->rsize) <= 64 * 1024;
+}
+
+// TileDev for shape k, uploading its halo offsets when the shape changes.
+int make_tile(uasd_ctx *c, int k, uasd::TileDev &t) {
+  const int *sh = uasd::kTileShapes[k - 1];
+  t = uasd::TileDev{};
+  t.N1 = c->lat_n1;
+  t.N2 = c->lat_n2;
+  t.N3 = c->lat_n3;
+  t.NA = c->lat_na;
+  t.TX = sh[0];
+  t.TY = sh[1];
+  t.TZ = sh[2];
+  t.RX = std::max(c->st_ex.rx, c->st_dm.rx);
+  t.RY = std::max(c->st_ex.ry, c->st_dm.ry);
+  t.RZ = std::max(c->st_ex.rz, c->st_dm.rz);
+  t.HX = t.TX + 2 * t.RX;
+  t.HY = t.TY + 2 * t.RY;
+  t.HZ = t.TZ + 2 * t.RZ;
+  t.HV = t.HX * t.HY * t.HZ;
+  t.ntx = (t.N1 + t.TX - 1) / t.TX;
+  t.nty = (t.N2 + t.TY - 1) / t.TY;
+  t.ntz = (t.N3 + t.TZ - 1) / t.TZ;
+  t.W = c->st_ex.W;
+  t.Wd = c->Wd > 0 ? c->st_dm.W : 0;
+  if (c->tile_off_shape != k) {
+    std::vector<int> off((size_t)t.NA * (t.W + t.Wd), 0);
+    auto fill = [&](const uasd_ctx::Stencil &st, int W, int base) {
+      for (int b = 0; b < t.NA; ++b)
+        for (int j = 0; j < W; ++j) {
+          const int q = b * W + j;
+          off[base + q] = st.bp[q] * t.HV + st.dz[q] * t.HX * t.HY + st.dy[q] * t.HX + st.dx[q];
+        }
+    };
+    fill(c->st_ex, t.W, 0);
+    if (t.Wd > 0) fill(c->st_dm, t.Wd, t.NA * t.W);
+    if (int r = upload_raw(c, c->tile_off, off)) return r;
+    c->tile_off_shape = k;
+  }
+  t.off = c->tile_off.as<int>();
+  t.doff = t.off + (size_t)t.NA * t.W;
+  // one site class per basis atom? (checked once per class rebuild)
+  if (c->ucls_version != c->cls_version) {
+    c->ucls = 0;
+    if (t.NA <= 8 && (int)c->cls_host.size() == c->N) {
+      c->ucls = 1;
+      for (int b = 0; b < t.NA; ++b) c->cls_b[b] = c->cls_host[slot_of(c, b)];
+      for (int i = 0; i < c->N && c->ucls; ++i)
+        if (c->cls_host[slot_of(c, i)] != c->cls_b[i % t.NA]) c->ucls = 0;
+    }
+    c->ucls_version = c->cls_version;
+  }
+  t.ucls = c->ucls;
+  for (int b = 0; b < 8; ++b) t.cls_b[b] = c->cls_b[b];
+  return UASD_OK;
 }
 
 // ------------------------------------------------------------ segments
@@ -795,6 +954,8 @@ int uasd_set_exchange(uasd_ctx *c, int mnn, int nham, const int *nlist, const in
   c->have_ex = true;
   c->classes_dirty = true;
   c->coloring_dirty = true;
+  c->stencil_dirty = true;
+  c->tile_off_shape = 0;
   return UASD_OK;
 }
 
@@ -808,6 +969,8 @@ int uasd_set_dm(uasd_ctx *c, int mdm, int nham, const int *dmlist, const int *dm
     c->dm_rows = 0;
     c->dm_row_of_site.clear();
     c->classes_dirty = c->coloring_dirty = true;
+  c->stencil_dirty = true;
+  c->tile_off_shape = 0;
     return UASD_OK;
   }
   if (nham <= 0 || !dmlistsize || !dm_vect) return fail(UASD_ERR_ARG, "uasd_set_dm: bad arguments");
@@ -858,6 +1021,8 @@ int uasd_set_dm(uasd_ctx *c, int mdm, int nham, const int *dmlist, const int *dm
   if (int r = upload_raw(c, c->dm_len, rs.len)) return r;
   if (int r = upload_real(c, c->dv, dv)) return r;
   c->classes_dirty = c->coloring_dirty = true;
+  c->stencil_dirty = true;
+  c->tile_off_shape = 0;
   return UASD_OK;
 }
 
@@ -1380,9 +1545,10 @@ int uasd_set_tuning(uasd_ctx *c, int kind, int gather, int block) {
   if (int r = check_ctx(c)) return r;
   c->gen += 1;
   if (kind < 0 || kind > 2) return fail(UASD_ERR_ARG, "kind must be 0 (LLG), 1 (field) or 2 (MC)");
-  // block may carry flags: 0x10000 non-temporal list loads, 0x20000 int32 entries
-  uasd::Tune t{gather, block & 0xffff, (block >> 16) & 1, (block >> 17) & 1};
-  if (!uasd::tune_valid(t) || (block >> 18) != 0)
+  // block may carry flags: 0x10000 non-temporal list loads, 0x20000 int32
+  // entries, (k << 18) LDS-tiled stencil shape k (LLG only)
+  uasd::Tune t{gather, block & 0xffff, (block >> 16) & 1, (block >> 17) & 1, (block >> 18) & 0xf};
+  if (!uasd::tune_valid(t) || (block >> 22) != 0 || (kind != 0 && t.tile != 0))
     return fail(UASD_ERR_ARG, "gather must be 2, 4, 8 or 16 and block 64, 128 or 256");
   c->tune[kind] = t;
   return UASD_OK;
@@ -1393,7 +1559,8 @@ int uasd_get_tuning(uasd_ctx *c, int kind, int *gather, int *block) {
   if (kind < 0 || kind > 2) return fail(UASD_ERR_ARG, "kind must be 0 (LLG), 1 (field) or 2 (MC)");
   if (gather) *gather = c->tune[kind].gather;
   if (block)
-    *block = c->tune[kind].block | (c->tune[kind].nt << 16) | (c->tune[kind].idx32 << 17);
+    *block = c->tune[kind].block | (c->tune[kind].nt << 16) | (c->tune[kind].idx32 << 17) |
+             (c->tune[kind].tile << 18);
   return UASD_OK;
 }
 
@@ -1446,12 +1613,20 @@ int tune_search(uasd_ctx *c, uasd::Tune *slot, Run &&run, double *best_ms) {
   const int blocks[] = {64, 128, 256};
   uasd::Tune best = *slot;
   float best_t = 1e30f;
+  std::vector<uasd::Tune> cand;
   const int nidx = c->use16 || c->use16_dm ? 2 : 1;
   for (int ix = 0; ix < nidx; ++ix)
-  for (int nt = 0; nt < 2; ++nt)
-  for (int g : gathers)
-    for (int bl : blocks) {
-      *slot = uasd::Tune{g, bl, nt, ix};
+    for (int nt = 0; nt < 2; ++nt)
+      for (int g : gathers)
+        for (int bl : blocks) cand.push_back(uasd::Tune{g, bl, nt, ix, 0});
+  if (slot == &c->tune[0]) {
+    build_stencil(c);
+    for (int k = 1; k <= uasd::kNumTileShapes; ++k)
+      if (tile_fits(c, k)) cand.push_back(uasd::Tune{4, 256, 0, 0, k});
+  }
+  for (const uasd::Tune &cd : cand) {
+    {
+      *slot = cd;
       if (int r = run()) return r;
       HIP_TRY(hipEventRecord(ev.a, c->stream));
       if (int r = run()) return r;
@@ -1464,6 +1639,7 @@ int tune_search(uasd_ctx *c, uasd::Tune *slot, Run &&run, double *best_ms) {
         best = *slot;
       }
     }
+  }
   *slot = best;
   if (best_ms) *best_ms = best_t;
   return UASD_OK;
@@ -1544,6 +1720,36 @@ int uasd_set_site_order(uasd_ctx *c, int na) {
   c->order_na = na;
   c->order_nc = c->N / na;
   c->gen += 1;
+  return UASD_OK;
+}
+
+int uasd_set_lattice_dims(uasd_ctx *c, int na, int n1, int n2, int n3) {
+  if (int r = check_ctx(c)) return r;
+  if (na == 0 && n1 == 0 && n2 == 0 && n3 == 0) {  // forget the lattice
+    c->lat_na = c->lat_n1 = c->lat_n2 = c->lat_n3 = 0;
+  } else {
+    if (na < 1 || n1 < 1 || n2 < 1 || n3 < 1 || (int64_t)na * n1 * n2 * n3 != c->N)
+      return fail(UASD_ERR_ARG, "NA*N1*N2*N3 must equal natom");
+    c->lat_na = na;
+    c->lat_n1 = n1;
+    c->lat_n2 = n2;
+    c->lat_n3 = n3;
+  }
+  c->stencil_dirty = true;
+  c->tile_off_shape = 0;
+  c->gen += 1;
+  return UASD_OK;
+}
+
+int uasd_has_stencil(uasd_ctx *c, int *has, int *radius) {
+  if (int r = check_ctx(c)) return r;
+  build_stencil(c);
+  if (has) *has = c->stencil_ok ? 1 : 0;
+  if (radius) {
+    radius[0] = std::max(c->st_ex.rx, c->st_dm.rx);
+    radius[1] = std::max(c->st_ex.ry, c->st_dm.ry);
+    radius[2] = std::max(c->st_ex.rz, c->st_dm.rz);
+  }
   return UASD_OK;
 }
 

@@ -596,23 +596,12 @@ __device__ __forceinline__ Quad<float> finish<float>(float o0, float o1, float o
 // Buffers: e = emom at step start, p = predicted emom (emom2 after the first
 // half), b2 = Depondt's b2eff.  Quads carry mmom in w.  The corrector writes
 // the new emom into `out` (e when moment_update is fused, p otherwise).
-template <typename R, int ALG, int HALF, bool FUSED, int G>
-__global__ void __launch_bounds__(256)
-    k_llg(HamDev<R> h, LLGDev<R> prm, const Quad<R> *e, Quad<R> *p, Quad<R> *b2,
-          const Quad<R> *__restrict__ beff, Quad<R> *out) {
-  const int64_t total = (int64_t)h.N * h.M;
-  const int64_t q = xcd_block() * blockDim.x + threadIdx.x;
-  if (q >= total) return;
-  const int site = (int)(q / h.M), ens = (int)(q - (int64_t)site * h.M);
-  R b0, b1, b2v;
-  if (FUSED) {
-    total_field<R, G>(h, HALF == 0 ? e : (const Quad<R> *)p, site, ens, b0, b1, b2v);
-  } else {
-    const Quad<R> b = beff[q];
-    b0 = b.x;
-    b1 = b.y;
-    b2v = b.z;
-  }
+// The LLG half-step of one (site, ens) once its effective field b is known.
+template <typename R, int ALG, int HALF>
+__device__ __forceinline__ void llg_update(const HamDev<R> &h, const LLGDev<R> &prm, int site,
+                                           int ens, int64_t q, R b0, R b1, R b2v,
+                                           const Quad<R> *e, Quad<R> *p, Quad<R> *b2,
+                                           Quad<R> *out) {
   const double lam = prm.lambda(site), T = prm.temperature(site);
   const Quad<R> eq = e[q];
   const R m = eq.w;
@@ -656,6 +645,228 @@ __global__ void __launch_bounds__(256)
       midpoint_solve(A0, A1, A2, eq.x, eq.y, eq.z, t0, t1, t2, detAi);
       out[q] = finish<R>(t0 * detAi, t1 * detAi, t2 * detAi, m);
     }
+  }
+}
+
+template <typename R, int ALG, int HALF, bool FUSED, int G>
+__global__ void __launch_bounds__(256)
+    k_llg(HamDev<R> h, LLGDev<R> prm, const Quad<R> *e, Quad<R> *p, Quad<R> *b2,
+          const Quad<R> *__restrict__ beff, Quad<R> *out) {
+  const int64_t total = (int64_t)h.N * h.M;
+  const int64_t q = xcd_block() * blockDim.x + threadIdx.x;
+  if (q >= total) return;
+  const int site = (int)(q / h.M), ens = (int)(q - (int64_t)site * h.M);
+  R b0, b1, b2v;
+  if (FUSED) {
+    total_field<R, G>(h, HALF == 0 ? e : (const Quad<R> *)p, site, ens, b0, b1, b2v);
+  } else {
+    const Quad<R> b = beff[q];
+    b0 = b.x;
+    b1 = b.y;
+    b2v = b.z;
+  }
+  llg_update<R, ALG, HALF>(h, prm, site, ens, q, b0, b1, b2v, e, p, b2, out);
+}
+
+// ------------------------------------------------------- LDS-tiled stencil
+// For a periodic lattice whose neighbour lists are one translation-invariant
+// stencil per basis atom (reference site i = b + NA*(x + N1*(y + N2*z)),
+// neighbour j of basis b at basis b' and cell offset (dx, dy, dz) for every
+// cell), a workgroup of TX*TY*TZ threads owns one tile of cells.  It stages
+// emomM = e*m of the tile plus a halo of (RX, RY, RZ) cells for all basis
+// atoms in LDS as three planes, then every thread takes the sites of its
+// cell and reads neighbour j at its halo position plus off[b*W + j]
+// (= b'*HV + dz*HX*HY + dy*HX + dx).  The terms are added in list order with
+// the same products as the ELL path, so results are identical; the gather
+// traffic moves from L2 to LDS.  M = 1.
+struct TileDev {
+  int N1, N2, N3, NA;
+  int TX, TY, TZ;
+  int RX, RY, RZ;
+  int HX, HY, HZ, HV;     // halo tile extents, HV = HX*HY*HZ
+  int ntx, nty, ntz;      // tiles per direction
+  int W, Wd;              // exchange / DM stencil widths
+  const int *off;         // (NA*W) exchange offsets into the halo tile
+  const int *doff;        // (NA*Wd) DM offsets
+  // ucls = 1: every site of basis b has site class cls_b[b] (a lattice of one
+  // species), so class rows, couplings and offsets are the same for all lanes
+  // of a wavefront and are read through the scalar cache
+  int ucls;
+  int cls_b[8];
+};
+
+template <typename R>
+__device__ __forceinline__ int dev_slot(const HamDev<R> &h, int i) {
+  return h.order_na == 1 ? i : (i % h.order_na) * h.order_nc + i / h.order_na;
+}
+
+// Load through the constant address space: the tables it is used for (class
+// rows, couplings, DM vectors) are not written by any kernel, and a
+// wave-uniform address is then read with a scalar load.
+template <typename T>
+__device__ __forceinline__ T cload(const T *p) {
+  return *(const __attribute__((address_space(4))) T *)p;
+}
+
+// One site of a tile: field from the LDS planes, then the LLG half-step.
+// UCLS: the site class is the per-basis one of TileDev (wave-uniform).
+template <typename R, int ALG, int HALF, bool UCLS>
+__device__ __forceinline__ void tile_site(const HamDev<R> &h, const LLGDev<R> &prm,
+                                          const TileDev &t, const R *sx, const R *sy,
+                                          const R *sz, int hbase, int b, int x, int y, int z,
+                                          const Quad<R> *e, Quad<R> *p, Quad<R> *b2,
+                                          Quad<R> *out) {
+  const int cell = x + t.N1 * (y + t.N2 * z);
+  const int site = h.order_na == 1 ? b + t.NA * cell : b * (t.N1 * t.N2 * t.N3) + cell;
+  const SiteClass sc = h.classes[UCLS ? t.cls_b[b] : h.cls[site]];
+    R s0 = 0, s1 = 0, s2 = 0, qf0 = 0, qf1 = 0, qf2 = 0;
+    {
+      // four neighbours per round: LDS reads first, then the terms in order
+      const int len = h.ex_len[sc.ex_row];
+      const R *cp = h.cpl + sc.ex_row;
+      const int *off = t.off + b * t.W;
+      int j = 0;
+      for (; j + 4 <= len; j += 4) {
+        R a0[4], a1[4], a2[4], J[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int k = hbase + off[j + u];
+          a0[u] = sx[k];
+          a1[u] = sy[k];
+          a2[u] = sz[k];
+          J[u] = UCLS ? cload(cp + (size_t)(j + u) * h.ex_rows) : cp[(size_t)(j + u) * h.ex_rows];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          s0 = s0 + J[u] * a0[u];
+          s1 = s1 + J[u] * a1[u];
+          s2 = s2 + J[u] * a2[u];
+        }
+      }
+      for (; j < len; ++j) {
+        const int k = hbase + off[j];
+        const R J = UCLS ? cload(cp + (size_t)j * h.ex_rows) : cp[(size_t)j * h.ex_rows];
+        s0 = s0 + J * sx[k];
+        s1 = s1 + J * sy[k];
+        s2 = s2 + J * sz[k];
+      }
+    }
+    if (h.Wd > 0 && sc.dm_row >= 0) {
+      const int len = h.dm_len[sc.dm_row];
+      const int *off = t.doff + b * t.Wd;
+      for (int j = 0; j < len; ++j) {
+        const int k = hbase + off[j];
+        const size_t r = (size_t)j * h.dm_rows + sc.dm_row;
+        const R D0 = UCLS ? cload(h.dvx + r) : h.dvx[r], D1 = UCLS ? cload(h.dvy + r) : h.dvy[r],
+                D2 = UCLS ? cload(h.dvz + r) : h.dvz[r];
+        const R m0 = sx[k], m1 = sy[k], m2 = sz[k];
+        s0 = s0 + D2 * m1 - D1 * m2;
+        s1 = s1 + D0 * m2 - D2 * m0;
+        s2 = s2 + D1 * m0 - D0 * m1;
+      }
+    }
+    if (sc.an_row >= 0) {
+      const int ty_ = h.an_type[sc.an_row];
+      const int k = hbase + b * t.HV;
+      const R m0 = sx[k], m1 = sy[k], m2 = sz[k];
+      if (ty_ == 1) {
+        uniaxial(h, sc.an_row, m0, m1, m2, s0, s1, s2);
+      } else if (ty_ == 2) {
+        cubic(h, sc.an_row, m0, m1, m2, s0, s1, s2);
+      } else if (ty_ == 7) {
+        R c0 = 0, c1 = 0, c2 = 0;
+        uniaxial(h, sc.an_row, m0, m1, m2, s0, s1, s2);
+        cubic(h, sc.an_row, m0, m1, m2, c0, c1, c2);
+        const R sb = h.an_sb[sc.an_row];
+        qf0 = qf0 + c0 * sb;
+        qf1 = qf1 + c1 * sb;
+        qf2 = qf2 + c2 * sb;
+      }
+    }
+    R x0 = 0, x1 = 0, x2 = 0;
+    if (h.ext_mode == 1) {
+      x0 = h.ext_u[0];
+      x1 = h.ext_u[1];
+      x2 = h.ext_u[2];
+    } else if (h.ext_mode == 2) {
+      const Quad<R> v = h.ext_q[site];
+      x0 = v.x;
+      x1 = v.y;
+      x2 = v.z;
+    }
+    const R b0 = (R)0 + s0 + (qf0 + x0 + (R)0);
+    const R b1 = (R)0 + s1 + (qf1 + x1 + (R)0);
+    const R b2v = (R)0 + s2 + (qf2 + x2 + (R)0);
+    llg_update<R, ALG, HALF>(h, prm, site, 0, site, b0, b1, b2v, e, p, b2, out);
+}
+
+template <typename R, int ALG, int HALF>
+__global__ void __launch_bounds__(256)
+    k_llg_tile(HamDev<R> h, LLGDev<R> prm, TileDev t, const Quad<R> *e, Quad<R> *p,
+               Quad<R> *b2, Quad<R> *out) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  R *sx = reinterpret_cast<R *>(lds_raw);
+  R *sy = sx + (size_t)t.HV * t.NA;
+  R *sz = sy + (size_t)t.HV * t.NA;
+  const Quad<R> *src = HALF == 0 ? e : (const Quad<R> *)p;
+  // tile coordinates (XCD-aware block order, z slowest)
+  const int64_t tb = xcd_block();
+  const int ntiles = t.ntx * t.nty * t.ntz;
+  if (tb >= ntiles) return;  // whole padding workgroups exit before any barrier
+  const int tz = (int)(tb / (t.ntx * t.nty));
+  const int ty = (int)((tb / t.ntx) % t.nty);
+  const int tx = (int)(tb % t.ntx);
+  const int X0 = tx * t.TX, Y0 = ty * t.TY, Z0 = tz * t.TZ;
+  // stage the halo tile: thread t takes halo cells t, t + blockDim, ...;
+  // the cell coordinates advance by mixed-radix addition (no divisions in
+  // the loop) and wrap periodically
+  {
+    const int nb = blockDim.x;
+    int hx = threadIdx.x % t.HX, hy = (threadIdx.x / t.HX) % t.HY, hz = threadIdx.x / (t.HX * t.HY);
+    const int ax = nb % t.HX, ay = (nb / t.HX) % t.HY, az = nb / (t.HX * t.HY);
+    const int plane = t.HX * t.HY;
+    const int ncell = t.N1 * t.N2 * t.N3;
+    while (hz < t.HZ) {
+      int x = X0 + hx - t.RX, y = Y0 + hy - t.RY, z = Z0 + hz - t.RZ;
+      while (x < 0) x += t.N1;
+      while (x >= t.N1) x -= t.N1;
+      while (y < 0) y += t.N2;
+      while (y >= t.N2) y -= t.N2;
+      while (z < 0) z += t.N3;
+      while (z >= t.N3) z -= t.N3;
+      const int cell = x + t.N1 * (y + t.N2 * z);
+      const int l = hz * plane + hy * t.HX + hx;
+      for (int b = 0; b < t.NA; ++b) {
+        const int slot = h.order_na == 1 ? b + t.NA * cell : b * ncell + cell;
+        const Quad<R> v = src[slot];
+        sx[b * t.HV + l] = v.x * v.w;
+        sy[b * t.HV + l] = v.y * v.w;
+        sz[b * t.HV + l] = v.z * v.w;
+      }
+      hx += ax;
+      hy += ay;
+      hz += az;
+      if (hx >= t.HX) {
+        hx -= t.HX;
+        hy += 1;
+      }
+      if (hy >= t.HY) {
+        hy -= t.HY;
+        hz += 1;
+      }
+    }
+  }
+  __syncthreads();
+  const int lx = threadIdx.x % t.TX, ly = (threadIdx.x / t.TX) % t.TY, lz = threadIdx.x / (t.TX * t.TY);
+  const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
+  if (lz >= t.TZ || x >= t.N1 || y >= t.N2 || z >= t.N3) return;
+  const int hbase = (lz + t.RZ) * t.HX * t.HY + (ly + t.RY) * t.HX + (lx + t.RX);
+  if (t.ucls) {
+    for (int b = 0; b < t.NA; ++b)
+      tile_site<R, ALG, HALF, true>(h, prm, t, sx, sy, sz, hbase, b, x, y, z, e, p, b2, out);
+  } else {
+    for (int b = 0; b < t.NA; ++b)
+      tile_site<R, ALG, HALF, false>(h, prm, t, sx, sy, sz, hbase, b, x, y, z, e, p, b2, out);
   }
 }
 

@@ -96,6 +96,29 @@ hipError_t launch_mc(hipStream_t s, const Tune &t, const HamDev<R> &h, const MCP
 }
 
 template <typename R>
+hipError_t launch_llg_tile(hipStream_t s, const TileDev &t, int alg, int half,
+                           const HamDev<R> &h, const LLGDev<R> &prm, const Quad<R> *e,
+                           Quad<R> *p, Quad<R> *b2, Quad<R> *out) {
+  const int ntiles = t.ntx * t.nty * t.ntz;
+  if (ntiles == 0) return hipSuccess;
+  const int threads = t.TX * t.TY * t.TZ;
+  if (threads > kBlock || h.M != 1) return hipErrorInvalidValue;
+  const size_t lds = tile_lds_bytes(t.HV, t.NA, sizeof(R));
+  const dim3 grid((ntiles + kXCD - 1) / kXCD * kXCD), block(threads);
+#define UASD_TILE_CASE(A, H)                                                                    \
+  if (alg == A && half == H) {                                                                  \
+    hipLaunchKernelGGL((k_llg_tile<R, A, H>), grid, block, lds, s, h, prm, t, e, p, b2, out);   \
+    return hipGetLastError();                                                                   \
+  }
+  UASD_TILE_CASE(1, 0)
+  UASD_TILE_CASE(1, 1)
+  UASD_TILE_CASE(5, 0)
+  UASD_TILE_CASE(5, 1)
+#undef UASD_TILE_CASE
+  return hipErrorInvalidValue;
+}
+
+template <typename R>
 hipError_t launch_reduce(hipStream_t s, int N, int M, const Quad<R> *spins, const double *en,
                          int nv, double *partial, int nblocks, double *out) {
   if (N == 0 || M == 0) return hipSuccess;
@@ -152,6 +175,9 @@ hipError_t launch_segsum(hipStream_t s, int M, int nseg, int nchunks, const int2
                                     Quad<R> *, Quad<R> *, const Quad<R> *, Quad<R> *);        \
   template hipError_t launch_mc<R>(hipStream_t, const Tune &, const HamDev<R> &,              \
                                    const MCParams &, Quad<R> *);                              \
+  template hipError_t launch_llg_tile<R>(hipStream_t, const TileDev &, int, int,              \
+                                         const HamDev<R> &, const LLGDev<R> &,                \
+                                         const Quad<R> *, Quad<R> *, Quad<R> *, Quad<R> *);   \
   template hipError_t launch_reduce<R>(hipStream_t, int, int, const Quad<R> *, const double *, \
                                        int, double *, int, double *);                         \
   template hipError_t launch_segsum<R>(hipStream_t, int, int, int, const int2 *, const int *,  \

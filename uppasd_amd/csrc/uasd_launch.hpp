@@ -14,17 +14,24 @@ constexpr int kBlock = 256;  // largest workgroup the kernels are built for
 //   block   workgroup size: 64, 128 or 256
 //   nt      1: neighbour lists read with non-temporal loads
 //   idx32   1: int32 list entries even where packed int16 entries exist
+//   tile    fused LLG only: 0 = list gathers, k > 0 = LDS-tiled stencil with
+//           tile shape kTileShapes[k-1] (periodic lattices with a stencil)
+constexpr int kNumTileShapes = 12;
+constexpr int kTileShapes[kNumTileShapes][3] = {
+    {8, 8, 4},  {16, 8, 2}, {32, 4, 2}, {16, 4, 4}, {16, 16, 1}, {32, 8, 1},
+    {64, 4, 1}, {8, 4, 4},  {4, 4, 4},  {8, 8, 2},  {16, 8, 1},  {8, 8, 1}};
 struct Tune {
   int gather = 4;
   int block = 256;
   int nt = 0;
   int idx32 = 0;
+  int tile = 0;
 };
 
 inline bool tune_valid(const Tune &t) {
   return (t.gather == 2 || t.gather == 4 || t.gather == 8 || t.gather == 16) &&
          (t.block == 64 || t.block == 128 || t.block == 256) && (t.nt == 0 || t.nt == 1) &&
-         (t.idx32 == 0 || t.idx32 == 1);
+         (t.idx32 == 0 || t.idx32 == 1) && t.tile >= 0 && t.tile <= kNumTileShapes;
 }
 
 // The Hamiltonian view a launch uses under a tuning choice.
@@ -53,6 +60,14 @@ hipError_t launch_llg(hipStream_t s, const Tune &t, int alg, int half, bool fuse
 template <typename R>
 hipError_t launch_mc(hipStream_t s, const Tune &t, const HamDev<R> &h, const MCParams &mp,
                      Quad<R> *spins);
+
+// Fused LLG half-step with the LDS-tiled stencil kernel (M = 1).
+template <typename R>
+hipError_t launch_llg_tile(hipStream_t s, const TileDev &t, int alg, int half,
+                           const HamDev<R> &h, const LLGDev<R> &prm, const Quad<R> *e,
+                           Quad<R> *p, Quad<R> *b2, Quad<R> *out);
+// LDS bytes the tile kernel needs for a halo tile of hv cells
+inline size_t tile_lds_bytes(int hv, int na, size_t rsize) { return 3 * (size_t)hv * na * rsize; }
 
 // FP64 sums per ensemble of emomM (nv = 3) or of emomM and en (nv = 4);
 // partial must hold M*nblocks*nv doubles, out M*nv doubles.

@@ -24,6 +24,10 @@ from . import _lib as B
 from ._lib import (UASD_FP32, UASD_FP64, UASD_NOISE_INJECTED, UASD_NOISE_NONE,
                    UASD_NOISE_PHILOX, UASD_SDE_DEPONDT, UASD_SDE_MIDPOINT, UasdError)
 
+# LDS tile shapes (cells x, y, z) selectable with set_tuning(TUNE_LLG, g, 256 | (k << 18))
+TILE_SHAPES = ((8, 8, 4), (16, 8, 2), (32, 4, 2), (16, 4, 4), (16, 16, 1), (32, 8, 1), (64, 4, 1),
+               (8, 4, 4), (4, 4, 4), (8, 8, 2), (16, 8, 1), (8, 8, 1))
+
 __all__ = ["Engine", "moment_update", "UasdError", "UASD_NOISE_NONE", "UASD_NOISE_PHILOX",
            "UASD_NOISE_INJECTED", "UASD_SDE_MIDPOINT", "UASD_SDE_DEPONDT"]
 
@@ -202,6 +206,19 @@ class Engine:
         B.check(B.lib().uasd_get_kernel_timing(self._h, kind, C.byref(n), C.byref(ms)),
                 "uasd_get_kernel_timing")
         return n.value, ms.value
+
+    def set_lattice_dims(self, na: int, n1: int, n2: int, n3: int):
+        """Site i = b + na*(x + n1*(y + n2*z)); enables the LDS-tiled stencil
+        kernel when the lists are translation invariant."""
+        B.check(B.lib().uasd_set_lattice_dims(self._h, int(na), int(n1), int(n2), int(n3)),
+                "uasd_set_lattice_dims")
+
+    def stencil(self):
+        """(has_stencil, (rx, ry, rz))."""
+        has = C.c_int(0)
+        r = np.zeros(3, dtype=np.int32)
+        B.check(B.lib().uasd_has_stencil(self._h, C.byref(has), B.ptr(r)), "uasd_has_stencil")
+        return bool(has.value), tuple(int(x) for x in r)
 
     def index_bytes(self):
         """(exchange, dm) bytes per neighbour entry read by the kernels (2 or 4)."""

@@ -324,6 +324,38 @@ def form_c5(U, CF, steps=2000):
     return out
 
 
+def list_kernel_compare(U, sysd, args, order, e0, steps=300):
+    """The same workload on the list-gather kernels (lattice not declared, so
+    no stencil), autotuned: its corrector time and roofline beside the tiled
+    kernel's.  Bounded to a few hundred steps."""
+    N = sysd["lattice"].natom
+    W = int(sysd["nlistsize"].max())
+    eng = U.Engine(N, 1, args.precision, site_order=order)
+    eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
+    eng.set_moments(np.ascontiguousarray(e0), sysd["mmom"][None])
+    eng.set_llg(0.1, 300.0, 1e-16)
+    eng.set_noise(U.UASD_NOISE_PHILOX, 0xC2C2C2C2)
+    eng.autotune_llg(args.solver, steps=3)
+    eng.sd_steps(args.solver, 20)
+    eng.sync()
+    eng.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    eng.sd_steps(args.solver, steps)
+    eng.sync()
+    elapsed = time.perf_counter() - t0
+    n, ms = eng.kernel_timing(1)
+    t = ms / max(n, 1) * 1e-3
+    s = 8 if args.precision == "fp64" else 4
+    ib = eng.index_bytes()[0]
+    kb = kernel_bytes(W, s, ib)
+    ach = kb["corrector"] * N / t / 1e9
+    g, b = eng.tuning(eng.TUNE_LLG)
+    return {"value": N * steps / elapsed, "corrector_ms": round(t * 1e3, 4),
+            "bytes_per_site": kb, "index_bytes": ib, "gather": g, "block": b & 0xFFFF,
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-calibration", action="store_true",
@@ -339,6 +371,8 @@ def main():
                     help="fixed gather batch (2/4/8/16) instead of autotuning")
     ap.add_argument("--block", type=int, default=256,
                     help="workgroup size with --gather; add 65536 for non-temporal index loads")
+    ap.add_argument("--no-list-compare", action="store_true",
+                    help="skip timing the list-gather kernels beside the tiled ones")
     ap.add_argument("--no-stencil", action="store_true",
                     help="do not declare the lattice (list-gather kernels only)")
     ap.add_argument("--site-order", type=int, default=0,
@@ -454,6 +488,8 @@ def main():
         },
     }
     del eng
+    if tile and not args.no_list_compare:
+        out["list_kernel"] = list_kernel_compare(U, sysd, args, order, e0)
     forms = [f for f in args.forms.split(",") if f and f != "none"]
     if forms:
         from uppasd_amd import configs as CF

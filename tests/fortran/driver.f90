@@ -5,7 +5,8 @@
 !>     -> moment_update (copym, updatemoments.f90:48-95, done here on the host)
 !>   kind 2 (MC): nsweeps calls of mc_evolve (mc_driver.f90:365).
 !> Input (stream, little endian): int32 kind, Natom, Mensemble, mnn, alg, nsteps,
-!> injected; float64 delta_t, lambda, temp; int32 nlist(mnn,Natom),
+!> injected, NA, N1, N2, N3 (NA > 1: basis-major device site order; N1 > 0:
+!> lattice dimensions passed to the library); float64 delta_t, lambda, temp; int32 nlist(mnn,Natom),
 !> nlistsize(Natom); float64 ncoup(mnn,Natom), mmom(Natom,Mensemble),
 !> emom(3,Natom,Mensemble), extfield(3); for injected noise float64
 !> normals(3,Natom,Mensemble,nsteps).  For kind 2, alg holds the mode
@@ -18,7 +19,7 @@ program driver
    implicit none
    integer, parameter :: dp = c_double
    character(len=512) :: fin, fout
-   integer(c_int32_t) :: hdr(7)
+   integer(c_int32_t) :: hdr(11)
    integer :: kind, Natom, Mensemble, mnn, alg, nsteps, injected, istep, i, k, u
    real(dp) :: par(3), delta_t, lam, temp, energy, temprescale
    integer, allocatable :: nlist(:,:), nlistsize(:), aHam(:), red_atom_list(:), iflip_a(:)
@@ -84,8 +85,10 @@ program driver
    maxc = 0
 
    call uasd_gpu_init(Natom, Mensemble, int(UASD_FP64))
+   if (hdr(8) > 1) call uasd_gpu_set_site_order(hdr(8))
    call uasd_gpu_set_hamiltonian(Natom, mnn, Natom, nlist, nlistsize, ncoup, aHam, 0, 1, &
       dmdum, dmsize, dmv, 0, tan, ean, kan, sb)
+   if (hdr(9) > 0) call uasd_gpu_set_lattice_dims(hdr(8), hdr(9), hdr(10), hdr(11))
 
    if (kind == 1) then
       if (injected == 1) then

@@ -28,14 +28,17 @@ DRIVER = os.path.join(HERE, "fortran", "uasd_driver")
 
 
 def _run_driver(tmp_path, kind, nlist, nlistsize, ncoup, mmom, emom, alg, nsteps, dt, lam,
-                temp, ext=(0.0, 0.0, 0.0), normals=None):
+                temp, ext=(0.0, 0.0, 0.0), normals=None, lattice=(0, 0, 0, 0)):
+    """lattice = (NA, N1, N2, N3): NA > 1 selects the basis-major site order,
+    N1 > 0 passes the lattice dimensions to the library."""
     if not os.path.exists(DRIVER):
         pytest.fail("Fortran driver not built (tests/fortran/build_driver.py)")
     M, N, _ = emom.shape
     W = nlist.shape[1]
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(fin, "wb") as f:
-        np.array([kind, N, M, W, alg, nsteps, 0 if normals is None else 1], np.int32).tofile(f)
+        np.array([kind, N, M, W, alg, nsteps, 0 if normals is None else 1, *lattice],
+                 np.int32).tofile(f)
         np.array([dt, lam, temp], np.float64).tofile(f)
         np.ascontiguousarray(nlist, np.int32).tofile(f)
         np.ascontiguousarray(nlistsize, np.int32).tofile(f)
@@ -70,8 +73,9 @@ def test_fortran_solvers_golden(O, tmp_path, solver, name):
     assert np.allclose([*m, np.linalg.norm(m)], g["expected"], atol=1e-8)
 
 
+@pytest.mark.parametrize("ordered", [False, True])
 @pytest.mark.parametrize("alg", [1, 5])
-def test_fortran_injected_noise_matches_oracle(O, tmp_path, alg):
+def test_fortran_injected_noise_matches_oracle(O, tmp_path, alg, ordered):
     rng = np.random.default_rng(21)
     sysd = CF.bcc_shells(3, CF.bcc5_couplings())
     lat = sysd["lattice"]
@@ -79,8 +83,10 @@ def test_fortran_injected_noise_matches_oracle(O, tmp_path, alg):
     emom = np.ascontiguousarray(CF.random_unit(N, rng, M))
     normals = rng.normal(size=(nsteps, M, N, 3))
     mm = np.ascontiguousarray(np.broadcast_to(sysd["mmom"], (M, N)))
+    lattice = (2, *lat.n) if ordered else (0, 0, 0, 0)
     e, eM, en = _run_driver(tmp_path, 1, sysd["nlist"], sysd["nlistsize"], sysd["ncoup"], mm,
-                            emom, alg, nsteps, 1e-16, 0.1, 300.0, normals=normals)
+                            emom, alg, nsteps, 1e-16, 0.1, 300.0, normals=normals,
+                            lattice=lattice)
     latO = O.LatticeSpec(lat.cell, lat.bas, lat.atype, lat.n, lat.bc)
     sim = RC.OracleSD(O, latO, sysd, sysd["mmom"], M=M)
     sim.set_state(emom)

@@ -41,9 +41,10 @@ module uasd_hip
       uasd_set_external_field, uasd_set_llg, uasd_set_temprescale, uasd_set_noise, &
       uasd_set_moments, uasd_get_moments, uasd_effective_field_host, uasd_evolve_first_host, &
       uasd_evolve_second_host, uasd_sd_steps, uasd_effective_field, uasd_mc_sweeps, &
-      uasd_measure, uasd_autotune_llg, uasd_autotune_mc
+      uasd_measure, uasd_autotune_llg, uasd_autotune_mc, uasd_set_site_order, &
+      uasd_set_lattice_dims
    public :: uasd_check, uasd_error_message, uasd_gpu_init, uasd_gpu_finalize, &
-      uasd_gpu_set_hamiltonian, uasd_effective_field_full, uasd_evolve_first, &
+      uasd_gpu_set_site_order, uasd_gpu_set_lattice_dims, uasd_gpu_set_hamiltonian, uasd_effective_field_full, uasd_evolve_first, &
       uasd_evolve_second, uasd_mc_evolve
 
    interface
@@ -75,6 +76,19 @@ module uasd_hip
       function uasd_ctx_sync(ctx) bind(C, name="uasd_ctx_sync") result(st)
          import :: c_ptr, c_int
          type(c_ptr), value :: ctx
+         integer(c_int) :: st
+      end function
+      function uasd_set_site_order(ctx, na) bind(C, name="uasd_set_site_order") result(st)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: na
+         integer(c_int) :: st
+      end function
+      function uasd_set_lattice_dims(ctx, na, n1, n2, n3) bind(C, name="uasd_set_lattice_dims") &
+            result(st)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: na, n1, n2, n3
          integer(c_int) :: st
       end function
       function uasd_set_exchange(ctx, mnn, nham, nlist, nlistsize, ncoup, aham) &
@@ -253,6 +267,24 @@ contains
       if (c_associated(uasd_gpu)) call uasd_check(uasd_ctx_destroy(uasd_gpu), 'uasd_ctx_destroy')
       uasd_gpu = c_null_ptr
    end subroutine uasd_gpu_finalize
+
+   !> Basis-major device site order for a crystal of NA basis atoms (the
+   !> host arrays keep the reference order).  Call before
+   !> uasd_gpu_set_hamiltonian.
+   subroutine uasd_gpu_set_site_order(NA)
+      integer, intent(in) :: NA
+      call uasd_check(uasd_set_site_order(uasd_gpu, int(NA, c_int)), 'uasd_set_site_order')
+   end subroutine uasd_gpu_set_site_order
+
+   !> The NA, N1, N2, N3 of effective_field_full (hamiltonianactions.f90:108-112):
+   !> on a periodic crystal whose lists are one stencil per basis atom the
+   !> resident LLG step may gather from LDS tiles.  Call after
+   !> uasd_gpu_set_hamiltonian.
+   subroutine uasd_gpu_set_lattice_dims(NA, N1, N2, N3)
+      integer, intent(in) :: NA, N1, N2, N3
+      call uasd_check(uasd_set_lattice_dims(uasd_gpu, int(NA, c_int), int(N1, c_int), &
+         int(N2, c_int), int(N3, c_int)), 'uasd_set_lattice_dims')
+   end subroutine uasd_gpu_set_lattice_dims
 
    !> Uploads the ham_t arrays (hamiltoniandatatype.f90:14-100): exchange
    !> (ncoup of configuration 1), DM when do_dm = 1, anisotropy when

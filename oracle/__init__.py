@@ -116,6 +116,10 @@ def _declare(L):
     L.or_mt_fill_real2.argtypes = [C.POINTER(MT), dp, C.c_long]
     L.or_zig_fill.argtypes = [C.POINTER(Zig), dp, C.c_long]
     L.or_noise3_f64.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, dp]
+    L.or_thermal3.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, fp]
+    L.or_bm_ln.argtypes = [C.c_float]
+    L.or_bm_ln.restype = C.c_float
+    L.or_bm_sincos.argtypes = [C.c_float, fp]
     L.or_noise3_f32.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, fp]
     L.or_uniform4.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, dp]
     L.or_philox4x32_10.argtypes = [C.POINTER(C.c_uint32 * 4), C.POINTER(C.c_uint32 * 2),
@@ -333,6 +337,13 @@ def mt(seed: int) -> MT:
     s = MT()
     lib().or_mt_init(C.byref(s), seed)
     return s
+
+
+def thermal3(seed, tag, step, site, ens):
+    """The LLG thermal deviates of (site, ens, step): three FP32 values."""
+    out = np.zeros(3, np.float32)
+    lib().or_thermal3(seed, tag, step, site, ens, out)
+    return out
 
 
 def noise3_f64(seed, tag, step, site, ens):

@@ -222,9 +222,9 @@ void or_sd_run(const or_ham *h, int N, int M, int alg, long nsteps, uint64_t ste
     uint64_t step = step0 + (uint64_t)s;
     for (int k = 0; k < M; ++k)
       for (int i = 0; i < N; ++i) {
-        double z[3] = {0.0, 0.0, 0.0};
-        if (noise == 1) or_noise3_f64(seed, 0x11u, step, (uint32_t)i, (uint32_t)k, z);
-        for (int c = 0; c < 3; ++c) V3(rv, c, i, k, N) = z[c];
+        float z[3] = {0.0f, 0.0f, 0.0f};
+        if (noise == 1) or_thermal3(seed, 0x11u, step, (uint32_t)i, (uint32_t)k, z);
+        for (int c = 0; c < 3; ++c) V3(rv, c, i, k, N) = (double)z[c];
       }
     or_effective_field(h, N, M, emomM, ext, beff, NULL, NULL, NULL);
     if (alg == 1) {

@@ -212,6 +212,85 @@ void or_noise3_f32(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
   out[2] = r1 * cosf(t1);
 }
 
+/* Thermal-field deviates of the LLG steps (both precisions of the HIP
+ * kernels): Box-Muller in single precision with polynomial ln, sin and cos,
+ * as the kernels compute it, operation for operation (IEEE adds, multiplies,
+ * fmaf, sqrtf, rintf; no contraction), so the deviates agree bit for bit.
+ * The reference's own deviates are single precision too (r4_nor,
+ * randomnumbers.f90:330-438).  Polynomials fitted by least squares on
+ * Chebyshev nodes (Lawson reweighting):
+ *   ln(u):  u = m 2^e, m in [sqrt(1/2), sqrt(2)], x = m - 1,
+ *           ln(1+x) = x + x^2 (-1/2 + x P(x))
+ *   angle:  2 pi u = (pi/2) v, v in (0, 4), q = rint(v), f = v - q,
+ *           sin((pi/2) f) = f S(f^2), cos((pi/2) f) = C(f^2). */
+static float bm_ln(float u)
+{
+  uint32_t bits;
+  memcpy(&bits, &u, 4);
+  int ex = (int)(bits >> 23) - 127;
+  uint32_t mb = (bits & 0x7fffffu) | 0x3f800000u;
+  float m;
+  memcpy(&m, &mb, 4);
+  if (m > 1.41421354f) {
+    m = m * 0.5f;
+    ex += 1;
+  }
+  const float x = m - 1.0f;
+  float p = -0.0830537900328636f;
+  p = fmaf(p, x, 0.127988979220390f);
+  p = fmaf(p, x, -0.129434198141098f);
+  p = fmaf(p, x, 0.141902372241020f);
+  p = fmaf(p, x, -0.166434884071350f);
+  p = fmaf(p, x, 0.200018793344498f);
+  p = fmaf(p, x, -0.250002801418304f);
+  p = fmaf(p, x, 0.333333253860474f);
+  const float t = fmaf(p, x, -0.5f);
+  const float x2 = x * x;
+  const float lnm = fmaf(t, x2, x);
+  return fmaf((float)ex, 0.693147182464599609375f, lnm);
+}
+
+static void bm_sincos_q(float v, float *s, float *c)
+{
+  const float qf = rintf(v);
+  const float f = v - qf;
+  const int q = (int)qf & 3;
+  const float f2 = f * f;
+  float sp = 0.000158195078256540f;
+  sp = fmaf(sp, f2, -0.00468126079067588f);
+  sp = fmaf(sp, f2, 0.0796925798058510f);
+  sp = fmaf(sp, f2, -0.645964086055756f);
+  sp = fmaf(sp, f2, 1.57079637050629f);
+  const float sn = sp * f;
+  float cs = 0.000903362990356982f;
+  cs = fmaf(cs, f2, -0.0208599362522364f);
+  cs = fmaf(cs, f2, 0.253669172525406f);
+  cs = fmaf(cs, f2, -1.23370051383972f);
+  cs = fmaf(cs, f2, 1.0f);
+  *s = q == 0 ? sn : q == 1 ? cs : q == 2 ? -sn : -cs;
+  *c = q == 0 ? cs : q == 1 ? -sn : q == 2 ? -cs : sn;
+}
+
+float or_bm_ln(float u) { return bm_ln(u); }
+void or_bm_sincos(float v, float out[2]) { bm_sincos_q(v, &out[0], &out[1]); }
+
+void or_thermal3(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site, uint32_t ens,
+                 float out[3])
+{
+  uint32_t w[4];
+  counter_words(seed, tag, step, site, ens, w);
+  const float s32 = 2.3283064365386963e-10f, s22 = 2.384185791015625e-07f;
+  const float u0 = ((float)w[0] + 0.5f) * s32, u2 = ((float)w[2] + 0.5f) * s32;
+  const float v1 = (float)((w[1] >> 9) * 2u + 1u) * s22, v3 = (float)((w[3] >> 9) * 2u + 1u) * s22;
+  const float r0 = sqrtf(-2.0f * bm_ln(u0)), r1 = sqrtf(-2.0f * bm_ln(u2));
+  float s0, c0, s1, c1;
+  bm_sincos_q(v1, &s0, &c0);
+  bm_sincos_q(v3, &s1, &c1);
+  out[0] = r0 * c0;
+  out[1] = r0 * s0;
+  out[2] = r1 * c1;
+}
+
 void or_uniform4(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
                  uint32_t ens, double out[4])
 {

@@ -51,6 +51,10 @@ void   or_noise3_f64(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
                      uint32_t ens, double out[3]);
 void   or_noise3_f32(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
                      uint32_t ens, float out[3]);
+void   or_thermal3(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
+                   uint32_t ens, float out[3]);        /* LLG thermal deviates */
+float  or_bm_ln(float u);
+void   or_bm_sincos(float v, float out[2]);           /* sin, cos of (pi/2) v */
 void   or_uniform4(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
                    uint32_t ens, double out[4]);
 

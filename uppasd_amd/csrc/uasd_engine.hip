@@ -489,9 +489,13 @@ uasd::LLGDev<R> llg_view(uasd_ctx *c) {
   p.temprescale = c->temprescale;
   p.bn = c->bn;
   p.noise_mode = c->noise_mode;
+  // a uniform T = 0 scales every deviate by zero: draw none
+  if (c->temp_uniform && c->temp0 == 0.0 && p.noise_mode == UASD_NOISE_PHILOX)
+    p.noise_mode = UASD_NOISE_NONE;
   p.seed = c->seed;
   p.step = c->step;
   if (c->noise_mode == UASD_NOISE_INJECTED) p.noise = quads<R>(c, c->noise);
+  p.lc0 = uasd::lam_const<R>(c->lam0, c->dt, c->bn);
   return p;
 }
 
@@ -743,13 +747,15 @@ int make_tile(uasd_ctx *c, int k, uasd::TileDev &t) {
   t.ntz = (t.N3 + t.TZ - 1) / t.TZ;
   t.W = c->st_ex.W;
   t.Wd = c->Wd > 0 ? c->st_dm.W : 0;
+  t.nbp = std::min(t.NA, 1024 / (t.TX * t.TY * t.TZ));
   if (c->tile_off_shape != k) {
     std::vector<int> off((size_t)t.NA * (t.W + t.Wd), 0);
     auto fill = [&](const uasd_ctx::Stencil &st, int W, int base) {
       for (int b = 0; b < t.NA; ++b)
         for (int j = 0; j < W; ++j) {
           const int q = b * W + j;
-          off[base + q] = st.bp[q] * t.HV + st.dz[q] * t.HX * t.HY + st.dy[q] * t.HX + st.dx[q];
+          off[base + q] =
+              3 * (st.bp[q] * t.HV + st.dz[q] * t.HX * t.HY + st.dy[q] * t.HX + st.dx[q]);
         }
     };
     fill(c->st_ex, t.W, 0);

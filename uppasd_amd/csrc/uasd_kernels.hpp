@@ -173,6 +173,81 @@ struct Noise<float> {
   }
 };
 
+// Thermal-field deviates of the LLG steps, FP32 for both precisions (the
+// reference draws them in single precision too: r4_nor,
+// randomnumbers.f90:330-438).  Box-Muller with ln, sin and cos evaluated by
+// polynomials fitted here (least squares with Lawson reweighting on Chebyshev
+// nodes; relative error below 3e-7): every operation is an IEEE add,
+// multiply, fmaf, sqrtf or rint, so the oracle reproduces the deviates bit
+// for bit.  Radius from all 32 bits of w0 (tails to 6.8 sigma).
+//   ln(u):  u = m 2^e, m in [sqrt(1/2), sqrt(2)], x = m - 1,
+//           ln(1+x) = x + x^2 (-1/2 + x P(x))
+//   angle:  2 pi u = (pi/2) v, v in (0, 4), q = rint(v), f = v - q in
+//           [-1/2, 1/2]: sin((pi/2) f) = f S(f^2), cos((pi/2) f) = C(f^2)
+namespace bm {
+__device__ __forceinline__ float ln(float u) {
+#pragma clang fp contract(off)
+  const uint32_t bits = __float_as_uint(u);
+  int ex = (int)(bits >> 23) - 127;
+  float m = __uint_as_float((bits & 0x7fffffu) | 0x3f800000u);
+  if (m > 1.41421354f) {
+    m = m * 0.5f;
+    ex += 1;
+  }
+  const float x = m - 1.0f;
+  float p = -0.0830537900328636f;
+  p = fmaf(p, x, 0.127988979220390f);
+  p = fmaf(p, x, -0.129434198141098f);
+  p = fmaf(p, x, 0.141902372241020f);
+  p = fmaf(p, x, -0.166434884071350f);
+  p = fmaf(p, x, 0.200018793344498f);
+  p = fmaf(p, x, -0.250002801418304f);
+  p = fmaf(p, x, 0.333333253860474f);
+  const float t = fmaf(p, x, -0.5f);
+  const float x2 = x * x;
+  const float lnm = fmaf(t, x2, x);
+  return fmaf((float)ex, 0.693147182464599609375f, lnm);
+}
+// sin and cos of (pi/2) v
+__device__ __forceinline__ void sincos_q(float v, float &s, float &c) {
+#pragma clang fp contract(off)
+  const float qf = rintf(v);
+  const float f = v - qf;
+  const int q = (int)qf & 3;
+  const float f2 = f * f;
+  float sp = 0.000158195078256540f;
+  sp = fmaf(sp, f2, -0.00468126079067588f);
+  sp = fmaf(sp, f2, 0.0796925798058510f);
+  sp = fmaf(sp, f2, -0.645964086055756f);
+  sp = fmaf(sp, f2, 1.57079637050629f);
+  const float sn = sp * f;
+  float cs = 0.000903362990356982f;
+  cs = fmaf(cs, f2, -0.0208599362522364f);
+  cs = fmaf(cs, f2, 0.253669172525406f);
+  cs = fmaf(cs, f2, -1.23370051383972f);
+  cs = fmaf(cs, f2, 1.0f);
+  s = q == 0 ? sn : q == 1 ? cs : q == 2 ? -sn : -cs;
+  c = q == 0 ? cs : q == 1 ? -sn : q == 2 ? -cs : sn;
+}
+}  // namespace bm
+
+__device__ __forceinline__ void thermal3(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
+                                         uint32_t ens, float &a, float &b, float &c) {
+#pragma clang fp contract(off)
+  uint32_t w0, w1, w2, w3;
+  counter_words(seed, tag, step, site, ens, w0, w1, w2, w3);
+  const float s32 = 2.3283064365386963e-10f, s22 = 2.384185791015625e-07f;
+  const float u0 = ((float)w0 + 0.5f) * s32, u2 = ((float)w2 + 0.5f) * s32;
+  const float v1 = (float)((w1 >> 9) * 2u + 1u) * s22, v3 = (float)((w3 >> 9) * 2u + 1u) * s22;
+  const float r0 = sqrtf(-2.0f * bm::ln(u0)), r1 = sqrtf(-2.0f * bm::ln(u2));
+  float s0, c0, s1, c1;
+  bm::sincos_q(v1, s0, c0);
+  bm::sincos_q(v3, s1, c1);
+  a = r0 * c0;
+  b = r0 * s0;
+  c = r1 * c1;
+}
+
 __device__ __forceinline__ void uniform4(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
                                          uint32_t ens, double &u0, double &u1, double &u2,
                                          double &u3) {
@@ -419,6 +494,28 @@ __device__ __forceinline__ void total_field(const HamDev<R> &h, const Quad<R> *_
 }
 
 // ------------------------------------------------------------ LLG params
+// Solver constants that depend on lam, dt and bn only: computed once on the
+// host when lam is uniform, per site otherwise (same operations either way).
+template <typename R>
+struct LamConst {
+  double Dk;       // midpoint_sigma: lam/(1+lam^2) kB/gamma/muB * gamma/bn
+  double Dp;       // depondt_sigma: 2 lam kB / (dt gamma muB)
+  R lam, lldamp;   // lam and 1/(1+lam^2)
+  R mdt, msqrtdt;  // midpoint_A: dt bn gamma lldamp and its square root
+};
+
+template <typename R>
+__host__ __device__ inline LamConst<R> lam_const(double lam, double dt, double bn) {
+  LamConst<R> c;
+  c.Dk = (lam / (1.0 + lam * lam) * kKBolt / kGama / (kMub)) * (kGama / bn);
+  c.Dp = (2.0 * lam * kKBolt) / (dt * kGama * kMub);
+  c.lam = (R)lam;
+  c.lldamp = (R)1.0 / ((R)1.0 + c.lam * c.lam);
+  c.mdt = (R)dt * (R)bn * (R)kGama * c.lldamp;
+  c.msqrtdt = (R)sqrt((double)c.mdt);  // correctly rounded in R as well
+  return c;
+}
+
 template <typename R>
 struct LLGDev {
   const double *lam, *temp, *lande;  // per site or nullptr
@@ -429,13 +526,21 @@ struct LLGDev {
   uint64_t step;              // step counter, or offset from *step_dev
   const uint64_t *step_dev;   // device-held step base (graph replays) or nullptr
   const Quad<R> *noise;  // injected N(0,1) per q
+  LamConst<R> lc0;       // lam_const(lam0, dt, bn)
+  __device__ __forceinline__ LamConst<R> lamc(int s) const {
+    return lam ? lam_const<R>(lam[s], dt, bn) : lc0;
+  }
   __device__ __forceinline__ double lambda(int s) const { return lam ? lam[s] : lam0; }
   __device__ __forceinline__ double temperature(int s) const { return temp ? temp[s] : temp0; }
   __device__ __forceinline__ double landeg(int s) const { return lande ? lande[s] : lande0; }
   __device__ __forceinline__ void normals(int site, int ens, size_t q, R &a, R &b, R &c) const {
     if (noise_mode == 1) {
       const uint64_t st = step_dev ? *step_dev + step : step;
-      Noise<R>::normal3(seed, kTagLLG, st, (uint32_t)site, (uint32_t)ens, a, b, c);
+      float fa, fb, fc;
+      thermal3(seed, kTagLLG, st, (uint32_t)site, (uint32_t)ens, fa, fb, fc);
+      a = (R)fa;
+      b = (R)fb;
+      c = (R)fc;
     } else if (noise_mode == 2) {
       const Quad<R> v = noise[q];
       a = v.x;
@@ -449,9 +554,8 @@ struct LLGDev {
 
 // rannum, llg = 1 (randomnumbers.f90:674-676, 735-745)
 template <typename R>
-__device__ __forceinline__ R midpoint_sigma(double lam, double bn, double mmom, double T,
+__device__ __forceinline__ R midpoint_sigma(double Dk, double mmom, double T,
                                             double temprescale) {
-  double Dk = (lam / (1.0 + lam * lam) * kKBolt / kGama / (kMub)) * (kGama / bn);
   double mmomi = 1.0 / mmom;
   double D = Dk * mmomi * T * temprescale;
   return (R)sqrt(2.0 * D);
@@ -459,9 +563,8 @@ __device__ __forceinline__ R midpoint_sigma(double lam, double bn, double mmom, 
 
 // depondt.f90:142-145
 template <typename R>
-__device__ __forceinline__ R depondt_sigma(double lam, double dt, double mmom, double T,
+__device__ __forceinline__ R depondt_sigma(double Dp, double mmom, double T,
                                            double temprescale) {
-  double Dp = (2.0 * lam * kKBolt) / (dt * kGama * kMub);
   return (R)sqrt(Dp * temprescale * T / mmom);
 }
 
@@ -480,9 +583,8 @@ __device__ __forceinline__ R fsin(R x) {
 
 // Rodrigues rotation of e about bd (depondt.f90:156-179)
 template <typename R>
-__device__ __forceinline__ void rodrigues(R bd0, R bd1, R bd2, R dt, R lam, R e0, R e1, R e2,
+__device__ __forceinline__ void rodrigues(R bd0, R bd1, R bd2, R dt, R lldamp, R e0, R e1, R e2,
                                           R &o0, R &o1, R &o2) {
-  R lldamp = (R)1.0 / ((R)1.0 + lam * lam);
   R Bnorm = bd0 * bd0 + bd1 * bd1 + bd2 * bd2;
   Bnorm = fsqrt(Bnorm) + (R)1.0e-15;
   R hx = bd0 / Bnorm, hy = bd1 / Bnorm, hz = bd2 / Bnorm;
@@ -507,11 +609,8 @@ __device__ __forceinline__ void depondt_bdup(R lam, R e0, R e1, R e2, R bl0, R b
 
 // midpoint A vector (midpoint.f90:127-151 / 270-296)
 template <typename R>
-__device__ __forceinline__ void midpoint_A(R lam, R deltat, R bn, R lande, R b0, R b1, R b2, R r0,
+__device__ __forceinline__ void midpoint_A(R lam, R dt, R sqrtdt, R lande, R b0, R b1, R b2, R r0,
                                            R r1, R r2, R p0, R p1, R p2, R &A0, R &A1, R &A2) {
-  R lldamp = (R)1.0 / ((R)1.0 + lam * lam);
-  R dt = deltat * bn * (R)kGama * lldamp;
-  R sqrtdt = fsqrt(dt);
   R dtg = dt * lande;
   R sqrtdtg = sqrtdt * lande;
   R a0 = -(R)0.0 - b0 - lam * (p1 * b2 - p2 * b1);
@@ -602,45 +701,46 @@ __device__ __forceinline__ void llg_update(const HamDev<R> &h, const LLGDev<R> &
                                            int ens, int64_t q, R b0, R b1, R b2v,
                                            const Quad<R> *e, Quad<R> *p, Quad<R> *b2,
                                            Quad<R> *out) {
-  const double lam = prm.lambda(site), T = prm.temperature(site);
+  const LamConst<R> lc = prm.lamc(site);
+  const double T = prm.temperature(site);
   const Quad<R> eq = e[q];
   const R m = eq.w;
   R n0, n1, n2;
   prm.normals((int)host_site(h, site), ens, (size_t)q, n0, n1, n2);
   if (ALG == 5) {
-    const R sig = depondt_sigma<R>(lam, prm.dt, (double)m, T, prm.temprescale);
+    const R sig = depondt_sigma<R>(lc.Dp, (double)m, T, prm.temprescale);
     const R t0 = n0 * sig, t1 = n1 * sig, t2 = n2 * sig;
     const R bl0 = b0 + t0, bl1 = b1 + t1, bl2 = b2v + t2;
     if (HALF == 0) {
       R d0, d1, d2, o0, o1, o2;
-      depondt_bdup((R)lam, eq.x, eq.y, eq.z, bl0, bl1, bl2, d0, d1, d2);
-      rodrigues(d0, d1, d2, (R)prm.dt, (R)lam, eq.x, eq.y, eq.z, o0, o1, o2);
+      depondt_bdup(lc.lam, eq.x, eq.y, eq.z, bl0, bl1, bl2, d0, d1, d2);
+      rodrigues(d0, d1, d2, (R)prm.dt, lc.lldamp, eq.x, eq.y, eq.z, o0, o1, o2);
       p[q] = Quad<R>{o0, o1, o2, m};
       b2[q] = Quad<R>{d0, d1, d2, (R)0};
     } else {
       const Quad<R> pq = p[q];
       const Quad<R> bq = b2[q];
       R d0, d1, d2, o0, o1, o2;
-      depondt_bdup((R)lam, pq.x, pq.y, pq.z, bl0, bl1, bl2, d0, d1, d2);
+      depondt_bdup(lc.lam, pq.x, pq.y, pq.z, bl0, bl1, bl2, d0, d1, d2);
       d0 = (R)0.5 * d0 + (R)0.5 * bq.x;
       d1 = (R)0.5 * d1 + (R)0.5 * bq.y;
       d2 = (R)0.5 * d2 + (R)0.5 * bq.z;
-      rodrigues(d0, d1, d2, (R)prm.dt, (R)lam, eq.x, eq.y, eq.z, o0, o1, o2);
+      rodrigues(d0, d1, d2, (R)prm.dt, lc.lldamp, eq.x, eq.y, eq.z, o0, o1, o2);
       out[q] = finish<R>(o0, o1, o2, m);
     }
   } else {
-    const R sig = midpoint_sigma<R>(lam, prm.bn, (double)m, T, prm.temprescale);
+    const R sig = midpoint_sigma<R>(lc.Dk, (double)m, T, prm.temprescale);
     const R r0 = n0 * sig, r1 = n1 * sig, r2 = n2 * sig;
     R A0, A1, A2, t0, t1, t2, detAi;
     if (HALF == 0) {
-      midpoint_A((R)lam, (R)prm.dt, (R)prm.bn, (R)prm.landeg(site), b0, b1, b2v, r0, r1, r2, eq.x,
+      midpoint_A(lc.lam, lc.mdt, lc.msqrtdt, (R)prm.landeg(site), b0, b1, b2v, r0, r1, r2, eq.x,
                  eq.y, eq.z, A0, A1, A2);
       midpoint_solve(A0, A1, A2, eq.x, eq.y, eq.z, t0, t1, t2, detAi);
       p[q] = Quad<R>{(R)0.5 * (eq.x + t0 * detAi), (R)0.5 * (eq.y + t1 * detAi),
                      (R)0.5 * (eq.z + t2 * detAi), m};
     } else {
       const Quad<R> pq = p[q];
-      midpoint_A((R)lam, (R)prm.dt, (R)prm.bn, (R)prm.landeg(site), b0, b1, b2v, r0, r1, r2, pq.x,
+      midpoint_A(lc.lam, lc.mdt, lc.msqrtdt, (R)prm.landeg(site), b0, b1, b2v, r0, r1, r2, pq.x,
                  pq.y, pq.z, A0, A1, A2);
       midpoint_solve(A0, A1, A2, eq.x, eq.y, eq.z, t0, t1, t2, detAi);
       out[q] = finish<R>(t0 * detAi, t1 * detAi, t2 * detAi, m);
@@ -672,13 +772,13 @@ __global__ void __launch_bounds__(256)
 // For a periodic lattice whose neighbour lists are one translation-invariant
 // stencil per basis atom (reference site i = b + NA*(x + N1*(y + N2*z)),
 // neighbour j of basis b at basis b' and cell offset (dx, dy, dz) for every
-// cell), a workgroup of TX*TY*TZ threads owns one tile of cells.  It stages
-// emomM = e*m of the tile plus a halo of (RX, RY, RZ) cells for all basis
-// atoms in LDS as three planes, then every thread takes the sites of its
-// cell and reads neighbour j at its halo position plus off[b*W + j]
-// (= b'*HV + dz*HX*HY + dy*HX + dx).  The terms are added in list order with
-// the same products as the ELL path, so results are identical; the gather
-// traffic moves from L2 to LDS.  M = 1.
+// cell), a workgroup owns one tile of TX*TY*TZ cells.  It stages emomM = e*m
+// of the tile plus a halo of (RX, RY, RZ) cells for all basis atoms in LDS,
+// the three components of a cell side by side; then every thread takes one
+// site (cell, basis b) and reads neighbour j at its halo position plus
+// off[b*W + j] (= 3*(b'*HV + dz*HX*HY + dy*HX + dx), in doubles of R).  The
+// terms are added in list order with the same products as the ELL path, so
+// results are identical; the gather traffic moves from L2 to LDS.  M = 1.
 struct TileDev {
   int N1, N2, N3, NA;
   int TX, TY, TZ;
@@ -686,8 +786,9 @@ struct TileDev {
   int HX, HY, HZ, HV;     // halo tile extents, HV = HX*HY*HZ
   int ntx, nty, ntz;      // tiles per direction
   int W, Wd;              // exchange / DM stencil widths
-  const int *off;         // (NA*W) exchange offsets into the halo tile
-  const int *doff;        // (NA*Wd) DM offsets
+  int nbp;                // basis atoms per pass: blockDim.x = TX*TY*TZ*nbp
+  const int *off;         // (NA*W) exchange offsets into the halo tile, x3
+  const int *doff;        // (NA*Wd) DM offsets, x3
   // ucls = 1: every site of basis b has site class cls_b[b] (a lattice of one
   // species), so class rows, couplings and offsets are the same for all lanes
   // of a wavefront and are read through the scalar cache
@@ -708,106 +809,107 @@ __device__ __forceinline__ T cload(const T *p) {
   return *(const __attribute__((address_space(4))) T *)p;
 }
 
-// One site of a tile: field from the LDS planes, then the LLG half-step.
+// One site of a tile: field from the LDS tile, then the LLG half-step.
+// hbase3 is 3 * (halo index of the site's cell).
 // UCLS: the site class is the per-basis one of TileDev (wave-uniform).
 template <typename R, int ALG, int HALF, bool UCLS>
 __device__ __forceinline__ void tile_site(const HamDev<R> &h, const LLGDev<R> &prm,
-                                          const TileDev &t, const R *sx, const R *sy,
-                                          const R *sz, int hbase, int b, int x, int y, int z,
-                                          const Quad<R> *e, Quad<R> *p, Quad<R> *b2,
-                                          Quad<R> *out) {
+                                          const TileDev &t, const R *sm, int hbase3, int b,
+                                          int x, int y, int z, const Quad<R> *e, Quad<R> *p,
+                                          Quad<R> *b2, Quad<R> *out) {
   const int cell = x + t.N1 * (y + t.N2 * z);
   const int site = h.order_na == 1 ? b + t.NA * cell : b * (t.N1 * t.N2 * t.N3) + cell;
   const SiteClass sc = h.classes[UCLS ? t.cls_b[b] : h.cls[site]];
-    R s0 = 0, s1 = 0, s2 = 0, qf0 = 0, qf1 = 0, qf2 = 0;
-    {
-      // four neighbours per round: LDS reads first, then the terms in order
-      const int len = h.ex_len[sc.ex_row];
-      const R *cp = h.cpl + sc.ex_row;
-      const int *off = t.off + b * t.W;
-      int j = 0;
-      for (; j + 4 <= len; j += 4) {
-        R a0[4], a1[4], a2[4], J[4];
+  R s0 = 0, s1 = 0, s2 = 0, qf0 = 0, qf1 = 0, qf2 = 0;
+  {
+    // four neighbours per round: LDS reads first, then the terms in order
+    const int len = h.ex_len[sc.ex_row];
+    const R *cp = h.cpl + sc.ex_row;
+    const int *off = t.off + b * t.W;
+    int j = 0;
+    for (; j + 4 <= len; j += 4) {
+      R a0[4], a1[4], a2[4], J[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int k = hbase + off[j + u];
-          a0[u] = sx[k];
-          a1[u] = sy[k];
-          a2[u] = sz[k];
-          J[u] = UCLS ? cload(cp + (size_t)(j + u) * h.ex_rows) : cp[(size_t)(j + u) * h.ex_rows];
-        }
+      for (int u = 0; u < 4; ++u) {
+        const R *v = sm + (hbase3 + off[j + u]);
+        a0[u] = v[0];
+        a1[u] = v[1];
+        a2[u] = v[2];
+        J[u] = UCLS ? cload(cp + (size_t)(j + u) * h.ex_rows) : cp[(size_t)(j + u) * h.ex_rows];
+      }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          s0 = s0 + J[u] * a0[u];
-          s1 = s1 + J[u] * a1[u];
-          s2 = s2 + J[u] * a2[u];
-        }
-      }
-      for (; j < len; ++j) {
-        const int k = hbase + off[j];
-        const R J = UCLS ? cload(cp + (size_t)j * h.ex_rows) : cp[(size_t)j * h.ex_rows];
-        s0 = s0 + J * sx[k];
-        s1 = s1 + J * sy[k];
-        s2 = s2 + J * sz[k];
+      for (int u = 0; u < 4; ++u) {
+        s0 = s0 + J[u] * a0[u];
+        s1 = s1 + J[u] * a1[u];
+        s2 = s2 + J[u] * a2[u];
       }
     }
-    if (h.Wd > 0 && sc.dm_row >= 0) {
-      const int len = h.dm_len[sc.dm_row];
-      const int *off = t.doff + b * t.Wd;
-      for (int j = 0; j < len; ++j) {
-        const int k = hbase + off[j];
-        const size_t r = (size_t)j * h.dm_rows + sc.dm_row;
-        const R D0 = UCLS ? cload(h.dvx + r) : h.dvx[r], D1 = UCLS ? cload(h.dvy + r) : h.dvy[r],
-                D2 = UCLS ? cload(h.dvz + r) : h.dvz[r];
-        const R m0 = sx[k], m1 = sy[k], m2 = sz[k];
-        s0 = s0 + D2 * m1 - D1 * m2;
-        s1 = s1 + D0 * m2 - D2 * m0;
-        s2 = s2 + D1 * m0 - D0 * m1;
-      }
+    for (; j < len; ++j) {
+      const R *v = sm + (hbase3 + off[j]);
+      const R J = UCLS ? cload(cp + (size_t)j * h.ex_rows) : cp[(size_t)j * h.ex_rows];
+      s0 = s0 + J * v[0];
+      s1 = s1 + J * v[1];
+      s2 = s2 + J * v[2];
     }
-    if (sc.an_row >= 0) {
-      const int ty_ = h.an_type[sc.an_row];
-      const int k = hbase + b * t.HV;
-      const R m0 = sx[k], m1 = sy[k], m2 = sz[k];
-      if (ty_ == 1) {
-        uniaxial(h, sc.an_row, m0, m1, m2, s0, s1, s2);
-      } else if (ty_ == 2) {
-        cubic(h, sc.an_row, m0, m1, m2, s0, s1, s2);
-      } else if (ty_ == 7) {
-        R c0 = 0, c1 = 0, c2 = 0;
-        uniaxial(h, sc.an_row, m0, m1, m2, s0, s1, s2);
-        cubic(h, sc.an_row, m0, m1, m2, c0, c1, c2);
-        const R sb = h.an_sb[sc.an_row];
-        qf0 = qf0 + c0 * sb;
-        qf1 = qf1 + c1 * sb;
-        qf2 = qf2 + c2 * sb;
-      }
+  }
+  if (h.Wd > 0 && sc.dm_row >= 0) {
+    const int len = h.dm_len[sc.dm_row];
+    const int *off = t.doff + b * t.Wd;
+    for (int j = 0; j < len; ++j) {
+      const R *v = sm + (hbase3 + off[j]);
+      const size_t r = (size_t)j * h.dm_rows + sc.dm_row;
+      const R D0 = UCLS ? cload(h.dvx + r) : h.dvx[r], D1 = UCLS ? cload(h.dvy + r) : h.dvy[r],
+              D2 = UCLS ? cload(h.dvz + r) : h.dvz[r];
+      const R m0 = v[0], m1 = v[1], m2 = v[2];
+      s0 = s0 + D2 * m1 - D1 * m2;
+      s1 = s1 + D0 * m2 - D2 * m0;
+      s2 = s2 + D1 * m0 - D0 * m1;
     }
-    R x0 = 0, x1 = 0, x2 = 0;
-    if (h.ext_mode == 1) {
-      x0 = h.ext_u[0];
-      x1 = h.ext_u[1];
-      x2 = h.ext_u[2];
-    } else if (h.ext_mode == 2) {
-      const Quad<R> v = h.ext_q[site];
-      x0 = v.x;
-      x1 = v.y;
-      x2 = v.z;
+  }
+  if (sc.an_row >= 0) {
+    const int ty_ = h.an_type[sc.an_row];
+    const R *v = sm + (hbase3 + 3 * b * t.HV);
+    const R m0 = v[0], m1 = v[1], m2 = v[2];
+    if (ty_ == 1) {
+      uniaxial(h, sc.an_row, m0, m1, m2, s0, s1, s2);
+    } else if (ty_ == 2) {
+      cubic(h, sc.an_row, m0, m1, m2, s0, s1, s2);
+    } else if (ty_ == 7) {
+      R c0 = 0, c1 = 0, c2 = 0;
+      uniaxial(h, sc.an_row, m0, m1, m2, s0, s1, s2);
+      cubic(h, sc.an_row, m0, m1, m2, c0, c1, c2);
+      const R sb = h.an_sb[sc.an_row];
+      qf0 = qf0 + c0 * sb;
+      qf1 = qf1 + c1 * sb;
+      qf2 = qf2 + c2 * sb;
     }
-    const R b0 = (R)0 + s0 + (qf0 + x0 + (R)0);
-    const R b1 = (R)0 + s1 + (qf1 + x1 + (R)0);
-    const R b2v = (R)0 + s2 + (qf2 + x2 + (R)0);
-    llg_update<R, ALG, HALF>(h, prm, site, 0, site, b0, b1, b2v, e, p, b2, out);
+  }
+  R x0 = 0, x1 = 0, x2 = 0;
+  if (h.ext_mode == 1) {
+    x0 = h.ext_u[0];
+    x1 = h.ext_u[1];
+    x2 = h.ext_u[2];
+  } else if (h.ext_mode == 2) {
+    const Quad<R> v = h.ext_q[site];
+    x0 = v.x;
+    x1 = v.y;
+    x2 = v.z;
+  }
+  const R b0 = (R)0 + s0 + (qf0 + x0 + (R)0);
+  const R b1 = (R)0 + s1 + (qf1 + x1 + (R)0);
+  const R b2v = (R)0 + s2 + (qf2 + x2 + (R)0);
+  llg_update<R, ALG, HALF>(h, prm, site, 0, site, b0, b1, b2v, e, p, b2, out);
 }
 
+// Staging items per thread and round (loads issued together, then stored).
+constexpr int kTileStage = 4;
+
 template <typename R, int ALG, int HALF>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
     k_llg_tile(HamDev<R> h, LLGDev<R> prm, TileDev t, const Quad<R> *e, Quad<R> *p,
                Quad<R> *b2, Quad<R> *out) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
-  R *sx = reinterpret_cast<R *>(lds_raw);
-  R *sy = sx + (size_t)t.HV * t.NA;
-  R *sz = sy + (size_t)t.HV * t.NA;
+  R *sm = reinterpret_cast<R *>(lds_raw);
   const Quad<R> *src = HALF == 0 ? e : (const Quad<R> *)p;
   // tile coordinates (XCD-aware block order, z slowest)
   const int64_t tb = xcd_block();
@@ -817,56 +919,76 @@ __global__ void __launch_bounds__(256)
   const int ty = (int)((tb / t.ntx) % t.nty);
   const int tx = (int)(tb % t.ntx);
   const int X0 = tx * t.TX, Y0 = ty * t.TY, Z0 = tz * t.TZ;
-  // stage the halo tile: thread t takes halo cells t, t + blockDim, ...;
-  // the cell coordinates advance by mixed-radix addition (no divisions in
-  // the loop) and wrap periodically
+  const int ncell = t.N1 * t.N2 * t.N3;
+  // stage the halo tile of all basis atoms: item = b*HV + l, l the halo
+  // cell; thread t takes items t, t + blockDim, ...  The digits (hx, hy,
+  // hz, hb) of the item advance by mixed-radix addition (no divisions in
+  // the loop) and the cell coordinates wrap periodically.
   {
-    const int nb = blockDim.x;
-    int hx = threadIdx.x % t.HX, hy = (threadIdx.x / t.HX) % t.HY, hz = threadIdx.x / (t.HX * t.HY);
-    const int ax = nb % t.HX, ay = (nb / t.HX) % t.HY, az = nb / (t.HX * t.HY);
-    const int plane = t.HX * t.HY;
-    const int ncell = t.N1 * t.N2 * t.N3;
-    while (hz < t.HZ) {
-      int x = X0 + hx - t.RX, y = Y0 + hy - t.RY, z = Z0 + hz - t.RZ;
-      while (x < 0) x += t.N1;
-      while (x >= t.N1) x -= t.N1;
-      while (y < 0) y += t.N2;
-      while (y >= t.N2) y -= t.N2;
-      while (z < 0) z += t.N3;
-      while (z >= t.N3) z -= t.N3;
-      const int cell = x + t.N1 * (y + t.N2 * z);
-      const int l = hz * plane + hy * t.HX + hx;
-      for (int b = 0; b < t.NA; ++b) {
-        const int slot = h.order_na == 1 ? b + t.NA * cell : b * ncell + cell;
-        const Quad<R> v = src[slot];
-        sx[b * t.HV + l] = v.x * v.w;
-        sy[b * t.HV + l] = v.y * v.w;
-        sz[b * t.HV + l] = v.z * v.w;
+    const int nth = blockDim.x, plane = t.HX * t.HY;
+    const int total = t.NA * t.HV;
+    const int i0 = threadIdx.x;
+    int hx = i0 % t.HX, hy = (i0 / t.HX) % t.HY, hz = (i0 / plane) % t.HZ, hb = i0 / t.HV;
+    const int ax = nth % t.HX, ay = (nth / t.HX) % t.HY, az = (nth / plane) % t.HZ,
+              ab = nth / t.HV;
+    for (int base = i0; base < total; base += kTileStage * nth) {
+      int slot[kTileStage], dst[kTileStage];
+#pragma unroll
+      for (int u = 0; u < kTileStage; ++u) {
+        int x = X0 + hx - t.RX, y = Y0 + hy - t.RY, z = Z0 + hz - t.RZ;
+        while (x < 0) x += t.N1;
+        while (x >= t.N1) x -= t.N1;
+        while (y < 0) y += t.N2;
+        while (y >= t.N2) y -= t.N2;
+        while (z < 0) z += t.N3;
+        while (z >= t.N3) z -= t.N3;
+        const int cell = x + t.N1 * (y + t.N2 * z);
+        const bool ok = base + u * nth < total;
+        slot[u] = !ok ? -1 : h.order_na == 1 ? hb + t.NA * cell : hb * ncell + cell;
+        dst[u] = 3 * (hb * t.HV + hz * plane + hy * t.HX + hx);
+        hx += ax;
+        hy += ay;
+        hz += az;
+        hb += ab;
+        if (hx >= t.HX) {
+          hx -= t.HX;
+          hy += 1;
+        }
+        if (hy >= t.HY) {
+          hy -= t.HY;
+          hz += 1;
+        }
+        if (hz >= t.HZ) {
+          hz -= t.HZ;
+          hb += 1;
+        }
       }
-      hx += ax;
-      hy += ay;
-      hz += az;
-      if (hx >= t.HX) {
-        hx -= t.HX;
-        hy += 1;
-      }
-      if (hy >= t.HY) {
-        hy -= t.HY;
-        hz += 1;
-      }
+      Quad<R> v[kTileStage];
+#pragma unroll
+      for (int u = 0; u < kTileStage; ++u)
+        if (slot[u] >= 0) v[u] = src[slot[u]];
+#pragma unroll
+      for (int u = 0; u < kTileStage; ++u)
+        if (slot[u] >= 0) {
+          R *d = sm + dst[u];
+          d[0] = v[u].x * v[u].w;
+          d[1] = v[u].y * v[u].w;
+          d[2] = v[u].z * v[u].w;
+        }
     }
   }
   __syncthreads();
-  const int lx = threadIdx.x % t.TX, ly = (threadIdx.x / t.TX) % t.TY, lz = threadIdx.x / (t.TX * t.TY);
+  const int tc = t.TX * t.TY * t.TZ;
+  const int lcell = threadIdx.x % tc;
+  const int lx = lcell % t.TX, ly = (lcell / t.TX) % t.TY, lz = lcell / (t.TX * t.TY);
   const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
-  if (lz >= t.TZ || x >= t.N1 || y >= t.N2 || z >= t.N3) return;
-  const int hbase = (lz + t.RZ) * t.HX * t.HY + (ly + t.RY) * t.HX + (lx + t.RX);
-  if (t.ucls) {
-    for (int b = 0; b < t.NA; ++b)
-      tile_site<R, ALG, HALF, true>(h, prm, t, sx, sy, sz, hbase, b, x, y, z, e, p, b2, out);
-  } else {
-    for (int b = 0; b < t.NA; ++b)
-      tile_site<R, ALG, HALF, false>(h, prm, t, sx, sy, sz, hbase, b, x, y, z, e, p, b2, out);
+  if (x >= t.N1 || y >= t.N2 || z >= t.N3) return;
+  const int hcell = (lz + t.RZ) * t.HX * t.HY + (ly + t.RY) * t.HX + (lx + t.RX);
+  for (int b = threadIdx.x / tc; b < t.NA; b += t.nbp) {
+    if (t.ucls)
+      tile_site<R, ALG, HALF, true>(h, prm, t, sm, 3 * hcell, b, x, y, z, e, p, b2, out);
+    else
+      tile_site<R, ALG, HALF, false>(h, prm, t, sm, 3 * hcell, b, x, y, z, e, p, b2, out);
   }
 }
 

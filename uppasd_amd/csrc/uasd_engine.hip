@@ -721,7 +721,9 @@ bool tile_fits(const uasd_ctx *c, int k) {
   const int rx = std::max(c->st_ex.rx, c->st_dm.rx), ry = std::max(c->st_ex.ry, c->st_dm.ry),
             rz = std::max(c->st_ex.rz, c->st_dm.rz);
   const int hv = (sh[0] + 2 * rx) * (sh[1] + 2 * ry) * (sh[2] + 2 * rz);
-  return uasd::tile_lds_bytes(hv, c->lat_na, c->rsize) <= 64 * 1024;
+  if (sh[0] * sh[1] * sh[2] * c->lat_na > 1024) return false;  // one thread per site
+  return uasd::tile_lds_bytes(hv, c->lat_na, c->rsize, sh[0] + sh[1] + sh[2] + 2 * (rx + ry + rz)) <=
+         64 * 1024;
 }
 
 // TileDev for shape k, uploading its halo offsets when the shape changes.
@@ -747,7 +749,6 @@ int make_tile(uasd_ctx *c, int k, uasd::TileDev &t) {
   t.ntz = (t.N3 + t.TZ - 1) / t.TZ;
   t.W = c->st_ex.W;
   t.Wd = c->Wd > 0 ? c->st_dm.W : 0;
-  t.nbp = std::min(t.NA, 1024 / (t.TX * t.TY * t.TZ));
   if (c->tile_off_shape != k) {
     std::vector<int> off((size_t)t.NA * (t.W + t.Wd), 0);
     auto fill = [&](const uasd_ctx::Stencil &st, int W, int base) {

@@ -774,8 +774,8 @@ __global__ void __launch_bounds__(256)
 // neighbour j of basis b at basis b' and cell offset (dx, dy, dz) for every
 // cell), a workgroup owns one tile of TX*TY*TZ cells.  It stages emomM = e*m
 // of the tile plus a halo of (RX, RY, RZ) cells for all basis atoms in LDS,
-// the three components of a cell side by side; then every thread takes one
-// site (cell, basis b) and reads neighbour j at its halo position plus
+// the three components of a cell side by side; then each of the
+// TX*TY*TZ*NA threads takes one site (cell, basis b) and reads neighbour j at its halo position plus
 // off[b*W + j] (= 3*(b'*HV + dz*HX*HY + dy*HX + dx), in doubles of R).  The
 // terms are added in list order with the same products as the ELL path, so
 // results are identical; the gather traffic moves from L2 to LDS.  M = 1.
@@ -786,7 +786,6 @@ struct TileDev {
   int HX, HY, HZ, HV;     // halo tile extents, HV = HX*HY*HZ
   int ntx, nty, ntz;      // tiles per direction
   int W, Wd;              // exchange / DM stencil widths
-  int nbp;                // basis atoms per pass: blockDim.x = TX*TY*TZ*nbp
   const int *off;         // (NA*W) exchange offsets into the halo tile, x3
   const int *doff;        // (NA*Wd) DM offsets, x3
   // ucls = 1: every site of basis b has site class cls_b[b] (a lattice of one
@@ -920,32 +919,40 @@ __global__ void __launch_bounds__(1024)
   const int tx = (int)(tb % t.ntx);
   const int X0 = tx * t.TX, Y0 = ty * t.TY, Z0 = tz * t.TZ;
   const int ncell = t.N1 * t.N2 * t.N3;
+  // wrapped lattice coordinates of the halo rows, pre-multiplied into cell
+  // index terms: x, N1*y, N1*N2*z (periodic images of any multiplicity)
+  int *wrap = reinterpret_cast<int *>(sm + (size_t)3 * t.NA * t.HV);
+  for (int i = threadIdx.x; i < t.HX + t.HY + t.HZ; i += blockDim.x) {
+    int v;
+    if (i < t.HX) {
+      v = ((X0 + i - t.RX) % t.N1 + t.N1) % t.N1;
+    } else if (i < t.HX + t.HY) {
+      v = t.N1 * (((Y0 + i - t.HX - t.RY) % t.N2 + t.N2) % t.N2);
+    } else {
+      v = t.N1 * t.N2 * (((Z0 + i - t.HX - t.HY - t.RZ) % t.N3 + t.N3) % t.N3);
+    }
+    wrap[i] = v;
+  }
+  __syncthreads();
   // stage the halo tile of all basis atoms: item = b*HV + l, l the halo
-  // cell; thread t takes items t, t + blockDim, ...  The digits (hx, hy,
-  // hz, hb) of the item advance by mixed-radix addition (no divisions in
-  // the loop) and the cell coordinates wrap periodically.
+  // cell, stored at 3*item; thread t takes items t, t + blockDim, ...  The
+  // digits (hx, hy, hz, hb) of the item advance by mixed-radix addition (no
+  // divisions in the loop).
   {
     const int nth = blockDim.x, plane = t.HX * t.HY;
     const int total = t.NA * t.HV;
     const int i0 = threadIdx.x;
+    const int *wy = wrap + t.HX, *wz = wrap + t.HX + t.HY;
     int hx = i0 % t.HX, hy = (i0 / t.HX) % t.HY, hz = (i0 / plane) % t.HZ, hb = i0 / t.HV;
     const int ax = nth % t.HX, ay = (nth / t.HX) % t.HY, az = (nth / plane) % t.HZ,
               ab = nth / t.HV;
     for (int base = i0; base < total; base += kTileStage * nth) {
-      int slot[kTileStage], dst[kTileStage];
+      int slot[kTileStage];
 #pragma unroll
       for (int u = 0; u < kTileStage; ++u) {
-        int x = X0 + hx - t.RX, y = Y0 + hy - t.RY, z = Z0 + hz - t.RZ;
-        while (x < 0) x += t.N1;
-        while (x >= t.N1) x -= t.N1;
-        while (y < 0) y += t.N2;
-        while (y >= t.N2) y -= t.N2;
-        while (z < 0) z += t.N3;
-        while (z >= t.N3) z -= t.N3;
-        const int cell = x + t.N1 * (y + t.N2 * z);
         const bool ok = base + u * nth < total;
+        const int cell = wrap[hx] + wy[hy] + wz[hz];  // digits stay in range
         slot[u] = !ok ? -1 : h.order_na == 1 ? hb + t.NA * cell : hb * ncell + cell;
-        dst[u] = 3 * (hb * t.HV + hz * plane + hy * t.HX + hx);
         hx += ax;
         hy += ay;
         hz += az;
@@ -970,7 +977,7 @@ __global__ void __launch_bounds__(1024)
 #pragma unroll
       for (int u = 0; u < kTileStage; ++u)
         if (slot[u] >= 0) {
-          R *d = sm + dst[u];
+          R *d = sm + 3 * (base + u * nth);
           d[0] = v[u].x * v[u].w;
           d[1] = v[u].y * v[u].w;
           d[2] = v[u].z * v[u].w;
@@ -984,12 +991,11 @@ __global__ void __launch_bounds__(1024)
   const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
   if (x >= t.N1 || y >= t.N2 || z >= t.N3) return;
   const int hcell = (lz + t.RZ) * t.HX * t.HY + (ly + t.RY) * t.HX + (lx + t.RX);
-  for (int b = threadIdx.x / tc; b < t.NA; b += t.nbp) {
-    if (t.ucls)
-      tile_site<R, ALG, HALF, true>(h, prm, t, sm, 3 * hcell, b, x, y, z, e, p, b2, out);
-    else
-      tile_site<R, ALG, HALF, false>(h, prm, t, sm, 3 * hcell, b, x, y, z, e, p, b2, out);
-  }
+  const int b = threadIdx.x / tc;  // blockDim.x = tc * NA
+  if (t.ucls)
+    tile_site<R, ALG, HALF, true>(h, prm, t, sm, 3 * hcell, b, x, y, z, e, p, b2, out);
+  else
+    tile_site<R, ALG, HALF, false>(h, prm, t, sm, 3 * hcell, b, x, y, z, e, p, b2, out);
 }
 
 // ----------------------------------------------------------------- MC

@@ -101,9 +101,9 @@ hipError_t launch_llg_tile(hipStream_t s, const TileDev &t, int alg, int half,
                            Quad<R> *p, Quad<R> *b2, Quad<R> *out) {
   const int ntiles = t.ntx * t.nty * t.ntz;
   if (ntiles == 0) return hipSuccess;
-  const int threads = t.TX * t.TY * t.TZ * t.nbp;
-  if (t.nbp < 1 || t.nbp > t.NA || threads > 1024 || h.M != 1) return hipErrorInvalidValue;
-  const size_t lds = tile_lds_bytes(t.HV, t.NA, sizeof(R));
+  const int threads = t.TX * t.TY * t.TZ * t.NA;
+  if (threads > 1024 || h.M != 1) return hipErrorInvalidValue;
+  const size_t lds = tile_lds_bytes(t.HV, t.NA, sizeof(R), t.HX + t.HY + t.HZ);
   const dim3 grid((ntiles + kXCD - 1) / kXCD * kXCD), block(threads);
 #define UASD_TILE_CASE(A, H)                                                                    \
   if (alg == A && half == H) {                                                                  \

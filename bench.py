@@ -338,7 +338,7 @@ def list_kernel_compare(U, sysd, args, order, e0, steps=300):
     eng.autotune_llg(args.solver, steps=3)
     eng.sd_steps(args.solver, 20)
     eng.sync()
-    eng.set_kernel_timing(True)
+    eng.set_kernel_timing(True, args.timing_every)
     t0 = time.perf_counter()
     eng.sd_steps(args.solver, steps)
     eng.sync()
@@ -367,6 +367,8 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--solver", type=int, choices=[1, 5], default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--timing-every", type=int, default=16,
+                    help="HIP events around one step in this many (events cost time of their own)")
     ap.add_argument("--gather", type=int, default=0,
                     help="fixed gather batch (2/4/8/16) instead of autotuning")
     ap.add_argument("--block", type=int, default=256,
@@ -421,7 +423,7 @@ def main():
 
     eng.sd_steps(args.solver, args.warmup)
     eng.sync()
-    eng.set_kernel_timing(True)
+    eng.set_kernel_timing(True, args.timing_every)
     eng.sync()
     t0 = time.perf_counter()
     eng.sd_steps(args.solver, args.steps)
@@ -473,7 +475,9 @@ def main():
                        "tile_shape": U.engine.TILE_SHAPES[tile - 1] if tile else None,
                        "site_order": order,
                        "autotuned": not args.gather, "tune_s": round(tune_s, 3)},
-            "kernel_ms": {"predictor": round(t_pred * 1e3, 4), "corrector": round(t_corr * 1e3, 4)},
+            "kernel_ms": {"predictor": round(t_pred * 1e3, 4), "corrector": round(t_corr * 1e3, 4),
+                          "timed_launches": [n_pred, n_corr],
+                          "method": f"HIP events around one step in {args.timing_every}"},
             "bytes_per_site": kb,
             "index_bytes": ib,
             "max_norm_drift": norm_drift,

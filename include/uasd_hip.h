@@ -170,7 +170,11 @@ int uasd_reset_accumulators(uasd_ctx *ctx);
 /* ---- colouring used by the MC sweeps ---- */
 int uasd_get_coloring(uasd_ctx *ctx, int *ncolors, int *color_of_site);
 
-/* ---- per-kernel timing with HIP events on the context stream ---- */
+/* ---- per-kernel timing with HIP events on the context stream ----
+ * enable = n > 0 records a pair of events around one launch in n of each
+ * kind (1: every launch); 0 switches timing off.  Events between kernels
+ * cost time of their own, so a sampled run keeps the stream close to an
+ * untimed one. */
 int uasd_set_kernel_timing(uasd_ctx *ctx, int enable);
 /* kind: 0 predictor, 1 corrector, 2 field, 3 MC colour pass, 4 LLG graph
  * replay (K steps), 5 MC graph replay (K sweeps) */

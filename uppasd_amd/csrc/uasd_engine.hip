@@ -184,6 +184,8 @@ struct uasd_ctx {
   // Per-kernel timing: an event pair is recorded around each launch without
   // a host wait; pairs are resolved when the totals are read.
   bool timing = false;
+  int timing_every = 1;       // time one launch in timing_every of each kind
+  int64_t timing_tick[6] = {};
   Timing timers[6];  // 0 predictor, 1 corrector, 2 field, 3 MC, 4 LLG graph, 5 MC graph
   std::vector<hipEvent_t> ev_pool;  // free events
   struct Pending {
@@ -445,8 +447,9 @@ hipEvent_t take_event(uasd_ctx *c) {
   return e;
 }
 
-void tic(uasd_ctx *c) {
+void tic(uasd_ctx *c, int kind) {
   if (!c->timing || c->capturing) return;
+  if (c->timing_tick[kind]++ % c->timing_every != 0) return;
   c->cur = take_event(c);
   if (c->cur) hipEventRecord(c->cur, c->stream);
 }
@@ -528,12 +531,12 @@ int run_llg_half(uasd_ctx *c, int alg, int half, bool fused, bool write_e,
     if (tile_fits(c, c->tune[0].tile)) {
       uasd::TileDev t;
       if (int r = make_tile(c, c->tune[0].tile, t)) return r;
-      tic(c);
+      tic(c, half);
       HIP_TRY(uasd::launch_llg_tile<R>(c->stream, t, alg, half, h, prm, e, p, b2, out));
       return toc(c, half);
     }
   }
-  tic(c);
+  tic(c, half);
   HIP_TRY(uasd::launch_llg<R>(c->stream, c->tune[0], alg, half, fused, h, prm, e, p, b2, beff,
                               out));
   return toc(c, half);
@@ -550,7 +553,7 @@ int run_field(uasd_ctx *c, bool from_scratch, bool want_energy, bool parts = fal
     HIP_TRY(c->energy.ensure((size_t)c->N * c->M * sizeof(double)));
     en = c->energy.as<double>();
   }
-  tic(c);
+  tic(c, 2);
   HIP_TRY(uasd::launch_field<R>(c->stream, c->tune[1], h, src, quads<R>(c, c->field), en,
                                 parts ? quads<R>(c, c->field, 1) : nullptr));
   return toc(c, 2);
@@ -721,7 +724,8 @@ bool tile_fits(const uasd_ctx *c, int k) {
   const int rx = std::max(c->st_ex.rx, c->st_dm.rx), ry = std::max(c->st_ex.ry, c->st_dm.ry),
             rz = std::max(c->st_ex.rz, c->st_dm.rz);
   const int hv = (sh[0] + 2 * rx) * (sh[1] + 2 * ry) * (sh[2] + 2 * rz);
-  if (sh[0] * sh[1] * sh[2] * c->lat_na > 1024) return false;  // one thread per site
+  // one thread per site; whole wavefronts per basis atom
+  if (sh[0] * sh[1] * sh[2] * c->lat_na > 1024 || (sh[0] * sh[1] * sh[2]) % 64 != 0) return false;
   return uasd::tile_lds_bytes(hv, c->lat_na, c->rsize, sh[0] + sh[1] + sh[2] + 2 * (rx + ry + rz)) <=
          64 * 1024;
 }
@@ -1356,7 +1360,7 @@ int replay_graph(uasd_ctx *c, const std::tuple<int, int, int, uint64_t> &key, in
     HIP_TRY(uasd::launch_counter(c->stream, idx, c->ctr.as<uint64_t>(), host_ctr, 0));
     c->ctr_valid[idx] = true;
   }
-  tic(c);
+  tic(c, timer);
   HIP_TRY(hipGraphLaunch(it->second.exec, c->stream));
   if (int r = toc(c, timer)) return r;
   host_ctr += (uint64_t)K;
@@ -1488,7 +1492,7 @@ static int mc_run(uasd_ctx *c, uasd::MCParams mp, int64_t nsweeps) {
     for (int col = 0; col < c->ncolors; ++col) {
       mp.sites = sites_dev + c->color_ptr[col];
       mp.nsites = c->color_ptr[col + 1] - c->color_ptr[col];
-      tic(c);
+      tic(c, 3);
       if (c->prec == UASD_FP64) {
         HIP_TRY(uasd::launch_mc<double>(c->stream, c->tune[2], ham_view<double>(c), mp,
                                         quads<double>(c, c->spins, 0)));
@@ -1852,8 +1856,11 @@ int uasd_get_coloring(uasd_ctx *c, int *ncolors, int *color_of_site) {
 int uasd_set_kernel_timing(uasd_ctx *c, int enable) {
   if (int r = check_ctx(c)) return r;
   if (int r = resolve_timing(c)) return r;
-  c->timing = enable != 0;
+  if (enable < 0) return fail(UASD_ERR_ARG, "enable must be >= 0");
+  c->timing = enable > 0;
+  c->timing_every = enable > 0 ? enable : 1;
   for (auto &t : c->timers) t = Timing{};
+  for (auto &k : c->timing_tick) k = 0;
   return UASD_OK;
 }
 

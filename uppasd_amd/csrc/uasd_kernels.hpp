@@ -818,7 +818,17 @@ __device__ __forceinline__ void tile_site(const HamDev<R> &h, const LLGDev<R> &p
                                           Quad<R> *b2, Quad<R> *out) {
   const int cell = x + t.N1 * (y + t.N2 * z);
   const int site = h.order_na == 1 ? b + t.NA * cell : b * (t.N1 * t.N2 * t.N3) + cell;
-  const SiteClass sc = h.classes[UCLS ? t.cls_b[b] : h.cls[site]];
+  SiteClass sc;
+  if (UCLS) {
+    // wave-uniform values kept in scalar registers
+    const SiteClass *cp = h.classes + __builtin_amdgcn_readfirstlane(t.cls_b[b]);
+    sc.ex_row = __builtin_amdgcn_readfirstlane(cload(&cp->ex_row));
+    sc.dm_row = __builtin_amdgcn_readfirstlane(cload(&cp->dm_row));
+    sc.an_row = __builtin_amdgcn_readfirstlane(cload(&cp->an_row));
+    sc.pad = 0;
+  } else {
+    sc = h.classes[h.cls[site]];
+  }
   R s0 = 0, s1 = 0, s2 = 0, qf0 = 0, qf1 = 0, qf2 = 0;
   {
     // four neighbours per round: LDS reads first, then the terms in order
@@ -830,7 +840,7 @@ __device__ __forceinline__ void tile_site(const HamDev<R> &h, const LLGDev<R> &p
       R a0[4], a1[4], a2[4], J[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const R *v = sm + (hbase3 + off[j + u]);
+        const R *v = sm + (hbase3 + cload(off + j + u));
         a0[u] = v[0];
         a1[u] = v[1];
         a2[u] = v[2];
@@ -844,7 +854,7 @@ __device__ __forceinline__ void tile_site(const HamDev<R> &h, const LLGDev<R> &p
       }
     }
     for (; j < len; ++j) {
-      const R *v = sm + (hbase3 + off[j]);
+      const R *v = sm + (hbase3 + cload(off + j));
       const R J = UCLS ? cload(cp + (size_t)j * h.ex_rows) : cp[(size_t)j * h.ex_rows];
       s0 = s0 + J * v[0];
       s1 = s1 + J * v[1];
@@ -855,7 +865,7 @@ __device__ __forceinline__ void tile_site(const HamDev<R> &h, const LLGDev<R> &p
     const int len = h.dm_len[sc.dm_row];
     const int *off = t.doff + b * t.Wd;
     for (int j = 0; j < len; ++j) {
-      const R *v = sm + (hbase3 + off[j]);
+      const R *v = sm + (hbase3 + cload(off + j));
       const size_t r = (size_t)j * h.dm_rows + sc.dm_row;
       const R D0 = UCLS ? cload(h.dvx + r) : h.dvx[r], D1 = UCLS ? cload(h.dvy + r) : h.dvy[r],
               D2 = UCLS ? cload(h.dvz + r) : h.dvz[r];
@@ -991,7 +1001,10 @@ __global__ void __launch_bounds__(1024)
   const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
   if (x >= t.N1 || y >= t.N2 || z >= t.N3) return;
   const int hcell = (lz + t.RZ) * t.HX * t.HY + (ly + t.RY) * t.HX + (lx + t.RX);
-  const int b = threadIdx.x / tc;  // blockDim.x = tc * NA
+  // blockDim.x = tc * NA with tc a multiple of 64: b is the same for all
+  // lanes of a wavefront, so the stencil offsets (and, for one species, the
+  // couplings) are read with scalar loads
+  const int b = __builtin_amdgcn_readfirstlane(threadIdx.x / tc);
   if (t.ucls)
     tile_site<R, ALG, HALF, true>(h, prm, t, sm, 3 * hcell, b, x, y, z, e, p, b2, out);
   else

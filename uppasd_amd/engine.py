@@ -197,8 +197,10 @@ class Engine:
         B.check(B.lib().uasd_get_coloring(self._h, C.byref(nc), B.ptr(col)), "uasd_get_coloring")
         return nc.value, col
 
-    def set_kernel_timing(self, enable: bool):
-        B.check(B.lib().uasd_set_kernel_timing(self._h, 1 if enable else 0), "uasd_set_kernel_timing")
+    def set_kernel_timing(self, enable: bool, every: int = 1):
+        """HIP events around one launch in `every` of each kernel kind."""
+        B.check(B.lib().uasd_set_kernel_timing(self._h, int(every) if enable else 0),
+                "uasd_set_kernel_timing")
 
     def kernel_timing(self, kind: int):
         n = C.c_int64(0)

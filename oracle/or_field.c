@@ -12,10 +12,14 @@
  * time-dependent field is taken as zero as in sd_mphase without pulses.
  */
 #include "uasd_oracle.h"
+#include <math.h>
 #include <stddef.h>
 
 #define V3(a, c, i, k, N) (a)[(c) + 3 * ((size_t)(i) + (size_t)(N) * (k))]
 
+/* The exchange sum accumulates by fused multiply-add, as the reference's
+ * release build (-Ofast -march=native, cmake/SetFortranFlags.cmake:77-157)
+ * contracts it; the HIP kernels do the same, so both round alike. */
 static void heisenberg_field(const or_ham *h, int i, int k, int N, const double *emomM,
                              double f[3])
 {
@@ -23,9 +27,9 @@ static void heisenberg_field(const or_ham *h, int i, int k, int N, const double 
   for (int j = 0; j < h->nlistsize[i]; ++j) {
     double J = h->ncoup[j + (size_t)mnn * i];
     int nb = h->nlist[j + (size_t)mnn * i] - 1;
-    f[0] = f[0] + J * V3(emomM, 0, nb, k, N);
-    f[1] = f[1] + J * V3(emomM, 1, nb, k, N);
-    f[2] = f[2] + J * V3(emomM, 2, nb, k, N);
+    f[0] = fma(J, V3(emomM, 0, nb, k, N), f[0]);
+    f[1] = fma(J, V3(emomM, 1, nb, k, N), f[1]);
+    f[2] = fma(J, V3(emomM, 2, nb, k, N), f[2]);
   }
 }
 

@@ -1,8 +1,9 @@
 """Builds libuasd_hip.so in-tree for gfx950 (hipcc, no JIT cache).
 
 The FP64 kernels and the context are compiled with -ffp-contract=off so that
-every multiply and add rounds on its own, as the reference Fortran does; the
-FP32 kernels allow contraction.  Packed FP32 instructions are disabled for all
+every multiply and add rounds on its own, except the exchange sums, which
+call fma() explicitly (the reference's release build contracts them too);
+the oracle rounds the same way.  The FP32 kernels allow contraction.  Packed FP32 instructions are disabled for all
 device code (-packed-fp32-ops).  The host set-up (neighbour map builder) is
 plain C++ with OpenMP, built by g++.
 """

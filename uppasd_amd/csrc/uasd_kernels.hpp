@@ -263,7 +263,10 @@ __device__ __forceinline__ void uniform4(uint64_t seed, uint32_t tag, uint64_t s
 // ------------------------------------------------------------- field terms
 // hamiltonianactions.f90:431-466 (exchange) and 546-580 (DM), summed in list
 // order; 842-879 / 883-919 anisotropy; 183-248 assembly.  The moments are
-// gathered as q.xyz * q.w (emomM).
+// gathered as q.xyz * q.w (emomM).  Exchange terms accumulate by fused
+// multiply-add, field = fma(J, emomM, field), as the reference's release
+// build contracts them (-Ofast -march=native, cmake/SetFortranFlags.cmake);
+// everything else rounds each multiply and add on its own.
 
 template <typename R>
 __device__ __forceinline__ void moment(const Quad<R> *__restrict__ s, size_t q, R &a, R &b, R &c) {
@@ -360,9 +363,9 @@ __device__ __forceinline__ void exchange_sum(const HamDev<R> &h, const I *__rest
     }
 #pragma unroll
     for (int u = 0; u < G; ++u) {
-      s0 = s0 + J[u] * (v[u].x * v[u].w);
-      s1 = s1 + J[u] * (v[u].y * v[u].w);
-      s2 = s2 + J[u] * (v[u].z * v[u].w);
+      s0 = fma(J[u], v[u].x * v[u].w, s0);
+      s1 = fma(J[u], v[u].y * v[u].w, s1);
+      s2 = fma(J[u], v[u].z * v[u].w, s2);
     }
   }
   if (j < len) {
@@ -382,9 +385,9 @@ __device__ __forceinline__ void exchange_sum(const HamDev<R> &h, const I *__rest
 #pragma unroll
     for (int u = 0; u < G; ++u) {
       if (j + u < len) {
-        s0 = s0 + J[u] * (v[u].x * v[u].w);
-        s1 = s1 + J[u] * (v[u].y * v[u].w);
-        s2 = s2 + J[u] * (v[u].z * v[u].w);
+        s0 = fma(J[u], v[u].x * v[u].w, s0);
+        s1 = fma(J[u], v[u].y * v[u].w, s1);
+        s2 = fma(J[u], v[u].z * v[u].w, s2);
       }
     }
   }
@@ -848,17 +851,17 @@ __device__ __forceinline__ void tile_site(const HamDev<R> &h, const LLGDev<R> &p
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        s0 = s0 + J[u] * a0[u];
-        s1 = s1 + J[u] * a1[u];
-        s2 = s2 + J[u] * a2[u];
+        s0 = fma(J[u], a0[u], s0);
+        s1 = fma(J[u], a1[u], s1);
+        s2 = fma(J[u], a2[u], s2);
       }
     }
     for (; j < len; ++j) {
       const R *v = sm + (hbase3 + cload(off + j));
       const R J = UCLS ? cload(cp + (size_t)j * h.ex_rows) : cp[(size_t)j * h.ex_rows];
-      s0 = s0 + J * v[0];
-      s1 = s1 + J * v[1];
-      s2 = s2 + J * v[2];
+      s0 = fma(J, v[0], s0);
+      s1 = fma(J, v[1], s1);
+      s2 = fma(J, v[2], s2);
     }
   }
   if (h.Wd > 0 && sc.dm_row >= 0) {

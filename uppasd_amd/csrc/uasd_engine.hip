@@ -140,6 +140,7 @@ struct uasd_ctx {
   // lattice dimensions (reference site i = b + NA*(x + N1*(y + N2*z))) and
   // the translation-invariant stencil of the lists, when there is one
   int lat_na = 0, lat_n1 = 0, lat_n2 = 0, lat_n3 = 0;
+  int64_t lds_max = 64 * 1024;  // LDS one workgroup may use (opt-in maximum)
   struct Stencil {
     bool ok = false;
     int W = 0, rx = 0, ry = 0, rz = 0;
@@ -726,8 +727,13 @@ bool tile_fits(const uasd_ctx *c, int k) {
   const int hv = (sh[0] + 2 * rx) * (sh[1] + 2 * ry) * (sh[2] + 2 * rz);
   // one thread per site; whole wavefronts per basis atom
   if (sh[0] * sh[1] * sh[2] * c->lat_na > 1024 || (sh[0] * sh[1] * sh[2]) % 64 != 0) return false;
-  return uasd::tile_lds_bytes(hv, c->lat_na, c->rsize, sh[0] + sh[1] + sh[2] + 2 * (rx + ry + rz)) <=
-         64 * 1024;
+  if (hv * c->lat_na > uasd::kTilePlane) return false;
+  // tile index arithmetic in FP32 reciprocals: fewer than 2^22 tiles
+  if ((int64_t)((c->lat_n1 + sh[0] - 1) / sh[0]) * ((c->lat_n2 + sh[1] - 1) / sh[1]) *
+          ((c->lat_n3 + sh[2] - 1) / sh[2]) >= (1 << 22))
+    return false;
+  return (int64_t)uasd::tile_lds_bytes(c->rsize, sh[0] + sh[1] + sh[2] + 2 * (rx + ry + rz)) <=
+         c->lds_max;
 }
 
 // TileDev for shape k, uploading its halo offsets when the shape changes.
@@ -748,9 +754,24 @@ int make_tile(uasd_ctx *c, int k, uasd::TileDev &t) {
   t.HY = t.TY + 2 * t.RY;
   t.HZ = t.TZ + 2 * t.RZ;
   t.HV = t.HX * t.HY * t.HZ;
+  t.rTX = 1.0f / (float)t.TX;
+  t.rTXY = 1.0f / (float)(t.TX * t.TY);
+  t.rTC = 1.0f / (float)(t.TX * t.TY * t.TZ);
+  t.rHX = 1.0f / (float)t.HX;
+  t.rHXY = 1.0f / (float)(t.HX * t.HY);
+  t.rHV = 1.0f / (float)t.HV;
+  {
+    const int nth = t.TX * t.TY * t.TZ * t.NA, plane = t.HX * t.HY;
+    t.ab = nth / t.HV;
+    t.az = (nth % t.HV) / plane;
+    t.ay = (nth % plane) / t.HX;
+    t.ax = nth % t.HX;
+  }
   t.ntx = (t.N1 + t.TX - 1) / t.TX;
   t.nty = (t.N2 + t.TY - 1) / t.TY;
   t.ntz = (t.N3 + t.TZ - 1) / t.TZ;
+  t.rntx = 1.0f / (float)t.ntx;
+  t.rntxy = 1.0f / (float)(t.ntx * t.nty);
   t.W = c->st_ex.W;
   t.Wd = c->Wd > 0 ? c->st_dm.W : 0;
   if (c->tile_off_shape != k) {
@@ -759,8 +780,7 @@ int make_tile(uasd_ctx *c, int k, uasd::TileDev &t) {
       for (int b = 0; b < t.NA; ++b)
         for (int j = 0; j < W; ++j) {
           const int q = b * W + j;
-          off[base + q] =
-              3 * (st.bp[q] * t.HV + st.dz[q] * t.HX * t.HY + st.dy[q] * t.HX + st.dx[q]);
+          off[base + q] = st.bp[q] * t.HV + st.dz[q] * t.HX * t.HY + st.dy[q] * t.HX + st.dx[q];
         }
     };
     fill(c->st_ex, t.W, 0);
@@ -874,6 +894,7 @@ int uasd_ctx_create(uasd_ctx **out, int natom, int mensemble, int precision) {
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return fail(UASD_ERR_NODEVICE, std::string("device is ") + prop.gcnArchName +
                                        ", the library is built for gfx950");
+  c->lds_max = (int64_t)std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
   HIP_TRY(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
   c->stream = c->own_stream;
   HIP_TRY(c->spins.ensure(3 * quad_bytes(c.get())));

@@ -107,8 +107,10 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
                                               uint32_t &o1, uint32_t &o2, uint32_t &o3) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-    uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    // one 32x32->64 multiply each (v_mad_u64_u32) rather than separate hi/lo
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
     uint32_t n0 = hi1 ^ c1 ^ k0;
     uint32_t n2 = hi0 ^ c3 ^ k1;
     c0 = n0;
@@ -698,18 +700,39 @@ __device__ __forceinline__ Quad<float> finish<float>(float o0, float o1, float o
 // Buffers: e = emom at step start, p = predicted emom (emom2 after the first
 // half), b2 = Depondt's b2eff.  Quads carry mmom in w.  The corrector writes
 // the new emom into `out` (e when moment_update is fused, p otherwise).
+// What the LLG half-step of one (site, ens) reads besides its field: the
+// moment at step start, the predicted moment and Depondt's b2eff (corrector),
+// and the thermal deviates.  Split from the update so a kernel can issue
+// these loads and draw the deviates before its field is ready.
+template <typename R>
+struct SiteIn {
+  Quad<R> eq, pq, bq;
+  R n0, n1, n2;
+};
+
+// hsite: the site's reference index (the noise counter), q its quad index.
+template <typename R, int ALG, int HALF>
+__device__ __forceinline__ SiteIn<R> llg_fetch(const LLGDev<R> &prm, int hsite, int ens,
+                                               int64_t q, const Quad<R> *e, const Quad<R> *p,
+                                               const Quad<R> *b2) {
+  SiteIn<R> in;
+  in.eq = e[q];
+  if (HALF == 1) in.pq = p[q];
+  if (ALG == 5 && HALF == 1) in.bq = b2[q];
+  prm.normals(hsite, ens, (size_t)q, in.n0, in.n1, in.n2);
+  return in;
+}
+
 // The LLG half-step of one (site, ens) once its effective field b is known.
 template <typename R, int ALG, int HALF>
-__device__ __forceinline__ void llg_update(const HamDev<R> &h, const LLGDev<R> &prm, int site,
-                                           int ens, int64_t q, R b0, R b1, R b2v,
-                                           const Quad<R> *e, Quad<R> *p, Quad<R> *b2,
-                                           Quad<R> *out) {
+__device__ __forceinline__ void llg_apply(const LLGDev<R> &prm, int site, int64_t q, R b0, R b1,
+                                          R b2v, const SiteIn<R> &in, Quad<R> *p, Quad<R> *b2,
+                                          Quad<R> *out) {
   const LamConst<R> lc = prm.lamc(site);
   const double T = prm.temperature(site);
-  const Quad<R> eq = e[q];
+  const Quad<R> eq = in.eq;
   const R m = eq.w;
-  R n0, n1, n2;
-  prm.normals((int)host_site(h, site), ens, (size_t)q, n0, n1, n2);
+  const R n0 = in.n0, n1 = in.n1, n2 = in.n2;
   if (ALG == 5) {
     const R sig = depondt_sigma<R>(lc.Dp, (double)m, T, prm.temprescale);
     const R t0 = n0 * sig, t1 = n1 * sig, t2 = n2 * sig;
@@ -721,8 +744,8 @@ __device__ __forceinline__ void llg_update(const HamDev<R> &h, const LLGDev<R> &
       p[q] = Quad<R>{o0, o1, o2, m};
       b2[q] = Quad<R>{d0, d1, d2, (R)0};
     } else {
-      const Quad<R> pq = p[q];
-      const Quad<R> bq = b2[q];
+      const Quad<R> pq = in.pq;
+      const Quad<R> bq = in.bq;
       R d0, d1, d2, o0, o1, o2;
       depondt_bdup(lc.lam, pq.x, pq.y, pq.z, bl0, bl1, bl2, d0, d1, d2);
       d0 = (R)0.5 * d0 + (R)0.5 * bq.x;
@@ -742,13 +765,22 @@ __device__ __forceinline__ void llg_update(const HamDev<R> &h, const LLGDev<R> &
       p[q] = Quad<R>{(R)0.5 * (eq.x + t0 * detAi), (R)0.5 * (eq.y + t1 * detAi),
                      (R)0.5 * (eq.z + t2 * detAi), m};
     } else {
-      const Quad<R> pq = p[q];
+      const Quad<R> pq = in.pq;
       midpoint_A(lc.lam, lc.mdt, lc.msqrtdt, (R)prm.landeg(site), b0, b1, b2v, r0, r1, r2, pq.x,
                  pq.y, pq.z, A0, A1, A2);
       midpoint_solve(A0, A1, A2, eq.x, eq.y, eq.z, t0, t1, t2, detAi);
       out[q] = finish<R>(t0 * detAi, t1 * detAi, t2 * detAi, m);
     }
   }
+}
+
+template <typename R, int ALG, int HALF>
+__device__ __forceinline__ void llg_update(const HamDev<R> &h, const LLGDev<R> &prm, int site,
+                                           int ens, int64_t q, R b0, R b1, R b2v,
+                                           const Quad<R> *e, Quad<R> *p, Quad<R> *b2,
+                                           Quad<R> *out) {
+  const SiteIn<R> in = llg_fetch<R, ALG, HALF>(prm, (int)host_site(h, site), ens, q, e, p, b2);
+  llg_apply<R, ALG, HALF>(prm, site, q, b0, b1, b2v, in, p, b2, out);
 }
 
 template <typename R, int ALG, int HALF, bool FUSED, int G>
@@ -777,9 +809,12 @@ __global__ void __launch_bounds__(256)
 // neighbour j of basis b at basis b' and cell offset (dx, dy, dz) for every
 // cell), a workgroup owns one tile of TX*TY*TZ cells.  It stages emomM = e*m
 // of the tile plus a halo of (RX, RY, RZ) cells for all basis atoms in LDS,
-// the three components of a cell side by side; then each of the
-// TX*TY*TZ*NA threads takes one site (cell, basis b) and reads neighbour j at its halo position plus
-// off[b*W + j] (= 3*(b'*HV + dz*HX*HY + dy*HX + dx), in doubles of R).  The
+// one plane per component at a fixed stride of kTilePlane values (so the
+// three reads of a neighbour share one address and differ by an immediate
+// offset, and lanes on consecutive cells read consecutive words: no bank
+// conflicts along a row of 16 cells); then each of the TX*TY*TZ*NA threads
+// takes one site (cell, basis b) and reads neighbour j at its halo position
+// plus off[b*W + j] (= b'*HV + dz*HX*HY + dy*HX + dx).  The
 // terms are added in list order with the same products as the ELL path, so
 // results are identical; the gather traffic moves from L2 to LDS.  M = 1.
 struct TileDev {
@@ -787,16 +822,27 @@ struct TileDev {
   int TX, TY, TZ;
   int RX, RY, RZ;
   int HX, HY, HZ, HV;     // halo tile extents, HV = HX*HY*HZ
+  // 1/d in FP32 for the small divisors d of the index arithmetic:
+  // fdiv(n, 1/d) = n/d exactly for 0 <= n < 2^22
+  float rTX, rTXY, rTC, rHX, rHXY, rHV, rntx, rntxy;
+  int ax, ay, az, ab;     // digits (x, y, z, basis) of the staging stride TX*TY*TZ*NA
   int ntx, nty, ntz;      // tiles per direction
   int W, Wd;              // exchange / DM stencil widths
-  const int *off;         // (NA*W) exchange offsets into the halo tile, x3
-  const int *doff;        // (NA*Wd) DM offsets, x3
+  const int *off;         // (NA*W) exchange offsets into the halo tile
+  const int *doff;        // (NA*Wd) DM offsets
   // ucls = 1: every site of basis b has site class cls_b[b] (a lattice of one
   // species), so class rows, couplings and offsets are the same for all lanes
   // of a wavefront and are read through the scalar cache
   int ucls;
   int cls_b[8];
 };
+
+// n / d for 0 <= n < 2^22 and rd = 1/d rounded to FP32: (n + 1/2) rd is
+// within 2^-23 (n + 1/2)/d < 1/(2d) of (n + 1/2)/d, whose fractional part
+// lies in [1/(2d), 1 - 1/(2d)], so the truncation is the exact quotient.
+__device__ __forceinline__ int fdiv(int n, float rd) {
+  return (int)(((float)n + 0.5f) * rd);
+}
 
 template <typename R>
 __device__ __forceinline__ int dev_slot(const HamDev<R> &h, int i) {
@@ -811,16 +857,17 @@ __device__ __forceinline__ T cload(const T *p) {
   return *(const __attribute__((address_space(4))) T *)p;
 }
 
+// Values per component plane of the LDS tile (NA*HV must not exceed it).
+constexpr int kTilePlane = 3072;
+
 // One site of a tile: field from the LDS tile, then the LLG half-step.
-// hbase3 is 3 * (halo index of the site's cell).
+// hbase is the halo index of the site's cell.
 // UCLS: the site class is the per-basis one of TileDev (wave-uniform).
 template <typename R, int ALG, int HALF, bool UCLS>
 __device__ __forceinline__ void tile_site(const HamDev<R> &h, const LLGDev<R> &prm,
-                                          const TileDev &t, const R *sm, int hbase3, int b,
-                                          int x, int y, int z, const Quad<R> *e, Quad<R> *p,
+                                          const TileDev &t, const R *sm, int hbase, int b,
+                                          int site, const SiteIn<R> &in, Quad<R> *p,
                                           Quad<R> *b2, Quad<R> *out) {
-  const int cell = x + t.N1 * (y + t.N2 * z);
-  const int site = h.order_na == 1 ? b + t.NA * cell : b * (t.N1 * t.N2 * t.N3) + cell;
   SiteClass sc;
   if (UCLS) {
     // wave-uniform values kept in scalar registers
@@ -843,10 +890,10 @@ __device__ __forceinline__ void tile_site(const HamDev<R> &h, const LLGDev<R> &p
       R a0[4], a1[4], a2[4], J[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const R *v = sm + (hbase3 + cload(off + j + u));
+        const R *v = sm + (hbase + cload(off + j + u));
         a0[u] = v[0];
-        a1[u] = v[1];
-        a2[u] = v[2];
+        a1[u] = v[kTilePlane];
+        a2[u] = v[2 * kTilePlane];
         J[u] = UCLS ? cload(cp + (size_t)(j + u) * h.ex_rows) : cp[(size_t)(j + u) * h.ex_rows];
       }
 #pragma unroll
@@ -857,22 +904,22 @@ __device__ __forceinline__ void tile_site(const HamDev<R> &h, const LLGDev<R> &p
       }
     }
     for (; j < len; ++j) {
-      const R *v = sm + (hbase3 + cload(off + j));
+      const R *v = sm + (hbase + cload(off + j));
       const R J = UCLS ? cload(cp + (size_t)j * h.ex_rows) : cp[(size_t)j * h.ex_rows];
       s0 = fma(J, v[0], s0);
-      s1 = fma(J, v[1], s1);
-      s2 = fma(J, v[2], s2);
+      s1 = fma(J, v[kTilePlane], s1);
+      s2 = fma(J, v[2 * kTilePlane], s2);
     }
   }
   if (h.Wd > 0 && sc.dm_row >= 0) {
     const int len = h.dm_len[sc.dm_row];
     const int *off = t.doff + b * t.Wd;
     for (int j = 0; j < len; ++j) {
-      const R *v = sm + (hbase3 + cload(off + j));
+      const R *v = sm + (hbase + cload(off + j));
       const size_t r = (size_t)j * h.dm_rows + sc.dm_row;
       const R D0 = UCLS ? cload(h.dvx + r) : h.dvx[r], D1 = UCLS ? cload(h.dvy + r) : h.dvy[r],
               D2 = UCLS ? cload(h.dvz + r) : h.dvz[r];
-      const R m0 = v[0], m1 = v[1], m2 = v[2];
+      const R m0 = v[0], m1 = v[kTilePlane], m2 = v[2 * kTilePlane];
       s0 = s0 + D2 * m1 - D1 * m2;
       s1 = s1 + D0 * m2 - D2 * m0;
       s2 = s2 + D1 * m0 - D0 * m1;
@@ -880,8 +927,8 @@ __device__ __forceinline__ void tile_site(const HamDev<R> &h, const LLGDev<R> &p
   }
   if (sc.an_row >= 0) {
     const int ty_ = h.an_type[sc.an_row];
-    const R *v = sm + (hbase3 + 3 * b * t.HV);
-    const R m0 = v[0], m1 = v[1], m2 = v[2];
+    const R *v = sm + (hbase + b * t.HV);
+    const R m0 = v[0], m1 = v[kTilePlane], m2 = v[2 * kTilePlane];
     if (ty_ == 1) {
       uniaxial(h, sc.an_row, m0, m1, m2, s0, s1, s2);
     } else if (ty_ == 2) {
@@ -910,7 +957,7 @@ __device__ __forceinline__ void tile_site(const HamDev<R> &h, const LLGDev<R> &p
   const R b0 = (R)0 + s0 + (qf0 + x0 + (R)0);
   const R b1 = (R)0 + s1 + (qf1 + x1 + (R)0);
   const R b2v = (R)0 + s2 + (qf2 + x2 + (R)0);
-  llg_update<R, ALG, HALF>(h, prm, site, 0, site, b0, b1, b2v, e, p, b2, out);
+  llg_apply<R, ALG, HALF>(prm, site, site, b0, b1, b2v, in, p, b2, out);
 }
 
 // Staging items per thread and round (loads issued together, then stored).
@@ -927,14 +974,31 @@ __global__ void __launch_bounds__(1024)
   const int64_t tb = xcd_block();
   const int ntiles = t.ntx * t.nty * t.ntz;
   if (tb >= ntiles) return;  // whole padding workgroups exit before any barrier
-  const int tz = (int)(tb / (t.ntx * t.nty));
-  const int ty = (int)((tb / t.ntx) % t.nty);
-  const int tx = (int)(tb % t.ntx);
+  const int tbi = (int)tb;
+  const int tz = fdiv(tbi, t.rntxy), txy = tbi - tz * t.ntx * t.nty;
+  const int ty = fdiv(txy, t.rntx), tx = txy - ty * t.ntx;
   const int X0 = tx * t.TX, Y0 = ty * t.TY, Z0 = tz * t.TZ;
   const int ncell = t.N1 * t.N2 * t.N3;
+  // this thread's site: cell (lx, ly, lz) of the tile, basis b.  blockDim.x =
+  // tc * NA with tc a multiple of 64, so b is the same for all lanes of a
+  // wavefront and the stencil offsets (and, for one species, the couplings)
+  // are read with scalar loads.  Its own moments and thermal deviates are
+  // fetched before the halo is staged, so that work overlaps the staging.
+  const int tc = t.TX * t.TY * t.TZ;
+  const int tid = threadIdx.x;
+  const int b = __builtin_amdgcn_readfirstlane(fdiv(tid, t.rTC));
+  const int lcell = tid - b * tc;
+  const int lz = fdiv(lcell, t.rTXY), lxy = lcell - lz * t.TX * t.TY;
+  const int ly = fdiv(lxy, t.rTX), lx = lxy - ly * t.TX;
+  const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
+  const bool valid = x < t.N1 && y < t.N2 && z < t.N3;
+  const int cell = x + t.N1 * (y + t.N2 * z);
+  const int site = h.order_na == 1 ? b + t.NA * cell : b * ncell + cell;
+  SiteIn<R> in{};
+  if (valid) in = llg_fetch<R, ALG, HALF>(prm, b + t.NA * cell, 0, site, e, p, b2);
   // wrapped lattice coordinates of the halo rows, pre-multiplied into cell
   // index terms: x, N1*y, N1*N2*z (periodic images of any multiplicity)
-  int *wrap = reinterpret_cast<int *>(sm + (size_t)3 * t.NA * t.HV);
+  int *wrap = reinterpret_cast<int *>(sm + 3 * kTilePlane);
   for (int i = threadIdx.x; i < t.HX + t.HY + t.HZ; i += blockDim.x) {
     int v;
     if (i < t.HX) {
@@ -948,7 +1012,7 @@ __global__ void __launch_bounds__(1024)
   }
   __syncthreads();
   // stage the halo tile of all basis atoms: item = b*HV + l, l the halo
-  // cell, stored at 3*item; thread t takes items t, t + blockDim, ...  The
+  // cell, stored at item of each plane; thread t takes items t, t + blockDim, ...  The
   // digits (hx, hy, hz, hb) of the item advance by mixed-radix addition (no
   // divisions in the loop).
   {
@@ -956,9 +1020,12 @@ __global__ void __launch_bounds__(1024)
     const int total = t.NA * t.HV;
     const int i0 = threadIdx.x;
     const int *wy = wrap + t.HX, *wz = wrap + t.HX + t.HY;
-    int hx = i0 % t.HX, hy = (i0 / t.HX) % t.HY, hz = (i0 / plane) % t.HZ, hb = i0 / t.HV;
-    const int ax = nth % t.HX, ay = (nth / t.HX) % t.HY, az = (nth / plane) % t.HZ,
-              ab = nth / t.HV;
+    // digits of i0 and of the stride nth (nth is wave-uniform)
+    int hb = fdiv(i0, t.rHV), l = i0 - hb * t.HV;
+    int hz = fdiv(l, t.rHXY);
+    l -= hz * plane;
+    int hy = fdiv(l, t.rHX), hx = l - hy * t.HX;
+    const int ax = t.ax, ay = t.ay, az = t.az, ab = t.ab;
     for (int base = i0; base < total; base += kTileStage * nth) {
       int slot[kTileStage];
 #pragma unroll
@@ -990,28 +1057,20 @@ __global__ void __launch_bounds__(1024)
 #pragma unroll
       for (int u = 0; u < kTileStage; ++u)
         if (slot[u] >= 0) {
-          R *d = sm + 3 * (base + u * nth);
+          R *d = sm + (base + u * nth);
           d[0] = v[u].x * v[u].w;
-          d[1] = v[u].y * v[u].w;
-          d[2] = v[u].z * v[u].w;
+          d[kTilePlane] = v[u].y * v[u].w;
+          d[2 * kTilePlane] = v[u].z * v[u].w;
         }
     }
   }
   __syncthreads();
-  const int tc = t.TX * t.TY * t.TZ;
-  const int lcell = threadIdx.x % tc;
-  const int lx = lcell % t.TX, ly = (lcell / t.TX) % t.TY, lz = lcell / (t.TX * t.TY);
-  const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
-  if (x >= t.N1 || y >= t.N2 || z >= t.N3) return;
+  if (!valid) return;
   const int hcell = (lz + t.RZ) * t.HX * t.HY + (ly + t.RY) * t.HX + (lx + t.RX);
-  // blockDim.x = tc * NA with tc a multiple of 64: b is the same for all
-  // lanes of a wavefront, so the stencil offsets (and, for one species, the
-  // couplings) are read with scalar loads
-  const int b = __builtin_amdgcn_readfirstlane(threadIdx.x / tc);
   if (t.ucls)
-    tile_site<R, ALG, HALF, true>(h, prm, t, sm, 3 * hcell, b, x, y, z, e, p, b2, out);
+    tile_site<R, ALG, HALF, true>(h, prm, t, sm, hcell, b, site, in, p, b2, out);
   else
-    tile_site<R, ALG, HALF, false>(h, prm, t, sm, 3 * hcell, b, x, y, z, e, p, b2, out);
+    tile_site<R, ALG, HALF, false>(h, prm, t, sm, hcell, b, site, in, p, b2, out);
 }
 
 // ----------------------------------------------------------------- MC

@@ -103,10 +103,18 @@ hipError_t launch_llg_tile(hipStream_t s, const TileDev &t, int alg, int half,
   if (ntiles == 0) return hipSuccess;
   const int threads = t.TX * t.TY * t.TZ * t.NA;
   if (threads > 1024 || h.M != 1) return hipErrorInvalidValue;
-  const size_t lds = tile_lds_bytes(t.HV, t.NA, sizeof(R), t.HX + t.HY + t.HZ);
+  if (t.NA * t.HV > kTilePlane) return hipErrorInvalidValue;
+  const size_t lds = tile_lds_bytes(sizeof(R), t.HX + t.HY + t.HZ);
   const dim3 grid((ntiles + kXCD - 1) / kXCD * kXCD), block(threads);
 #define UASD_TILE_CASE(A, H)                                                                    \
   if (alg == A && half == H) {                                                                  \
+    static size_t lds_set = 0; /* dynamic LDS above 64 KiB must be requested */                \
+    if (lds > lds_set) {                                                                        \
+      hipError_t err = hipFuncSetAttribute((const void *)&k_llg_tile<R, A, H>,                  \
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+      if (err != hipSuccess) return err;                                                        \
+      lds_set = lds;                                                                            \
+    }                                                                                           \
     hipLaunchKernelGGL((k_llg_tile<R, A, H>), grid, block, lds, s, h, prm, t, e, p, b2, out);   \
     return hipGetLastError();                                                                   \
   }

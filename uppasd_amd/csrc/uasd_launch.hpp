@@ -67,10 +67,10 @@ hipError_t launch_llg_tile(hipStream_t s, const TileDev &t, int alg, int half,
                            const HamDev<R> &h, const LLGDev<R> &prm, const Quad<R> *e,
                            Quad<R> *p, Quad<R> *b2, Quad<R> *out);
 // LDS bytes the tile kernel needs for a halo tile of hv cells
-// LDS of k_llg_tile: 3 components per halo site, then the halo's wrapped
-// coordinate table (nwrap = HX + HY + HZ ints)
-inline size_t tile_lds_bytes(int hv, int na, size_t rsize, int nwrap) {
-  return 3 * (size_t)hv * na * rsize + (size_t)nwrap * sizeof(int);
+// LDS of k_llg_tile: three planes of kTilePlane values (NA*HV used), then
+// the halo's wrapped coordinate table (nwrap = HX + HY + HZ ints)
+inline size_t tile_lds_bytes(size_t rsize, int nwrap) {
+  return 3 * (size_t)kTilePlane * rsize + (size_t)nwrap * sizeof(int);
 }
 
 // FP64 sums per ensemble of emomM (nv = 3) or of emomM and en (nv = 4);

@@ -858,7 +858,7 @@ __device__ __forceinline__ T cload(const T *p) {
 }
 
 // Values per component plane of the LDS tile (NA*HV must not exceed it).
-constexpr int kTilePlane = 3072;
+constexpr int kTilePlane = 3080;
 
 // One site of a tile: field from the LDS tile, then the LLG half-step.
 // hbase is the halo index of the site's cell.

@@ -117,6 +117,7 @@ struct uasd_ctx {
   DevBuf field;   // beff quads for the unfused path
   DevBuf scratch; // emomM quads (w = 1) for host field calls
   DevBuf noise;   // injected N(0,1) quads
+  DevBuf nsave;   // fused steps: deviates drawn by the predictor for the corrector
   DevBuf llgarr;  // lambda, temp, lande (double, N each)
   DevBuf energy;  // per q energy terms
   DevBuf partial, redout;
@@ -509,6 +510,11 @@ int ready_for_llg(uasd_ctx *c) {
   if (!c->have_llg) return fail(UASD_ERR_STATE, "LLG parameters not set (uasd_set_llg)");
   if (c->noise_mode == UASD_NOISE_INJECTED && !c->noise.p)
     return fail(UASD_ERR_STATE, "injected noise selected but no normals uploaded");
+  if (c->noise_mode == UASD_NOISE_PHILOX) {
+    void *old = c->nsave.p;
+    HIP_TRY(c->nsave.ensure((size_t)c->N * c->M * sizeof(float4)));
+    if (c->nsave.p != old) c->gen += 1;  // captured graphs hold the old address
+  }
   return rebuild_classes(c);
 }
 
@@ -523,6 +529,9 @@ int run_llg_half(uasd_ctx *c, int alg, int half, bool fused, bool write_e,
     prm.step = (uint64_t)step_off;
     prm.step_dev = c->ctr.as<uint64_t>();
   }
+  // a fused predictor/corrector pair draws the deviates once: the predictor
+  // stores them, the corrector of the same step reads them back
+  if (fused && prm.noise_mode == UASD_NOISE_PHILOX) prm.nsave = c->nsave.as<float4>();
   uasd::Quad<R> *e = quads<R>(c, c->spins, 0), *p = quads<R>(c, c->spins, 1),
                 *b2 = quads<R>(c, c->spins, 2);
   const uasd::Quad<R> *beff = fused ? nullptr : quads<R>(c, c->field);

@@ -531,6 +531,9 @@ struct LLGDev {
   uint64_t step;              // step counter, or offset from *step_dev
   const uint64_t *step_dev;   // device-held step base (graph replays) or nullptr
   const Quad<R> *noise;  // injected N(0,1) per q
+  // Philox deviates of a fused step: stored by the predictor, read back by
+  // the corrector (the same values it would draw), or nullptr
+  float4 *nsave;
   LamConst<R> lc0;       // lam_const(lam0, dt, bn)
   __device__ __forceinline__ LamConst<R> lamc(int s) const {
     return lam ? lam_const<R>(lam[s], dt, bn) : lc0;
@@ -719,7 +722,17 @@ __device__ __forceinline__ SiteIn<R> llg_fetch(const LLGDev<R> &prm, int hsite, 
   in.eq = e[q];
   if (HALF == 1) in.pq = p[q];
   if (ALG == 5 && HALF == 1) in.bq = b2[q];
-  prm.normals(hsite, ens, (size_t)q, in.n0, in.n1, in.n2);
+  if (HALF == 1 && prm.nsave) {
+    const float4 v = prm.nsave[q];
+    in.n0 = (R)v.x;
+    in.n1 = (R)v.y;
+    in.n2 = (R)v.z;
+  } else {
+    prm.normals(hsite, ens, (size_t)q, in.n0, in.n1, in.n2);
+    // Philox deviates are FP32 values, so the conversion back is exact
+    if (HALF == 0 && prm.nsave)
+      prm.nsave[q] = make_float4((float)in.n0, (float)in.n1, (float)in.n2, 0.0f);
+  }
   return in;
 }
 

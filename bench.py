@@ -32,19 +32,24 @@ METRIC = "spin-updates/s (LLG step, MC sweep); FP32+batched vs FP64 at stated er
 HBM_PEAK_GBS = 8000.0
 
 
-def kernel_bytes(W: int, s: int, ib: int = 4) -> dict:
+def kernel_bytes(W: int, s: int, ib: int = 4, cls: int = 4, nsave: int = 16) -> dict:
     """Algorithmic HBM bytes per site of the fused midpoint kernels.
 
     Spins are stored as quads {x, y, z, mmom} of s-byte reals; neighbour
     entries take ib bytes (4 for int32 indices, 2 for int16 differences, 0 for
-    the LDS-tiled stencil kernels, which read no lists).
-    predictor: gather e_j quads (4s), index stream ib*W, class id 4, write p (4s)
-    corrector: gather p_j quads (4s), own e (4s), index ib*W, class id 4, write e (4s)
+    the LDS-tiled stencil kernels, which read no lists); cls bytes of site
+    class per site (0 for the tiled kernels on a one-species lattice, whose
+    classes come with the launch); nsave bytes of thermal deviates the
+    predictor stores for its corrector (a float4; 0 without Philox noise).
+    predictor: gather e_j quads (4s), index stream ib*W, class, write p (4s),
+               write deviates
+    corrector: gather p_j quads (4s), own e (4s), index ib*W, class, write e (4s),
+               read deviates
     Each site's own spin is one of its neighbours' gathers, so the gathered
     quads count once per site (perfect reuse through L2 / MALL).
     """
-    pred = 4 * s + ib * W + 4 + 4 * s
-    corr = 4 * s + 4 * s + ib * W + 4 + 4 * s
+    pred = 4 * s + ib * W + cls + 4 * s + nsave
+    corr = 4 * s + 4 * s + ib * W + cls + 4 * s + nsave
     return {"predictor": pred, "corrector": corr, "step": pred + corr}
 
 
@@ -419,6 +424,7 @@ def main():
     tune_s = time.perf_counter() - t_tune
     gather, block = eng.tuning(eng.TUNE_LLG)
     nt, idx32, tile = (block >> 16) & 1, (block >> 17) & 1, (block >> 18) & 0xF
+    pipe = (block >> 22) & 1
     block &= 0xFFFF
 
     eng.sd_steps(args.solver, args.warmup)
@@ -435,7 +441,7 @@ def main():
 
     s = 8 if args.precision == "fp64" else 4
     ib = 0 if tile else eng.index_bytes()[0]     # the tile kernel reads no lists
-    kb = kernel_bytes(W, s, ib)
+    kb = kernel_bytes(W, s, ib, cls=0 if tile else 4)   # one species: classes per basis
     t_corr = ms_corr / max(n_corr, 1) * 1e-3
     t_pred = ms_pred / max(n_pred, 1) * 1e-3
     achieved = kb["corrector"] * N / t_corr / 1e9
@@ -473,6 +479,7 @@ def main():
             "launch": {"gather": gather, "block": block, "nt_index_loads": bool(nt),
                        "int32_index": bool(idx32),
                        "tile_shape": U.engine.TILE_SHAPES[tile - 1] if tile else None,
+                       "resident_tile_loop": bool(pipe),
                        "site_order": order,
                        "autotuned": not args.gather, "tune_s": round(tune_s, 3)},
             "kernel_ms": {"predictor": round(t_pred * 1e3, 4), "corrector": round(t_corr * 1e3, 4),

@@ -193,7 +193,10 @@ int uasd_index_bytes(uasd_ctx *ctx, int *exchange, int *dm);
  * (64, 128, 256), plus 0x10000 to read the neighbour lists with
  * non-temporal loads, 0x20000 to read int32 entries where packed int16
  * entries exist, and (k << 18) for the LDS-tiled stencil kernel with tile
- * shape k = 1..12 (kind 0 only; needs uasd_set_lattice_dims and a stencil).
+ * shape k = 1..15 (kind 0 only; needs uasd_set_lattice_dims and a stencil):
+ * cells (x, y, z) = (8,8,4) (16,8,2) (32,4,2) (16,4,4) (16,16,1) (32,8,1)
+ * (64,4,1) (8,4,4) (4,4,4) (8,8,2) (16,8,1) (8,8,1) (16,8,4) (8,16,4) (8,8,8),
+ * one thread per site (cells x basis atoms <= 1024).
  * Results do not depend on any of them. */
 int uasd_set_tuning(uasd_ctx *ctx, int kind, int gather, int block);
 int uasd_get_tuning(uasd_ctx *ctx, int kind, int *gather, int *block);

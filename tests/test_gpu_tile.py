@@ -12,7 +12,7 @@ from uppasd_amd import configs as CF
 
 pytestmark = pytest.mark.gpu
 
-TILE = lambda k: 256 | (k << 18)
+TILE = lambda k, pipe=0: 256 | (k << 18) | (pipe << 22)
 
 
 def _eng(sysd, order, precision="fp64"):
@@ -29,12 +29,12 @@ def _eng(sysd, order, precision="fp64"):
     return eng
 
 
-def _run(sysd, e0, order, tile, alg, precision="fp64", steps=12):
+def _run(sysd, e0, order, tile, alg, precision="fp64", steps=12, pipe=0):
     eng = _eng(sysd, order, precision)
     eng.set_moments(e0, sysd["mmom"][None])
     eng.set_llg(0.1, 200.0, 1e-16)
     eng.set_noise(U.UASD_NOISE_PHILOX, 17)
-    eng.set_tuning(eng.TUNE_LLG, 4, TILE(tile) if tile else 256)
+    eng.set_tuning(eng.TUNE_LLG, 4, TILE(tile, pipe) if tile else 256)
     eng.sd_steps(alg, steps)
     return eng.get_moments()[0], eng
 
@@ -60,8 +60,25 @@ def test_tile_equals_list_kernels(case, alg):
     shapes = [k for k, sh in enumerate(U.engine.TILE_SHAPES, 1) if (lat.n[2] > 1) == (sh[2] > 1)]
     for order in (1, lat.NA):
         for k in shapes:
-            got, _ = _run(sysd, e0, order, k, alg)
-            assert np.array_equal(got, ref), (case, order, k)
+            for pipe in (0, 1):
+                got, _ = _run(sysd, e0, order, k, alg, pipe=pipe)
+                assert np.array_equal(got, ref), (case, order, k, pipe)
+
+
+@pytest.mark.parametrize("alg", [1, 5])
+def test_resident_workgroups_loop_over_tiles(alg):
+    """A lattice with more tiles than resident workgroups (bcc 70^3 x 2,
+    partial tiles at the upper edges): with the resident-workgroup launch
+    every workgroup runs through several tiles, prefetching the next halo
+    while it computes; the result equals the list-gather kernels bit for bit."""
+    sysd = CF.bcc_shells(70, CF.bcc5_couplings())
+    lat = sysd["lattice"]
+    rng = np.random.default_rng(94)
+    e0 = np.ascontiguousarray(CF.random_unit(lat.natom, rng, 1))
+    ref, _ = _run(sysd, e0, lat.NA, 0, alg, steps=3)
+    for k in (1, 4):                       # (8,8,4): 1458 tiles, (16,4,4): 1620 tiles
+        got, _ = _run(sysd, e0, lat.NA, k, alg, steps=3, pipe=1)
+        assert np.array_equal(got, ref), k
 
 
 def test_tile_fp32_equals_list_fp32():

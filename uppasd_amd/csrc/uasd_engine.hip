@@ -542,7 +542,8 @@ int run_llg_half(uasd_ctx *c, int alg, int half, bool fused, bool write_e,
       uasd::TileDev t;
       if (int r = make_tile(c, c->tune[0].tile, t)) return r;
       tic(c, half);
-      HIP_TRY(uasd::launch_llg_tile<R>(c->stream, t, alg, half, h, prm, e, p, b2, out));
+      HIP_TRY(uasd::launch_llg_tile<R>(c->stream, t, c->tune[0].pipe, alg, half, h, prm, e, p, b2,
+                                       out));
       return toc(c, half);
     }
   }
@@ -736,12 +737,13 @@ bool tile_fits(const uasd_ctx *c, int k) {
   const int hv = (sh[0] + 2 * rx) * (sh[1] + 2 * ry) * (sh[2] + 2 * rz);
   // one thread per site; whole wavefronts per basis atom
   if (sh[0] * sh[1] * sh[2] * c->lat_na > 1024 || (sh[0] * sh[1] * sh[2]) % 64 != 0) return false;
-  if (hv * c->lat_na > uasd::kTilePlane) return false;
+  const int plane = uasd::tile_plane(hv * c->lat_na);
+  if (plane == 0) return false;
   // tile index arithmetic in FP32 reciprocals: fewer than 2^22 tiles
   if ((int64_t)((c->lat_n1 + sh[0] - 1) / sh[0]) * ((c->lat_n2 + sh[1] - 1) / sh[1]) *
           ((c->lat_n3 + sh[2] - 1) / sh[2]) >= (1 << 22))
     return false;
-  return (int64_t)uasd::tile_lds_bytes(c->rsize, sh[0] + sh[1] + sh[2] + 2 * (rx + ry + rz)) <=
+  return (int64_t)uasd::tile_lds_bytes(c->rsize, plane, sh[0] + sh[1] + sh[2] + 2 * (rx + ry + rz)) <=
          c->lds_max;
 }
 
@@ -1587,9 +1589,11 @@ int uasd_set_tuning(uasd_ctx *c, int kind, int gather, int block) {
   c->gen += 1;
   if (kind < 0 || kind > 2) return fail(UASD_ERR_ARG, "kind must be 0 (LLG), 1 (field) or 2 (MC)");
   // block may carry flags: 0x10000 non-temporal list loads, 0x20000 int32
-  // entries, (k << 18) LDS-tiled stencil shape k (LLG only)
-  uasd::Tune t{gather, block & 0xffff, (block >> 16) & 1, (block >> 17) & 1, (block >> 18) & 0xf};
-  if (!uasd::tune_valid(t) || (block >> 22) != 0 || (kind != 0 && t.tile != 0))
+  // entries, (k << 18) LDS-tiled stencil shape k and 0x400000 resident
+  // workgroups looping over tiles (LLG only)
+  uasd::Tune t{gather, block & 0xffff, (block >> 16) & 1, (block >> 17) & 1, (block >> 18) & 0xf,
+               (block >> 22) & 1};
+  if (!uasd::tune_valid(t) || (block >> 23) != 0 || (kind != 0 && t.tile != 0))
     return fail(UASD_ERR_ARG, "gather must be 2, 4, 8 or 16 and block 64, 128 or 256");
   c->tune[kind] = t;
   return UASD_OK;
@@ -1601,7 +1605,7 @@ int uasd_get_tuning(uasd_ctx *c, int kind, int *gather, int *block) {
   if (gather) *gather = c->tune[kind].gather;
   if (block)
     *block = c->tune[kind].block | (c->tune[kind].nt << 16) | (c->tune[kind].idx32 << 17) |
-             (c->tune[kind].tile << 18);
+             (c->tune[kind].tile << 18) | (c->tune[kind].pipe << 22);
   return UASD_OK;
 }
 
@@ -1663,7 +1667,8 @@ int tune_search(uasd_ctx *c, uasd::Tune *slot, Run &&run, double *best_ms) {
   if (slot == &c->tune[0]) {
     build_stencil(c);
     for (int k = 1; k <= uasd::kNumTileShapes; ++k)
-      if (tile_fits(c, k)) cand.push_back(uasd::Tune{4, 256, 0, 0, k});
+      if (tile_fits(c, k))
+        for (int pipe = 0; pipe < 2; ++pipe) cand.push_back(uasd::Tune{4, 256, 0, 0, k, pipe});
   }
   for (const uasd::Tune &cd : cand) {
     {

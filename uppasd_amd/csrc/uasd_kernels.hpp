@@ -822,7 +822,7 @@ __global__ void __launch_bounds__(256)
 // neighbour j of basis b at basis b' and cell offset (dx, dy, dz) for every
 // cell), a workgroup owns one tile of TX*TY*TZ cells.  It stages emomM = e*m
 // of the tile plus a halo of (RX, RY, RZ) cells for all basis atoms in LDS,
-// one plane per component at a fixed stride of kTilePlane values (so the
+// one plane per component at a fixed stride of PL values (so the
 // three reads of a neighbour share one address and differ by an immediate
 // offset, and lanes on consecutive cells read consecutive words: no bank
 // conflicts along a row of 16 cells); then each of the TX*TY*TZ*NA threads
@@ -839,6 +839,7 @@ struct TileDev {
   // fdiv(n, 1/d) = n/d exactly for 0 <= n < 2^22
   float rTX, rTXY, rTC, rHX, rHXY, rHV, rntx, rntxy;
   int ax, ay, az, ab;     // digits (x, y, z, basis) of the staging stride TX*TY*TZ*NA
+  int per;                // tiles per workgroup (consecutive in the XCD-aware order)
   int ntx, nty, ntz;      // tiles per direction
   int W, Wd;              // exchange / DM stencil widths
   const int *off;         // (NA*W) exchange offsets into the halo tile
@@ -870,17 +871,20 @@ __device__ __forceinline__ T cload(const T *p) {
   return *(const __attribute__((address_space(4))) T *)p;
 }
 
-// Values per component plane of the LDS tile (NA*HV must not exceed it).
-constexpr int kTilePlane = 3080;
+// Values per component plane of the LDS tile: the launch takes the smallest
+// that holds NA*HV.  Each is 8 more than a multiple of 64, so the three
+// component reads of a neighbour never pair into one two-address LDS read
+// (whose halves would share banks); 2*PL doubles stay within the 16-bit
+// immediate offset of a DS instruction.
+constexpr int kNumTilePlanes = 4;
+constexpr int kTilePlanes[kNumTilePlanes] = {1160, 2312, 3080, 3848};
 
-// One site of a tile: field from the LDS tile, then the LLG half-step.
-// hbase is the halo index of the site's cell.
+// Effective field of one site of a tile from the LDS halo.  hbase is the
+// halo index of the site's cell.
 // UCLS: the site class is the per-basis one of TileDev (wave-uniform).
-template <typename R, int ALG, int HALF, bool UCLS>
-__device__ __forceinline__ void tile_site(const HamDev<R> &h, const LLGDev<R> &prm,
-                                          const TileDev &t, const R *sm, int hbase, int b,
-                                          int site, const SiteIn<R> &in, Quad<R> *p,
-                                          Quad<R> *b2, Quad<R> *out) {
+template <typename R, int PL, bool UCLS>
+__device__ __forceinline__ void tile_field(const HamDev<R> &h, const TileDev &t, const R *sm,
+                                           int hbase, int b, int site, R &b0, R &b1, R &b2v) {
   SiteClass sc;
   if (UCLS) {
     // wave-uniform values kept in scalar registers
@@ -905,8 +909,8 @@ __device__ __forceinline__ void tile_site(const HamDev<R> &h, const LLGDev<R> &p
       for (int u = 0; u < 4; ++u) {
         const R *v = sm + (hbase + cload(off + j + u));
         a0[u] = v[0];
-        a1[u] = v[kTilePlane];
-        a2[u] = v[2 * kTilePlane];
+        a1[u] = v[PL];
+        a2[u] = v[2 * PL];
         J[u] = UCLS ? cload(cp + (size_t)(j + u) * h.ex_rows) : cp[(size_t)(j + u) * h.ex_rows];
       }
 #pragma unroll
@@ -920,8 +924,8 @@ __device__ __forceinline__ void tile_site(const HamDev<R> &h, const LLGDev<R> &p
       const R *v = sm + (hbase + cload(off + j));
       const R J = UCLS ? cload(cp + (size_t)j * h.ex_rows) : cp[(size_t)j * h.ex_rows];
       s0 = fma(J, v[0], s0);
-      s1 = fma(J, v[kTilePlane], s1);
-      s2 = fma(J, v[2 * kTilePlane], s2);
+      s1 = fma(J, v[PL], s1);
+      s2 = fma(J, v[2 * PL], s2);
     }
   }
   if (h.Wd > 0 && sc.dm_row >= 0) {
@@ -932,7 +936,7 @@ __device__ __forceinline__ void tile_site(const HamDev<R> &h, const LLGDev<R> &p
       const size_t r = (size_t)j * h.dm_rows + sc.dm_row;
       const R D0 = UCLS ? cload(h.dvx + r) : h.dvx[r], D1 = UCLS ? cload(h.dvy + r) : h.dvy[r],
               D2 = UCLS ? cload(h.dvz + r) : h.dvz[r];
-      const R m0 = v[0], m1 = v[kTilePlane], m2 = v[2 * kTilePlane];
+      const R m0 = v[0], m1 = v[PL], m2 = v[2 * PL];
       s0 = s0 + D2 * m1 - D1 * m2;
       s1 = s1 + D0 * m2 - D2 * m0;
       s2 = s2 + D1 * m0 - D0 * m1;
@@ -941,7 +945,7 @@ __device__ __forceinline__ void tile_site(const HamDev<R> &h, const LLGDev<R> &p
   if (sc.an_row >= 0) {
     const int ty_ = h.an_type[sc.an_row];
     const R *v = sm + (hbase + b * t.HV);
-    const R m0 = v[0], m1 = v[kTilePlane], m2 = v[2 * kTilePlane];
+    const R m0 = v[0], m1 = v[PL], m2 = v[2 * PL];
     if (ty_ == 1) {
       uniaxial(h, sc.an_row, m0, m1, m2, s0, s1, s2);
     } else if (ty_ == 2) {
@@ -967,51 +971,66 @@ __device__ __forceinline__ void tile_site(const HamDev<R> &h, const LLGDev<R> &p
     x1 = v.y;
     x2 = v.z;
   }
-  const R b0 = (R)0 + s0 + (qf0 + x0 + (R)0);
-  const R b1 = (R)0 + s1 + (qf1 + x1 + (R)0);
-  const R b2v = (R)0 + s2 + (qf2 + x2 + (R)0);
-  llg_apply<R, ALG, HALF>(prm, site, site, b0, b1, b2v, in, p, b2, out);
+  b0 = (R)0 + s0 + (qf0 + x0 + (R)0);
+  b1 = (R)0 + s1 + (qf1 + x1 + (R)0);
+  b2v = (R)0 + s2 + (qf2 + x2 + (R)0);
 }
 
-// Staging items per thread and round (loads issued together, then stored).
-constexpr int kTileStage = 4;
+// Halo items per thread whose loads are held in registers while the previous
+// tile is computed (more would spill); the rest is staged after it, in
+// rounds of kTileBatch loads issued together.
+constexpr int kTilePre = 2;
+constexpr int kTileBatch = 4;
 
-template <typename R, int ALG, int HALF>
-__global__ void __launch_bounds__(1024)
-    k_llg_tile(HamDev<R> h, LLGDev<R> prm, TileDev t, const Quad<R> *e, Quad<R> *p,
-               Quad<R> *b2, Quad<R> *out) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
-  R *sm = reinterpret_cast<R *>(lds_raw);
-  const Quad<R> *src = HALF == 0 ? e : (const Quad<R> *)p;
-  // tile coordinates (XCD-aware block order, z slowest)
-  const int64_t tb = xcd_block();
-  const int ntiles = t.ntx * t.nty * t.ntz;
-  if (tb >= ntiles) return;  // whole padding workgroups exit before any barrier
-  const int tbi = (int)tb;
-  const int tz = fdiv(tbi, t.rntxy), txy = tbi - tz * t.ntx * t.nty;
+// Digits (hx, hy, hz, hb) of halo item i = hb*HV + (hz*HY + hy)*HX + hx.
+struct HaloDigits {
+  int hx, hy, hz, hb;
+};
+
+__device__ __forceinline__ HaloDigits halo_digits(const TileDev &t, int i) {
+  HaloDigits d;
+  d.hb = fdiv(i, t.rHV);
+  int l = i - d.hb * t.HV;
+  d.hz = fdiv(l, t.rHXY);
+  l -= d.hz * t.HX * t.HY;
+  d.hy = fdiv(l, t.rHX);
+  d.hx = l - d.hy * t.HX;
+  return d;
+}
+
+// Advance the digits by the stride (t.ax, t.ay, t.az, t.ab) = blockDim.x.
+__device__ __forceinline__ void halo_step(const TileDev &t, HaloDigits &d) {
+  d.hx += t.ax;
+  d.hy += t.ay;
+  d.hz += t.az;
+  d.hb += t.ab;
+  if (d.hx >= t.HX) {
+    d.hx -= t.HX;
+    d.hy += 1;
+  }
+  if (d.hy >= t.HY) {
+    d.hy -= t.HY;
+    d.hz += 1;
+  }
+  if (d.hz >= t.HZ) {
+    d.hz -= t.HZ;
+    d.hb += 1;
+  }
+}
+
+// Origin cell of tile `tile` (x fastest, z slowest).
+__device__ __forceinline__ void tile_origin(const TileDev &t, int tile, int &X0, int &Y0, int &Z0) {
+  const int tz = fdiv(tile, t.rntxy), txy = tile - tz * t.ntx * t.nty;
   const int ty = fdiv(txy, t.rntx), tx = txy - ty * t.ntx;
-  const int X0 = tx * t.TX, Y0 = ty * t.TY, Z0 = tz * t.TZ;
-  const int ncell = t.N1 * t.N2 * t.N3;
-  // this thread's site: cell (lx, ly, lz) of the tile, basis b.  blockDim.x =
-  // tc * NA with tc a multiple of 64, so b is the same for all lanes of a
-  // wavefront and the stencil offsets (and, for one species, the couplings)
-  // are read with scalar loads.  Its own moments and thermal deviates are
-  // fetched before the halo is staged, so that work overlaps the staging.
-  const int tc = t.TX * t.TY * t.TZ;
-  const int tid = threadIdx.x;
-  const int b = __builtin_amdgcn_readfirstlane(fdiv(tid, t.rTC));
-  const int lcell = tid - b * tc;
-  const int lz = fdiv(lcell, t.rTXY), lxy = lcell - lz * t.TX * t.TY;
-  const int ly = fdiv(lxy, t.rTX), lx = lxy - ly * t.TX;
-  const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
-  const bool valid = x < t.N1 && y < t.N2 && z < t.N3;
-  const int cell = x + t.N1 * (y + t.N2 * z);
-  const int site = h.order_na == 1 ? b + t.NA * cell : b * ncell + cell;
-  SiteIn<R> in{};
-  if (valid) in = llg_fetch<R, ALG, HALF>(prm, b + t.NA * cell, 0, site, e, p, b2);
-  // wrapped lattice coordinates of the halo rows, pre-multiplied into cell
-  // index terms: x, N1*y, N1*N2*z (periodic images of any multiplicity)
-  int *wrap = reinterpret_cast<int *>(sm + 3 * kTilePlane);
+  X0 = tx * t.TX;
+  Y0 = ty * t.TY;
+  Z0 = tz * t.TZ;
+}
+
+// Wrapped lattice coordinates of the halo rows of a tile, pre-multiplied into
+// cell index terms: x, N1*y, N1*N2*z (periodic images of any multiplicity).
+__device__ __forceinline__ void halo_wrap_table(const TileDev &t, int X0, int Y0, int Z0,
+                                                int *wrap) {
   for (int i = threadIdx.x; i < t.HX + t.HY + t.HZ; i += blockDim.x) {
     int v;
     if (i < t.HX) {
@@ -1023,67 +1042,149 @@ __global__ void __launch_bounds__(1024)
     }
     wrap[i] = v;
   }
-  __syncthreads();
-  // stage the halo tile of all basis atoms: item = b*HV + l, l the halo
-  // cell, stored at item of each plane; thread t takes items t, t + blockDim, ...  The
-  // digits (hx, hy, hz, hb) of the item advance by mixed-radix addition (no
-  // divisions in the loop).
-  {
-    const int nth = blockDim.x, plane = t.HX * t.HY;
-    const int total = t.NA * t.HV;
-    const int i0 = threadIdx.x;
-    const int *wy = wrap + t.HX, *wz = wrap + t.HX + t.HY;
-    // digits of i0 and of the stride nth (nth is wave-uniform)
-    int hb = fdiv(i0, t.rHV), l = i0 - hb * t.HV;
-    int hz = fdiv(l, t.rHXY);
-    l -= hz * plane;
-    int hy = fdiv(l, t.rHX), hx = l - hy * t.HX;
-    const int ax = t.ax, ay = t.ay, az = t.az, ab = t.ab;
-    for (int base = i0; base < total; base += kTileStage * nth) {
-      int slot[kTileStage];
+}
+
+// Device slot of halo item d through the wrap table.
+template <typename R>
+__device__ __forceinline__ int halo_slot(const HamDev<R> &h, const TileDev &t, const int *wrap,
+                                         const HaloDigits &d, int ncell) {
+  const int cell = wrap[d.hx] + wrap[t.HX + d.hy] + wrap[t.HX + t.HY + d.hz];
+  return h.order_na == 1 ? d.hb + t.NA * cell : d.hb * ncell + cell;
+}
+
+template <typename R, int PL>
+__device__ __forceinline__ void halo_put(R *sm, int item, const Quad<R> &v) {
+  R *d = sm + item;
+  d[0] = v.x * v.w;
+  d[PL] = v.y * v.w;
+  d[2 * PL] = v.z * v.w;
+}
+
+// Halo items of this thread past the first kTilePre, loaded and stored in
+// rounds of kTileBatch.
+template <typename R, int PL>
+__device__ __forceinline__ void halo_rest(const HamDev<R> &h, const TileDev &t, const int *wrap,
+                                          const Quad<R> *src, R *sm, int ncell) {
+  const int nth = blockDim.x, total = t.NA * t.HV;
+  int base = threadIdx.x + kTilePre * nth;
+  if (base >= total) return;
+  HaloDigits d = halo_digits(t, base);
+  for (; base < total; base += kTileBatch * nth) {
+    int slot[kTileBatch];
 #pragma unroll
-      for (int u = 0; u < kTileStage; ++u) {
-        const bool ok = base + u * nth < total;
-        const int cell = wrap[hx] + wy[hy] + wz[hz];  // digits stay in range
-        slot[u] = !ok ? -1 : h.order_na == 1 ? hb + t.NA * cell : hb * ncell + cell;
-        hx += ax;
-        hy += ay;
-        hz += az;
-        hb += ab;
-        if (hx >= t.HX) {
-          hx -= t.HX;
-          hy += 1;
-        }
-        if (hy >= t.HY) {
-          hy -= t.HY;
-          hz += 1;
-        }
-        if (hz >= t.HZ) {
-          hz -= t.HZ;
-          hb += 1;
-        }
-      }
-      Quad<R> v[kTileStage];
-#pragma unroll
-      for (int u = 0; u < kTileStage; ++u)
-        if (slot[u] >= 0) v[u] = src[slot[u]];
-#pragma unroll
-      for (int u = 0; u < kTileStage; ++u)
-        if (slot[u] >= 0) {
-          R *d = sm + (base + u * nth);
-          d[0] = v[u].x * v[u].w;
-          d[kTilePlane] = v[u].y * v[u].w;
-          d[2 * kTilePlane] = v[u].z * v[u].w;
-        }
+    for (int u = 0; u < kTileBatch; ++u) {
+      slot[u] = base + u * nth < total ? halo_slot(h, t, wrap, d, ncell) : -1;
+      halo_step(t, d);
     }
+    Quad<R> v[kTileBatch];
+#pragma unroll
+    for (int u = 0; u < kTileBatch; ++u)
+      if (slot[u] >= 0) v[u] = src[slot[u]];
+#pragma unroll
+    for (int u = 0; u < kTileBatch; ++u)
+      if (slot[u] >= 0) halo_put<R, PL>(sm, base + u * nth, v[u]);
   }
-  __syncthreads();
-  if (!valid) return;
+}
+
+// Persistent tiles: workgroup w takes the tiles [w*per, (w+1)*per) of the
+// XCD-aware block order, adjacent in x.  The halo loads of its next tile are
+// issued before the current tile is computed and stored after it, so their
+// latency is covered by that computation.
+template <typename R, int ALG, int HALF, int PL>
+__global__ void __launch_bounds__(1024)
+    k_llg_tile(HamDev<R> h, LLGDev<R> prm, TileDev t, const Quad<R> *e, Quad<R> *p,
+               Quad<R> *b2, Quad<R> *out) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  R *sm = reinterpret_cast<R *>(lds_raw);
+  const Quad<R> *src = HALF == 0 ? e : (const Quad<R> *)p;
+  const int ntiles = t.ntx * t.nty * t.ntz;
+  const int first = (int)xcd_block() * t.per;
+  const int last = min(ntiles, first + t.per);
+  if (first >= last) return;  // whole workgroups exit before any barrier
+  const int ncell = t.N1 * t.N2 * t.N3;
+  const int nw = t.HX + t.HY + t.HZ;
+  int *wraps = reinterpret_cast<int *>(sm + 3 * PL);  // two tables, alternating by tile
+  // this thread's site in every tile: cell (lx, ly, lz), basis b.  blockDim.x
+  // = tc * NA with tc a multiple of 64, so b is the same for all lanes of a
+  // wavefront and the stencil offsets (and, for one species, the couplings)
+  // are read with scalar loads.
+  const int tc = t.TX * t.TY * t.TZ;
+  const int tid = threadIdx.x;
+  const int b = __builtin_amdgcn_readfirstlane(fdiv(tid, t.rTC));
+  const int lcell = tid - b * tc;
+  const int lz = fdiv(lcell, t.rTXY), lxy = lcell - lz * t.TX * t.TY;
+  const int ly = fdiv(lxy, t.rTX), lx = lxy - ly * t.TX;
   const int hcell = (lz + t.RZ) * t.HX * t.HY + (ly + t.RY) * t.HX + (lx + t.RX);
-  if (t.ucls)
-    tile_site<R, ALG, HALF, true>(h, prm, t, sm, hcell, b, site, in, p, b2, out);
-  else
-    tile_site<R, ALG, HALF, false>(h, prm, t, sm, hcell, b, site, in, p, b2, out);
+  const HaloDigits d0 = halo_digits(t, tid);
+  const int total = t.NA * t.HV, nth = blockDim.x;
+  // the first tile: wrap table, halo, own data
+  int X0, Y0, Z0;
+  tile_origin(t, first, X0, Y0, Z0);
+  halo_wrap_table(t, X0, Y0, Z0, wraps);
+  __syncthreads();
+  Quad<R> v[kTilePre];
+  {
+    HaloDigits d = d0;
+    int slot[kTilePre];
+#pragma unroll
+    for (int u = 0; u < kTilePre; ++u) {
+      slot[u] = tid + u * nth < total ? halo_slot(h, t, wraps, d, ncell) : -1;
+      halo_step(t, d);
+    }
+#pragma unroll
+    for (int u = 0; u < kTilePre; ++u)
+      if (slot[u] >= 0) v[u] = src[slot[u]];
+#pragma unroll
+    for (int u = 0; u < kTilePre; ++u)
+      if (slot[u] >= 0) halo_put<R, PL>(sm, tid + u * nth, v[u]);
+    halo_rest<R, PL>(h, t, wraps, src, sm, ncell);
+  }
+  for (int tile = first;;) {
+    const int k = tile - first;
+    const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
+    const bool valid = x < t.N1 && y < t.N2 && z < t.N3;
+    const int cell = x + t.N1 * (y + t.N2 * z);
+    const int site = h.order_na == 1 ? b + t.NA * cell : b * ncell + cell;
+    const int next = tile + 1 < last ? tile + 1 : -1;
+    int *wnext = wraps + ((k + 1) & 1) * nw;
+    int nX0 = 0, nY0 = 0, nZ0 = 0;
+    if (next >= 0) {
+      tile_origin(t, next, nX0, nY0, nZ0);
+      halo_wrap_table(t, nX0, nY0, nZ0, wnext);
+    }
+    __syncthreads();  // this tile's halo and the next tile's table are in place
+    int slot[kTilePre];
+    if (next >= 0) {
+      HaloDigits d = d0;
+#pragma unroll
+      for (int u = 0; u < kTilePre; ++u) {
+        slot[u] = tid + u * nth < total ? halo_slot(h, t, wnext, d, ncell) : -1;
+        halo_step(t, d);
+      }
+#pragma unroll
+      for (int u = 0; u < kTilePre; ++u)
+        if (slot[u] >= 0) v[u] = src[slot[u]];
+    }
+    if (valid) {
+      R b0, b1, b2v;
+      if (t.ucls)
+        tile_field<R, PL, true>(h, t, sm, hcell, b, site, b0, b1, b2v);
+      else
+        tile_field<R, PL, false>(h, t, sm, hcell, b, site, b0, b1, b2v);
+      const SiteIn<R> in = llg_fetch<R, ALG, HALF>(prm, b + t.NA * cell, 0, site, e, p, b2);
+      llg_apply<R, ALG, HALF>(prm, site, site, b0, b1, b2v, in, p, b2, out);
+    }
+    if (next < 0) break;
+    __syncthreads();  // every read of this tile's halo is done
+#pragma unroll
+    for (int u = 0; u < kTilePre; ++u)
+      if (slot[u] >= 0) halo_put<R, PL>(sm, tid + u * nth, v[u]);
+    halo_rest<R, PL>(h, t, wnext, src, sm, ncell);
+    tile = next;
+    X0 = nX0;
+    Y0 = nY0;
+    Z0 = nZ0;
+  }
 }
 
 // ----------------------------------------------------------------- MC

@@ -96,32 +96,66 @@ hipError_t launch_mc(hipStream_t s, const Tune &t, const HamDev<R> &h, const MCP
 }
 
 template <typename R>
-hipError_t launch_llg_tile(hipStream_t s, const TileDev &t, int alg, int half,
+hipError_t launch_llg_tile(hipStream_t s, TileDev t, int pipe, int alg, int half,
                            const HamDev<R> &h, const LLGDev<R> &prm, const Quad<R> *e,
                            Quad<R> *p, Quad<R> *b2, Quad<R> *out) {
   const int ntiles = t.ntx * t.nty * t.ntz;
   if (ntiles == 0) return hipSuccess;
   const int threads = t.TX * t.TY * t.TZ * t.NA;
   if (threads > 1024 || h.M != 1) return hipErrorInvalidValue;
-  if (t.NA * t.HV > kTilePlane) return hipErrorInvalidValue;
-  const size_t lds = tile_lds_bytes(sizeof(R), t.HX + t.HY + t.HZ);
-  const dim3 grid((ntiles + kXCD - 1) / kXCD * kXCD), block(threads);
-#define UASD_TILE_CASE(A, H)                                                                    \
-  if (alg == A && half == H) {                                                                  \
+  const int plane = tile_plane(t.NA * t.HV);
+  if (plane == 0) return hipErrorInvalidValue;
+  const size_t lds = tile_lds_bytes(sizeof(R), plane, t.HX + t.HY + t.HZ);
+  // one tile per workgroup, or (pipe) as many workgroups as are resident at
+  // once (occupancy x CUs, cached per kernel and shape), each looping over a
+  // run of consecutive tiles
+  auto tiles_per_wg = [&](const void *kern, size_t &c_lds, int &c_thr, int &c_slots) -> hipError_t {
+    t.per = 1;
+    if (!pipe) return hipSuccess;
+    if (c_lds != lds || c_thr != threads) {
+      int dev = 0, ncu = 0, occ = 0;
+      hipError_t err = hipGetDevice(&dev);
+      if (err == hipSuccess)
+        err = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+      if (err == hipSuccess)
+        err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, threads, lds);
+      if (err != hipSuccess) return err;
+      c_lds = lds;
+      c_thr = threads;
+      c_slots = std::max(1, occ) * std::max(1, ncu);
+    }
+    t.per = (ntiles + c_slots - 1) / c_slots;
+    return hipSuccess;
+  };
+#define UASD_TILE_CASE(A, H, PL)                                                                \
+  if (alg == A && half == H && plane == PL) {                                                   \
     static size_t lds_set = 0; /* dynamic LDS above 64 KiB must be requested */                \
     if (lds > lds_set) {                                                                        \
-      hipError_t err = hipFuncSetAttribute((const void *)&k_llg_tile<R, A, H>,                  \
+      hipError_t err = hipFuncSetAttribute((const void *)&k_llg_tile<R, A, H, PL>,              \
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
       if (err != hipSuccess) return err;                                                        \
       lds_set = lds;                                                                            \
     }                                                                                           \
-    hipLaunchKernelGGL((k_llg_tile<R, A, H>), grid, block, lds, s, h, prm, t, e, p, b2, out);   \
+    static size_t c_lds = 0;                                                                    \
+    static int c_thr = 0, c_slots = 0;                                                          \
+    hipError_t gerr = tiles_per_wg((const void *)&k_llg_tile<R, A, H, PL>, c_lds, c_thr, c_slots); \
+    if (gerr != hipSuccess) return gerr;                                                        \
+    const int nwg = (ntiles + t.per - 1) / t.per;                                               \
+    const dim3 grid((nwg + kXCD - 1) / kXCD * kXCD), block(threads);                            \
+    hipLaunchKernelGGL((k_llg_tile<R, A, H, PL>), grid, block, lds, s, h, prm, t, e, p, b2, out); \
     return hipGetLastError();                                                                   \
   }
-  UASD_TILE_CASE(1, 0)
-  UASD_TILE_CASE(1, 1)
-  UASD_TILE_CASE(5, 0)
-  UASD_TILE_CASE(5, 1)
+#define UASD_TILE_PLANES(A, H)                                                                  \
+  UASD_TILE_CASE(A, H, kTilePlanes[0])                                                          \
+  UASD_TILE_CASE(A, H, kTilePlanes[1])                                                          \
+  UASD_TILE_CASE(A, H, kTilePlanes[2])                                                          \
+  UASD_TILE_CASE(A, H, kTilePlanes[3])
+  static_assert(kNumTilePlanes == 4, "one case per plane size");
+  UASD_TILE_PLANES(1, 0)
+  UASD_TILE_PLANES(1, 1)
+  UASD_TILE_PLANES(5, 0)
+  UASD_TILE_PLANES(5, 1)
+#undef UASD_TILE_PLANES
 #undef UASD_TILE_CASE
   return hipErrorInvalidValue;
 }
@@ -183,7 +217,7 @@ hipError_t launch_segsum(hipStream_t s, int M, int nseg, int nchunks, const int2
                                     Quad<R> *, Quad<R> *, const Quad<R> *, Quad<R> *);        \
   template hipError_t launch_mc<R>(hipStream_t, const Tune &, const HamDev<R> &,              \
                                    const MCParams &, Quad<R> *);                              \
-  template hipError_t launch_llg_tile<R>(hipStream_t, const TileDev &, int, int,              \
+  template hipError_t launch_llg_tile<R>(hipStream_t, TileDev, int, int, int,              \
                                          const HamDev<R> &, const LLGDev<R> &,                \
                                          const Quad<R> *, Quad<R> *, Quad<R> *, Quad<R> *);   \
   template hipError_t launch_reduce<R>(hipStream_t, int, int, const Quad<R> *, const double *, \

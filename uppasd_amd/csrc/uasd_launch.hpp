@@ -16,22 +16,24 @@ constexpr int kBlock = 256;  // largest workgroup the kernels are built for
 //   idx32   1: int32 list entries even where packed int16 entries exist
 //   tile    fused LLG only: 0 = list gathers, k > 0 = LDS-tiled stencil with
 //           tile shape kTileShapes[k-1] (periodic lattices with a stencil)
-constexpr int kNumTileShapes = 12;
+constexpr int kNumTileShapes = 15;
 constexpr int kTileShapes[kNumTileShapes][3] = {
-    {8, 8, 4},  {16, 8, 2}, {32, 4, 2}, {16, 4, 4}, {16, 16, 1}, {32, 8, 1},
-    {64, 4, 1}, {8, 4, 4},  {4, 4, 4},  {8, 8, 2},  {16, 8, 1},  {8, 8, 1}};
+    {8, 8, 4}, {16, 8, 2}, {32, 4, 2}, {16, 4, 4}, {16, 16, 1}, {32, 8, 1}, {64, 4, 1}, {8, 4, 4},
+    {4, 4, 4}, {8, 8, 2},  {16, 8, 1}, {8, 8, 1},  {16, 8, 4},  {8, 16, 4}, {8, 8, 8}};
 struct Tune {
   int gather = 4;
   int block = 256;
   int nt = 0;
   int idx32 = 0;
   int tile = 0;
+  int pipe = 0;  // tile kernel: 1 = resident workgroups loop over several tiles
 };
 
 inline bool tune_valid(const Tune &t) {
   return (t.gather == 2 || t.gather == 4 || t.gather == 8 || t.gather == 16) &&
          (t.block == 64 || t.block == 128 || t.block == 256) && (t.nt == 0 || t.nt == 1) &&
-         (t.idx32 == 0 || t.idx32 == 1) && t.tile >= 0 && t.tile <= kNumTileShapes;
+         (t.idx32 == 0 || t.idx32 == 1) && t.tile >= 0 && t.tile <= kNumTileShapes &&
+         (t.pipe == 0 || (t.pipe == 1 && t.tile > 0));
 }
 
 // The Hamiltonian view a launch uses under a tuning choice.
@@ -63,14 +65,21 @@ hipError_t launch_mc(hipStream_t s, const Tune &t, const HamDev<R> &h, const MCP
 
 // Fused LLG half-step with the LDS-tiled stencil kernel (M = 1).
 template <typename R>
-hipError_t launch_llg_tile(hipStream_t s, const TileDev &t, int alg, int half,
+hipError_t launch_llg_tile(hipStream_t s, TileDev t, int pipe, int alg, int half,
                            const HamDev<R> &h, const LLGDev<R> &prm, const Quad<R> *e,
                            Quad<R> *p, Quad<R> *b2, Quad<R> *out);
 // LDS bytes the tile kernel needs for a halo tile of hv cells
-// LDS of k_llg_tile: three planes of kTilePlane values (NA*HV used), then
-// the halo's wrapped coordinate table (nwrap = HX + HY + HZ ints)
-inline size_t tile_lds_bytes(size_t rsize, int nwrap) {
-  return 3 * (size_t)kTilePlane * rsize + (size_t)nwrap * sizeof(int);
+// Component plane size of k_llg_tile for NA*HV halo values: the smallest of
+// kTilePlanes that holds them, or 0.
+inline int tile_plane(int nahv) {
+  for (int k = 0; k < kNumTilePlanes; ++k)
+    if (nahv <= kTilePlanes[k]) return kTilePlanes[k];
+  return 0;
+}
+// LDS of k_llg_tile: three planes of `plane` values (NA*HV used), then two
+// wrapped coordinate tables of the halo (nwrap = HX + HY + HZ ints each)
+inline size_t tile_lds_bytes(size_t rsize, int plane, int nwrap) {
+  return 3 * (size_t)plane * rsize + 2 * (size_t)nwrap * sizeof(int);
 }
 
 // FP64 sums per ensemble of emomM (nv = 3) or of emomM and en (nv = 4);

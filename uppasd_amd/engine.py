@@ -26,7 +26,8 @@ from ._lib import (UASD_FP32, UASD_FP64, UASD_NOISE_INJECTED, UASD_NOISE_NONE,
 
 # LDS tile shapes (cells x, y, z) selectable with set_tuning(TUNE_LLG, g, 256 | (k << 18))
 TILE_SHAPES = ((8, 8, 4), (16, 8, 2), (32, 4, 2), (16, 4, 4), (16, 16, 1), (32, 8, 1), (64, 4, 1),
-               (8, 4, 4), (4, 4, 4), (8, 8, 2), (16, 8, 1), (8, 8, 1))
+               (8, 4, 4), (4, 4, 4), (8, 8, 2), (16, 8, 1), (8, 8, 1), (16, 8, 4), (8, 16, 4),
+               (8, 8, 8))
 
 __all__ = ["Engine", "moment_update", "UasdError", "UASD_NOISE_NONE", "UASD_NOISE_PHILOX",
            "UASD_NOISE_INJECTED", "UASD_SDE_MIDPOINT", "UASD_SDE_DEPONDT"]

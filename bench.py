@@ -55,7 +55,8 @@ def kernel_bytes(W: int, s: int, ib: int = 4, cls: int = 4, nsave: int = 16) -> 
 
 def profile_path(order: int, variant: str) -> str:
     """Committed rocprofv3 summary of the FP64 midpoint workload for a device
-    site order and kernel variant (idx4 / idx2 list gathers, tileK stencil)."""
+    site order and kernel variant (idx4 / idx2 list gathers, tileK stencil with
+    one tile per workgroup, tileKr with resident workgroups looping over tiles)."""
     return os.path.join(ROOT, "profiles",
                         f"k_llg_fp64_midpoint_bcc64_58nn_order{order}_{variant}.md")
 
@@ -72,6 +73,16 @@ def profiled_traffic(kernel: str, order: int, variant: str):
         if len(cells) > 6 and cells[1] == f"`{kernel}`":
             return (float(cells[4]) + float(cells[5])) * 1e6
     return None
+
+
+def tile_plane(sysd, tile: int) -> int:
+    """Component plane size the tile kernel launches with (its template
+    argument): the smallest of 1160, 2312, 3080, 3848 holding NA * HV, for the
+    bcc five-shell stencil (reach 2 cells each way)."""
+    from uppasd_amd.engine import TILE_SHAPES
+    tx, ty, tz = TILE_SHAPES[tile - 1]
+    nahv = sysd["lattice"].NA * (tx + 4) * (ty + 4) * (tz + 4)
+    return next(p for p in (1160, 2312, 3080, 3848) if nahv <= p)
 
 
 def build_c2(L: int):
@@ -449,7 +460,9 @@ def main():
     traffic = None
     if args.precision == "fp64" and args.L == 64 and args.solver == 1 and not nt:
         if tile:
-            traffic = profiled_traffic("uasd::k_llg_tile<double, 1, 1>", order, f"tile{tile}")
+            plane = tile_plane(sysd, tile)
+            traffic = profiled_traffic(f"uasd::k_llg_tile<double, 1, 1, {plane}>", order,
+                                       f"tile{tile}{'r' if pipe else ''}")
         else:
             traffic = profiled_traffic(f"uasd::k_llg<double, 1, 1, true, {gather}>", order,
                                        f"idx{ib}")

@@ -99,14 +99,18 @@ def test_colouring_is_proper():
     assert ncol >= 2
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("mode,label", [("M", "bcc Fe (MC)"), ("H", "bcc Fe (HB)")])
-def test_bccfe_mc_statistics_vs_reference(mode, label):
+def test_bccfe_mc_statistics_vs_reference(mode, label, precision):
+    """bccFe at 500 K through the device sweeps (both precisions; the FP32
+    engine draws its trial moves in single precision): the cumulant row of
+    the reference's regression run within 2 %."""
     c = RC.read_case("bccFe", "inpsd.dat.base")
     lat = U.Lattice(c["cell"], c["bas"], c["atype"], c["n"], c["bc"])
     sh = U.Shells.from_lines(lat, c["jlines"])
     nl, ns, nc = U.build_pair_list(lat, sh, c["sym"], c["ammom"].reshape(-1, 1))
     N, M = lat.natom, 16
-    eng = U.Engine(N, M)
+    eng = U.Engine(N, M, precision)
     eng.set_exchange(nl, ns, nc)
     e0 = np.ascontiguousarray(np.broadcast_to(np.tile(c["edir"], (N // lat.NA, 1)), (M, N, 3)))
     eng.set_moments(e0, np.full((M, N), 2.23))

@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace uasd {
 
 // ---------------------------------------------------------------- constants
@@ -1365,6 +1367,42 @@ __global__ void __launch_bounds__(256) k_mc_colour(HamDev<R> h, MCParams mp, Qua
     const R f1 = (R)0 + s1 + (q1 + x1 + (R)0);
     const R f2 = (R)0 + s2 + (q2 + x2 + (R)0);
     heat_bath<R>(mp.hbeta[ens], me.w, f0, f1, f2, u1, u0, o0, o1, o2);
+  } else if (std::is_same<R, float>::value) {
+    // FP32 engine: trial moves and acceptance in single precision from the
+    // same uniforms (24 bits each).  Uniform direction by z = 1 - 2 u2,
+    // sin(theta) = sqrt(1 - z^2) (the same distribution as acos / sin), the
+    // azimuth by the polynomial sincos and the Gaussian step by the thermal
+    // deviates' Box-Muller, both of the thermal field.
+    const float v1 = (float)u1, v2 = (float)u2, v3 = (float)u3;
+    const int ftype = (int)floor(3 * u0);
+    float n0, n1, n2;
+    if (ftype == 0) {
+      const float z = 1.0f - 2.0f * v2;
+      const float st = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+      float sp, cp;
+      bm::sincos_q(4.0f * v1, sp, cp);
+      n0 = st * cp;
+      n1 = st * sp;
+      n2 = z;
+    } else if (ftype == 1) {
+      float g0, g1, g2;
+      thermal3(mp.seed, kTagMCG, sweep, hs, (uint32_t)ens, g0, g1, g2);
+      const float dl = (float)mp.delta[ens];
+      const float a0 = (float)me.x + g0 * dl, a1 = (float)me.y + g1 * dl, a2 = (float)me.z + g2 * dl;
+      const float il = 1.0f / sqrtf(a0 * a0 + a1 * a1 + a2 * a2);
+      n0 = a0 * il;
+      n1 = a1 * il;
+      n2 = a2 * il;
+    } else {
+      n0 = 0.0f - (float)me.x;
+      n1 = 0.0f - (float)me.y;
+      n2 = 0.0f - (float)me.z;
+    }
+    const R de = mc_delta_e<R, G>(h, (const Quad<R> *)s, site, ens, (R)n0, (R)n1, (R)n2, x0, x1, x2);
+    if (!(de <= (R)0 || v3 < expf(-(float)mp.beta[ens] * (float)de))) return;
+    o0 = (R)n0;
+    o1 = (R)n1;
+    o2 = (R)n2;
   } else {
     double g0 = 0.0, g1 = 0.0, g2 = 0.0;
     if ((int)floor(3 * u0) == 1)

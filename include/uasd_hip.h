@@ -196,7 +196,9 @@ int uasd_index_bytes(uasd_ctx *ctx, int *exchange, int *dm);
  * shape k = 1..15 (kind 0 only; needs uasd_set_lattice_dims and a stencil):
  * cells (x, y, z) = (8,8,4) (16,8,2) (32,4,2) (16,4,4) (16,16,1) (32,8,1)
  * (64,4,1) (8,4,4) (4,4,4) (8,8,2) (16,8,1) (8,8,1) (16,8,4) (8,16,4) (8,8,8),
- * one thread per site (cells x basis atoms <= 1024).
+ * one thread per site (cells x basis atoms <= 1024); plus 0x400000 for
+ * resident workgroups that each loop over a run of adjacent tiles, with the
+ * next tile's halo loads in flight during the current tile's computation.
  * Results do not depend on any of them. */
 int uasd_set_tuning(uasd_ctx *ctx, int kind, int gather, int block);
 int uasd_get_tuning(uasd_ctx *ctx, int kind, int *gather, int *block);

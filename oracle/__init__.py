@@ -120,7 +120,6 @@ def _declare(L):
     L.or_bm_ln.argtypes = [C.c_float]
     L.or_bm_ln.restype = C.c_float
     L.or_bm_sincos.argtypes = [C.c_float, fp]
-    L.or_noise3_f32.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, fp]
     L.or_uniform4.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, dp]
     L.or_philox4x32_10.argtypes = [C.POINTER(C.c_uint32 * 4), C.POINTER(C.c_uint32 * 2),
                                    C.POINTER(C.c_uint32 * 4)]

@@ -197,21 +197,6 @@ void or_noise3_f64(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
   out[2] = r1 * cos(t1);
 }
 
-void or_noise3_f32(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
-                   uint32_t ens, float out[3])
-{
-  const float two_pi = 6.28318530717958647692f;
-  uint32_t w[4];
-  counter_words(seed, tag, step, site, ens, w);
-  float u[4];
-  for (int i = 0; i < 4; ++i) u[i] = ((float)(w[i] >> 8) + 0.5f) * 5.9604644775390625e-08f;
-  float r0 = sqrtf(-2.0f * logf(u[0])), r1 = sqrtf(-2.0f * logf(u[2]));
-  float t0 = two_pi * u[1], t1 = two_pi * u[3];
-  out[0] = r0 * cosf(t0);
-  out[1] = r0 * sinf(t0);
-  out[2] = r1 * cosf(t1);
-}
-
 /* Thermal-field deviates of the LLG steps (both precisions of the HIP
  * kernels): Box-Muller in single precision with polynomial ln, sin and cos,
  * as the kernels compute it, operation for operation (IEEE adds, multiplies,

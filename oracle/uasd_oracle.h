@@ -49,8 +49,6 @@ void   or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t o
 double or_u32_to_open01(uint32_t x);                       /* (x+0.5)*2^-32 */
 void   or_noise3_f64(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
                      uint32_t ens, double out[3]);
-void   or_noise3_f32(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
-                     uint32_t ens, float out[3]);
 void   or_thermal3(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
                    uint32_t ens, float out[3]);        /* LLG thermal deviates */
 float  or_bm_ln(float u);

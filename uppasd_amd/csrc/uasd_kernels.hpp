@@ -137,45 +137,23 @@ __device__ __forceinline__ void counter_words(uint64_t seed, uint32_t tag, uint6
                 (uint32_t)seed, (uint32_t)(seed >> 32), w0, w1, w2, w3);
 }
 
-template <typename R>
-struct Noise;
-
-template <>
-struct Noise<double> {
-  // three N(0,1) deviates: Box-Muller on (u0,u1) and (u2,u3)
-  __device__ __forceinline__ static void normal3(uint64_t seed, uint32_t tag, uint64_t step,
-                                                 uint32_t site, uint32_t ens, double &a,
-                                                 double &b, double &c) {
-    uint32_t w0, w1, w2, w3;
-    counter_words(seed, tag, step, site, ens, w0, w1, w2, w3);
-    const double s = 2.3283064365386963e-10, two_pi = 6.283185307179586;
-    double u0 = ((double)w0 + 0.5) * s, u1 = ((double)w1 + 0.5) * s;
-    double u2 = ((double)w2 + 0.5) * s, u3 = ((double)w3 + 0.5) * s;
-    double r0 = sqrt(-2.0 * log(u0)), r1 = sqrt(-2.0 * log(u2));
-    double t0 = two_pi * u1, t1 = two_pi * u3;
-    a = r0 * cos(t0);
-    b = r0 * sin(t0);
-    c = r1 * cos(t1);
-  }
-};
-
-template <>
-struct Noise<float> {
-  __device__ __forceinline__ static void normal3(uint64_t seed, uint32_t tag, uint64_t step,
-                                                 uint32_t site, uint32_t ens, float &a, float &b,
-                                                 float &c) {
-    uint32_t w0, w1, w2, w3;
-    counter_words(seed, tag, step, site, ens, w0, w1, w2, w3);
-    const float s = 5.9604644775390625e-08f, two_pi = 6.28318530717958647692f;
-    float u0 = ((float)(w0 >> 8) + 0.5f) * s, u1 = ((float)(w1 >> 8) + 0.5f) * s;
-    float u2 = ((float)(w2 >> 8) + 0.5f) * s, u3 = ((float)(w3 >> 8) + 0.5f) * s;
-    float r0 = sqrtf(-2.0f * logf(u0)), r1 = sqrtf(-2.0f * logf(u2));
-    float t0 = two_pi * u1, t1 = two_pi * u3;
-    a = r0 * cosf(t0);
-    b = r0 * sinf(t0);
-    c = r1 * cosf(t1);
-  }
-};
+// Three N(0,1) deviates in FP64 for the FP64 MC engine's Gaussian trial
+// step: Box-Muller on (u0,u1) and (u2,u3) with libm (the oracle restates it,
+// or_noise3_f64, equal to libm ulps).
+__device__ __forceinline__ void normal3_f64(uint64_t seed, uint32_t tag, uint64_t step,
+                                            uint32_t site, uint32_t ens, double &a, double &b,
+                                            double &c) {
+  uint32_t w0, w1, w2, w3;
+  counter_words(seed, tag, step, site, ens, w0, w1, w2, w3);
+  const double s = 2.3283064365386963e-10, two_pi = 6.283185307179586;
+  double u0 = ((double)w0 + 0.5) * s, u1 = ((double)w1 + 0.5) * s;
+  double u2 = ((double)w2 + 0.5) * s, u3 = ((double)w3 + 0.5) * s;
+  double r0 = sqrt(-2.0 * log(u0)), r1 = sqrt(-2.0 * log(u2));
+  double t0 = two_pi * u1, t1 = two_pi * u3;
+  a = r0 * cos(t0);
+  b = r0 * sin(t0);
+  c = r1 * cos(t1);
+}
 
 // Thermal-field deviates of the LLG steps, FP32 for both precisions (the
 // reference draws them in single precision too: r4_nor,
@@ -1406,8 +1384,7 @@ __global__ void __launch_bounds__(256) k_mc_colour(HamDev<R> h, MCParams mp, Qua
   } else {
     double g0 = 0.0, g1 = 0.0, g2 = 0.0;
     if ((int)floor(3 * u0) == 1)
-      Noise<double>::normal3(mp.seed, kTagMCG, sweep, hs, (uint32_t)ens, g0, g1,
-                             g2);
+      normal3_f64(mp.seed, kTagMCG, sweep, hs, (uint32_t)ens, g0, g1, g2);
     R n0, n1, n2;
     trial_move<R>(me.x, me.y, me.z, mp.delta[ens], u0, u1, u2, g0, g1, g2, n0, n1, n2);
     const R de = mc_delta_e<R, G>(h, (const Quad<R> *)s, site, ens, n0, n1, n2, x0, x1, x2);

@@ -800,7 +800,8 @@ __global__ void __launch_bounds__(256)
 // For a periodic lattice whose neighbour lists are one translation-invariant
 // stencil per basis atom (reference site i = b + NA*(x + N1*(y + N2*z)),
 // neighbour j of basis b at basis b' and cell offset (dx, dy, dz) for every
-// cell), a workgroup owns one tile of TX*TY*TZ cells.  It stages emomM = e*m
+// cell), a workgroup owns a tile of TX*TY*TZ cells (or, in the resident
+// launch, a run of adjacent tiles one after the other).  It stages emomM = e*m
 // of the tile plus a halo of (RX, RY, RZ) cells for all basis atoms in LDS,
 // one plane per component at a fixed stride of PL values (so the
 // three reads of a neighbour share one address and differ by an immediate

@@ -126,3 +126,28 @@ def test_autotune_mc_leaves_chain_unchanged():
         return eng.get_moments()[0]
 
     assert np.array_equal(run(True), run(False))
+
+
+def test_sampled_kernel_timing_counts_and_leaves_results():
+    """HIP-event timing of one launch in n of each kind: the launch counts are
+    the sampled ones, the durations positive, and the trajectory is the one
+    of an untimed run."""
+    rng = np.random.default_rng(61)
+    sysd = CF.bcc_shells(6, CF.bcc5_couplings())
+    emom = np.ascontiguousarray(CF.random_unit(sysd["lattice"].natom, rng, 1))
+    ref, _, _ = _llg_run(sysd, emom, 4, 256, "fp64", 1, steps=24)
+    N = sysd["lattice"].natom
+    eng = U.Engine(N, 1)
+    eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
+    eng.set_moments(emom, np.broadcast_to(sysd["mmom"], (1, N)))
+    eng.set_llg(0.1, 300.0, 1e-16)
+    eng.set_noise(U.UASD_NOISE_PHILOX, 1234)
+    eng.set_tuning(eng.TUNE_LLG, 4, 256)
+    eng.set_kernel_timing(True, every=4)
+    eng.sd_steps(1, 24)
+    (n0, ms0), (n1, ms1) = eng.kernel_timing(0), eng.kernel_timing(1)
+    eng.set_kernel_timing(False)
+    assert (n0, n1) == (6, 6) and ms0 > 0.0 and ms1 > 0.0
+    assert np.array_equal(eng.get_moments()[0], ref)
+    with pytest.raises(U.UasdError):
+        eng.set_kernel_timing(True, every=-1)

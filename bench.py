@@ -68,9 +68,13 @@ def profiled_traffic(kernel: str, order: int, variant: str):
     path = profile_path(order, variant)
     if not os.path.exists(path):
         return None
+    in_table = False          # rows of the FETCH_SIZE / WRITE_SIZE table only
     for line in open(path):
         cells = [c.strip() for c in line.split("|")]
-        if len(cells) > 6 and cells[1] == f"`{kernel}`":
+        if len(cells) > 3 and cells[1] == "kernel":
+            in_table = len(cells) > 3 and cells[2].startswith("FETCH_SIZE")
+            continue
+        if in_table and len(cells) > 6 and cells[1] == f"`{kernel}`":
             return (float(cells[4]) + float(cells[5])) * 1e6
     return None
 

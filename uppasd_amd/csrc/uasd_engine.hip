@@ -130,6 +130,8 @@ struct uasd_ctx {
   double dt = 1e-16, temprescale = 1.0, bn = 1.0;
   double lam0 = 0, temp0 = 0, lande0 = 1;
   bool lam_uniform = true, temp_uniform = true, lande_uniform = true;
+  bool m_uniform = false;  // every moment magnitude equal (m0), as uploaded
+  double m0 = 0;
   int noise_mode = UASD_NOISE_NONE;
   uint64_t seed = 0;
   uint64_t step = 0;      // LLG step counter (Philox counter)
@@ -501,6 +503,12 @@ uasd::LLGDev<R> llg_view(uasd_ctx *c) {
   p.step = c->step;
   if (c->noise_mode == UASD_NOISE_INJECTED) p.noise = quads<R>(c, c->noise);
   p.lc0 = uasd::lam_const<R>(c->lam0, c->dt, c->bn);
+  if (c->lam_uniform && c->temp_uniform && c->m_uniform) {
+    const double m = (double)(R)c->m0;  // as the kernels read it from the quads
+    p.sig_uniform = 1;
+    p.sigk0 = uasd::midpoint_sigma<R>(p.lc0.Dk, m, c->temp0, c->temprescale);
+    p.sigp0 = uasd::depondt_sigma<R>(p.lc0.Dp, m, c->temp0, c->temprescale);
+  }
   return p;
 }
 
@@ -1211,6 +1219,10 @@ int uasd_set_moments(uasd_ctx *c, const double *emom, const double *mmom) {
   if (int r = upload_quads(c, c->spins, 0, hq)) return r;
   if (int r = upload_quads(c, c->spins, 1, hq)) return r;  // emom2 starts as emom
   c->have_state = true;
+  const size_t nm = (size_t)c->N * c->M;
+  c->m_uniform = std::all_of(mmom, mmom + nm, [&](double v) { return v == mmom[0]; });
+  c->m0 = mmom[0];
+  c->gen += 1;  // captured steps hold the noise amplitudes
   return UASD_OK;
 }
 

@@ -515,6 +515,10 @@ struct LLGDev {
   // the corrector (the same values it would draw), or nullptr
   float4 *nsave;
   LamConst<R> lc0;       // lam_const(lam0, dt, bn)
+  // uniform lam, temperature and moment: the noise amplitudes, computed on
+  // the host by the same functions (sig_uniform = 1)
+  int sig_uniform;
+  R sigk0, sigp0;        // midpoint_sigma, depondt_sigma
   __device__ __forceinline__ LamConst<R> lamc(int s) const {
     return lam ? lam_const<R>(lam[s], dt, bn) : lc0;
   }
@@ -542,7 +546,7 @@ struct LLGDev {
 
 // rannum, llg = 1 (randomnumbers.f90:674-676, 735-745)
 template <typename R>
-__device__ __forceinline__ R midpoint_sigma(double Dk, double mmom, double T,
+__host__ __device__ inline R midpoint_sigma(double Dk, double mmom, double T,
                                             double temprescale) {
   double mmomi = 1.0 / mmom;
   double D = Dk * mmomi * T * temprescale;
@@ -551,7 +555,7 @@ __device__ __forceinline__ R midpoint_sigma(double Dk, double mmom, double T,
 
 // depondt.f90:142-145
 template <typename R>
-__device__ __forceinline__ R depondt_sigma(double Dp, double mmom, double T,
+__host__ __device__ inline R depondt_sigma(double Dp, double mmom, double T,
                                            double temprescale) {
   return (R)sqrt(Dp * temprescale * T / mmom);
 }
@@ -727,7 +731,7 @@ __device__ __forceinline__ void llg_apply(const LLGDev<R> &prm, int site, int64_
   const R m = eq.w;
   const R n0 = in.n0, n1 = in.n1, n2 = in.n2;
   if (ALG == 5) {
-    const R sig = depondt_sigma<R>(lc.Dp, (double)m, T, prm.temprescale);
+    const R sig = prm.sig_uniform ? prm.sigp0 : depondt_sigma<R>(lc.Dp, (double)m, T, prm.temprescale);
     const R t0 = n0 * sig, t1 = n1 * sig, t2 = n2 * sig;
     const R bl0 = b0 + t0, bl1 = b1 + t1, bl2 = b2v + t2;
     if (HALF == 0) {
@@ -748,7 +752,7 @@ __device__ __forceinline__ void llg_apply(const LLGDev<R> &prm, int site, int64_
       out[q] = finish<R>(o0, o1, o2, m);
     }
   } else {
-    const R sig = midpoint_sigma<R>(lc.Dk, (double)m, T, prm.temprescale);
+    const R sig = prm.sig_uniform ? prm.sigk0 : midpoint_sigma<R>(lc.Dk, (double)m, T, prm.temprescale);
     const R r0 = n0 * sig, r1 = n1 * sig, r2 = n2 * sig;
     R A0, A1, A2, t0, t1, t2, detAi;
     if (HALF == 0) {

@@ -465,7 +465,8 @@ def main():
     if args.precision == "fp64" and args.L == 64 and args.solver == 1 and not nt:
         if tile:
             plane = tile_plane(sysd, tile)
-            traffic = profiled_traffic(f"uasd::k_llg_tile<double, 1, 1, {plane}>", order,
+            # exchange-only Hamiltonian: the kernel without DM/anisotropy/Zeeman terms
+            traffic = profiled_traffic(f"uasd::k_llg_tile<double, 1, 1, {plane}, false>", order,
                                        f"tile{tile}{'r' if pipe else ''}")
         else:
             traffic = profiled_traffic(f"uasd::k_llg<double, 1, 1, true, {gather}>", order,

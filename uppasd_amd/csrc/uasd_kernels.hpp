@@ -867,7 +867,9 @@ constexpr int kTilePlanes[kNumTilePlanes] = {1160, 2312, 3080, 3848};
 // Effective field of one site of a tile from the LDS halo.  hbase is the
 // halo index of the site's cell.
 // UCLS: the site class is the per-basis one of TileDev (wave-uniform).
-template <typename R, int PL, bool UCLS>
+// FULL: DM, anisotropy and Zeeman terms may be present; without them the
+// kernel holds fewer launch arguments live (exchange-only Hamiltonians).
+template <typename R, int PL, bool UCLS, bool FULL>
 __device__ __forceinline__ void tile_field(const HamDev<R> &h, const TileDev &t, const R *sm,
                                            int hbase, int b, int site, R &b0, R &b1, R &b2v) {
   SiteClass sc;
@@ -913,7 +915,7 @@ __device__ __forceinline__ void tile_field(const HamDev<R> &h, const TileDev &t,
       s2 = fma(J, v[2 * PL], s2);
     }
   }
-  if (h.Wd > 0 && sc.dm_row >= 0) {
+  if (FULL && h.Wd > 0 && sc.dm_row >= 0) {
     const int len = h.dm_len[sc.dm_row];
     const int *off = t.doff + b * t.Wd;
     for (int j = 0; j < len; ++j) {
@@ -927,7 +929,7 @@ __device__ __forceinline__ void tile_field(const HamDev<R> &h, const TileDev &t,
       s2 = s2 + D1 * m0 - D0 * m1;
     }
   }
-  if (sc.an_row >= 0) {
+  if (FULL && sc.an_row >= 0) {
     const int ty_ = h.an_type[sc.an_row];
     const R *v = sm + (hbase + b * t.HV);
     const R m0 = v[0], m1 = v[PL], m2 = v[2 * PL];
@@ -946,11 +948,11 @@ __device__ __forceinline__ void tile_field(const HamDev<R> &h, const TileDev &t,
     }
   }
   R x0 = 0, x1 = 0, x2 = 0;
-  if (h.ext_mode == 1) {
+  if (FULL && h.ext_mode == 1) {
     x0 = h.ext_u[0];
     x1 = h.ext_u[1];
     x2 = h.ext_u[2];
-  } else if (h.ext_mode == 2) {
+  } else if (FULL && h.ext_mode == 2) {
     const Quad<R> v = h.ext_q[site];
     x0 = v.x;
     x1 = v.y;
@@ -1075,7 +1077,7 @@ __device__ __forceinline__ void halo_rest(const HamDev<R> &h, const TileDev &t, 
 // XCD-aware block order, adjacent in x.  The halo loads of its next tile are
 // issued before the current tile is computed and stored after it, so their
 // latency is covered by that computation.
-template <typename R, int ALG, int HALF, int PL>
+template <typename R, int ALG, int HALF, int PL, bool FULL>
 __global__ void __launch_bounds__(1024)
     k_llg_tile(HamDev<R> h, LLGDev<R> prm, TileDev t, const Quad<R> *e, Quad<R> *p,
                Quad<R> *b2, Quad<R> *out) {
@@ -1153,9 +1155,9 @@ __global__ void __launch_bounds__(1024)
     if (valid) {
       R b0, b1, b2v;
       if (t.ucls)
-        tile_field<R, PL, true>(h, t, sm, hcell, b, site, b0, b1, b2v);
+        tile_field<R, PL, true, FULL>(h, t, sm, hcell, b, site, b0, b1, b2v);
       else
-        tile_field<R, PL, false>(h, t, sm, hcell, b, site, b0, b1, b2v);
+        tile_field<R, PL, false, FULL>(h, t, sm, hcell, b, site, b0, b1, b2v);
       const SiteIn<R> in = llg_fetch<R, ALG, HALF>(prm, b + t.NA * cell, 0, site, e, p, b2);
       llg_apply<R, ALG, HALF>(prm, site, site, b0, b1, b2v, in, p, b2, out);
     }

@@ -104,6 +104,8 @@ hipError_t launch_llg_tile(hipStream_t s, TileDev t, int pipe, int alg, int half
   const int threads = t.TX * t.TY * t.TZ * t.NA;
   if (threads > 1024 || h.M != 1) return hipErrorInvalidValue;
   const int plane = tile_plane(t.NA * t.HV);
+  // DM, anisotropy or Zeeman terms present: the full kernel
+  const bool full = h.Wd > 0 || h.an_type != nullptr || h.ext_mode != 0;
   if (plane == 0) return hipErrorInvalidValue;
   const size_t lds = tile_lds_bytes(sizeof(R), plane, t.HX + t.HY + t.HZ);
   // one tile per workgroup, or (pipe) as many workgroups as are resident at
@@ -127,35 +129,39 @@ hipError_t launch_llg_tile(hipStream_t s, TileDev t, int pipe, int alg, int half
     t.per = (ntiles + c_slots - 1) / c_slots;
     return hipSuccess;
   };
-#define UASD_TILE_CASE(A, H, PL)                                                                \
-  if (alg == A && half == H && plane == PL) {                                                   \
+#define UASD_TILE_CASE(A, H, PL, F)                                                             \
+  if (alg == A && half == H && plane == PL && full == F) {                                                   \
     static size_t lds_set = 0; /* dynamic LDS above 64 KiB must be requested */                \
     if (lds > lds_set) {                                                                        \
-      hipError_t err = hipFuncSetAttribute((const void *)&k_llg_tile<R, A, H, PL>,              \
+      hipError_t err = hipFuncSetAttribute((const void *)&k_llg_tile<R, A, H, PL, F>,              \
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
       if (err != hipSuccess) return err;                                                        \
       lds_set = lds;                                                                            \
     }                                                                                           \
     static size_t c_lds = 0;                                                                    \
     static int c_thr = 0, c_slots = 0;                                                          \
-    hipError_t gerr = tiles_per_wg((const void *)&k_llg_tile<R, A, H, PL>, c_lds, c_thr, c_slots); \
+    hipError_t gerr = tiles_per_wg((const void *)&k_llg_tile<R, A, H, PL, F>, c_lds, c_thr, c_slots); \
     if (gerr != hipSuccess) return gerr;                                                        \
     const int nwg = (ntiles + t.per - 1) / t.per;                                               \
     const dim3 grid((nwg + kXCD - 1) / kXCD * kXCD), block(threads);                            \
-    hipLaunchKernelGGL((k_llg_tile<R, A, H, PL>), grid, block, lds, s, h, prm, t, e, p, b2, out); \
+    hipLaunchKernelGGL((k_llg_tile<R, A, H, PL, F>), grid, block, lds, s, h, prm, t, e, p, b2, out); \
     return hipGetLastError();                                                                   \
   }
+#define UASD_TILE_FULL(A, H, PL)                                                                \
+  UASD_TILE_CASE(A, H, PL, false)                                                               \
+  UASD_TILE_CASE(A, H, PL, true)
 #define UASD_TILE_PLANES(A, H)                                                                  \
-  UASD_TILE_CASE(A, H, kTilePlanes[0])                                                          \
-  UASD_TILE_CASE(A, H, kTilePlanes[1])                                                          \
-  UASD_TILE_CASE(A, H, kTilePlanes[2])                                                          \
-  UASD_TILE_CASE(A, H, kTilePlanes[3])
+  UASD_TILE_FULL(A, H, kTilePlanes[0])                                                          \
+  UASD_TILE_FULL(A, H, kTilePlanes[1])                                                          \
+  UASD_TILE_FULL(A, H, kTilePlanes[2])                                                          \
+  UASD_TILE_FULL(A, H, kTilePlanes[3])
   static_assert(kNumTilePlanes == 4, "one case per plane size");
   UASD_TILE_PLANES(1, 0)
   UASD_TILE_PLANES(1, 1)
   UASD_TILE_PLANES(5, 0)
   UASD_TILE_PLANES(5, 1)
 #undef UASD_TILE_PLANES
+#undef UASD_TILE_FULL
 #undef UASD_TILE_CASE
   return hipErrorInvalidValue;
 }

@@ -751,8 +751,9 @@ bool tile_fits(const uasd_ctx *c, int k) {
   if ((int64_t)((c->lat_n1 + sh[0] - 1) / sh[0]) * ((c->lat_n2 + sh[1] - 1) / sh[1]) *
           ((c->lat_n3 + sh[2] - 1) / sh[2]) >= (1 << 22))
     return false;
-  return (int64_t)uasd::tile_lds_bytes(c->rsize, plane, sh[0] + sh[1] + sh[2] + 2 * (rx + ry + rz)) <=
-         c->lds_max;
+  const int ntab = sh[0] + 2 * rx + c->lat_na * (sh[1] + 2 * ry) * (sh[2] + 2 * rz);
+  if ((sh[0] + 2 * rx) > sh[0] * sh[1] * sh[2] * c->lat_na) return false;  // a row per round
+  return (int64_t)uasd::tile_lds_bytes(c->rsize, plane, ntab) <= c->lds_max;
 }
 
 // TileDev for shape k, uploading its halo offsets when the shape changes.
@@ -777,15 +778,7 @@ int make_tile(uasd_ctx *c, int k, uasd::TileDev &t) {
   t.rTXY = 1.0f / (float)(t.TX * t.TY);
   t.rTC = 1.0f / (float)(t.TX * t.TY * t.TZ);
   t.rHX = 1.0f / (float)t.HX;
-  t.rHXY = 1.0f / (float)(t.HX * t.HY);
-  t.rHV = 1.0f / (float)t.HV;
-  {
-    const int nth = t.TX * t.TY * t.TZ * t.NA, plane = t.HX * t.HY;
-    t.ab = nth / t.HV;
-    t.az = (nth % t.HV) / plane;
-    t.ay = (nth % plane) / t.HX;
-    t.ax = nth % t.HX;
-  }
+  t.rs = t.TX * t.TY * t.TZ * t.NA / t.HX;  // halo rows staged per round
   t.ntx = (t.N1 + t.TX - 1) / t.TX;
   t.nty = (t.N2 + t.TY - 1) / t.TY;
   t.ntz = (t.N3 + t.TZ - 1) / t.TZ;

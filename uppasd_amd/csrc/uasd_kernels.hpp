@@ -822,8 +822,8 @@ struct TileDev {
   int HX, HY, HZ, HV;     // halo tile extents, HV = HX*HY*HZ
   // 1/d in FP32 for the small divisors d of the index arithmetic:
   // fdiv(n, 1/d) = n/d exactly for 0 <= n < 2^22
-  float rTX, rTXY, rTC, rHX, rHXY, rHV, rntx, rntxy;
-  int ax, ay, az, ab;     // digits (x, y, z, basis) of the staging stride TX*TY*TZ*NA
+  float rTX, rTXY, rTC, rHX, rntx, rntxy;
+  int rs;                 // halo rows staged per round: blockDim.x / HX
   int per;                // tiles per workgroup (consecutive in the XCD-aware order)
   int ntx, nty, ntz;      // tiles per direction
   int W, Wd;              // exchange / DM stencil widths
@@ -963,47 +963,17 @@ __device__ __forceinline__ void tile_field(const HamDev<R> &h, const TileDev &t,
   b2v = (R)0 + s2 + (qf2 + x2 + (R)0);
 }
 
-// Halo items per thread whose loads are held in registers while the previous
-// tile is computed (more would spill); the rest is staged after it, in
-// rounds of kTileBatch loads issued together.
+// Halo staging.  The halo of a tile is NR = NA*HZ*HY rows of HX cells; item
+// i = row*HX + hx goes to value i of each LDS plane.  Thread t stages column
+// t % HX of rows t / HX, t / HX + RS, ... with RS = blockDim.x / HX rows per
+// round (the last blockDim.x % HX threads stage nothing).  A table per tile
+// holds the x term of every column and the slot of every row's first cell,
+// so an item's device slot is one LDS read and an add.
+// Items per thread whose loads are held in registers while the previous tile
+// is computed (more would spill); the rest are staged after it, in rounds of
+// kTileBatch loads issued together.
 constexpr int kTilePre = 2;
 constexpr int kTileBatch = 4;
-
-// Digits (hx, hy, hz, hb) of halo item i = hb*HV + (hz*HY + hy)*HX + hx.
-struct HaloDigits {
-  int hx, hy, hz, hb;
-};
-
-__device__ __forceinline__ HaloDigits halo_digits(const TileDev &t, int i) {
-  HaloDigits d;
-  d.hb = fdiv(i, t.rHV);
-  int l = i - d.hb * t.HV;
-  d.hz = fdiv(l, t.rHXY);
-  l -= d.hz * t.HX * t.HY;
-  d.hy = fdiv(l, t.rHX);
-  d.hx = l - d.hy * t.HX;
-  return d;
-}
-
-// Advance the digits by the stride (t.ax, t.ay, t.az, t.ab) = blockDim.x.
-__device__ __forceinline__ void halo_step(const TileDev &t, HaloDigits &d) {
-  d.hx += t.ax;
-  d.hy += t.ay;
-  d.hz += t.az;
-  d.hb += t.ab;
-  if (d.hx >= t.HX) {
-    d.hx -= t.HX;
-    d.hy += 1;
-  }
-  if (d.hy >= t.HY) {
-    d.hy -= t.HY;
-    d.hz += 1;
-  }
-  if (d.hz >= t.HZ) {
-    d.hz -= t.HZ;
-    d.hb += 1;
-  }
-}
 
 // Origin cell of tile `tile` (x fastest, z slowest).
 __device__ __forceinline__ void tile_origin(const TileDev &t, int tile, int &X0, int &Y0, int &Z0) {
@@ -1014,29 +984,44 @@ __device__ __forceinline__ void tile_origin(const TileDev &t, int tile, int &X0,
   Z0 = tz * t.TZ;
 }
 
-// Wrapped lattice coordinates of the halo rows of a tile, pre-multiplied into
-// cell index terms: x, N1*y, N1*N2*z (periodic images of any multiplicity).
-__device__ __forceinline__ void halo_wrap_table(const TileDev &t, int X0, int Y0, int Z0,
-                                                int *wrap) {
-  for (int i = threadIdx.x; i < t.HX + t.HY + t.HZ; i += blockDim.x) {
+// The staging table of a tile: tab[hx] is the x term of halo column hx,
+// tab[HX + row] the device slot of the row's basis atom at its wrapped (y, z)
+// and x = 0, so that slot = tab[hx] + tab[HX + row] (periodic images of any
+// multiplicity).
+template <typename R>
+__device__ __forceinline__ void halo_table(const HamDev<R> &h, const TileDev &t, int X0, int Y0,
+                                           int Z0, int ncell, int *tab) {
+  const int nr = t.NA * t.HY * t.HZ, hyz = t.HY * t.HZ;
+  for (int i = threadIdx.x; i < t.HX + nr; i += blockDim.x) {
     int v;
     if (i < t.HX) {
-      v = ((X0 + i - t.RX) % t.N1 + t.N1) % t.N1;
-    } else if (i < t.HX + t.HY) {
-      v = t.N1 * (((Y0 + i - t.HX - t.RY) % t.N2 + t.N2) % t.N2);
+      const int x = ((X0 + i - t.RX) % t.N1 + t.N1) % t.N1;
+      v = h.order_na == 1 ? t.NA * x : x;
     } else {
-      v = t.N1 * t.N2 * (((Z0 + i - t.HX - t.HY - t.RZ) % t.N3 + t.N3) % t.N3);
+      const int r = i - t.HX;
+      const int hb = r / hyz, rz = r - hb * hyz;
+      const int hz = rz / t.HY, hy = rz - hz * t.HY;
+      const int y = ((Y0 + hy - t.RY) % t.N2 + t.N2) % t.N2;
+      const int z = ((Z0 + hz - t.RZ) % t.N3 + t.N3) % t.N3;
+      const int cyz = t.N1 * (y + t.N2 * z);
+      v = h.order_na == 1 ? hb + t.NA * cyz : hb * ncell + cyz;
     }
-    wrap[i] = v;
+    tab[i] = v;
   }
 }
 
-// Device slot of halo item d through the wrap table.
-template <typename R>
-__device__ __forceinline__ int halo_slot(const HamDev<R> &h, const TileDev &t, const int *wrap,
-                                         const HaloDigits &d, int ncell) {
-  const int cell = wrap[d.hx] + wrap[t.HX + d.hy] + wrap[t.HX + t.HY + d.hz];
-  return h.order_na == 1 ? d.hb + t.NA * cell : d.hb * ncell + cell;
+// This thread's staging column and first row.
+struct HaloCol {
+  int hx, r0;
+  bool on;  // the thread stages items
+};
+
+__device__ __forceinline__ HaloCol halo_col(const TileDev &t) {
+  HaloCol c;
+  c.r0 = fdiv(threadIdx.x, t.rHX);
+  c.hx = threadIdx.x - c.r0 * t.HX;
+  c.on = c.r0 < t.rs;
+  return c;
 }
 
 template <typename R, int PL>
@@ -1047,29 +1032,36 @@ __device__ __forceinline__ void halo_put(R *sm, int item, const Quad<R> &v) {
   d[2 * PL] = v.z * v.w;
 }
 
-// Halo items of this thread past the first kTilePre, loaded and stored in
-// rounds of kTileBatch.
+// Issue the loads of K rounds of this thread's items, from round k0.  item[u]
+// is -1 for rounds past the last row.
+template <typename R, int K>
+__device__ __forceinline__ void halo_load(const TileDev &t, const int *tab, const HaloCol &c,
+                                          int k0, const Quad<R> *src, Quad<R> (&v)[K],
+                                          int (&item)[K]) {
+  const int nr = t.NA * t.HY * t.HZ;
+  const int xterm = tab[c.hx];
+#pragma unroll
+  for (int u = 0; u < K; ++u) {
+    const int r = c.r0 + (k0 + u) * t.rs;
+    item[u] = c.on && r < nr ? r * t.HX + c.hx : -1;
+  }
+#pragma unroll
+  for (int u = 0; u < K; ++u)
+    if (item[u] >= 0) v[u] = src[tab[t.HX + c.r0 + (k0 + u) * t.rs] + xterm];
+}
+
+// Items of this thread from round kTilePre on, in batches of kTileBatch.
 template <typename R, int PL>
-__device__ __forceinline__ void halo_rest(const HamDev<R> &h, const TileDev &t, const int *wrap,
-                                          const Quad<R> *src, R *sm, int ncell) {
-  const int nth = blockDim.x, total = t.NA * t.HV;
-  int base = threadIdx.x + kTilePre * nth;
-  if (base >= total) return;
-  HaloDigits d = halo_digits(t, base);
-  for (; base < total; base += kTileBatch * nth) {
-    int slot[kTileBatch];
-#pragma unroll
-    for (int u = 0; u < kTileBatch; ++u) {
-      slot[u] = base + u * nth < total ? halo_slot(h, t, wrap, d, ncell) : -1;
-      halo_step(t, d);
-    }
+__device__ __forceinline__ void halo_rest(const TileDev &t, const int *tab, const HaloCol &c,
+                                          const Quad<R> *src, R *sm) {
+  const int nr = t.NA * t.HY * t.HZ;
+  for (int k = kTilePre; c.on && c.r0 + k * t.rs < nr; k += kTileBatch) {
     Quad<R> v[kTileBatch];
+    int item[kTileBatch];
+    halo_load<R, kTileBatch>(t, tab, c, k, src, v, item);
 #pragma unroll
     for (int u = 0; u < kTileBatch; ++u)
-      if (slot[u] >= 0) v[u] = src[slot[u]];
-#pragma unroll
-    for (int u = 0; u < kTileBatch; ++u)
-      if (slot[u] >= 0) halo_put<R, PL>(sm, base + u * nth, v[u]);
+      if (item[u] >= 0) halo_put<R, PL>(sm, item[u], v[u]);
   }
 }
 
@@ -1089,8 +1081,8 @@ __global__ void __launch_bounds__(1024)
   const int last = min(ntiles, first + t.per);
   if (first >= last) return;  // whole workgroups exit before any barrier
   const int ncell = t.N1 * t.N2 * t.N3;
-  const int nw = t.HX + t.HY + t.HZ;
-  int *wraps = reinterpret_cast<int *>(sm + 3 * PL);  // two tables, alternating by tile
+  const int nw = t.HX + t.NA * t.HY * t.HZ;
+  int *tabs = reinterpret_cast<int *>(sm + 3 * PL);  // two tables, alternating by tile
   // this thread's site in every tile: cell (lx, ly, lz), basis b.  blockDim.x
   // = tc * NA with tc a multiple of 64, so b is the same for all lanes of a
   // wavefront and the stencil offsets (and, for one species, the couplings)
@@ -1102,30 +1094,19 @@ __global__ void __launch_bounds__(1024)
   const int lz = fdiv(lcell, t.rTXY), lxy = lcell - lz * t.TX * t.TY;
   const int ly = fdiv(lxy, t.rTX), lx = lxy - ly * t.TX;
   const int hcell = (lz + t.RZ) * t.HX * t.HY + (ly + t.RY) * t.HX + (lx + t.RX);
-  const HaloDigits d0 = halo_digits(t, tid);
-  const int total = t.NA * t.HV, nth = blockDim.x;
-  // the first tile: wrap table, halo, own data
+  const HaloCol col = halo_col(t);
+  // the first tile: table, halo
   int X0, Y0, Z0;
   tile_origin(t, first, X0, Y0, Z0);
-  halo_wrap_table(t, X0, Y0, Z0, wraps);
+  halo_table(h, t, X0, Y0, Z0, ncell, tabs);
   __syncthreads();
   Quad<R> v[kTilePre];
-  {
-    HaloDigits d = d0;
-    int slot[kTilePre];
+  int item[kTilePre];
+  halo_load<R, kTilePre>(t, tabs, col, 0, src, v, item);
 #pragma unroll
-    for (int u = 0; u < kTilePre; ++u) {
-      slot[u] = tid + u * nth < total ? halo_slot(h, t, wraps, d, ncell) : -1;
-      halo_step(t, d);
-    }
-#pragma unroll
-    for (int u = 0; u < kTilePre; ++u)
-      if (slot[u] >= 0) v[u] = src[slot[u]];
-#pragma unroll
-    for (int u = 0; u < kTilePre; ++u)
-      if (slot[u] >= 0) halo_put<R, PL>(sm, tid + u * nth, v[u]);
-    halo_rest<R, PL>(h, t, wraps, src, sm, ncell);
-  }
+  for (int u = 0; u < kTilePre; ++u)
+    if (item[u] >= 0) halo_put<R, PL>(sm, item[u], v[u]);
+  halo_rest<R, PL>(t, tabs, col, src, sm);
   for (int tile = first;;) {
     const int k = tile - first;
     const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
@@ -1133,25 +1114,14 @@ __global__ void __launch_bounds__(1024)
     const int cell = x + t.N1 * (y + t.N2 * z);
     const int site = h.order_na == 1 ? b + t.NA * cell : b * ncell + cell;
     const int next = tile + 1 < last ? tile + 1 : -1;
-    int *wnext = wraps + ((k + 1) & 1) * nw;
+    int *tnext = tabs + ((k + 1) & 1) * nw;
     int nX0 = 0, nY0 = 0, nZ0 = 0;
     if (next >= 0) {
       tile_origin(t, next, nX0, nY0, nZ0);
-      halo_wrap_table(t, nX0, nY0, nZ0, wnext);
+      halo_table(h, t, nX0, nY0, nZ0, ncell, tnext);
     }
     __syncthreads();  // this tile's halo and the next tile's table are in place
-    int slot[kTilePre];
-    if (next >= 0) {
-      HaloDigits d = d0;
-#pragma unroll
-      for (int u = 0; u < kTilePre; ++u) {
-        slot[u] = tid + u * nth < total ? halo_slot(h, t, wnext, d, ncell) : -1;
-        halo_step(t, d);
-      }
-#pragma unroll
-      for (int u = 0; u < kTilePre; ++u)
-        if (slot[u] >= 0) v[u] = src[slot[u]];
-    }
+    if (next >= 0) halo_load<R, kTilePre>(t, tnext, col, 0, src, v, item);
     if (valid) {
       R b0, b1, b2v;
       if (t.ucls)
@@ -1165,8 +1135,8 @@ __global__ void __launch_bounds__(1024)
     __syncthreads();  // every read of this tile's halo is done
 #pragma unroll
     for (int u = 0; u < kTilePre; ++u)
-      if (slot[u] >= 0) halo_put<R, PL>(sm, tid + u * nth, v[u]);
-    halo_rest<R, PL>(h, t, wnext, src, sm, ncell);
+      if (item[u] >= 0) halo_put<R, PL>(sm, item[u], v[u]);
+    halo_rest<R, PL>(t, tnext, col, src, sm);
     tile = next;
     X0 = nX0;
     Y0 = nY0;

@@ -107,7 +107,8 @@ hipError_t launch_llg_tile(hipStream_t s, TileDev t, int pipe, int alg, int half
   // DM, anisotropy or Zeeman terms present: the full kernel
   const bool full = h.Wd > 0 || h.an_type != nullptr || h.ext_mode != 0;
   if (plane == 0) return hipErrorInvalidValue;
-  const size_t lds = tile_lds_bytes(sizeof(R), plane, t.HX + t.HY + t.HZ);
+  if (t.rs < 1) return hipErrorInvalidValue;  // at least one halo row per round
+  const size_t lds = tile_lds_bytes(sizeof(R), plane, t.HX + t.NA * t.HY * t.HZ);
   // one tile per workgroup, or (pipe) as many workgroups as are resident at
   // once (occupancy x CUs, cached per kernel and shape), each looping over a
   // run of consecutive tiles

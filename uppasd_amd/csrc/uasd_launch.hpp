@@ -77,9 +77,9 @@ inline int tile_plane(int nahv) {
   return 0;
 }
 // LDS of k_llg_tile: three planes of `plane` values (NA*HV used), then two
-// wrapped coordinate tables of the halo (nwrap = HX + HY + HZ ints each)
-inline size_t tile_lds_bytes(size_t rsize, int plane, int nwrap) {
-  return 3 * (size_t)plane * rsize + 2 * (size_t)nwrap * sizeof(int);
+// staging tables of the halo (ntab = HX + NA*HY*HZ ints each)
+inline size_t tile_lds_bytes(size_t rsize, int plane, int ntab) {
+  return 3 * (size_t)plane * rsize + 2 * (size_t)ntab * sizeof(int);
 }
 
 // FP64 sums per ensemble of emomM (nv = 3) or of emomM and en (nv = 4);

@@ -18,7 +18,7 @@ CASES = {  # case dir -> files to copy
     "HeisStripe": ["inpsd.dat", "jfile", "kfile", "momfile", "posfile"],
     "kagome": ["inpsd.dat", "jfile", "dmfile", "momfile", "posfile"],
     "SCsurf": ["inpsd.dat", "jij", "dmdata", "kdata", "momfile", "posfile"],
-    "HeisChain": ["inpsd.dat", "jfile", "momfile", "posfile"],
+    "HeisChain": ["inpsd.dat", "jfile", "momfile", "posfile", "startmom"],
 }
 
 

@@ -9,6 +9,7 @@ Tolerances (FP64 path, -ffp-contract=off on both sides):
     reference's own 'similar' comparison
 """
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -327,3 +328,38 @@ def test_index_width_selection_and_int32_path(O):
     bref, eref = O.OracleHam(N, nlist, nsize, ncoup).field(emomM)
     assert rel(beff, bref) <= 1e-13
     assert abs(en - eref) <= 1e-12 * abs(eref)
+
+
+def test_heischain_from_restart_file_on_gpu(O):
+    """The reference's Heisenberg-chain case on the device: the start read from
+    its restart file (uppasd_amd.restart), 20000 midpoint steps at 0.1 K with
+    damping 4 on the injected ziggurat stream (default tseed 1), then T = 0
+    without damping: <M> at iteration 1000 and the energy per atom at
+    iteration 1400 against the regression values."""
+    from uppasd_amd import restart as RS
+    c = RC.read_case("HeisChain", "inpsd.dat")
+    lat = U.Lattice(c["cell"], c["bas"], c["atype"], c["n"], c["bc"])
+    sh = U.Shells.from_lines(lat, c["jlines"])
+    nl, ns, nc = U.build_pair_list(lat, sh, c["sym"], c["ammom"].reshape(-1, 1))
+    N = lat.natom
+    _, mmom, emom = RS.read_restart(os.path.join(RC.CASES, "HeisChain", "startmom"), N, 1)
+    eng = U.Engine(N, 1)
+    eng.set_exchange(nl, ns, nc)
+    eng.set_moments(emom, mmom)
+    eng.set_llg(4.0, 0.1, 1e-16)
+    z = O.zig(1)
+    noise = np.zeros((1, N, 3))
+    for _ in range(20000):
+        O.lib().or_zig_fill(ctypes.byref(z), noise.reshape(-1), noise.size)
+        eng.set_noise(U.UASD_NOISE_INJECTED, 0, noise)
+        eng.sd_steps(1, 1)
+    eng.set_llg(0.0, 0.0, 1e-16)
+    eng.set_noise(U.UASD_NOISE_NONE)
+    exp = {g["type"]: g["expected"] for g in RC.golden("HeisChain")}
+    eng.sd_steps(1, 1000)
+    ms, _ = eng.measure()
+    m = ms[0] / N
+    assert np.allclose([*m, np.linalg.norm(m)], exp["Magnetization"], rtol=0, atol=1e-8)
+    eng.sd_steps(1, 400)
+    _, en = eng.measure(with_energy=True)
+    assert abs(en[0] / N - exp["Total energy"][0]) <= 1e-8

@@ -7,6 +7,7 @@ the printed digits.  Tolerance: the yaml prints 9 significant digits; its
 'similar' comparison allows |delta| <= 1e-8.
 """
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -171,3 +172,36 @@ def test_kagome_golden_rejects_flipped_dm(O):
         sim.step(1, 1.0e-16, 0.5, 0.0)
     m = sim.mavg()[0]
     assert not np.allclose([*m, np.linalg.norm(m)], gm["expected"], atol=1e-4)
+
+
+def test_heischain_from_restart_file_golden(O):
+    """Heisenberg chain (100 spins, J = 2.5 mRy, vacuum in x and y): the start
+    comes from the reference's restart file `startmom` through
+    uppasd_amd.restart.read_restart (header skipped, moments normalised as
+    read_mag_conf_std does).  20000 initial-phase midpoint steps at 0.1 K with
+    damping 4 on the ziggurat stream (default tseed 1), then the measurement
+    phase at T = 0 without damping: <M> at iteration 1000 and the total
+    energy per atom at iteration 1400 against the regression values."""
+    import ctypes
+    from uppasd_amd import restart as RS
+    c = RC.read_case("HeisChain", "inpsd.dat")
+    lat = _lattice(O, c)
+    N = lat.natom
+    rstep, mmom, emom = RS.read_restart(os.path.join(RC.CASES, "HeisChain", "startmom"), N, 1)
+    assert rstep == 0 and np.all(mmom == 1.0)
+    sim = RC.OracleSD(O, lat, _pairs(O, c, lat), mmom[0])
+    sim.set_state(emom)
+    z = O.zig(1)
+    noise = np.zeros_like(sim.e)
+    for _ in range(20000):
+        O.lib().or_zig_fill(ctypes.byref(z), noise.reshape(-1), noise.size)
+        sim.step(1, 1e-16, 4.0, 0.1, normals=noise)
+    zero = np.zeros_like(sim.e)
+    exp = {g["type"]: g["expected"] for g in RC.golden("HeisChain")}
+    for _ in range(1000):
+        sim.step(1, 1e-16, 0.0, 0.0, normals=zero)
+    m = sim.mavg()[0]
+    assert np.allclose([*m, np.linalg.norm(m)], exp["Magnetization"], rtol=0, atol=1e-8)
+    for _ in range(400):
+        sim.step(1, 1e-16, 0.0, 0.0, normals=zero)
+    assert abs(sim.ham.field(sim.eM)[1] / N - exp["Total energy"][0]) <= 1e-8

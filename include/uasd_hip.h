@@ -156,6 +156,14 @@ int uasd_set_segments(uasd_ctx *ctx, int nseg, const int *offsets);
 /* FP64 per (segment, ensemble) sums: msum (3,M,nseg) = sum emomM over the
  * segment, energy (M,nseg) the field-routine energy in mRy, or NULL. */
 int uasd_measure_segments(uasd_ctx *ctx, double *msum, double *energy);
+/* Term-resolved energies of calc_energy (energy.f90:60-366 with do_lsf 'N'
+ * and scalar exchange; the terms of the accelerated Hamiltonian):
+ * ene (4,M,nseg), ene[4*(seg*M + k) + t] = energy per site in mRy of term t
+ * (0 exchange "Exc", 1 anisotropy "Ani", 2 DM, 3 external field "Zeeman") in
+ * ensemble k of segment seg (one segment = the whole lattice), i.e. the
+ * reference's ene%ene_xc(k) ... times mub/mry.  Anisotropy type 7 counts
+ * 0.5*(uniaxial + sb*cubic), as calc_energy does. */
+int uasd_energy_terms(uasd_ctx *ctx, double *ene);
 /* One cumulant sample per (segment, ensemble), kept on the device in FP64:
  * m = |sum emomM|/n and e = energy/n (mRy per site), as running means with
  * the linearly increasing weights of calc_and_print_cumulant

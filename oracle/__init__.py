@@ -137,6 +137,7 @@ def _declare(L):
     L.or_effective_field.argtypes = [C.POINTER(Ham), C.c_int, C.c_int, dp, vp, dp, vp, vp,
                                      C.POINTER(C.c_double)]
     L.or_effective_field_single.argtypes = [C.POINTER(Ham), C.c_int, dp, vp, C.c_int, C.c_int, dp]
+    L.or_energy_terms.argtypes = [C.POINTER(Ham), C.c_int, C.c_int, dp, vp, dp]
     L.or_depondt_first.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, C.c_double, dp, dp,
                                    dp, dp, dp, dp, dp, vp]
     L.or_depondt_second.argtypes = [C.c_int, C.c_int, dp, C.c_double, dp, dp, dp, dp, dp]
@@ -317,6 +318,17 @@ class OracleHam:
         """(beff, energy, beff1, beff2) as effective_field_full returns them."""
         beff, en = self.field(emomM, ext)
         return beff, en, self.last_parts[0], self.last_parts[1]
+
+    def energy_terms(self, emomM, ext=None):
+        """calc_energy terms per ensemble: (M, 4) per atom in field units,
+        columns exchange, anisotropy, DM, external field (times mub/mry for
+        mRy)."""
+        M = emomM.shape[0]
+        out = np.zeros((M, 4))
+        ext_arr = None if ext is None else np.ascontiguousarray(ext, dtype=np.float64)
+        lib().or_energy_terms(C.byref(self.h), self.N, M, np.ascontiguousarray(emomM),
+                              ext_arr.ctypes.data if ext_arr is not None else None, out)
+        return out
 
     def field_single(self, emomM, i, k, ext_uniform=None):
         out = np.zeros(3)

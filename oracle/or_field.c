@@ -7,6 +7,10 @@
  *   dzyaloshinskii_moriya_field 546-580
  *   uniaxial_anisotropy_field   842-879
  *   cubic_anisotropy_field      883-919
+ * and the term-resolved energies of source/Hamiltonian/energy.f90:
+ *   calc_energy site loop       183-324 (do_lsf 'N', scalar exchange)
+ *   division per atom           353-366
+ *   update_ene                  559-571
  * Only the terms on the accelerated path are restated (exchange, DM,
  * uniaxial / cubic / type-7 anisotropy, static external field); the
  * time-dependent field is taken as zero as in sd_mphase without pulses.
@@ -129,5 +133,59 @@ void or_effective_field_single(const or_ham *h, int N, const double *emomM,
   for (int c = 0; c < 3; ++c) {
     double b2 = bq[c] + be[c] + 0.0;
     out[c] = 0.0 + bs[c] + b2;
+  }
+}
+
+/* update_ene (energy.f90:559-571): -factor * sum(m * b). */
+static double update_ene(const double m[3], const double b[3], double factor)
+{
+  return -factor * (m[0] * b[0] + m[1] * b[1] + m[2] * b[2]);
+}
+
+/* calc_energy for the terms on the accelerated path (energy.f90:183-366):
+ * per ensemble k, each field built from zero at every site and dotted with
+ * emomM, summed in site order and divided by Natom.  ene[4*k + t] in field
+ * units (the caller scales by mub/mry), t = 0 exchange, 1 anisotropy, 2 DM,
+ * 3 external field.  Anisotropy type 7 is uniaxial + sb * cubic
+ * (energy.f90:311-316); the external field is external_field +
+ * time_external_field with the latter zero. */
+void or_energy_terms(const or_ham *h, int N, int M, const double *emomM, const double *ext,
+                     double *ene)
+{
+  for (int k = 0; k < M; ++k) {
+    double exc = 0.0, eani = 0.0, edm = 0.0, eext = 0.0;
+    for (int i = 0; i < N; ++i) {
+      double m[3] = {V3(emomM, 0, i, k, N), V3(emomM, 1, i, k, N), V3(emomM, 2, i, k, N)};
+      double bxc[3] = {0.0, 0.0, 0.0};
+      heisenberg_field(h, i, k, N, emomM, bxc);
+      exc = exc + update_ene(m, bxc, 0.5);
+      if (h->do_dm) {
+        double bdm[3] = {0.0, 0.0, 0.0};
+        dm_field(h, i, k, N, emomM, bdm);
+        edm = edm + update_ene(m, bdm, 0.5);
+      }
+      if (h->do_anisotropy) {
+        int t = h->taniso[i];
+        double ba[3] = {0.0, 0.0, 0.0};
+        if (t == 1) {
+          uniaxial_field(h, i, k, N, emomM, ba);
+        } else if (t == 2) {
+          cubic_field(h, i, k, N, emomM, ba);
+        } else if (t == 7) {
+          double bc[3] = {0.0, 0.0, 0.0};
+          uniaxial_field(h, i, k, N, emomM, ba);
+          cubic_field(h, i, k, N, emomM, bc);
+          for (int c = 0; c < 3; ++c) ba[c] = ba[c] + bc[c] * h->sb[i];
+        }
+        if (t == 1 || t == 2 || t == 7) eani = eani + update_ene(m, ba, 0.5);
+      }
+      double be[3];
+      for (int c = 0; c < 3; ++c) be[c] = 0.0 + (ext ? V3(ext, c, i, k, N) : 0.0);
+      eext = eext + update_ene(m, be, 1.0);
+    }
+    ene[4 * k + 0] = exc / N;
+    ene[4 * k + 1] = eani / N;
+    ene[4 * k + 2] = edm / N;
+    ene[4 * k + 3] = eext / N;
   }
 }

@@ -172,6 +172,10 @@ double or_mc_energy_change(const or_ham *h, int N, const double *emomM,
 /* ext_uniform: one field vector applied to every site, or NULL */
 void   or_effective_field_single(const or_ham *h, int N, const double *emomM,
                                  const double *ext_uniform, int i, int k, double out[3]);
+/* calc_energy terms (energy.f90:183-366): ene (M,4) per atom in field units,
+ * columns exchange, anisotropy, DM, external field. */
+void   or_energy_terms(const or_ham *h, int N, int M, const double *emomM,
+                       const double *ext, double *ene);
 void   or_flip_h(double *e_out, double mmom, double temperature, double temprescale,
                  double flipprob, const double totfield[3], double q);
 void   or_choose_random_atom_x(or_mt *st, int N, int *iflip_a);

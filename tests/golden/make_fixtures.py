@@ -19,6 +19,13 @@ CASES = {  # case dir -> files to copy
     "kagome": ["inpsd.dat", "jfile", "dmfile", "momfile", "posfile"],
     "SCsurf": ["inpsd.dat", "jij", "dmdata", "kdata", "momfile", "posfile"],
     "HeisChain": ["inpsd.dat", "jfile", "momfile", "posfile", "startmom"],
+    "Regression": ["inpsd.dat", "jfile", "momfile", "posfile"],
+}
+
+# Cases whose golden values come from another descriptor file than
+# regulartests.yaml, and the outputs of them that the accelerated path covers.
+EXTRA = {
+    "Regression": ("regressionResaro.yaml", ("averages", "cumulants", "totenergy")),
 }
 
 
@@ -48,6 +55,23 @@ def main(ref):
                                 "expected": t["expected"],
                                 "comparison": t.get("comparison_func", "similar"),
                                 "type": t.get("type", "")})
+    for case, (yml, kinds) in EXTRA.items():
+        with open(os.path.join(ref, "tests", yml)) as fh:
+            y = yaml.safe_load(fh)
+        entries = golden.setdefault(case, [])
+        for c in y["caselist"]:
+            if os.path.basename(os.path.normpath(c["workingdir"])) != case:
+                continue
+            for comp in c["comparisons"]:
+                if os.path.basename(comp["outfile"]).split(".")[0] not in kinds:
+                    continue
+                for t in comp["testlist"]:
+                    entries.append({"name": c["name"], "outfile": os.path.basename(comp["outfile"]),
+                                    "headers": comp.get("headers", []),
+                                    "select": t["select"], "extract": t["extract"],
+                                    "expected": t["expected"],
+                                    "comparison": t.get("comparison_func", "similar"),
+                                    "type": t.get("type", ""), "source": yml})
     with open(os.path.join(out, "golden.json"), "w") as fh:
         json.dump(golden, fh, indent=1, sort_keys=True)
 

@@ -6,6 +6,7 @@ The file formats follow the reference readers:
   momfile  isite ichem |m| ex ey ez             (inputhandler_ext.f90:228-320)
   jfile    isite jsite rx ry rz J               (inputhandler_ext.f90:444-618)
   dmfile   isite jsite rx ry rz Dx Dy Dz        (inputhandler_ext.f90:1095-1251)
+  (maptype 2: r counts cell vectors from atom i to atom j, inputhandler_ext.f90:1039-1043)
   kfile    iat type K1 K2 ex ey ez ratio        (inputhandler_ext.f90:1069-1089)
 """
 from __future__ import annotations
@@ -76,6 +77,20 @@ def read_case(case, base="inpsd.dat"):
                cell=cell_of(lines), n=tuple(int(x) for x in kv["ncell"][:3]),
                bc=tuple(kv.get("bc", ["0", "0", "0"])[:3]),
                sym=int(kv.get("sym", ["0"])[0]))
+    maptype = int(kv.get("maptype", ["1"])[0])
+    if maptype == 2:
+        cell, bas = out["cell"], out["bas"]
+
+        def bgfm(r):
+            i, j = r[0], r[1]
+            v = np.zeros(3)
+            for a in range(3):
+                v[a] = bas[j - 1][a] - bas[i - 1][a] + cell[0][a] * r[4] + cell[1][a] * r[5] + \
+                    cell[2][a] * r[6]
+            return (*r[:4], *v, *r[7:])
+        out["jlines"] = [bgfm(r) for r in out["jlines"]]
+    if "hfield" in kv:
+        out["hfield"] = np.array([_num(x) for x in kv["hfield"][:3]])
     if "anisotropy" in kv:
         k = _rows(os.path.join(d, os.path.basename(kv["anisotropy"][0])))
         out["ktype"] = np.array([int(r[1]) for r in k], dtype=np.int32)
@@ -83,6 +98,8 @@ def read_case(case, base="inpsd.dat"):
     if "dm" in kv:
         dml = _rows(os.path.join(d, os.path.basename(kv["dm"][0])))
         out["dmlines"] = [(int(r[0]), int(r[1]), 1, 1, *[_num(x) for x in r[2:8]]) for r in dml]
+        if maptype == 2:
+            out["dmlines"] = [bgfm(r) for r in out["dmlines"]]
     return out
 
 

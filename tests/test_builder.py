@@ -89,3 +89,24 @@ def test_kagome_and_alloy(O):
     a = CF.bcc_alloy(5, occ)
     q = _oracle_pairs(O, a["lattice"], a["lines"], 1, a["ammom"], achem=occ, nchmax=2)
     assert _same(a["nlist"], q["nlist"]) and _same(a["ncoup"], q["ncoup"])
+
+
+def test_maptype2_regression_case_bitexact(O):
+    """The reference's measurement regression case gives its exchange in
+    maptype 2 (cell-vector counts from atom i to atom j).  The product's
+    reader converts the raw jfile rows; the test-side reader converts them on
+    its own; the lists agree bit for bit, and every atom has 14 neighbours
+    (8 of the (0.5, 0.5, 0.51) shell under cubic symmetry, 6 of (0, 0, 1)),
+    as the case's golden exchange energy -6.1 mRy requires."""
+    import refcase as RC
+    c = RC.read_case("Regression", "inpsd.dat")
+    lat = U.Lattice(c["cell"], c["bas"], c["atype"], c["n"], c["bc"])
+    raw = [(int(r[0]), int(r[1]), 1, 1, *[float(x) for x in r[2:6]])
+           for r in RC._rows(RC.os.path.join(RC.CASES, "Regression", "jfile"))]
+    sh = U.Shells.from_lines(lat, raw, maptype=2)
+    nl, ns, nc = U.build_pair_list(lat, sh, c["sym"], c["ammom"].reshape(-1, 1))
+    p = _oracle_pairs(O, lat, c["jlines"], c["sym"], c["ammom"].reshape(-1, 1))
+    assert _same(nl, p["nlist"]) and _same(ns, p["nlistsize"]) and _same(nc, p["ncoup"])
+    assert np.all(ns == 14)
+    with pytest.raises(ValueError):
+        U.Shells.from_lines(lat, raw, maptype=3)

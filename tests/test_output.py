@@ -52,10 +52,42 @@ class _OracleRun:
         en = np.array([self.sim.ham.field(self.sim.eM)[1]]) if with_energy else None
         return ms, en
 
+    def energy_terms(self):
+        return self.sim.ham.energy_terms(self.sim.eM) * (OUT.MUB / OUT.MRY)
+
     def advance(self, n, T=500.0, dt=float(np.float32(1.0e-16)), lam=0.5):
         for _ in range(n):
             self.O.lib().or_zig_fill(ctypes.byref(self.z), self.noise.reshape(-1), self.noise.size)
             self.sim.step(1, dt, lam, T, normals=self.noise)
+
+
+def regression_case(O):
+    """The reference's measurement regression case (tests/Regression, simid
+    megaTest): 4x4x4 cells of a two-type basis, maptype-2 exchange with
+    cubic symmetry, moments 1.3 and 2.5 from the momfile, midpoint at T = 0,
+    damping 0.3, dt 1e-16, a 0.05 T field along y."""
+    c = RC.read_case("Regression", "inpsd.dat")
+    lat = O.LatticeSpec(c["cell"], c["bas"], c["atype"], c["n"], c["bc"])
+    pairs = O.build_pairs(lat, c["jlines"], c["sym"], ammom=c["ammom"].reshape(-1, 1))
+    sim = RC.OracleSD(O, lat, pairs, np.tile(c["ammom"], lat.natom // lat.NA))
+    sim.set_state(np.tile(c["edir"], (lat.natom // lat.NA, 1))[None])
+    ext = np.broadcast_to(c["hfield"], sim.e.shape).copy()
+    return c, lat, sim, ext
+
+
+class _OracleFieldRun(_OracleRun):
+    """T = 0 midpoint under a static external field."""
+
+    def __init__(self, O, sim, ext):
+        super().__init__(O, sim, None, None)
+        self.ext = ext
+
+    def energy_terms(self):
+        return self.sim.ham.energy_terms(self.sim.eM, self.ext) * (OUT.MUB / OUT.MRY)
+
+    def advance(self, n):
+        for _ in range(n):
+            self.sim.step(1, 1e-16, 0.3, 0.0, normals=None, ext=self.ext)
 
 
 def test_bccfe_files_match_golden(O, tmp_path):
@@ -85,3 +117,51 @@ def test_bccfe_files_match_golden(O, tmp_path):
     assert lines[0].split()[:2] == ["#Iter", "<M>"]
     assert all(len(l) == 8 + 9 * 16 for l in lines[1:])   # (i8, 9es16.8) + lsf column
     assert [int(l.split()[0]) for l in lines[1:]][:3] == [1, 11, 21]  # cumu_buff 10
+
+
+def test_regression_case_files_match_golden(O, tmp_path):
+    """Averages at iteration 11000 (written by the measurement after the
+    loop), the totenergy row at 10900 (Tot, Exc, Ani, DM, PD, BiqDM, BQ, Dip,
+    Zeeman, LSF) and cumulant row 211 (<E> and the exchange-only <E_exc>
+    differ here) of the reference's measurement regression case, its values
+    of the current revision (regressionResaro.yaml)."""
+    c, lat, sim, ext = regression_case(O)
+    run = _OracleFieldRun(O, sim, ext)
+    ph = OUT.MeasurementPhase(run, "megaTest", 0.0, avrg_step=100, cumu_step=50,
+                              outdir=str(tmp_path), cumu_buff=10)
+    ph.run(11000, run.advance)
+    checked = 0
+    for g in RC.golden("Regression"):
+        path = tmp_path / g["outfile"]
+        got = read_selected(path, g["headers"], g["select"], g["extract"])
+        assert np.allclose(got, g["expected"], rtol=1e-8, atol=1e-8), (g["type"], got,
+                                                                       g["expected"])
+        checked += 1
+    assert checked == 3
+    lines = open(tmp_path / "totenergy.megaTest.out").read().splitlines()
+    assert lines[0].split() == ["#Iter", "Tot", "Exc", "Ani", "DM", "PD", "BiqDM", "BQ", "Dip",
+                                "Zeeman", "LSF", "Chir", "Ring", "SA"]
+    assert len(lines[0]) == 8 + 13 * 20                     # (a8, 16a20)
+    assert all(len(l) == 8 + 13 * 16 for l in lines[1:])    # (i8, 14es16.8)
+    assert [int(l.split()[0]) for l in lines[1:4]] == [0, 100, 200]
+    assert not os.path.exists(tmp_path / "stdenergy.megaTest.out")   # Mensemble = 1
+
+
+def test_totenergy_mean_and_deviation():
+    """calculate_mean_and_deviation over ensembles: the mean, and the sample
+    variance of the field-unit energies times mub/mry (not a standard
+    deviation; energy.f90:577-602); Tot = Exc + DM + Zeeman + Ani."""
+    fc = OUT.MUB / OUT.MRY
+    terms = np.array([[-2.0, 0.1, -0.3, -1e-4], [-2.2, 0.3, -0.1, -3e-4]])
+    te = OUT.TotEnergy("x")
+    means, devs = te.row(terms)
+    raw = terms / fc                                       # field units
+    tot = raw[:, 0] + raw[:, 2] + raw[:, 3] + raw[:, 1]
+    assert np.isclose(means[0], tot.mean() * fc, rtol=1e-14)
+    assert np.isclose(devs[0], np.var(tot, ddof=1) * fc, rtol=1e-12)
+    for j, col in ((1, 0), (2, 1), (3, 2), (8, 3)):
+        assert np.isclose(means[j], raw[:, col].mean() * fc, rtol=1e-14)
+        assert np.isclose(devs[j], np.var(raw[:, col], ddof=1) * fc, rtol=1e-12)
+    assert means[4:8] == [0.0] * 4 and means[9:] == [0.0] * 4
+    m1, d1 = te.row(terms[:1])
+    assert d1 == [0.0] * 13 and m1[1] == -2.0

@@ -31,8 +31,8 @@ EXPORTS = (
     "uasd_sd_steps", "uasd_effective_field", "uasd_mc_sweeps", "uasd_measure",
     "uasd_get_coloring", "uasd_set_kernel_timing", "uasd_get_kernel_timing",
     "uasd_set_tuning", "uasd_get_tuning", "uasd_autotune_llg", "uasd_autotune_mc",
-    "uasd_set_temprescale", "uasd_set_segments", "uasd_measure_segments", "uasd_accumulate",
-    "uasd_get_accumulators", "uasd_reset_accumulators", "uasd_set_graph_steps",
+    "uasd_set_temprescale", "uasd_set_segments", "uasd_measure_segments", "uasd_energy_terms",
+    "uasd_accumulate", "uasd_get_accumulators", "uasd_reset_accumulators", "uasd_set_graph_steps",
     "uasd_index_bytes", "uasd_set_site_order", "uasd_set_lattice_dims", "uasd_has_stencil",
     "uasd_build_pair_list",
 )
@@ -87,6 +87,7 @@ def lib() -> C.CDLL:
         L.uasd_set_segments.argtypes = [vp, C.c_int, vp]
         L.uasd_measure_segments.argtypes = [vp, vp, vp]
         L.uasd_accumulate.argtypes = [vp, C.c_int]
+        L.uasd_energy_terms.argtypes = [vp, vp]
         L.uasd_get_accumulators.argtypes = [vp, vp, C.POINTER(C.c_int64)]
         L.uasd_reset_accumulators.argtypes = [vp]
         L.uasd_set_graph_steps.argtypes = [vp, C.c_int]

@@ -1843,6 +1843,49 @@ int uasd_measure_segments(uasd_ctx *c, double *msum, double *energy) {
   return UASD_OK;
 }
 
+int uasd_energy_terms(uasd_ctx *c, double *ene) {
+  if (int r = check_ctx(c)) return r;
+  if (!c->have_state) return fail(UASD_ERR_STATE, "moments not set");
+  if (!c->have_ex) return fail(UASD_ERR_STATE, "exchange not set");
+  if (!ene) return fail(UASD_ERR_ARG, "ene is null");
+  if (c->M > 65535) return fail(UASD_ERR_UNSUPPORTED, "more than 65535 ensembles");
+  if (int r = ensure_segments(c)) return r;
+  if (int r = rebuild_classes(c)) return r;
+  HIP_TRY(c->seg_partial.ensure((size_t)c->nchunks * c->M * 4 * sizeof(double)));
+  HIP_TRY(c->seg_out.ensure((size_t)c->nseg * c->M * 4 * sizeof(double)));
+  const int2 *chunks = c->seg_dev.as<int2>();
+  const int *seg_chunk = reinterpret_cast<const int *>(chunks + c->nchunks);
+  if (c->prec == UASD_FP64) {
+    HIP_TRY(uasd::launch_energy_terms<double>(c->stream, ham_view<double>(c), c->nseg,
+                                              c->nchunks, chunks, seg_chunk,
+                                              quads<double>(c, c->spins, 0),
+                                              c->seg_partial.as<double>(),
+                                              c->seg_out.as<double>()));
+  } else {
+    HIP_TRY(uasd::launch_energy_terms<float>(c->stream, ham_view<float>(c), c->nseg,
+                                             c->nchunks, chunks, seg_chunk,
+                                             quads<float>(c, c->spins, 0),
+                                             c->seg_partial.as<double>(),
+                                             c->seg_out.as<double>()));
+  }
+  const size_t n = (size_t)c->nseg * c->M * 4;
+  std::vector<double> out(n);
+  HIP_TRY(hipMemcpyAsync(out.data(), c->seg_out.p, n * sizeof(double), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  // per site in mRy: energy.f90:353-366 divides by the number of atoms,
+  // calculate_mean_and_deviation then scales by mub/mry
+  for (int sg = 0; sg < c->nseg; ++sg) {
+    const double na = (double)(c->seg_off[sg + 1] - c->seg_off[sg]);
+    for (int k = 0; k < c->M; ++k)
+      for (int t = 0; t < 4; ++t) {
+        const size_t i = ((size_t)sg * c->M + k) * 4 + t;
+        ene[i] = out[i] / na * (uasd::kMub / uasd::kMry);
+      }
+  }
+  return UASD_OK;
+}
+
 int uasd_accumulate(uasd_ctx *c, int with_energy) {
   if (int r = check_ctx(c)) return r;
   if (!c->have_state) return fail(UASD_ERR_STATE, "moments not set");

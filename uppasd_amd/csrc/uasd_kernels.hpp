@@ -1509,6 +1509,86 @@ __global__ void k_segsum_final(int nseg, int M, const int *__restrict__ seg_chun
   out[t] = x;
 }
 
+// ------------------------------------------------- term-resolved energies
+// The site terms of calc_energy (energy.f90:183-324, do_lsf 'N', scalar
+// exchange): every field is built from zero and dotted with emomM by
+// update_ene (energy.f90:559-571), -factor * m.b, with factor 0.5 for
+// exchange, DM and anisotropy (type 7: uniaxial + sb * cubic field,
+// energy.f90:311-316) and 1 for the external field.  Grid (chunks, M):
+// workgroup (c, ens) sums the chunk's sites in FP64 into
+// partial[(c*M + ens)*4 + t], t = exchange, anisotropy, DM, Zeeman;
+// k_segsum_final<4> then adds the chunks of each segment.
+template <typename R>
+__device__ __forceinline__ double update_ene(R m0, R m1, R m2, R b0, R b1, R b2, double f) {
+  return -f * ((double)m0 * (double)b0 + (double)m1 * (double)b1 + (double)m2 * (double)b2);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_energy_terms(HamDev<R> h, const int2 *__restrict__ chunks, const Quad<R> *__restrict__ src,
+                   double *__restrict__ partial) {
+  constexpr int G = 4;
+  const int2 ch = chunks[blockIdx.x];
+  const int ens = blockIdx.y, M = h.M;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int site = ch.x + (int)threadIdx.x; site < ch.y; site += blockDim.x) {
+    const size_t q = (size_t)site * M + ens;
+    const SiteClass sc = h.classes[h.cls[site]];
+    R m0, m1, m2;
+    moment(src, q, m0, m1, m2);
+    R b0 = (R)0, b1 = (R)0, b2 = (R)0;
+    if (h.nbr16)
+      exchange_sum<R, G, false>(h, h.nbr16 + site, src, site, ens, sc.ex_row, b0, b1, b2);
+    else
+      exchange_sum<R, G, false>(h, h.nbr + site, src, site, ens, sc.ex_row, b0, b1, b2);
+    a[0] += update_ene(m0, m1, m2, b0, b1, b2, 0.5);
+    if (sc.an_row >= 0) {
+      const int t = h.an_type[sc.an_row];
+      if (t == 1 || t == 2 || t == 7) {
+        R u0 = (R)0, u1 = (R)0, u2 = (R)0;
+        if (t == 1) {
+          uniaxial(h, sc.an_row, m0, m1, m2, u0, u1, u2);
+        } else if (t == 2) {
+          cubic(h, sc.an_row, m0, m1, m2, u0, u1, u2);
+        } else {
+          R c0 = (R)0, c1 = (R)0, c2 = (R)0;
+          uniaxial(h, sc.an_row, m0, m1, m2, u0, u1, u2);
+          cubic(h, sc.an_row, m0, m1, m2, c0, c1, c2);
+          const R sb = h.an_sb[sc.an_row];
+          u0 = u0 + c0 * sb;
+          u1 = u1 + c1 * sb;
+          u2 = u2 + c2 * sb;
+        }
+        a[1] += update_ene(m0, m1, m2, u0, u1, u2, 0.5);
+      }
+    }
+    if (h.Wd > 0 && sc.dm_row >= 0) {
+      R d0 = (R)0, d1 = (R)0, d2 = (R)0;
+      if (h.dnbr16)
+        dm_sum<R, G>(h, h.dnbr16 + site, src, site, ens, sc.dm_row, d0, d1, d2);
+      else
+        dm_sum<R, G>(h, h.dnbr + site, src, site, ens, sc.dm_row, d0, d1, d2);
+      a[2] += update_ene(m0, m1, m2, d0, d1, d2, 0.5);
+    }
+    if (h.ext_mode == 1) {
+      a[3] += update_ene(m0, m1, m2, h.ext_u[3 * ens], h.ext_u[3 * ens + 1],
+                         h.ext_u[3 * ens + 2], 1.0);
+    } else if (h.ext_mode == 2) {
+      const Quad<R> v = h.ext_q[q];
+      a[3] += update_ene(m0, m1, m2, v.x, v.y, v.z, 1.0);
+    }
+  }
+  __shared__ double red[256][4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = a[v];
+  __syncthreads();
+  if ((int)threadIdx.x < 4) {
+    double x = 0.0;
+    for (int j = 0; j < (int)blockDim.x; ++j) x += red[j][threadIdx.x];
+    partial[((size_t)blockIdx.x * M + ens) * 4 + threadIdx.x] = x;
+  }
+}
+
 // Cumulant accumulators per (segment, ensemble), the running weighted means
 // of calc_and_print_cumulant (prn_averages.f90:956-1000): sample n has weight
 // n, mean_n = (mean_{n-1} * W_{n-1} + x_n * n) / (W_{n-1} + n).

@@ -216,6 +216,18 @@ hipError_t launch_segsum(hipStream_t s, int M, int nseg, int nchunks, const int2
   return hipGetLastError();
 }
 
+template <typename R>
+hipError_t launch_energy_terms(hipStream_t s, const HamDev<R> &h, int nseg, int nchunks,
+                               const int2 *chunks, const int *seg_chunk, const Quad<R> *spins,
+                               double *partial, double *out) {
+  if (nseg == 0 || h.M == 0 || nchunks == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_energy_terms<R>), dim3(nchunks, h.M), dim3(kBlock), 0, s, h, chunks,
+                     spins, partial);
+  hipLaunchKernelGGL(k_segsum_final<4>, dim3((nseg * h.M * 4 + 255) / 256), dim3(256), 0, s,
+                     nseg, h.M, seg_chunk, partial, out);
+  return hipGetLastError();
+}
+
 #define UASD_INSTANTIATE(R)                                                                   \
   template hipError_t launch_field<R>(hipStream_t, const Tune &, const HamDev<R> &,           \
                                       const Quad<R> *, Quad<R> *, double *, Quad<R> *);       \
@@ -231,6 +243,9 @@ hipError_t launch_segsum(hipStream_t s, int M, int nseg, int nchunks, const int2
                                        int, double *, int, double *);                         \
   template hipError_t launch_segsum<R>(hipStream_t, int, int, int, const int2 *, const int *,  \
                                        const Quad<R> *, const double *, int, double *,        \
-                                       double *);
+                                       double *);                                             \
+  template hipError_t launch_energy_terms<R>(hipStream_t, const HamDev<R> &, int, int,        \
+                                             const int2 *, const int *, const Quad<R> *,      \
+                                             double *, double *);
 
 }  // namespace uasd

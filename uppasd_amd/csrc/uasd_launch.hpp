@@ -96,6 +96,14 @@ hipError_t launch_segsum(hipStream_t s, int M, int nseg, int nchunks, const int2
                          const int *seg_chunk, const Quad<R> *spins, const double *en, int nv,
                          double *partial, double *out);
 
+// Term-resolved energies (see k_energy_terms) per (segment, ensemble):
+// out[(seg*M + ens)*4 + t] for t = exchange, anisotropy, DM, Zeeman, sums
+// over the segment's sites in field units; partial holds nchunks*M*4 doubles.
+template <typename R>
+hipError_t launch_energy_terms(hipStream_t s, const HamDev<R> &h, int nseg, int nchunks,
+                               const int2 *chunks, const int *seg_chunk, const Quad<R> *spins,
+                               double *partial, double *out);
+
 // ctr[idx] = value (add = 0) or ctr[idx] += value (add = 1), on the stream.
 hipError_t launch_counter(hipStream_t s, int idx, uint64_t *ctr, uint64_t value, int add);
 

@@ -192,6 +192,15 @@ class Engine:
         B.check(B.lib().uasd_measure(self._h, B.ptr(ms), B.ptr(en)), "uasd_measure")
         return ms, en
 
+    def energy_terms(self):
+        """calc_energy's terms per ensemble, energy per site in mRy: (M, 4)
+        with columns exchange, anisotropy, DM, Zeeman; (nseg, M, 4) when the
+        lattice holds several segments."""
+        nseg = getattr(self, "nseg", 1)
+        out = np.zeros((nseg, self.M, 4))
+        B.check(B.lib().uasd_energy_terms(self._h, B.ptr(out)), "uasd_energy_terms")
+        return out[0] if nseg == 1 else out
+
     def coloring(self):
         nc = C.c_int(0)
         col = np.zeros(self.N, dtype=np.int32)

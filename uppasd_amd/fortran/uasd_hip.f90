@@ -41,7 +41,7 @@ module uasd_hip
       uasd_set_external_field, uasd_set_llg, uasd_set_temprescale, uasd_set_noise, &
       uasd_set_moments, uasd_get_moments, uasd_effective_field_host, uasd_evolve_first_host, &
       uasd_evolve_second_host, uasd_sd_steps, uasd_effective_field, uasd_mc_sweeps, &
-      uasd_measure, uasd_autotune_llg, uasd_autotune_mc, uasd_set_site_order, &
+      uasd_measure, uasd_energy_terms, uasd_autotune_llg, uasd_autotune_mc, uasd_set_site_order, &
       uasd_set_lattice_dims
    public :: uasd_check, uasd_error_message, uasd_gpu_init, uasd_gpu_finalize, &
       uasd_gpu_set_site_order, uasd_gpu_set_lattice_dims, uasd_gpu_set_hamiltonian, uasd_effective_field_full, uasd_evolve_first, &
@@ -192,6 +192,13 @@ module uasd_hip
       function uasd_measure(ctx, msum, energy) bind(C, name="uasd_measure") result(st)
          import :: c_ptr, c_int
          type(c_ptr), value :: ctx, msum, energy
+         integer(c_int) :: st
+      end function
+      !> calc_energy's terms (energy.f90:60-366): ene(4,Mensemble) per site in
+      !> mRy, rows exchange, anisotropy, DM, Zeeman.
+      function uasd_energy_terms(ctx, ene) bind(C, name="uasd_energy_terms") result(st)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx, ene
          integer(c_int) :: st
       end function
       function uasd_autotune_llg(ctx, sdealgh, steps, best_ms) bind(C, name="uasd_autotune_llg") &

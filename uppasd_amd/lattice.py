@@ -57,8 +57,16 @@ class Shells:
 
     @classmethod
     def from_lines(cls, lat: Lattice, lines, hdim: int = 1, nchmax: int = 1,
-                   alloy: bool = False) -> "Shells":
-        """lines: (isite, jsite, ichem, jchem, rx, ry, rz, value[, value2, value3])."""
+                   alloy: bool = False, maptype: int = 1) -> "Shells":
+        """lines: (isite, jsite, ichem, jchem, rx, ry, rz, value[, value2, value3]).
+
+        maptype 1: (rx, ry, rz) is the Cartesian bond vector.  maptype 2
+        ("bgfm"): it counts cell vectors, and the bond vector is
+        bas_j - bas_i + C1*rx + C2*ry + C3*rz, evaluated component by
+        component in that order (read_exchange_getNeighVec,
+        inputhandler_ext.f90:1039-1043)."""
+        if maptype not in (1, 2):
+            raise ValueError("maptype must be 1 or 2")
         tol = 1.0e-5
         NT = lat.NT
         red = [[] for _ in range(NT)]
@@ -68,6 +76,10 @@ class Shells:
             isite, jsite = int(ln[0]), int(ln[1])
             ic, jcm = (int(ln[2]), int(ln[3])) if alloy else (1, 1)
             r = np.array([float(x) for x in ln[4:7]])
+            if maptype == 2:
+                c, bi, bj = lat.cell, lat.bas[isite - 1], lat.bas[jsite - 1]
+                r = np.array([bj[a] - bi[a] + c[0, a] * r[0] + c[1, a] * r[1] + c[2, a] * r[2]
+                              for a in range(3)])
             v = tuple(float(x) for x in ln[7:7 + hdim])
             it = int(lat.atype[isite - 1]) - 1
             unique = True

@@ -3,6 +3,8 @@
   averages.<simid>.out    prn_avrg (prn_averages.f90:569-656)
   cumulants.<simid>.out   calc_and_print_cumulant (prn_averages.f90:919-1095)
   cumulants.<simid>.json
+  totenergy.<simid>.out   calc_energy (energy.f90:353-455)
+  stdenergy.<simid>.out
 
 Columns, Fortran edit descriptors and the running means follow the reference,
 so the reference's own comparisons (fixed column positions) read these files
@@ -87,12 +89,15 @@ class Cumulants:
         self.plotenergy, self.trs, self.trg = plotenergy, temprescale, temprescalegrad
         self.cumu_buff = cumu_buff
         self.cumuw = self.cumutotw = 0.0
-        self.m = self.m2 = self.m4 = self.e = self.e2 = 0.0
+        self.m = self.m2 = self.m4 = self.e = self.e2 = self.et = 0.0
         self.navrgcum = 0
         self.last = None
 
-    def sample(self, msum: np.ndarray, energy_per_atom=None, write: bool = True):
-        """msum (M, 3); energy_per_atom (M,) in mRy (or None)."""
+    def sample(self, msum: np.ndarray, energy_per_atom=None, write: bool = True,
+               exchange_per_atom=None):
+        """msum (M, 3); energy_per_atom (M,) total energy per atom in mRy (or
+        None); exchange_per_atom (M,) the exchange term alone, the running
+        mean printed as <E_{exc}> (the total when None)."""
         ms = np.asarray(msum, dtype=np.float64)
         M, N, T = ms.shape[0], self.natom, self.temp
         for k in range(M):
@@ -110,18 +115,20 @@ class Cumulants:
                 chi = (mt2 - mt ** 2) * MUB ** 2 * N / (KBOLT ** 2) / T
             else:
                 chi = (mt2 - mt ** 2) * N * MUB ** 2 / KBOLT
-            cv = en = 0.0
+            cv = en = ent = 0.0
             if self.plotenergy and energy_per_atom is not None:
                 ek = float(energy_per_atom[k])
+                xk = ek if exchange_per_atom is None else float(exchange_per_atom[k])
                 en = (self.e * tot + ek * w) / (tot + w)
                 en2 = (self.e2 * tot + ek ** 2 * w) / (tot + w)
+                ent = (self.et * tot + xk * w) / (tot + w)
                 if T > 0.0:
                     cv = (en2 - en ** 2) * MRY ** 2 * N / (KBOLT ** 2) / T ** 2 * \
                         (self.trg * T + self.trs) / self.trs ** 2
-                self.e, self.e2 = en, en2
+                self.e, self.e2, self.et = en, en2, ent
             self.navrgcum += 1
             self.cumutotw += w
-            self.last = (mt, mt2, mt4, u4, chi, cv, en)
+            self.last = (mt, mt2, mt4, u4, chi, cv, en, ent)
         if write:
             self._write(M)
         return self.last
@@ -130,13 +137,13 @@ class Cumulants:
         it = self.navrgcum // M
         if (it - 1) % self.cumu_buff != 0:
             return
-        mt, mt2, mt4, u4, chi, cv, en = self.last
+        mt, mt2, mt4, u4, chi, cv, en, ent = self.last
         with open(self.path, "a") as fh:
             if it == 1:
                 fh.write(fmt_a("#Iter", 8) + "".join(fmt_a(h, 16) for h in (
                     "<M>", "<M^2>", "<M^4>", "U_{Binder}", "\\chi", "C_v(tot)", "<E>",
                     "<E_{exc}>", "<E_{lsf}>")) + "\n")
-            fh.write(fmt_i(it) + "".join(fmt_es(v) for v in (mt, mt2, mt4, u4, chi, cv, en, en,
+            fh.write(fmt_i(it) + "".join(fmt_es(v) for v in (mt, mt2, mt4, u4, chi, cv, en, ent,
                                                              0.0)) + "\n")
         f = lambda x: f"{x:16.8f}"
         with open(self.json_path, "w") as fh:
@@ -153,21 +160,98 @@ class Cumulants:
             fh.write("}\n")
 
 
+class TotEnergy:
+    """calc_energy's files (energy.f90:353-455) from the term-resolved
+    energies per ensemble (Engine.energy_terms: columns exchange, anisotropy,
+    DM, Zeeman, per atom in mRy).
+
+    totenergy holds the ensemble mean of each term and of their total; with
+    Mensemble > 1 stdenergy holds what calculate_mean_and_deviation
+    (energy.f90:577-602) calls the deviation: the sample variance of the
+    field-unit energies times mub/mry, i.e. the variance in mRy^2 divided by
+    mub/mry.  Terms outside the accelerated Hamiltonian are printed as 0."""
+
+    COLS = ("Tot", "Exc", "Ani", "DM", "PD", "BiqDM", "BQ", "Dip", "Zeeman", "LSF", "Chir",
+            "Ring", "SA")
+
+    def __init__(self, simid: str, outdir: str = ".", real_time: bool = False,
+                 delta_t: float = 0.0):
+        sid = simid.strip()
+        self.path = os.path.join(outdir, f"totenergy.{sid}.out")
+        self.std_path = os.path.join(outdir, f"stdenergy.{sid}.out")
+        self.real_time, self.delta_t = real_time, float(delta_t)
+
+    @staticmethod
+    def per_ensemble(terms: np.ndarray) -> np.ndarray:
+        """(M, 4) terms -> (M, 13) in column order; Tot summed in the order of
+        ene%energy (energy.f90:368-371): xc+dm+pd+bq+ring+ext+ani+dip+bqdm+lsf+chir+sa."""
+        t = np.asarray(terms, dtype=np.float64).reshape(-1, 4)
+        M = t.shape[0]
+        out = np.zeros((M, 13))
+        for k in range(M):
+            xc, ani, dm, ext = (float(v) for v in t[k])
+            tot = xc + dm + 0.0 + 0.0 + 0.0 + ext + ani + 0.0 + 0.0 + 0.0 + 0.0 + 0.0
+            out[k] = (tot, xc, ani, dm, 0.0, 0.0, 0.0, 0.0, ext, 0.0, 0.0, 0.0, 0.0)
+        return out
+
+    @staticmethod
+    def mean_dev(v: np.ndarray):
+        """calculate_mean_and_deviation on values already in mRy."""
+        v = np.asarray(v, dtype=np.float64)
+        n = v.shape[0]
+        if n > 1:
+            mean = float(np.sum(v)) / n
+            dev = float(np.sum((v - mean) ** 2)) / (n - 1.0) / (MUB / MRY)
+        else:
+            mean, dev = float(v[0]), 0.0
+        return mean, dev
+
+    def row(self, terms: np.ndarray):
+        e = self.per_ensemble(terms)
+        md = [self.mean_dev(e[:, j]) for j in range(13)]
+        return [m for m, _ in md], [d for _, d in md]
+
+    def _line(self, fh, it, vals):
+        if self.real_time:
+            fh.write(fmt_es(it * self.delta_t, 12, 4) + "".join(fmt_es(v) for v in vals) + "\n")
+        else:
+            fh.write(fmt_i(it) + "".join(fmt_es(v) for v in vals) + "\n")
+
+    def write(self, it: int, terms: np.ndarray):
+        """One measurement at iteration it (= mstep-1); header at it == 0."""
+        means, devs = self.row(terms)
+        first = "#Time" if self.real_time else "#Iter"
+        with open(self.path, "a") as fh:
+            if it == 0:
+                fh.write(fmt_a(first, 8) + "".join(fmt_a(h, 20) for h in self.COLS) + "\n")
+            self._line(fh, it, means)
+        if np.asarray(terms).reshape(-1, 4).shape[0] > 1:
+            with open(self.std_path, "a") as fh:
+                if it == 0:
+                    fh.write(fmt_a(first, 8) + "".join(fmt_a(h, 20) for h in self.COLS) + "\n")
+                self._line(fh, it, devs)
+        return means, devs
+
+
 class MeasurementPhase:
     """Resident measurement phase with the reference's cadence.
 
     step(n) advances the engine n steps (LLG, SDEalgh `alg`) or sweeps (MC,
     mode 'M'/'H'); measurements are taken before the step of iteration mstep
-    exactly where sd_mphase / mc_mphase take them."""
+    exactly where sd_mphase / mc_mphase take them.  With plotenergy the
+    term-resolved energies (Engine.energy_terms) are written to totenergy and
+    feed the cumulants' <E> and <E_{exc}>."""
 
     def __init__(self, eng, simid: str, temp: float, avrg_step: int = 100,
                  cumu_step: int = 50, plotenergy: bool = True, outdir: str = ".",
-                 cumu_buff: int = 1):
+                 cumu_buff: int = 1, final_measure: bool = True):
         self.eng, self.temp = eng, float(temp)
         self.avrg_step, self.cumu_step, self.plotenergy = avrg_step, cumu_step, plotenergy
         self.avg = Averages(simid, outdir)
         self.cum = Cumulants(simid, temp, eng.N, outdir, plotenergy, cumu_buff=cumu_buff)
-        self.energy = None
+        self.tot = TotEnergy(simid, outdir) if plotenergy else None
+        self.energy = self.exchange = None
+        self.final_measure = final_measure
 
     def run(self, nstep: int, advance):
         """advance(n) moves the state n steps; iterations 1..nstep."""
@@ -178,10 +262,12 @@ class MeasurementPhase:
                 self.avg.write(mstep - 1, ms, self.eng.N)
             if mstep % self.cumu_step == 0:
                 ms, _ = self.eng.measure()
-                self.cum.sample(ms, self.energy)
+                self.cum.sample(ms, self.energy, exchange_per_atom=self.exchange)
             if self.plotenergy and (mstep - 1) % self.avrg_step == 0:
-                _, en = self.eng.measure(with_energy=True)
-                self.energy = en / self.eng.N
+                terms = self.eng.energy_terms()
+                self.tot.write(mstep - 1, terms)
+                e = TotEnergy.per_ensemble(terms)
+                self.energy, self.exchange = e[:, 0], e[:, 1]
             # next iteration that measures anything
             nxt = nstep + 1
             for m in range(mstep + 1, min(nstep, mstep + max(self.avrg_step, self.cumu_step)) + 1):
@@ -190,3 +276,14 @@ class MeasurementPhase:
                     break
             advance(nxt - mstep)
             mstep = nxt
+        # sd_mphase measures once more after the loop (sd_driver.f90:839),
+        # at mstep = nstep + 1, without an energy evaluation; mc_mphase does
+        # not (mc_driver.f90:430-440): final_measure=False
+        if not self.final_measure:
+            return
+        if (mstep - 1) % self.avrg_step == 0:
+            ms, _ = self.eng.measure()
+            self.avg.write(mstep - 1, ms, self.eng.N)
+        if mstep % self.cumu_step == 0:
+            ms, _ = self.eng.measure()
+            self.cum.sample(ms, self.energy, exchange_per_atom=self.exchange)

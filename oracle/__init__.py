@@ -145,6 +145,9 @@ def _declare(L):
                                   C.c_double, dp]
     L.or_midpoint_first.argtypes = [C.c_int, C.c_int, dp, C.c_double, dp, dp, dp, dp, dp, dp,
                                     C.c_double, dp, vp]
+    L.or_heun_first.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_double, dp, dp,
+                                dp, dp, dp, dp, dp]
+    L.or_heun_second.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, dp, dp, dp, dp, dp]
     L.or_midpoint_second.argtypes = [C.c_int, C.c_int, dp, C.c_double, dp, dp, dp, dp,
                                      C.c_double, dp]
     L.or_moment_update.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, vp]

@@ -3,6 +3,8 @@
  *   source/Evolution/depondt.f90      depondt_evolve_first 50-193, _second 203-333
  *   source/Evolution/midpoint.f90     smodeulermpt 34-181, modeulermpf 188-323
  *   source/RNG/randomnumbers.f90      rannum scaling 625-771 (llg = 1 branch)
+ *   source/Evolution/heun_proper.f90  evolve4p 28-161, heun4p 261-369,
+ *                                     evolve4f 168-254, heun4f 376-530
  *   source/Evolution/updatemoments.f90 copym 48-95 (mompar = 0, initexc /= 'I')
  * Spin-transfer torques are off (stt='N', do_she='N', do_sot='N'), so the
  * torque accumulators are the literal 0.0 they start from.
@@ -204,6 +206,126 @@ void or_moment_update(int N, int M, const double *mmom, double *emom,
     }
 }
 
+/* Heun predictor-corrector (SDEalgh 4).  evolve4p (heun_proper.f90:28-161):
+ * ranv (N(0,1) on entry) is scaled in place by sigma = sqrt(dt*2*D),
+ * D = lam/(1+lam^2) kB/muB / mmom * T * temprescale, dt = delta_t*gama
+ * (compensate_drift 0); heun4p (261-369) starts from emom2 (the current
+ * state), keeps b2eff = beff and writes the Euler predictor into emom and
+ * emomM = emom*mmom.  The noise enters through dW = ranv*Landeg. */
+void or_heun_first(int N, int M, const double *landeg, const double *lambda, const double *temp,
+                   double temprescale, double delta_t, const double *beff, double *ranv,
+                   const double *mmom, double *emom, const double *emom2, double *emomM,
+                   double *b2eff)
+{
+  const double dt = delta_t * OR_GAMA;
+  for (int k = 0; k < M; ++k)
+    for (int i = 0; i < N; ++i) {
+      double Dk = lambda[i] / (1 + lambda[i] * lambda[i]) * OR_K_BOLT / OR_MUB;
+      double D = Dk * (1.0 / S2(mmom, i, k, N)) * temp[i] * temprescale;
+      double sigma = sqrt(dt * 2 * D);
+      for (int c = 0; c < 3; ++c) V3(ranv, c, i, k, N) = V3(ranv, c, i, k, N) * sigma;
+    }
+  for (int i = 0; i < N; ++i)
+    for (int k = 0; k < M; ++k) {
+      const double lam = lambda[i], lg = landeg[i];
+      for (int c = 0; c < 3; ++c) V3(b2eff, c, i, k, N) = V3(beff, c, i, k, N);
+      double e1x = V3(emom2, 0, i, k, N), e1y = V3(emom2, 1, i, k, N), e1z = V3(emom2, 2, i, k, N);
+      double lldamp = 1.0 / (1.0 + lam * lam);
+      double dtg = dt * lg * lldamp;
+      double dwx = V3(ranv, 0, i, k, N) * lg, dwy = V3(ranv, 1, i, k, N) * lg,
+             dwz = V3(ranv, 2, i, k, N) * lg;
+      double hesx = V3(beff, 0, i, k, N), hesy = V3(beff, 1, i, k, N), hesz = V3(beff, 2, i, k, N);
+      double hlpx = lam * hesx, hlpy = lam * hesy, hlpz = lam * hesz;
+      double prot = e1x * hlpx + e1y * hlpy + e1z * hlpz;
+      /* the spin-torque field is 0: (hes + 0.0) */
+      double a1x = hlpx + e1z * (hesy + 0.0) - e1y * (hesz + 0.0) - e1x * prot;
+      double a1y = hlpy - e1z * (hesx + 0.0) + e1x * (hesz + 0.0) - e1y * prot;
+      double a1z = hlpz + e1y * (hesx + 0.0) - e1x * (hesy + 0.0) - e1z * prot;
+      double b1xx = lam - e1x * e1x * lam, b1xy = e1z - e1x * e1y * lam,
+             b1xz = -e1y - e1x * e1z * lam;
+      double b1yx = -e1z - e1y * e1x * lam, b1yy = lam - e1y * e1y * lam,
+             b1yz = e1x - e1y * e1z * lam;
+      double b1zx = e1y - e1z * e1x * lam, b1zy = -e1x - e1z * e1y * lam,
+             b1zz = lam - e1z * e1z * lam;
+      double etx = e1x + a1x * dtg + b1xx * dwx + b1xy * dwy + b1xz * dwz;
+      double ety = e1y + a1y * dtg + b1yx * dwx + b1yy * dwy + b1yz * dwz;
+      double etz = e1z + a1z * dtg + b1zx * dwx + b1zy * dwy + b1zz * dwz;
+      double m = S2(mmom, i, k, N);
+      V3(emom, 0, i, k, N) = etx;
+      V3(emom, 1, i, k, N) = ety;
+      V3(emom, 2, i, k, N) = etz;
+      V3(emomM, 0, i, k, N) = etx * m;
+      V3(emomM, 1, i, k, N) = ety * m;
+      V3(emomM, 2, i, k, N) = etz * m;
+    }
+}
+
+/* heun4f (heun_proper.f90:376-530) and the renormalisation of evolve4f
+ * (236-245): emom = predictor, emom2 = state at step start, beff the field
+ * of the predictor, b2eff that of the start; the drift and diffusion of
+ * both points are averaged.  The reference's b1zz reads
+ * lam - e1z*e1z - lam (heun_proper.f90:480), kept as written: it multiplies
+ * the noise only. */
+void or_heun_second(int N, int M, const double *landeg, const double *lambda, double delta_t,
+                    const double *beff, const double *b2eff, const double *ranv,
+                    const double *emom, double *emom2)
+{
+  const double dt = delta_t * OR_GAMA;
+  for (int i = 0; i < N; ++i)
+    for (int k = 0; k < M; ++k) {
+      const double lam = lambda[i], lg = landeg[i];
+      double lldamp = 1.0 / (1.0 + lam * lam);
+      double dtg = dt * lg * lldamp;
+      double dwx = V3(ranv, 0, i, k, N) * lg, dwy = V3(ranv, 1, i, k, N) * lg,
+             dwz = V3(ranv, 2, i, k, N) * lg;
+      double e1x = V3(emom, 0, i, k, N), e1y = V3(emom, 1, i, k, N), e1z = V3(emom, 2, i, k, N);
+      double e2x = V3(emom2, 0, i, k, N), e2y = V3(emom2, 1, i, k, N), e2z = V3(emom2, 2, i, k, N);
+      double h1esx = V3(beff, 0, i, k, N), h1esy = V3(beff, 1, i, k, N), h1esz = V3(beff, 2, i, k, N);
+      double h1px = lam * h1esx, h1py = lam * h1esy, h1pz = lam * h1esz;
+      double h2esx = V3(b2eff, 0, i, k, N), h2esy = V3(b2eff, 1, i, k, N),
+             h2esz = V3(b2eff, 2, i, k, N);
+      double h2px = lam * h2esx, h2py = lam * h2esy, h2pz = lam * h2esz;
+      double prot = e1x * h1px + e1y * h1py + e1z * h1pz;
+      double a1x = h1px + e1z * (h1esy + 0.0) - e1y * (h1esz + 0.0) - e1x * prot;
+      double a1y = h1py - e1z * (h1esx + 0.0) + e1x * (h1esz + 0.0) - e1y * prot;
+      double a1z = h1pz + e1y * (h1esx + 0.0) - e1x * (h1esy + 0.0) - e1z * prot;
+      double b1xx = lam - e1x * e1x * lam, b1xy = e1z - e1x * e1y * lam,
+             b1xz = -e1y - e1x * e1z * lam;
+      double b1yx = -e1z - e1y * e1x * lam, b1yy = lam - e1y * e1y * lam,
+             b1yz = e1x - e1y * e1z * lam;
+      double b1zx = e1y - e1z * e1x * lam, b1zy = -e1x - e1z * e1y * lam,
+             b1zz = lam - e1z * e1z - lam;
+      prot = e2x * h2px + e2y * h2py + e2z * h2pz;
+      double a2x = h2px + e2z * h2esy - e2y * h2esz - e2x * prot;
+      double a2y = h2py - e2z * h2esx + e2x * h2esz - e2y * prot;
+      double a2z = h2pz + e2y * h2esx - e2x * h2esy - e2z * prot;
+      double b2xx = lam - e2x * e2x * lam, b2xy = e2z - e2x * e2y * lam,
+             b2xz = -e2y - e2x * e2z * lam;
+      double b2yx = -e2z - e2y * e2x * lam, b2yy = lam - e2y * e2y * lam,
+             b2yz = e2x - e2y * e2z * lam;
+      double b2zx = e2y - e2z * e2x * lam, b2zy = -e2x - e2z * e2y * lam,
+             b2zz = lam - e2z * e2z * lam;
+      double ax = a1x + a2x, ay = a1y + a2y, az = a1z + a2z;
+      double bxx = b1xx + b2xx, bxy = b1xy + b2xy, bxz = b1xz + b2xz;
+      double byx = b1yx + b2yx, byy = b1yy + b2yy, byz = b1yz + b2yz;
+      double bzx = b1zx + b2zx, bzy = b1zy + b2zy, bzz = b1zz + b2zz;
+      double etx = ax * dtg + bxx * dwx + bxy * dwy + bxz * dwz;
+      double ety = ay * dtg + byx * dwx + byy * dwy + byz * dwz;
+      double etz = az * dtg + bzx * dwx + bzy * dwy + bzz * dwz;
+      V3(emom2, 0, i, k, N) = e2x + 0.5 * etx;
+      V3(emom2, 1, i, k, N) = e2y + 0.5 * ety;
+      V3(emom2, 2, i, k, N) = e2z + 0.5 * etz;
+    }
+  for (int i = 0; i < N; ++i)
+    for (int k = 0; k < M; ++k) {
+      double x = V3(emom2, 0, i, k, N), y = V3(emom2, 1, i, k, N), z = V3(emom2, 2, i, k, N);
+      double etot = 1.0 / sqrt(x * x + y * y + z * z);
+      V3(emom2, 0, i, k, N) = x * etot;
+      V3(emom2, 1, i, k, N) = y * etot;
+      V3(emom2, 2, i, k, N) = z * etot;
+    }
+}
+
 /* The sd_mphase step body (sd_driver.f90:668-765) for nsteps, with the
  * thermal field drawn from the Philox counter stream the HIP kernels use
  * (noise = 1) or absent (noise = 0).  Used as the CPU baseline and for long
@@ -232,6 +354,12 @@ void or_sd_run(const or_ham *h, int N, int M, int alg, long nsteps, uint64_t ste
       or_midpoint_first(N, M, landeg, 1.0, lam, beff, emom, e2, emomM, mmom, dt, rv, NULL);
       or_effective_field(h, N, M, emomM, ext, beff, NULL, NULL, NULL);
       or_midpoint_second(N, M, landeg, 1.0, lam, beff, emom, e2, dt, rv);
+    } else if (alg == 4) {
+      /* at step start emom2 equals emom (moment_update copies it) */
+      for (size_t x = 0; x < n3; ++x) e2[x] = emom[x];
+      or_heun_first(N, M, landeg, lam, temp, 1.0, dt, beff, rv, mmom, emom, e2, emomM, b2);
+      or_effective_field(h, N, M, emomM, ext, beff, NULL, NULL, NULL);
+      or_heun_second(N, M, landeg, lam, dt, beff, b2, rv, emom, e2);
     } else {
       or_depondt_first(N, M, lam, temp, 1.0, dt, beff, rv, mmom, emom, e2, emomM, b2, NULL);
       or_effective_field(h, N, M, emomM, ext, beff, NULL, NULL, NULL);

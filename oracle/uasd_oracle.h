@@ -150,6 +150,14 @@ void or_midpoint_first(int N, int M, const double *landeg, double bn,
 void or_midpoint_second(int N, int M, const double *landeg, double bn,
                         const double *lambda, const double *beff, const double *emom,
                         double *emom2, double dt, const double *ranv);
+/* Heun (SDEalgh 4): heun_proper.f90 evolve4p/heun4p and heun4f/evolve4f */
+void or_heun_first(int N, int M, const double *landeg, const double *lambda, const double *temp,
+                   double temprescale, double delta_t, const double *beff, double *ranv,
+                   const double *mmom, double *emom, const double *emom2, double *emomM,
+                   double *b2eff);
+void or_heun_second(int N, int M, const double *landeg, const double *lambda, double delta_t,
+                    const double *beff, const double *b2eff, const double *ranv,
+                    const double *emom, double *emom2);
 void or_moment_update(int N, int M, const double *mmom, double *emom,
                       const double *emom2, double *emomM, double *mmomi);
 /* nsteps of the sd_mphase body; noise 1 = Philox stream of the HIP kernels */

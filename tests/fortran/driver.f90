@@ -80,6 +80,7 @@ program driver
          ext(:, i, k) = extfield
          mmomi(i, k) = 1.0_dp / mmom(i, k)
          emomM(:, i, k) = emom(:, i, k) * mmom(i, k)
+         emom2(:, i, k) = emom(:, i, k)   ! as after moment_update; Heun starts from emom2
       end do
    end do
    maxc = 0

@@ -146,6 +146,13 @@ class OracleSD:
                                 self.noise, None)
             beff, _ = self.ham.field(self.eM, ext)
             L.or_midpoint_second(N, M, lg, 1.0, lam, beff, self.e, self.e2, dt, self.noise)
+        elif alg == 4:
+            # at step start emom2 equals emom (moment_update copied it)
+            self.e2[...] = self.e
+            L.or_heun_first(N, M, lg, lam, T, 1.0, dt, beff, self.noise, self.mmom, self.e,
+                            self.e2, self.eM, self.b2)
+            beff, _ = self.ham.field(self.eM, ext)
+            L.or_heun_second(N, M, lg, lam, dt, beff, self.b2, self.noise, self.e, self.e2)
         else:
             L.or_depondt_first(N, M, lam, T, 1.0, dt, beff, self.noise, self.mmom, self.e, self.e2,
                                self.eM, self.b2, None)

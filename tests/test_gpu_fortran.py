@@ -56,7 +56,7 @@ def _run_driver(tmp_path, kind, nlist, nlistsize, ncoup, mmom, emom, alg, nsteps
     return (out[:n3].reshape(M, N, 3), out[n3:2 * n3].reshape(M, N, 3), out[2 * n3:])
 
 
-@pytest.mark.parametrize("solver,name", [(1, "Midpoint"), (5, "Depondt")])
+@pytest.mark.parametrize("solver,name", [(1, "Midpoint"), (4, "Heun"), (5, "Depondt")])
 def test_fortran_solvers_golden(O, tmp_path, solver, name):
     c = RC.read_case("Solvers", "inpsd.dat.base")
     lat = U.Lattice(c["cell"], c["bas"], c["atype"], c["n"], c["bc"])
@@ -74,7 +74,7 @@ def test_fortran_solvers_golden(O, tmp_path, solver, name):
 
 
 @pytest.mark.parametrize("ordered", [False, True])
-@pytest.mark.parametrize("alg", [1, 5])
+@pytest.mark.parametrize("alg", [1, 4, 5])
 def test_fortran_injected_noise_matches_oracle(O, tmp_path, alg, ordered):
     rng = np.random.default_rng(21)
     sysd = CF.bcc_shells(3, CF.bcc5_couplings())

@@ -22,7 +22,7 @@ def _llg_engine(sysd, emom, precision="fp64"):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-@pytest.mark.parametrize("alg", [1, 5])
+@pytest.mark.parametrize("alg", [1, 4, 5])
 def test_llg_graph_replays_equal_eager(precision, alg):
     rng = np.random.default_rng(40)
     sysd = CF.bcc_shells(5, CF.bcc5_couplings())

@@ -99,7 +99,7 @@ def _oracle_step(O, sim, alg, dt, lam, T, normals, ext):
     sim.step(alg, dt, lam, T, normals=normals, ext=ext)
 
 
-@pytest.mark.parametrize("alg", [1, 5])
+@pytest.mark.parametrize("alg", [1, 4, 5])
 def test_dropin_step_matches_oracle(O, alg):
     """field -> evolve_first -> field -> evolve_second -> moment_update through
     the host-array entry points, with an injected noise stream at 300 K."""
@@ -120,7 +120,7 @@ def test_dropin_step_matches_oracle(O, alg):
     sim.set_state(emom)
     ext = ext_array(sysd, M)
     e, eM = emom.copy(), emom * mm[..., None]
-    e2, b2 = np.zeros_like(e), np.zeros_like(e)
+    e2, b2 = emom.copy(), np.zeros_like(e)   # after moment_update emom2 equals emom
     for it in range(5):
         normals = rng.normal(size=(M, N, 3))
         eng.set_noise(U.UASD_NOISE_INJECTED, 0, normals)
@@ -133,7 +133,7 @@ def test_dropin_step_matches_oracle(O, alg):
         assert np.max(np.abs(e - sim.e)) <= 1e-13, it
 
 
-@pytest.mark.parametrize("alg", [1, 5])
+@pytest.mark.parametrize("alg", [1, 4, 5])
 def test_resident_T0_trajectory(O, alg):
     """Deterministic T = 0 trajectory, fused kernels vs oracle, 500 steps."""
     rng = np.random.default_rng(7)
@@ -154,7 +154,7 @@ def test_resident_T0_trajectory(O, alg):
     assert np.max(np.abs(eM - sim.eM)) <= 1e-12 * 2.2
 
 
-@pytest.mark.parametrize("alg", [1, 5])
+@pytest.mark.parametrize("alg", [1, 4, 5])
 def test_resident_injected_noise_trajectory(O, alg):
     """Stochastic LLG at 300 K on an identical injected noise stream."""
     rng = np.random.default_rng(9)
@@ -175,7 +175,8 @@ def test_resident_injected_noise_trajectory(O, alg):
     assert np.max(np.abs(e - sim.e)) <= 1e-12
 
 
-def test_philox_noise_stream_matches_oracle(O):
+@pytest.mark.parametrize("alg", [4, 5])
+def test_philox_noise_stream_matches_oracle(O, alg):
     """In-kernel Philox deviates reproduce the oracle's restatement of the
     same counter stream bit for bit (single-precision Box-Muller made of
     IEEE operations only), so the trajectories agree to FP64 rounding."""
@@ -196,13 +197,13 @@ def test_philox_noise_stream_matches_oracle(O):
         for k in range(M):
             for i in range(N):
                 normals[k, i] = O.thermal3(seed, 0x11, step, i, k)
-        sim.step(5, 1e-16, 0.1, 300.0, normals=normals)
-    eng.sd_steps(5, nsteps)
+        sim.step(alg, 1e-16, 0.1, 300.0, normals=normals)
+    eng.sd_steps(alg, nsteps)
     e, _ = eng.get_moments()
     assert np.max(np.abs(e - sim.e)) <= 1e-12
 
 
-@pytest.mark.parametrize("solver,name", [(1, "Midpoint"), (5, "Depondt")])
+@pytest.mark.parametrize("solver,name", [(1, "Midpoint"), (4, "Heun"), (5, "Depondt")])
 def test_solvers_golden_on_gpu(O, solver, name):
     """The reference's Solvers chain (regulartests.yaml) on the device: the
     random start comes from the oracle's MT19937 stream, T = 0."""

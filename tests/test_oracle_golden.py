@@ -33,7 +33,7 @@ def _expected(case, typ):
     return [g for g in RC.golden(case) if g["type"] == typ]
 
 
-@pytest.mark.parametrize("solver,name", [(1, "Midpoint"), (5, "Depondt")])
+@pytest.mark.parametrize("solver,name", [(1, "Midpoint"), (4, "Heun"), (5, "Depondt")])
 def test_solvers_chain_golden(O, solver, name):
     c = RC.read_case("Solvers", "inpsd.dat.base")
     lat = _lattice(O, c)

@@ -508,6 +508,7 @@ uasd::LLGDev<R> llg_view(uasd_ctx *c) {
     p.sig_uniform = 1;
     p.sigk0 = uasd::midpoint_sigma<R>(p.lc0.Dk, m, c->temp0, c->temprescale);
     p.sigp0 = uasd::depondt_sigma<R>(p.lc0.Dp, m, c->temp0, c->temprescale);
+    p.sigh0 = uasd::heun_sigma<R>(p.lc0.Dh, m, c->temp0, c->temprescale, c->dt);
   }
   return p;
 }
@@ -528,6 +529,10 @@ int ready_for_llg(uasd_ctx *c) {
 
 // Quad arrays in c->spins: 0 e (emom), 1 p (emom2), 2 b2 (b2eff).
 // step_off >= 0: inside a graph capture, the step is *ctr[0] + step_off.
+bool sde_supported(int alg) {
+  return alg == UASD_SDE_MIDPOINT || alg == UASD_SDE_HEUN || alg == UASD_SDE_DEPONDT;
+}
+
 template <typename R>
 int run_llg_half(uasd_ctx *c, int alg, int half, bool fused, bool write_e,
                  int64_t step_off = -1) {
@@ -544,7 +549,8 @@ int run_llg_half(uasd_ctx *c, int alg, int half, bool fused, bool write_e,
                 *b2 = quads<R>(c, c->spins, 2);
   const uasd::Quad<R> *beff = fused ? nullptr : quads<R>(c, c->field);
   uasd::Quad<R> *out = write_e ? e : p;
-  if (fused && c->tune[0].tile > 0) {
+  // the tile kernels carry the midpoint and Depondt solvers; Heun runs on the lists
+  if (fused && c->tune[0].tile > 0 && alg != UASD_SDE_HEUN) {
     build_stencil(c);
     if (tile_fits(c, c->tune[0].tile)) {
       uasd::TileDev t;
@@ -1277,13 +1283,14 @@ int uasd_effective_field_host(uasd_ctx *c, const double *emomM, double *beff, do
 int uasd_evolve_first_host(uasd_ctx *c, int alg, const double *beff, double *emom, double *emom2,
                            double *emomM, double *b2eff) {
   if (int r = check_ctx(c)) return r;
-  if (alg != UASD_SDE_MIDPOINT && alg != UASD_SDE_DEPONDT)
-    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint) or 5 (Depondt)");
-  if (!beff || !emom || !emom2 || !emomM || (alg == 5 && !b2eff))
+  if (!sde_supported(alg))
+    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint), 4 (Heun) or 5 (Depondt)");
+  if (!beff || !emom || !emom2 || !emomM || (alg != UASD_SDE_MIDPOINT && !b2eff))
     return fail(UASD_ERR_ARG, "null array");
   if (int r = ready_for_llg(c)) return r;
   if (int r = upload_quads(c, c->field, 0, to_quads(c, beff, nullptr, 0.0))) return r;
-  if (int r = upload_vec_keep_w(c, 0, emom)) return r;
+  // heun4p starts from emom2 (heun_proper.f90:309-311), the others from emom
+  if (int r = upload_vec_keep_w(c, 0, alg == UASD_SDE_HEUN ? emom2 : emom)) return r;
   int r = c->prec == UASD_FP64 ? run_llg_half<double>(c, alg, 0, false, false)
                                : run_llg_half<float>(c, alg, 0, false, false);
   if (r) return r;
@@ -1292,6 +1299,14 @@ int uasd_evolve_first_host(uasd_ctx *c, int alg, const double *beff, double *emo
   if (alg == UASD_SDE_DEPONDT) {
     // depondt.f90:182-187: emom2 = m(t), emom = rotated, emomM = emom*mmom, b2eff = bdup
     std::memcpy(emom2, emom, sizeof(double) * 3 * (size_t)c->N * c->M);
+    from_quads(c, pq, emom, false);
+    from_quads(c, pq, emomM, true);
+    std::vector<double> bq;
+    if ((r = download_quads(c, c->spins, 2, bq))) return r;
+    from_quads(c, bq, b2eff, false);
+  } else if (alg == UASD_SDE_HEUN) {
+    // heun_proper.f90:303-365: b2eff = beff, emom = predictor, emomM =
+    // emom*mmom, emom2 unchanged
     from_quads(c, pq, emom, false);
     from_quads(c, pq, emomM, true);
     std::vector<double> bq;
@@ -1308,12 +1323,13 @@ int uasd_evolve_first_host(uasd_ctx *c, int alg, const double *beff, double *emo
 int uasd_evolve_second_host(uasd_ctx *c, int alg, const double *beff, const double *b2eff,
                             double *emom, double *emom2) {
   if (int r = check_ctx(c)) return r;
-  if (alg != UASD_SDE_MIDPOINT && alg != UASD_SDE_DEPONDT)
-    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint) or 5 (Depondt)");
-  if (!beff || !emom || !emom2 || (alg == 5 && !b2eff)) return fail(UASD_ERR_ARG, "null array");
+  if (!sde_supported(alg))
+    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint), 4 (Heun) or 5 (Depondt)");
+  if (!beff || !emom || !emom2 || (alg != UASD_SDE_MIDPOINT && !b2eff))
+    return fail(UASD_ERR_ARG, "null array");
   if (int r = ready_for_llg(c)) return r;
   if (int r = upload_quads(c, c->field, 0, to_quads(c, beff, nullptr, 0.0))) return r;
-  if (alg == UASD_SDE_DEPONDT) {
+  if (alg == UASD_SDE_DEPONDT || alg == UASD_SDE_HEUN) {
     // reference on entry: emom = predicted, emom2 = m(t)
     if (int r = upload_vec_keep_w(c, 1, emom)) return r;
     if (int r = upload_vec_keep_w(c, 0, emom2)) return r;
@@ -1419,8 +1435,8 @@ int uasd_set_graph_steps(uasd_ctx *c, int steps_per_graph) {
 
 int uasd_sd_steps(uasd_ctx *c, int alg, int64_t nsteps) {
   if (int r = check_ctx(c)) return r;
-  if (alg != UASD_SDE_MIDPOINT && alg != UASD_SDE_DEPONDT)
-    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint) or 5 (Depondt)");
+  if (!sde_supported(alg))
+    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint), 4 (Heun) or 5 (Depondt)");
   if (nsteps < 0) return fail(UASD_ERR_ARG, "nsteps < 0");
   if (int r = ready_for_llg(c)) return r;
   int64_t s = 0;
@@ -1702,8 +1718,8 @@ extern "C" {
 
 int uasd_autotune_llg(uasd_ctx *c, int alg, int steps, double *best_ms) {
   if (int r = check_ctx(c)) return r;
-  if (alg != UASD_SDE_MIDPOINT && alg != UASD_SDE_DEPONDT)
-    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint) or 5 (Depondt)");
+  if (!sde_supported(alg))
+    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint), 4 (Heun) or 5 (Depondt)");
   if (steps <= 0) return fail(UASD_ERR_ARG, "steps must be > 0");
   if (int r = ready_for_llg(c)) return r;
   StateGuard g(c);

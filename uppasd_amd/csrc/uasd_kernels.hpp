@@ -485,6 +485,7 @@ template <typename R>
 struct LamConst {
   double Dk;       // midpoint_sigma: lam/(1+lam^2) kB/gamma/muB * gamma/bn
   double Dp;       // depondt_sigma: 2 lam kB / (dt gamma muB)
+  double Dh;       // heun_sigma: lam/(1+lam^2) kB/muB
   R lam, lldamp;   // lam and 1/(1+lam^2)
   R mdt, msqrtdt;  // midpoint_A: dt bn gamma lldamp and its square root
 };
@@ -494,6 +495,7 @@ __host__ __device__ inline LamConst<R> lam_const(double lam, double dt, double b
   LamConst<R> c;
   c.Dk = (lam / (1.0 + lam * lam) * kKBolt / kGama / (kMub)) * (kGama / bn);
   c.Dp = (2.0 * lam * kKBolt) / (dt * kGama * kMub);
+  c.Dh = lam / (1.0 + lam * lam) * kKBolt / kMub;
   c.lam = (R)lam;
   c.lldamp = (R)1.0 / ((R)1.0 + c.lam * c.lam);
   c.mdt = (R)dt * (R)bn * (R)kGama * c.lldamp;
@@ -518,7 +520,7 @@ struct LLGDev {
   // uniform lam, temperature and moment: the noise amplitudes, computed on
   // the host by the same functions (sig_uniform = 1)
   int sig_uniform;
-  R sigk0, sigp0;        // midpoint_sigma, depondt_sigma
+  R sigk0, sigp0, sigh0;  // midpoint_sigma, depondt_sigma, heun_sigma
   __device__ __forceinline__ LamConst<R> lamc(int s) const {
     return lam ? lam_const<R>(lam[s], dt, bn) : lc0;
   }
@@ -558,6 +560,16 @@ template <typename R>
 __host__ __device__ inline R depondt_sigma(double Dp, double mmom, double T,
                                            double temprescale) {
   return (R)sqrt(Dp * temprescale * T / mmom);
+}
+
+// evolve4p (heun_proper.f90:115-126): D = Dk/mmom*T*temprescale,
+// sigma = sqrt(dt*2*D) with dt = delta_t*gama
+template <typename R>
+__host__ __device__ inline R heun_sigma(double Dh, double mmom, double T, double temprescale,
+                                        double delta_t) {
+  const double mmomi = 1.0 / mmom;
+  const double D = Dh * mmomi * T * temprescale;
+  return (R)sqrt(delta_t * kGama * 2.0 * D);
 }
 
 template <typename R>
@@ -614,6 +626,34 @@ __device__ __forceinline__ void midpoint_A(R lam, R dt, R sqrtdt, R lande, R b0,
   A0 = (R)0.5 * dtg * a0 + (R)0.5 * sqrtdtg * s0;
   A1 = (R)0.5 * dtg * a1 + (R)0.5 * sqrtdtg * s1;
   A2 = (R)0.5 * dtg * a2 + (R)0.5 * sqrtdtg * s2;
+}
+
+// Heun drift a(e, h) = lam h + e x h - e (e . lam h) (heun_proper.f90:319-325,
+// 448-455; the spin torques are zero)
+template <typename R>
+__device__ __forceinline__ void heun_drift(R lam, R ex, R ey, R ez, R hx, R hy, R hz, R &ax,
+                                           R &ay, R &az) {
+  const R px = lam * hx, py = lam * hy, pz = lam * hz;
+  const R prot = ex * px + ey * py + ez * pz;
+  ax = px + ez * hy - ey * hz - ex * prot;
+  ay = py - ez * hx + ex * hz - ey * prot;
+  az = pz + ey * hx - ex * hy - ez * prot;
+}
+
+// Heun diffusion B(e) dW with B = lam (I - e e^T) + [e]x^T (heun_proper.f90:
+// 327-335); QUIRK reproduces the corrector's b1zz = lam - ez*ez - lam
+// (heun_proper.f90:480).
+template <typename R, bool QUIRK>
+__device__ __forceinline__ void heun_diffusion(R lam, R ex, R ey, R ez, R (&b)[9]) {
+  b[0] = lam - ex * ex * lam;
+  b[1] = ez - ex * ey * lam;
+  b[2] = -ey - ex * ez * lam;
+  b[3] = -ez - ey * ex * lam;
+  b[4] = lam - ey * ey * lam;
+  b[5] = ex - ey * ez * lam;
+  b[6] = ey - ez * ex * lam;
+  b[7] = -ex - ez * ey * lam;
+  b[8] = QUIRK ? lam - ez * ez - lam : lam - ez * ez * lam;
 }
 
 // Cramer solve of (I + [A]x) e_t = (I - [A]x) e  (midpoint.f90:153-164)
@@ -705,7 +745,7 @@ __device__ __forceinline__ SiteIn<R> llg_fetch(const LLGDev<R> &prm, int hsite, 
   SiteIn<R> in;
   in.eq = e[q];
   if (HALF == 1) in.pq = p[q];
-  if (ALG == 5 && HALF == 1) in.bq = b2[q];
+  if ((ALG == 5 || ALG == 4) && HALF == 1) in.bq = b2[q];
   if (HALF == 1 && prm.nsave) {
     const float4 v = prm.nsave[q];
     in.n0 = (R)v.x;
@@ -750,6 +790,46 @@ __device__ __forceinline__ void llg_apply(const LLGDev<R> &prm, int site, int64_
       d2 = (R)0.5 * d2 + (R)0.5 * bq.z;
       rodrigues(d0, d1, d2, (R)prm.dt, lc.lldamp, eq.x, eq.y, eq.z, o0, o1, o2);
       out[q] = finish<R>(o0, o1, o2, m);
+    }
+  } else if (ALG == 4) {
+    // Heun predictor-corrector (heun_proper.f90): dW = ranv*Landeg,
+    // dtg = dt*Landeg/(1+lam^2) with dt = delta_t*gama
+    const R sig = prm.sig_uniform
+                      ? prm.sigh0
+                      : heun_sigma<R>(lc.Dh, (double)m, T, prm.temprescale, prm.dt);
+    const R lg = (R)prm.landeg(site);
+    const R w0 = n0 * sig * lg, w1 = n1 * sig * lg, w2 = n2 * sig * lg;
+    const R dtg = (R)(prm.dt * kGama) * lg * lc.lldamp;
+    const R lam = lc.lam;
+    if (HALF == 0) {
+      // heun4p: Euler predictor from the step's start, b2eff = beff
+      R ax, ay, az, B[9];
+      heun_drift(lam, eq.x, eq.y, eq.z, b0, b1, b2v, ax, ay, az);
+      heun_diffusion<R, false>(lam, eq.x, eq.y, eq.z, B);
+      const R o0 = eq.x + ax * dtg + B[0] * w0 + B[1] * w1 + B[2] * w2;
+      const R o1 = eq.y + ay * dtg + B[3] * w0 + B[4] * w1 + B[5] * w2;
+      const R o2 = eq.z + az * dtg + B[6] * w0 + B[7] * w1 + B[8] * w2;
+      p[q] = Quad<R>{o0, o1, o2, m};
+      b2[q] = Quad<R>{b0, b1, b2v, (R)0};
+    } else {
+      // heun4f + evolve4f: average of the drift and diffusion at the
+      // predictor (field b) and at the start (field b2eff), renormalised
+      const Quad<R> pq = in.pq, bq = in.bq;
+      R a1x, a1y, a1z, a2x, a2y, a2z, B1[9], B2[9];
+      heun_drift(lam, pq.x, pq.y, pq.z, b0, b1, b2v, a1x, a1y, a1z);
+      heun_diffusion<R, true>(lam, pq.x, pq.y, pq.z, B1);
+      heun_drift(lam, eq.x, eq.y, eq.z, bq.x, bq.y, bq.z, a2x, a2y, a2z);
+      heun_diffusion<R, false>(lam, eq.x, eq.y, eq.z, B2);
+      R B[9];
+#pragma unroll
+      for (int j = 0; j < 9; ++j) B[j] = B1[j] + B2[j];
+      const R ax = a1x + a2x, ay = a1y + a2y, az = a1z + a2z;
+      const R t0 = ax * dtg + B[0] * w0 + B[1] * w1 + B[2] * w2;
+      const R t1 = ay * dtg + B[3] * w0 + B[4] * w1 + B[5] * w2;
+      const R t2 = az * dtg + B[6] * w0 + B[7] * w1 + B[8] * w2;
+      const R o0 = eq.x + (R)0.5 * t0, o1 = eq.y + (R)0.5 * t1, o2 = eq.z + (R)0.5 * t2;
+      const R etot = (R)1.0 / sqrt(o0 * o0 + o1 * o1 + o2 * o2);
+      out[q] = Quad<R>{o0 * etot, o1 * etot, o2 * etot, m};
     }
   } else {
     const R sig = prm.sig_uniform ? prm.sigk0 : midpoint_sigma<R>(lc.Dk, (double)m, T, prm.temprescale);

@@ -73,6 +73,10 @@ hipError_t launch_llg(hipStream_t s, const Tune &t, int alg, int half, bool fuse
   UASD_LLG_CASE(1, 1, true)
   UASD_LLG_CASE(1, 0, false)
   UASD_LLG_CASE(1, 1, false)
+  UASD_LLG_CASE(4, 0, true)
+  UASD_LLG_CASE(4, 1, true)
+  UASD_LLG_CASE(4, 0, false)
+  UASD_LLG_CASE(4, 1, false)
   UASD_LLG_CASE(5, 0, true)
   UASD_LLG_CASE(5, 1, true)
   UASD_LLG_CASE(5, 0, false)

@@ -369,16 +369,18 @@ contains
       integer, intent(in) :: llg, SDEalgh, Nred, Natom
       character(len=1), intent(in) :: stt, do_she, do_sot
       if (llg /= 1) call uasd_unsupported('llg other than 1 (LLG, one damping)')
-      if (SDEalgh /= 1 .and. SDEalgh /= 5) call uasd_unsupported('SDEalgh other than 1 and 5')
+      if (SDEalgh /= 1 .and. SDEalgh /= 4 .and. SDEalgh /= 5) &
+         call uasd_unsupported('SDEalgh other than 1, 4 and 5')
       if (stt /= 'N' .or. do_she /= 'N' .or. do_sot /= 'N') &
          call uasd_unsupported('spin-transfer / spin-Hall / spin-orbit torques')
       if (Nred /= Natom) call uasd_unsupported('a reduced list of updated moments')
    end subroutine check_llg_options
 
    !----------------------------------------------------------------------------
-   !> evolve_first with the reference's argument list, SDEalgh 1 (midpoint)
-   !> and 5 (Depondt).  thermal_field is returned as zero (the thermal field is
-   !> drawn and applied inside the kernel and not stored).
+   !> evolve_first with the reference's argument list, SDEalgh 1 (midpoint),
+   !> 4 (Heun) and 5 (Depondt).  thermal_field is returned as zero (the thermal
+   !> field is drawn and applied inside the kernel and not stored).  Heun
+   !> starts from emom2, as heun4p does, so emom2 is read as well.
    !----------------------------------------------------------------------------
    subroutine uasd_evolve_first(Natom,Mensemble,Landeg,llg,SDEalgh,bn,lambda1_array,     &
          lambda2_array,NA,compensate_drift,delta_t,relaxtime,Temp_array,temprescale,   &
@@ -402,7 +404,8 @@ contains
       real(dp), intent(inout) :: temprescale
       real(dp), dimension(3,Natom,Mensemble), intent(inout), target :: beff, emom
       real(dp), dimension(3,Natom,Mensemble), intent(inout) :: thermal_field
-      real(dp), dimension(3,Natom,Mensemble), intent(out), target :: b2eff, emom2, emomM
+      real(dp), dimension(3,Natom,Mensemble), intent(out), target :: b2eff, emomM
+      real(dp), dimension(3,Natom,Mensemble), intent(inout), target :: emom2
       integer, intent(in) :: max_no_constellations
       real(dp), intent(in) :: cos_thr
       logical, intent(in) :: OPT_flag
@@ -434,7 +437,7 @@ contains
    end subroutine uasd_evolve_first
 
    !----------------------------------------------------------------------------
-   !> evolve_second with the reference's argument list (SDEalgh 1 and 5).
+   !> evolve_second with the reference's argument list (SDEalgh 1, 4 and 5).
    !----------------------------------------------------------------------------
    subroutine uasd_evolve_second(Natom,Mensemble,Landeg,llg,SDEalgh,bn,lambda1_array,    &
          delta_t,relaxtime,beff,beff2,b2eff,btorque,emom,emom2,stt,nlist,nlistsize,    &

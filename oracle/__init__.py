@@ -138,6 +138,10 @@ def _declare(L):
                                      C.POINTER(C.c_double)]
     L.or_effective_field_single.argtypes = [C.POINTER(Ham), C.c_int, dp, vp, C.c_int, C.c_int, dp]
     L.or_energy_terms.argtypes = [C.POINTER(Ham), C.c_int, C.c_int, dp, vp, dp]
+    L.or_pt_exchange.argtypes = [C.c_int, dp, dp, C.c_int, dp, C.c_double,
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.or_pt_select.argtypes = [C.c_int, dp]
+    L.or_pt_select.restype = C.c_int
     L.or_depondt_first.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, C.c_double, dp, dp,
                                    dp, dp, dp, dp, dp, vp]
     L.or_depondt_second.argtypes = [C.c_int, C.c_int, dp, C.c_double, dp, dp, dp, dp, dp]

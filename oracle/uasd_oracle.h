@@ -205,6 +205,12 @@ void   or_zig_fill(or_zig *z, double *out, long n);
 void   or_sum_moments(int N, int M, const double *emomM, double *msum /*(3,M)*/);
 double or_calc_mavrg(int N, int M, const double *emomM);
 
+/* parallel tempering exchange step (pt_driver.f90:205-232) and the final
+ * replica choice (246-250) */
+void or_pt_exchange(int nrep, double *temps, const double *repene, int natom,
+                    const double *flipprob, double temprescale, double *n_hit, double *n_trial);
+int  or_pt_select(int nrep, const double *repene);
+
 #ifdef __cplusplus
 }
 #endif

@@ -48,7 +48,13 @@ enum {
 };
 
 /* Solvers (the reference's SDEalgh values, evolution.f90:134-217). */
-enum { UASD_SDE_MIDPOINT = 1, UASD_SDE_HEUN = 4, UASD_SDE_DEPONDT = 5 };
+enum {
+  UASD_SDE_MIDPOINT = 1,
+  UASD_SDE_HEUN = 4,
+  UASD_SDE_DEPONDT = 5,
+  UASD_SDE_SIB = 6,     /* sibt/sibf, sphmidpoint.f90:36-477 */
+  UASD_SDE_SPHMID = 7   /* sispht/sisphf, sphmidpoint.f90:485-923 */
+};
 
 typedef struct uasd_ctx uasd_ctx;
 
@@ -119,13 +125,13 @@ int uasd_get_moments(uasd_ctx *ctx, double *emom, double *emomM);
  * external field, energy in mRy as the reference's field routine reports it. */
 int uasd_effective_field_host(uasd_ctx *ctx, const double *emomM, double *beff, double *beff1,
                               double *beff2, double *energy);
-/* evolve_first (evolution.f90:38-273) for SDEalgh 1, 4 and 5: reads beff,
+/* evolve_first (evolution.f90:38-273) for SDEalgh 1, 4, 5, 6, 7: reads beff,
  * emom, mmom; writes emom2, emomM (and, for Depondt, b2eff and the updated
  * emom).  Heun (evolve4p, heun_proper.f90:28-369) starts from emom2 and
  * writes the predictor into emom and emomM and b2eff = beff. */
 int uasd_evolve_first_host(uasd_ctx *ctx, int sdealgh, const double *beff, double *emom,
                            double *emom2, double *emomM, double *b2eff);
-/* evolve_second (evolution.f90:280-406) for SDEalgh 1, 4 and 5; Heun
+/* evolve_second (evolution.f90:280-406) for SDEalgh 1, 4, 5, 6, 7; Heun
  * (evolve4f, heun_proper.f90:168-254) reads the predictor in emom and the
  * start in emom2 and writes the renormalised result into emom2. */
 int uasd_evolve_second_host(uasd_ctx *ctx, int sdealgh, const double *beff,

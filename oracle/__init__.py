@@ -151,6 +151,10 @@ def _declare(L):
                                     C.c_double, dp, vp]
     L.or_heun_first.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_double, dp, dp,
                                 dp, dp, dp, dp, dp]
+    L.or_sib_first.argtypes = [C.c_int, C.c_int, C.c_int, dp, C.c_double, dp, dp, dp, dp, dp,
+                               dp, C.c_double, dp]
+    L.or_sib_second.argtypes = [C.c_int, C.c_int, C.c_int, dp, C.c_double, dp, dp, dp, dp,
+                                C.c_double, dp]
     L.or_heun_second.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, dp, dp, dp, dp, dp]
     L.or_midpoint_second.argtypes = [C.c_int, C.c_int, dp, C.c_double, dp, dp, dp, dp,
                                      C.c_double, dp]

@@ -5,6 +5,8 @@
  *   source/RNG/randomnumbers.f90      rannum scaling 625-771 (llg = 1 branch)
  *   source/Evolution/heun_proper.f90  evolve4p 28-161, heun4p 261-369,
  *                                     evolve4f 168-254, heun4f 376-530
+ *   source/Evolution/sphmidpoint.f90  sibt 36-262, sibf 270-477 (SDEalgh 6),
+ *                                     sispht 485-713, sisphf 721-923 (SDEalgh 7)
  *   source/Evolution/updatemoments.f90 copym 48-95 (mompar = 0, initexc /= 'I')
  * Spin-transfer torques are off (stt='N', do_she='N', do_sot='N'), so the
  * torque accumulators are the literal 0.0 they start from.
@@ -326,6 +328,116 @@ void or_heun_second(int N, int M, const double *landeg, const double *lambda, do
     }
 }
 
+/* Semi-implicit midpoint solvers with fixed-point iteration (SDEalgh 6,
+ * sibt/sibf) and its spherical variant (SDEalgh 7, sispht/sisphf).  Both
+ * halves solve e = e1 + dtg u x a1 + sqrtdtg u x s1 by at most 100
+ * fixed-point iterations from e = e1, u = (e1 + e)/2 (SIB) or
+ * (e1 + e)/|e1 + e| (spherical), stopping when every component moved by
+ * less than epsilona = 1e-12, a default-real literal (so float(1e-12)).
+ * a1 = -b - lam v x b and s1 = -f - lam v x f with f the rannum-scaled
+ * noise and v = e1 (predictor) or the predicted midpoint (corrector).
+ * Predictor: SIB stores (e1 + e)/2, spherical (e1 + e0)/|e1 + e0| with e0
+ * the iterate before the last (the loop exits before e0 = e); emomM =
+ * emom2*mmom.  Corrector: emom2 = the last iterate. */
+static void sib_solve(int sph, const double e1[3], const double a1[3], const double s1[3],
+                      double dtg, double sqrtdtg, double out[3], double prev[3])
+{
+  const double epsilona = (double)1e-12f;
+  double e0[3] = {e1[0], e1[1], e1[2]}, ef[3] = {e1[0], e1[1], e1[2]};
+  for (int k = 1; k <= 100; ++k) {
+    double u[3];
+    if (sph) {
+      double unorm = sqrt((e1[0] + e0[0]) * (e1[0] + e0[0]) + (e1[1] + e0[1]) * (e1[1] + e0[1]) +
+                          (e1[2] + e0[2]) * (e1[2] + e0[2]));
+      for (int c = 0; c < 3; ++c) u[c] = (e1[c] + e0[c]) / unorm;
+    } else {
+      for (int c = 0; c < 3; ++c) u[c] = 0.5 * (e1[c] + e0[c]);
+    }
+    double d[3] = {dtg * u[0], dtg * u[1], dtg * u[2]};
+    double q[3] = {sqrtdtg * u[0], sqrtdtg * u[1], sqrtdtg * u[2]};
+    ef[0] = e1[0] + d[1] * a1[2] - d[2] * a1[1] + q[1] * s1[2] - q[2] * s1[1];
+    ef[1] = e1[1] + d[2] * a1[0] - d[0] * a1[2] + q[2] * s1[0] - q[0] * s1[2];
+    ef[2] = e1[2] + d[0] * a1[1] - d[1] * a1[0] + q[0] * s1[1] - q[1] * s1[0];
+    double dx = sqrt((e0[0] - ef[0]) * (e0[0] - ef[0]));
+    double dy = sqrt((e0[1] - ef[1]) * (e0[1] - ef[1]));
+    double dz = sqrt((e0[2] - ef[2]) * (e0[2] - ef[2]));
+    if (dx < epsilona && dy < epsilona && dz < epsilona) break;
+    for (int c = 0; c < 3; ++c) e0[c] = ef[c];
+  }
+  for (int c = 0; c < 3; ++c) {
+    out[c] = ef[c];
+    prev[c] = e0[c];
+  }
+}
+
+static void sib_drift(double lam, const double v[3], const double b[3], const double f[3],
+                      double a1[3], double s1[3])
+{
+  a1[0] = -b[0] - lam * (v[1] * b[2] - v[2] * b[1]);
+  a1[1] = -b[1] - lam * (v[2] * b[0] - v[0] * b[2]);
+  a1[2] = -b[2] - lam * (v[0] * b[1] - v[1] * b[0]);
+  s1[0] = -f[0] - lam * (v[1] * f[2] - v[2] * f[1]);
+  s1[1] = -f[1] - lam * (v[2] * f[0] - v[0] * f[2]);
+  s1[2] = -f[2] - lam * (v[0] * f[1] - v[1] * f[0]);
+}
+
+void or_sib_first(int alg, int N, int M, const double *landeg, double bn, const double *lambda,
+                  const double *beff, const double *emom, double *emom2, double *emomM,
+                  const double *mmom, double deltat, const double *ranv)
+{
+  for (int k = 0; k < M; ++k)
+    for (int i = 0; i < N; ++i) {
+      double lldamp = 1.0 / (1.0 + lambda[i] * lambda[i]);
+      double dt = deltat * bn * OR_GAMA * lldamp;
+      double sqrtdt = sqrt(dt);
+      double dtg = dt * landeg[i], sqrtdtg = sqrtdt * landeg[i];
+      double e1[3], b[3], f[3], a1[3], s1[3], ef[3], e0[3];
+      for (int c = 0; c < 3; ++c) {
+        e1[c] = V3(emom, c, i, k, N);
+        b[c] = V3(beff, c, i, k, N);
+        f[c] = V3(ranv, c, i, k, N);
+      }
+      sib_drift(lambda[i], e1, b, f, a1, s1);
+      sib_solve(alg == 7, e1, a1, s1, dtg, sqrtdtg, ef, e0);
+      double o[3];
+      if (alg == 7) {
+        double unorm = sqrt((e1[0] + e0[0]) * (e1[0] + e0[0]) + (e1[1] + e0[1]) * (e1[1] + e0[1]) +
+                            (e1[2] + e0[2]) * (e1[2] + e0[2]));
+        for (int c = 0; c < 3; ++c) o[c] = (e1[c] + e0[c]) / unorm;
+      } else {
+        for (int c = 0; c < 3; ++c) o[c] = 0.5 * (e1[c] + ef[c]);
+      }
+      double m = S2(mmom, i, k, N);
+      for (int c = 0; c < 3; ++c) {
+        V3(emom2, c, i, k, N) = o[c];
+        V3(emomM, c, i, k, N) = o[c] * m;
+      }
+    }
+}
+
+void or_sib_second(int alg, int N, int M, const double *landeg, double bn, const double *lambda,
+                   const double *beff, const double *emom, double *emom2, double deltat,
+                   const double *ranv)
+{
+  for (int k = 0; k < M; ++k)
+    for (int i = 0; i < N; ++i) {
+      double lldamp = 1.0 / (1.0 + lambda[i] * lambda[i]);
+      double dt = deltat * bn * OR_GAMA * lldamp;
+      double sqrtdt = sqrt(dt);
+      double dtg = dt * landeg[i], sqrtdtg = sqrtdt * landeg[i];
+      double e1[3], etp[3], b[3], f[3], a1[3], s1[3], ef[3], e0[3];
+      for (int c = 0; c < 3; ++c) {
+        e1[c] = V3(emom, c, i, k, N);
+        etp[c] = V3(emom2, c, i, k, N);
+        b[c] = V3(beff, c, i, k, N);
+        f[c] = V3(ranv, c, i, k, N);
+      }
+      sib_drift(lambda[i], etp, b, f, a1, s1);
+      sib_solve(alg == 7, e1, a1, s1, dtg, sqrtdtg, ef, e0);
+      for (int c = 0; c < 3; ++c) V3(emom2, c, i, k, N) = ef[c];
+    }
+}
+
 /* The sd_mphase step body (sd_driver.f90:668-765) for nsteps, with the
  * thermal field drawn from the Philox counter stream the HIP kernels use
  * (noise = 1) or absent (noise = 0).  Used as the CPU baseline and for long
@@ -354,6 +466,11 @@ void or_sd_run(const or_ham *h, int N, int M, int alg, long nsteps, uint64_t ste
       or_midpoint_first(N, M, landeg, 1.0, lam, beff, emom, e2, emomM, mmom, dt, rv, NULL);
       or_effective_field(h, N, M, emomM, ext, beff, NULL, NULL, NULL);
       or_midpoint_second(N, M, landeg, 1.0, lam, beff, emom, e2, dt, rv);
+    } else if (alg == 6 || alg == 7) {
+      or_rannum_scale(N, M, 1, lam, 1.0, mmomi, temp, 1.0, rv);
+      or_sib_first(alg, N, M, landeg, 1.0, lam, beff, emom, e2, emomM, mmom, dt, rv);
+      or_effective_field(h, N, M, emomM, ext, beff, NULL, NULL, NULL);
+      or_sib_second(alg, N, M, landeg, 1.0, lam, beff, emom, e2, dt, rv);
     } else if (alg == 4) {
       /* at step start emom2 equals emom (moment_update copies it) */
       for (size_t x = 0; x < n3; ++x) e2[x] = emom[x];

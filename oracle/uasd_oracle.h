@@ -158,6 +158,13 @@ void or_heun_first(int N, int M, const double *landeg, const double *lambda, con
 void or_heun_second(int N, int M, const double *landeg, const double *lambda, double delta_t,
                     const double *beff, const double *b2eff, const double *ranv,
                     const double *emom, double *emom2);
+/* SIB (alg 6) and spherical midpoint (alg 7): sphmidpoint.f90 */
+void or_sib_first(int alg, int N, int M, const double *landeg, double bn, const double *lambda,
+                  const double *beff, const double *emom, double *emom2, double *emomM,
+                  const double *mmom, double deltat, const double *ranv);
+void or_sib_second(int alg, int N, int M, const double *landeg, double bn, const double *lambda,
+                   const double *beff, const double *emom, double *emom2, double deltat,
+                   const double *ranv);
 void or_moment_update(int N, int M, const double *mmom, double *emom,
                       const double *emom2, double *emomM, double *mmomi);
 /* nsteps of the sd_mphase body; noise 1 = Philox stream of the HIP kernels */

@@ -146,6 +146,12 @@ class OracleSD:
                                 self.noise, None)
             beff, _ = self.ham.field(self.eM, ext)
             L.or_midpoint_second(N, M, lg, 1.0, lam, beff, self.e, self.e2, dt, self.noise)
+        elif alg in (6, 7):
+            L.or_rannum_scale(N, M, 1, lam, 1.0, self.mmomi, T, 1.0, self.noise.reshape(-1))
+            L.or_sib_first(alg, N, M, lg, 1.0, lam, beff, self.e, self.e2, self.eM, self.mmom,
+                           dt, self.noise)
+            beff, _ = self.ham.field(self.eM, ext)
+            L.or_sib_second(alg, N, M, lg, 1.0, lam, beff, self.e, self.e2, dt, self.noise)
         elif alg == 4:
             # at step start emom2 equals emom (moment_update copied it)
             self.e2[...] = self.e

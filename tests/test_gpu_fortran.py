@@ -74,7 +74,7 @@ def test_fortran_solvers_golden(O, tmp_path, solver, name):
 
 
 @pytest.mark.parametrize("ordered", [False, True])
-@pytest.mark.parametrize("alg", [1, 4, 5])
+@pytest.mark.parametrize("alg", [1, 4, 5, 6, 7])
 def test_fortran_injected_noise_matches_oracle(O, tmp_path, alg, ordered):
     rng = np.random.default_rng(21)
     sysd = CF.bcc_shells(3, CF.bcc5_couplings())

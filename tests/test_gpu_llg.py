@@ -99,7 +99,7 @@ def _oracle_step(O, sim, alg, dt, lam, T, normals, ext):
     sim.step(alg, dt, lam, T, normals=normals, ext=ext)
 
 
-@pytest.mark.parametrize("alg", [1, 4, 5])
+@pytest.mark.parametrize("alg", [1, 4, 5, 6, 7])
 def test_dropin_step_matches_oracle(O, alg):
     """field -> evolve_first -> field -> evolve_second -> moment_update through
     the host-array entry points, with an injected noise stream at 300 K."""
@@ -133,7 +133,7 @@ def test_dropin_step_matches_oracle(O, alg):
         assert np.max(np.abs(e - sim.e)) <= 1e-13, it
 
 
-@pytest.mark.parametrize("alg", [1, 4, 5])
+@pytest.mark.parametrize("alg", [1, 4, 5, 6, 7])
 def test_resident_T0_trajectory(O, alg):
     """Deterministic T = 0 trajectory, fused kernels vs oracle, 500 steps."""
     rng = np.random.default_rng(7)
@@ -154,7 +154,7 @@ def test_resident_T0_trajectory(O, alg):
     assert np.max(np.abs(eM - sim.eM)) <= 1e-12 * 2.2
 
 
-@pytest.mark.parametrize("alg", [1, 4, 5])
+@pytest.mark.parametrize("alg", [1, 4, 5, 6, 7])
 def test_resident_injected_noise_trajectory(O, alg):
     """Stochastic LLG at 300 K on an identical injected noise stream."""
     rng = np.random.default_rng(9)
@@ -175,7 +175,7 @@ def test_resident_injected_noise_trajectory(O, alg):
     assert np.max(np.abs(e - sim.e)) <= 1e-12
 
 
-@pytest.mark.parametrize("alg", [4, 5])
+@pytest.mark.parametrize("alg", [4, 5, 7])
 def test_philox_noise_stream_matches_oracle(O, alg):
     """In-kernel Philox deviates reproduce the oracle's restatement of the
     same counter stream bit for bit (single-precision Box-Muller made of

@@ -114,14 +114,16 @@ def test_autotune_may_pick_tile_and_keeps_results():
     assert np.array_equal(eng.get_moments()[0], ref)
 
 
-def test_heun_keeps_the_list_kernels_under_a_tile_tuning():
+@pytest.mark.parametrize("alg", [4, 6, 7])
+def test_other_solvers_keep_the_list_kernels_under_a_tile_tuning(alg):
     """The tile kernels carry the midpoint and Depondt solvers only: with a
-    tile shape selected, Heun (SDEalgh 4) runs on the list kernels and gives
-    the list result bit for bit, in both site orders."""
-    sysd = CASES["tri_dmi"]()
+    tile shape selected, Heun (SDEalgh 4), SIB (6) and the spherical midpoint
+    (7) run on the list kernels and give the list result bit for bit, in
+    both site orders."""
+    sysd = CASES["kagome"]()
     lat = sysd["lattice"]
     e0 = np.ascontiguousarray(CF.random_unit(lat.natom, np.random.default_rng(91), 1))
-    ref, _ = _run(sysd, e0, 1, 0, 4)
+    ref, _ = _run(sysd, e0, 1, 0, alg)
     for order in (1, lat.NA):
-        got, _ = _run(sysd, e0, order, 12, 4, pipe=1)
+        got, _ = _run(sysd, e0, order, 12, alg, pipe=1)
         assert np.array_equal(got, ref), order

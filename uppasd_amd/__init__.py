@@ -7,5 +7,6 @@ behind the C-ABI of ``libuasd_hip.so`` (include/uasd_hip.h).
 """
 from .engine import (Engine, UasdError, moment_update, device_count,  # noqa: F401
                      UASD_NOISE_NONE, UASD_NOISE_PHILOX, UASD_NOISE_INJECTED,
-                     UASD_SDE_MIDPOINT, UASD_SDE_HEUN, UASD_SDE_DEPONDT)
+                     UASD_SDE_MIDPOINT, UASD_SDE_HEUN, UASD_SDE_DEPONDT, UASD_SDE_SIB,
+                     UASD_SDE_SPHMID)
 from .lattice import Lattice, Shells, build_pair_list, bcc, triangular, kagome  # noqa: F401

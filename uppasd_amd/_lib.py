@@ -17,6 +17,7 @@ UASD_OK = 0
 UASD_FP64, UASD_FP32 = 0, 1
 UASD_NOISE_NONE, UASD_NOISE_PHILOX, UASD_NOISE_INJECTED = 0, 1, 2
 UASD_SDE_MIDPOINT, UASD_SDE_HEUN, UASD_SDE_DEPONDT = 1, 4, 5
+UASD_SDE_SIB, UASD_SDE_SPHMID = 6, 7
 
 ERRORS = {-1: "argument", -2: "hip", -3: "out of memory", -4: "unsupported", -5: "state",
           -6: "no device"}

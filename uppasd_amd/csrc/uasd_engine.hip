@@ -530,8 +530,12 @@ int ready_for_llg(uasd_ctx *c) {
 // Quad arrays in c->spins: 0 e (emom), 1 p (emom2), 2 b2 (b2eff).
 // step_off >= 0: inside a graph capture, the step is *ctr[0] + step_off.
 bool sde_supported(int alg) {
-  return alg == UASD_SDE_MIDPOINT || alg == UASD_SDE_HEUN || alg == UASD_SDE_DEPONDT;
+  return alg == UASD_SDE_MIDPOINT || alg == UASD_SDE_HEUN || alg == UASD_SDE_DEPONDT ||
+         alg == UASD_SDE_SIB || alg == UASD_SDE_SPHMID;
 }
+
+// solvers that keep b2eff between the halves
+bool sde_b2eff(int alg) { return alg == UASD_SDE_HEUN || alg == UASD_SDE_DEPONDT; }
 
 template <typename R>
 int run_llg_half(uasd_ctx *c, int alg, int half, bool fused, bool write_e,
@@ -549,8 +553,8 @@ int run_llg_half(uasd_ctx *c, int alg, int half, bool fused, bool write_e,
                 *b2 = quads<R>(c, c->spins, 2);
   const uasd::Quad<R> *beff = fused ? nullptr : quads<R>(c, c->field);
   uasd::Quad<R> *out = write_e ? e : p;
-  // the tile kernels carry the midpoint and Depondt solvers; Heun runs on the lists
-  if (fused && c->tune[0].tile > 0 && alg != UASD_SDE_HEUN) {
+  // the tile kernels carry the midpoint and Depondt solvers; the others run on the lists
+  if (fused && c->tune[0].tile > 0 && (alg == UASD_SDE_MIDPOINT || alg == UASD_SDE_DEPONDT)) {
     build_stencil(c);
     if (tile_fits(c, c->tune[0].tile)) {
       uasd::TileDev t;
@@ -1284,8 +1288,8 @@ int uasd_evolve_first_host(uasd_ctx *c, int alg, const double *beff, double *emo
                            double *emomM, double *b2eff) {
   if (int r = check_ctx(c)) return r;
   if (!sde_supported(alg))
-    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint), 4 (Heun) or 5 (Depondt)");
-  if (!beff || !emom || !emom2 || !emomM || (alg != UASD_SDE_MIDPOINT && !b2eff))
+    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint), 4 (Heun), 5 (Depondt), 6 (SIB) or 7 (spherical midpoint)");
+  if (!beff || !emom || !emom2 || !emomM || (sde_b2eff(alg) && !b2eff))
     return fail(UASD_ERR_ARG, "null array");
   if (int r = ready_for_llg(c)) return r;
   if (int r = upload_quads(c, c->field, 0, to_quads(c, beff, nullptr, 0.0))) return r;
@@ -1313,7 +1317,8 @@ int uasd_evolve_first_host(uasd_ctx *c, int alg, const double *beff, double *emo
     if ((r = download_quads(c, c->spins, 2, bq))) return r;
     from_quads(c, bq, b2eff, false);
   } else {
-    // midpoint.f90:167-175: emom2 = et', emomM = et'*mmom, emom unchanged
+    // midpoint.f90:167-175 (and sibt / sispht, sphmidpoint.f90:247-258,
+    // 698-710): emom2 = et', emomM = et'*mmom, emom unchanged
     from_quads(c, pq, emom2, false);
     from_quads(c, pq, emomM, true);
   }
@@ -1324,8 +1329,8 @@ int uasd_evolve_second_host(uasd_ctx *c, int alg, const double *beff, const doub
                             double *emom, double *emom2) {
   if (int r = check_ctx(c)) return r;
   if (!sde_supported(alg))
-    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint), 4 (Heun) or 5 (Depondt)");
-  if (!beff || !emom || !emom2 || (alg != UASD_SDE_MIDPOINT && !b2eff))
+    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint), 4 (Heun), 5 (Depondt), 6 (SIB) or 7 (spherical midpoint)");
+  if (!beff || !emom || !emom2 || (sde_b2eff(alg) && !b2eff))
     return fail(UASD_ERR_ARG, "null array");
   if (int r = ready_for_llg(c)) return r;
   if (int r = upload_quads(c, c->field, 0, to_quads(c, beff, nullptr, 0.0))) return r;
@@ -1436,7 +1441,7 @@ int uasd_set_graph_steps(uasd_ctx *c, int steps_per_graph) {
 int uasd_sd_steps(uasd_ctx *c, int alg, int64_t nsteps) {
   if (int r = check_ctx(c)) return r;
   if (!sde_supported(alg))
-    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint), 4 (Heun) or 5 (Depondt)");
+    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint), 4 (Heun), 5 (Depondt), 6 (SIB) or 7 (spherical midpoint)");
   if (nsteps < 0) return fail(UASD_ERR_ARG, "nsteps < 0");
   if (int r = ready_for_llg(c)) return r;
   int64_t s = 0;
@@ -1719,7 +1724,7 @@ extern "C" {
 int uasd_autotune_llg(uasd_ctx *c, int alg, int steps, double *best_ms) {
   if (int r = check_ctx(c)) return r;
   if (!sde_supported(alg))
-    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint), 4 (Heun) or 5 (Depondt)");
+    return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint), 4 (Heun), 5 (Depondt), 6 (SIB) or 7 (spherical midpoint)");
   if (steps <= 0) return fail(UASD_ERR_ARG, "steps must be > 0");
   if (int r = ready_for_llg(c)) return r;
   StateGuard g(c);

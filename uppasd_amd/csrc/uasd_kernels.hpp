@@ -656,6 +656,51 @@ __device__ __forceinline__ void heun_diffusion(R lam, R ex, R ey, R ez, R (&b)[9
   b[8] = QUIRK ? lam - ez * ez - lam : lam - ez * ez * lam;
 }
 
+// Fixed-point solve of the semi-implicit midpoint solvers (sphmidpoint.f90,
+// sibt/sibf 36-477 for SPH = false, sispht/sisphf 485-923 for SPH = true):
+// e = e1 + dtg u x a + sqrtdtg u x s with u = (e1 + e0)/2 or
+// (e1 + e0)/|e1 + e0|, at most 100 iterations from e0 = e1, stopping when
+// every component moved by less than float(1e-12) (a default-real literal
+// in the reference).  f = the last iterate, p = the one before it when the
+// loop stopped early (else the last).
+template <typename R, bool SPH>
+__device__ __forceinline__ void sib_solve(R e1x, R e1y, R e1z, R a0, R a1, R a2, R s0, R s1,
+                                          R s2, R dtg, R sqrtdtg, R &fx, R &fy, R &fz, R &px,
+                                          R &py, R &pz) {
+  const R eps = (R)(double)1e-12f;
+  R e0x = e1x, e0y = e1y, e0z = e1z;
+  fx = e1x;
+  fy = e1y;
+  fz = e1z;
+  for (int k = 0; k < 100; ++k) {
+    R ux, uy, uz;
+    if (SPH) {
+      const R sx = e1x + e0x, sy = e1y + e0y, sz = e1z + e0z;
+      const R un = sqrt(sx * sx + sy * sy + sz * sz);
+      ux = sx / un;
+      uy = sy / un;
+      uz = sz / un;
+    } else {
+      ux = (R)0.5 * (e1x + e0x);
+      uy = (R)0.5 * (e1y + e0y);
+      uz = (R)0.5 * (e1z + e0z);
+    }
+    const R dx = dtg * ux, dy = dtg * uy, dz = dtg * uz;
+    const R qx = sqrtdtg * ux, qy = sqrtdtg * uy, qz = sqrtdtg * uz;
+    fx = e1x + dy * a2 - dz * a1 + qy * s2 - qz * s1;
+    fy = e1y + dz * a0 - dx * a2 + qz * s0 - qx * s2;
+    fz = e1z + dx * a1 - dy * a0 + qx * s1 - qy * s0;
+    const R cx = e0x - fx, cy = e0y - fy, cz = e0z - fz;
+    if (sqrt(cx * cx) < eps && sqrt(cy * cy) < eps && sqrt(cz * cz) < eps) break;
+    e0x = fx;
+    e0y = fy;
+    e0z = fz;
+  }
+  px = e0x;
+  py = e0y;
+  pz = e0z;
+}
+
 // Cramer solve of (I + [A]x) e_t = (I - [A]x) e  (midpoint.f90:153-164)
 template <typename R>
 __device__ __forceinline__ void midpoint_solve(R A0, R A1, R A2, R e0, R e1, R e2, R &t0, R &t1,
@@ -790,6 +835,39 @@ __device__ __forceinline__ void llg_apply(const LLGDev<R> &prm, int site, int64_
       d2 = (R)0.5 * d2 + (R)0.5 * bq.z;
       rodrigues(d0, d1, d2, (R)prm.dt, lc.lldamp, eq.x, eq.y, eq.z, o0, o1, o2);
       out[q] = finish<R>(o0, o1, o2, m);
+    }
+  } else if (ALG == 6 || ALG == 7) {
+    // SIB (6) and spherical midpoint (7), sphmidpoint.f90: the rannum noise
+    // of the midpoint solver, a = -b - lam v x b, s = -f - lam v x f with
+    // v = e (predictor) or the predicted midpoint (corrector)
+    const R sig = prm.sig_uniform ? prm.sigk0 : midpoint_sigma<R>(lc.Dk, (double)m, T, prm.temprescale);
+    const R f0 = n0 * sig, f1 = n1 * sig, f2 = n2 * sig;
+    const R lg = (R)prm.landeg(site);
+    const R dtg = lc.mdt * lg, sqrtdtg = lc.msqrtdt * lg;
+    const R lam = lc.lam;
+    R vx = eq.x, vy = eq.y, vz = eq.z;
+    if (HALF == 1) {
+      vx = in.pq.x;
+      vy = in.pq.y;
+      vz = in.pq.z;
+    }
+    const R a0 = -b0 - lam * (vy * b2v - vz * b1), a1 = -b1 - lam * (vz * b0 - vx * b2v),
+            a2 = -b2v - lam * (vx * b1 - vy * b0);
+    const R s0 = -f0 - lam * (vy * f2 - vz * f1), s1 = -f1 - lam * (vz * f0 - vx * f2),
+            s2 = -f2 - lam * (vx * f1 - vy * f0);
+    R fx, fy, fz, px, py, pz;
+    sib_solve<R, ALG == 7>(eq.x, eq.y, eq.z, a0, a1, a2, s0, s1, s2, dtg, sqrtdtg, fx, fy, fz,
+                           px, py, pz);
+    if (HALF == 0) {
+      if (ALG == 7) {
+        const R sx = eq.x + px, sy = eq.y + py, sz = eq.z + pz;
+        const R un = sqrt(sx * sx + sy * sy + sz * sz);
+        p[q] = Quad<R>{sx / un, sy / un, sz / un, m};
+      } else {
+        p[q] = Quad<R>{(R)0.5 * (eq.x + fx), (R)0.5 * (eq.y + fy), (R)0.5 * (eq.z + fz), m};
+      }
+    } else {
+      out[q] = finish<R>(fx, fy, fz, m);
     }
   } else if (ALG == 4) {
     // Heun predictor-corrector (heun_proper.f90): dW = ranv*Landeg,

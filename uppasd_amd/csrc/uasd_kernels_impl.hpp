@@ -77,6 +77,14 @@ hipError_t launch_llg(hipStream_t s, const Tune &t, int alg, int half, bool fuse
   UASD_LLG_CASE(4, 1, true)
   UASD_LLG_CASE(4, 0, false)
   UASD_LLG_CASE(4, 1, false)
+  UASD_LLG_CASE(6, 0, true)
+  UASD_LLG_CASE(6, 1, true)
+  UASD_LLG_CASE(6, 0, false)
+  UASD_LLG_CASE(6, 1, false)
+  UASD_LLG_CASE(7, 0, true)
+  UASD_LLG_CASE(7, 1, true)
+  UASD_LLG_CASE(7, 0, false)
+  UASD_LLG_CASE(7, 1, false)
   UASD_LLG_CASE(5, 0, true)
   UASD_LLG_CASE(5, 1, true)
   UASD_LLG_CASE(5, 0, false)

@@ -23,7 +23,7 @@ import numpy as np
 from . import _lib as B
 from ._lib import (UASD_FP32, UASD_FP64, UASD_NOISE_INJECTED, UASD_NOISE_NONE,
                    UASD_NOISE_PHILOX, UASD_SDE_DEPONDT, UASD_SDE_HEUN, UASD_SDE_MIDPOINT,
-                   UasdError)
+                   UASD_SDE_SIB, UASD_SDE_SPHMID, UasdError)
 
 # LDS tile shapes (cells x, y, z) selectable with set_tuning(TUNE_LLG, g, 256 | (k << 18))
 TILE_SHAPES = ((8, 8, 4), (16, 8, 2), (32, 4, 2), (16, 4, 4), (16, 16, 1), (32, 8, 1), (64, 4, 1),
@@ -31,7 +31,8 @@ TILE_SHAPES = ((8, 8, 4), (16, 8, 2), (32, 4, 2), (16, 4, 4), (16, 16, 1), (32, 
                (8, 8, 8))
 
 __all__ = ["Engine", "moment_update", "UasdError", "UASD_NOISE_NONE", "UASD_NOISE_PHILOX",
-           "UASD_NOISE_INJECTED", "UASD_SDE_MIDPOINT", "UASD_SDE_HEUN", "UASD_SDE_DEPONDT"]
+           "UASD_NOISE_INJECTED", "UASD_SDE_MIDPOINT", "UASD_SDE_HEUN", "UASD_SDE_DEPONDT",
+           "UASD_SDE_SIB", "UASD_SDE_SPHMID"]
 
 
 def device_count() -> int:

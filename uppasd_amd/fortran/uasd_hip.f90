@@ -369,8 +369,8 @@ contains
       integer, intent(in) :: llg, SDEalgh, Nred, Natom
       character(len=1), intent(in) :: stt, do_she, do_sot
       if (llg /= 1) call uasd_unsupported('llg other than 1 (LLG, one damping)')
-      if (SDEalgh /= 1 .and. SDEalgh /= 4 .and. SDEalgh /= 5) &
-         call uasd_unsupported('SDEalgh other than 1, 4 and 5')
+      if (SDEalgh /= 1 .and. (SDEalgh < 4 .or. SDEalgh > 7)) &
+         call uasd_unsupported('SDEalgh other than 1, 4, 5, 6 and 7')
       if (stt /= 'N' .or. do_she /= 'N' .or. do_sot /= 'N') &
          call uasd_unsupported('spin-transfer / spin-Hall / spin-orbit torques')
       if (Nred /= Natom) call uasd_unsupported('a reduced list of updated moments')
@@ -378,7 +378,7 @@ contains
 
    !----------------------------------------------------------------------------
    !> evolve_first with the reference's argument list, SDEalgh 1 (midpoint),
-   !> 4 (Heun) and 5 (Depondt).  thermal_field is returned as zero (the thermal
+   !> 4 (Heun), 5 (Depondt), 6 (SIB) and 7 (spherical midpoint).  thermal_field is returned as zero (the thermal
    !> field is drawn and applied inside the kernel and not stored).  Heun
    !> starts from emom2, as heun4p does, so emom2 is read as well.
    !----------------------------------------------------------------------------
@@ -437,7 +437,7 @@ contains
    end subroutine uasd_evolve_first
 
    !----------------------------------------------------------------------------
-   !> evolve_second with the reference's argument list (SDEalgh 1, 4 and 5).
+   !> evolve_second with the reference's argument list (SDEalgh 1, 4-7).
    !----------------------------------------------------------------------------
    subroutine uasd_evolve_second(Natom,Mensemble,Landeg,llg,SDEalgh,bn,lambda1_array,    &
          delta_t,relaxtime,beff,beff2,b2eff,btorque,emom,emom2,stt,nlist,nlistsize,    &

@@ -32,8 +32,12 @@ METRIC = "spin-updates/s (LLG step, MC sweep); FP32+batched vs FP64 at stated er
 HBM_PEAK_GBS = 8000.0
 
 
-def kernel_bytes(W: int, s: int, ib: int = 4, cls: int = 4, nsave: int = 16) -> dict:
-    """Algorithmic HBM bytes per site of the fused midpoint kernels.
+SOLVER_NAMES = {1: "midpoint", 4: "Heun", 5: "Depondt", 6: "SIB", 7: "spherical midpoint"}
+
+
+def kernel_bytes(W: int, s: int, ib: int = 4, cls: int = 4, nsave: int = 16,
+                 b2: bool = False) -> dict:
+    """Algorithmic HBM bytes per site of the fused LLG kernels.
 
     Spins are stored as quads {x, y, z, mmom} of s-byte reals; neighbour
     entries take ib bytes (4 for int32 indices, 2 for int16 differences, 0 for
@@ -46,10 +50,13 @@ def kernel_bytes(W: int, s: int, ib: int = 4, cls: int = 4, nsave: int = 16) -> 
     corrector: gather p_j quads (4s), own e (4s), index ib*W, class, write e (4s),
                read deviates
     Each site's own spin is one of its neighbours' gathers, so the gathered
-    quads count once per site (perfect reuse through L2 / MALL).
+    quads count once per site (perfect reuse through L2 / MALL).  b2: the
+    solver keeps a field quad between the halves (Depondt's bdup, Heun's
+    b2eff), written by the predictor and read by the corrector (4s each).
     """
-    pred = 4 * s + ib * W + cls + 4 * s + nsave
-    corr = 4 * s + 4 * s + ib * W + cls + 4 * s + nsave
+    q = 4 * s if b2 else 0
+    pred = 4 * s + ib * W + cls + 4 * s + nsave + q
+    corr = 4 * s + 4 * s + ib * W + cls + 4 * s + nsave + q
     return {"predictor": pred, "corrector": corr, "step": pred + corr}
 
 
@@ -367,7 +374,7 @@ def list_kernel_compare(U, sysd, args, order, e0, steps=300):
     t = ms / max(n, 1) * 1e-3
     s = 8 if args.precision == "fp64" else 4
     ib = eng.index_bytes()[0]
-    kb = kernel_bytes(W, s, ib)
+    kb = kernel_bytes(W, s, ib, b2=args.solver in (4, 5))
     ach = kb["corrector"] * N / t / 1e9
     g, b = eng.tuning(eng.TUNE_LLG)
     return {"value": N * steps / elapsed, "corrector_ms": round(t * 1e3, 4),
@@ -385,7 +392,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--L", type=int, default=64)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    ap.add_argument("--solver", type=int, choices=[1, 5], default=1)
+    ap.add_argument("--solver", type=int, choices=[1, 4, 5, 6, 7], default=1,
+                    help="SDEalgh: 1 midpoint, 4 Heun, 5 Depondt, 6 SIB, 7 spherical midpoint")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-every", type=int, default=16,
                     help="HIP events around one step in this many (events cost time of their own)")
@@ -456,7 +464,8 @@ def main():
 
     s = 8 if args.precision == "fp64" else 4
     ib = 0 if tile else eng.index_bytes()[0]     # the tile kernel reads no lists
-    kb = kernel_bytes(W, s, ib, cls=0 if tile else 4)   # one species: classes per basis
+    kb = kernel_bytes(W, s, ib, cls=0 if tile else 4,   # one species: classes per basis
+                      b2=args.solver in (4, 5))
     t_corr = ms_corr / max(n_corr, 1) * 1e-3
     t_pred = ms_pred / max(n_pred, 1) * 1e-3
     achieved = kb["corrector"] * N / t_corr / 1e9
@@ -489,7 +498,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": (f"bcc {args.L}^3 x 2 ({N} spins), 5 exchange shells ({W} neighbours), "
-                         f"{'midpoint' if args.solver == 1 else 'Depondt'} LLG, alpha 0.1, "
+                         f"{SOLVER_NAMES[args.solver]} LLG, alpha 0.1, "
                          f"dt 1e-16 s, T 300 K Philox thermal field, {args.precision}, "
                          "device-resident"),
             "spins": N, "neighbours": W, "solver": args.solver, "precision": args.precision,

@@ -48,7 +48,7 @@ CASES = {
 
 
 @pytest.mark.parametrize("case", list(CASES))
-@pytest.mark.parametrize("alg", [1, 5])
+@pytest.mark.parametrize("alg", [1, 4, 5, 6, 7])
 def test_tile_equals_list_kernels(case, alg):
     sysd = CASES[case]()
     lat = sysd["lattice"]
@@ -65,7 +65,7 @@ def test_tile_equals_list_kernels(case, alg):
                 assert np.array_equal(got, ref), (case, order, k, pipe)
 
 
-@pytest.mark.parametrize("alg", [1, 5])
+@pytest.mark.parametrize("alg", [1, 4, 5, 6, 7])
 def test_resident_workgroups_loop_over_tiles(alg):
     """A lattice with more tiles than resident workgroups (bcc 70^3 x 2,
     partial tiles at the upper edges): with the resident-workgroup launch
@@ -112,18 +112,3 @@ def test_autotune_may_pick_tile_and_keeps_results():
     eng.autotune_llg(1, steps=2)
     eng.sd_steps(1, 6)
     assert np.array_equal(eng.get_moments()[0], ref)
-
-
-@pytest.mark.parametrize("alg", [4, 6, 7])
-def test_other_solvers_keep_the_list_kernels_under_a_tile_tuning(alg):
-    """The tile kernels carry the midpoint and Depondt solvers only: with a
-    tile shape selected, Heun (SDEalgh 4), SIB (6) and the spherical midpoint
-    (7) run on the list kernels and give the list result bit for bit, in
-    both site orders."""
-    sysd = CASES["kagome"]()
-    lat = sysd["lattice"]
-    e0 = np.ascontiguousarray(CF.random_unit(lat.natom, np.random.default_rng(91), 1))
-    ref, _ = _run(sysd, e0, 1, 0, alg)
-    for order in (1, lat.NA):
-        got, _ = _run(sysd, e0, order, 12, alg, pipe=1)
-        assert np.array_equal(got, ref), order

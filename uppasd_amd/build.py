@@ -12,6 +12,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -42,15 +43,19 @@ def _run(cmd, verbose):
 def build(verbose: bool = False) -> str:
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    # the translation units are independent: compile them side by side
+    cmds, objs = [], []
     for src, extra in UNITS:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        _run([HIPCC, *HIP_COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        cmds.append([HIPCC, *HIP_COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     setup_obj = os.path.join(objdir, "uasd_setup.o")
-    _run(["g++", "-O2", "-fPIC", "-std=c++17", "-fopenmp", "-ffp-contract=off",
-          "-c", os.path.join(CSRC, "uasd_setup.cpp"), "-o", setup_obj], verbose)
+    cmds.append(["g++", "-O2", "-fPIC", "-std=c++17", "-fopenmp", "-ffp-contract=off",
+                 "-c", os.path.join(CSRC, "uasd_setup.cpp"), "-o", setup_obj])
     objs.append(setup_obj)
+    with ThreadPoolExecutor(max_workers=len(cmds)) as pool:
+        for f in [pool.submit(_run, c, verbose) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     _run(["g++", "-shared", "-o", tmp, *objs, f"-L{ROCM}/lib", "-lamdhip64", "-fopenmp",
           f"-Wl,-rpath,{ROCM}/lib"], verbose)

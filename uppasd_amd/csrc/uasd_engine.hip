@@ -553,8 +553,7 @@ int run_llg_half(uasd_ctx *c, int alg, int half, bool fused, bool write_e,
                 *b2 = quads<R>(c, c->spins, 2);
   const uasd::Quad<R> *beff = fused ? nullptr : quads<R>(c, c->field);
   uasd::Quad<R> *out = write_e ? e : p;
-  // the tile kernels carry the midpoint and Depondt solvers; the others run on the lists
-  if (fused && c->tune[0].tile > 0 && (alg == UASD_SDE_MIDPOINT || alg == UASD_SDE_DEPONDT)) {
+  if (fused && c->tune[0].tile > 0) {
     build_stencil(c);
     if (tile_fits(c, c->tune[0].tile)) {
       uasd::TileDev t;

@@ -171,8 +171,14 @@ hipError_t launch_llg_tile(hipStream_t s, TileDev t, int pipe, int alg, int half
   static_assert(kNumTilePlanes == 4, "one case per plane size");
   UASD_TILE_PLANES(1, 0)
   UASD_TILE_PLANES(1, 1)
+  UASD_TILE_PLANES(4, 0)
+  UASD_TILE_PLANES(4, 1)
   UASD_TILE_PLANES(5, 0)
   UASD_TILE_PLANES(5, 1)
+  UASD_TILE_PLANES(6, 0)
+  UASD_TILE_PLANES(6, 1)
+  UASD_TILE_PLANES(7, 0)
+  UASD_TILE_PLANES(7, 1)
 #undef UASD_TILE_PLANES
 #undef UASD_TILE_FULL
 #undef UASD_TILE_CASE

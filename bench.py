@@ -414,7 +414,7 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
               "(the engine is single-GPU; a multi-GPU run would shard the lattice by slabs).",
               file=sys.stderr)
         sys.exit(2)

@@ -19,6 +19,7 @@ CASES = {  # case dir -> files to copy
     "kagome": ["inpsd.dat", "jfile", "dmfile", "momfile", "posfile"],
     "SCsurf": ["inpsd.dat", "jij", "dmdata", "kdata", "momfile", "posfile"],
     "HeisChain": ["inpsd.dat", "jfile", "momfile", "posfile", "startmom"],
+    "HeisChainAF": ["inpsd.dat", "jfile", "momfile", "posfile"],
     "Regression": ["inpsd.dat", "jfile", "momfile", "posfile"],
 }
 

@@ -364,3 +364,39 @@ def test_heischain_from_restart_file_on_gpu(O):
     eng.sd_steps(1, 400)
     _, en = eng.measure(with_energy=True)
     assert abs(en[0] / N - exp["Total energy"][0]) <= 1e-8
+
+
+def test_heischain_af_golden_on_gpu(O):
+    """The antiferromagnetic two-sublattice chain on the device: the random
+    start from the oracle's MT19937 stream (default tseed 1), midpoint at
+    T = 0 with damping 0.5; <M> at iteration 1000, calc_energy's total per
+    atom at 1500 (uasd_energy_terms) and the per-type averages at 5000
+    against the regression values."""
+    c = RC.read_case("HeisChainAF", "inpsd.dat")
+    lat = U.Lattice(c["cell"], c["bas"], c["atype"], c["n"], c["bc"])
+    nl, ns, nc = U.build_pair_list(lat, U.Shells.from_lines(lat, c["jlines"]), c["sym"],
+                                   c["ammom"].reshape(-1, 1))
+    latO = O.LatticeSpec(c["cell"], c["bas"], c["atype"], c["n"], c["bc"])
+    N, ncell = lat.natom, lat.natom // lat.NA
+    mm = np.tile(c["ammom"], ncell)
+    sim = RC.OracleSD(O, latO, dict(nlist=nl, nlistsize=ns, ncoup=nc), mm)
+    sim.random_init(O.mt(1), latO)
+    print(f"AF chain N={N}", flush=True)
+    eng = U.Engine(N, 1)
+    eng.set_exchange(nl, ns, nc)
+    eng.set_moments(sim.e, mm[None])
+    eng.set_llg(0.5, 0.0, 1e-16)
+    eng.set_noise(U.UASD_NOISE_NONE)
+    exp = {g["type"]: g["expected"] for g in RC.golden("HeisChainAF")}
+    eng.sd_steps(1, 1000)
+    ms, _ = eng.measure()
+    m = ms[0] / N
+    assert np.allclose([*m, np.linalg.norm(m)], exp["Magnetization"], rtol=0, atol=1e-8)
+    eng.sd_steps(1, 500)
+    xc, ani, dm, ze = eng.energy_terms()[0]
+    assert abs((xc + dm + ze + ani) - exp["Total energy"][0]) <= 1e-8
+    eng.sd_steps(1, 3500)
+    _, eM = eng.get_moments()
+    for k in (1, 2):
+        v = eM[0].reshape(ncell, lat.NA, 3)[:, k - 1].sum(axis=0) / ncell
+        assert np.allclose([np.linalg.norm(v), *v], exp[f"Sublattice {k}"], rtol=0, atol=1e-8), k

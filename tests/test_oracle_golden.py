@@ -205,3 +205,31 @@ def test_heischain_from_restart_file_golden(O):
     for _ in range(400):
         sim.step(1, 1e-16, 0.0, 0.0, normals=zero)
     assert abs(sim.ham.field(sim.eM)[1] / N - exp["Total energy"][0]) <= 1e-8
+
+
+def test_heischain_af_golden(O):
+    """Antiferromagnetic two-sublattice chain (100 cells, |m| = 1 and 2,
+    vacuum in x and y): random start on the MT19937 stream (default tseed 1),
+    midpoint at T = 0 with damping 0.5.  <M> at iteration 1000, the total
+    energy per atom at 1500 (calc_energy's terms, here exchange only) and the
+    per-type projected averages (do_proj_avrg Y: the sum of emomM over a
+    type's sites per cell, prn_averages.f90:704-716) at 5000."""
+    from uppasd_amd import output as OUT
+    c = RC.read_case("HeisChainAF", "inpsd.dat")
+    lat = _lattice(O, c)
+    sim = RC.OracleSD(O, lat, _pairs(O, c, lat), _sites(lat, c["ammom"]))
+    sim.random_init(O.mt(1), lat)
+    exp = {g["type"]: g["expected"] for g in RC.golden("HeisChainAF")}
+    ncell = lat.natom // lat.NA
+    for it in range(1, 5001):
+        sim.step(1, 1e-16, 0.5, 0.0)
+        if it == 1000:
+            m = sim.mavg()[0]
+            assert np.allclose([*m, np.linalg.norm(m)], exp["Magnetization"], rtol=0,
+                               atol=1e-8)
+        if it == 1500:
+            t = OUT.TotEnergy.per_ensemble(sim.ham.energy_terms(sim.eM) * OUT.MUB / OUT.MRY)
+            assert abs(t[0, 0] - exp["Total energy"][0]) <= 1e-8
+    for k in (1, 2):
+        v = sim.eM[0].reshape(ncell, lat.NA, 3)[:, k - 1].sum(axis=0) / ncell
+        assert np.allclose([np.linalg.norm(v), *v], exp[f"Sublattice {k}"], rtol=0, atol=1e-8), k

@@ -20,6 +20,7 @@ CASES = {  # case dir -> files to copy
     "SCsurf": ["inpsd.dat", "jij", "dmdata", "kdata", "momfile", "posfile"],
     "HeisChain": ["inpsd.dat", "jfile", "momfile", "posfile", "startmom"],
     "HeisChainAF": ["inpsd.dat", "jfile", "momfile", "posfile"],
+    "FeCo": ["inpsd.dat", "jASD", "momfile", "posfile"],
     "Regression": ["inpsd.dat", "jfile", "momfile", "posfile"],
 }
 

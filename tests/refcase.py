@@ -26,6 +26,16 @@ def golden(case):
         return json.load(fh)[case]
 
 
+# the reference harness's comparison functions (tests/bergtest.py:42-73)
+def matches(got, expected, comparison="similar"):
+    d = abs(float(got) - float(expected))
+    if comparison == "sloppy":
+        return d <= 2e-2 and d / abs(float(expected)) < 2e-2
+    if comparison == "almost":
+        return d <= 5e-4
+    return d <= 1e-8
+
+
 def _rows(path):
     out = []
     with open(path) as fh:

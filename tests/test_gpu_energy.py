@@ -123,3 +123,29 @@ def test_regression_case_files_on_gpu(tmp_path):
                                                                        g["expected"])
         checked += 1
     assert checked == 3
+
+
+def test_feco_case_files_on_gpu(tmp_path):
+    """FeCo B2 on the device (maptype-2 exchange read by the product's
+    reader, 255 neighbours, T = 0 midpoint, damping 0.001, 5000 steps)
+    through MeasurementPhase: the averages and cumulant rows against the
+    reference's values with its own comparison functions, the cumulant row
+    also to 1e-8."""
+    from test_output import check_feco_files
+    c = RC.read_case("FeCo", "inpsd.dat")
+    lat = U.Lattice(c["cell"], c["bas"], c["atype"], c["n"], c["bc"])
+    raw = [(int(r[0]), int(r[1]), 1, 1, *[float(x) for x in r[2:6]])
+           for r in RC._rows(RC.os.path.join(RC.CASES, "FeCo", "jASD"))]
+    nl, ns, nc = U.build_pair_list(lat, U.Shells.from_lines(lat, raw, maptype=2), c["sym"],
+                                   c["ammom"].reshape(-1, 1))
+    N = lat.natom
+    print(f"FeCo B2 N={N} steps=5000", flush=True)
+    eng = U.Engine(N, 1)
+    eng.set_exchange(nl, ns, nc)
+    eng.set_moments(np.tile(c["edir"], (N // lat.NA, 1))[None], np.tile(c["ammom"], N // lat.NA))
+    eng.set_llg(0.001, 0.0, 1e-16)
+    eng.set_noise(U.UASD_NOISE_NONE)
+    ph = OUT.MeasurementPhase(eng, "test", 0.0, avrg_step=100, cumu_step=20,
+                              outdir=str(tmp_path), cumu_buff=10)
+    ph.run(5000, lambda n: eng.sd_steps(1, n))
+    check_feco_files(str(tmp_path))

@@ -165,3 +165,46 @@ def test_totenergy_mean_and_deviation():
     assert means[4:8] == [0.0] * 4 and means[9:] == [0.0] * 4
     m1, d1 = te.row(terms[:1])
     assert d1 == [0.0] * 13 and m1[1] == -2.0
+
+
+def feco_case(O):
+    """FeCo B2 (6^3 cells, two types, 255 neighbours from a maptype-2 jfile,
+    moments 2.72 / 1.72 canted by +-45 degrees from the momfile), midpoint at
+    T = 0 with damping 0.001."""
+    c = RC.read_case("FeCo", "inpsd.dat")
+    lat = O.LatticeSpec(c["cell"], c["bas"], c["atype"], c["n"], c["bc"])
+    pairs = O.build_pairs(lat, c["jlines"], c["sym"], ammom=c["ammom"].reshape(-1, 1))
+    sim = RC.OracleSD(O, lat, pairs, np.tile(c["ammom"], lat.natom // lat.NA))
+    sim.set_state(np.tile(c["edir"], (lat.natom // lat.NA, 1))[None])
+    return c, lat, sim
+
+
+def check_feco_files(outdir):
+    """Every FeCo golden entry with the comparison function the reference's
+    harness applies to it (sloppy for both), and the cumulant row, which
+    agrees to every printed digit, also to 1e-8."""
+    checked = 0
+    for g in RC.golden("FeCo"):
+        kind = "averages" if g["outfile"].startswith("averages") else "cumulants"
+        path = os.path.join(outdir, f"{kind}.test.out")
+        got = read_selected(path, g["headers"], g["select"], g["extract"])
+        assert all(RC.matches(a, b, g["comparison"]) for a, b in zip(got, g["expected"])), \
+            (g["type"], got, g["expected"])
+        if kind == "cumulants":
+            assert np.allclose(got, g["expected"], rtol=0, atol=1e-8), (got, g["expected"])
+        checked += 1
+    assert checked == 2
+
+
+def test_feco_files_match_golden(O, tmp_path):
+    c, lat, sim = feco_case(O)
+    run = _OracleFieldRun(O, sim, None)
+
+    def advance(n):
+        for _ in range(n):
+            sim.step(1, 1e-16, 0.001, 0.0)
+
+    ph = OUT.MeasurementPhase(run, "test", 0.0, avrg_step=100, cumu_step=20,
+                              outdir=str(tmp_path), cumu_buff=10)
+    ph.run(5000, advance)
+    check_feco_files(str(tmp_path))

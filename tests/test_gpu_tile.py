@@ -81,12 +81,13 @@ def test_resident_workgroups_loop_over_tiles(alg):
         assert np.array_equal(got, ref), k
 
 
-def test_tile_fp32_equals_list_fp32():
+@pytest.mark.parametrize("alg", [1, 4, 5, 6, 7])
+def test_tile_fp32_equals_list_fp32(alg):
     sysd = CF.triangular_dmi(24)
     rng = np.random.default_rng(91)
     e0 = np.ascontiguousarray(CF.random_unit(sysd["lattice"].natom, rng, 1))
-    ref, _ = _run(sysd, e0, 1, 0, 5, "fp32")
-    got, _ = _run(sysd, e0, 1, 5, 5, "fp32")
+    ref, _ = _run(sysd, e0, 1, 0, alg, "fp32")
+    got, _ = _run(sysd, e0, 1, 5, alg, "fp32")
     assert np.array_equal(got, ref)
 
 

@@ -73,3 +73,33 @@ def test_damping_relaxes_towards_the_field(O, alg):
         sim.step(alg, 1e-15, 0.5, 0.0, ext=ext)
         ez.append(sim.e[0, 0, 2])
     assert np.all(np.diff(ez) > 0) and ez[-1] > 0.5
+
+
+@pytest.mark.parametrize("alg", [1, 6, 7])
+def test_free_spins_reach_boltzmann_equilibrium(O, alg):
+    """Thermal noise of SIB and the spherical midpoint (the rannum-scaled
+    fluctuations in a1 and s1): free spins of 2 muB in 20 T along z at 15 K
+    relax to the Boltzmann distribution, <cos theta> = L(x) = coth x - 1/x
+    with x = m B / (kB T) = 1.79 (L = 0.50).  A noise amplitude off by sqrt(2)
+    would give L(x/2) = 0.28 or L(2x) = 0.72.  The midpoint solver, whose
+    noise the bccFe golden run pins, is the control."""
+    rng = np.random.default_rng(alg)
+    N, m, B, T = 4000, 2.0, 20.0, 15.0
+    sim = _free_spins(O, N)          # moments of 2 muB
+    e0 = rng.normal(size=(1, N, 3))
+    e0 /= np.linalg.norm(e0, axis=2, keepdims=True)
+    sim.set_state(e0)
+    ext = np.zeros_like(e0)
+    ext[..., 2] = B
+    samples = []
+    for it in range(3000):
+        sim.step(alg, 1e-15, 1.0, T, normals=rng.normal(size=(1, N, 3)), ext=ext)
+        if it >= 1000 and it % 20 == 0:
+            samples.append(sim.e[0, :, 2].mean())
+    mub = ctypes.c_double.in_dll(O.lib(), "OR_MUB").value
+    kb = ctypes.c_double.in_dll(O.lib(), "OR_K_BOLT").value
+    x = m * mub * B / (kb * T)
+    want = 1.0 / np.tanh(x) - 1.0 / x
+    got = float(np.mean(samples))
+    print(f"alg {alg}: <cos theta> {got:.4f}, L(x) {want:.4f}")
+    assert abs(got - want) < 0.02, (got, want)

@@ -103,3 +103,31 @@ def test_free_spins_reach_boltzmann_equilibrium(O, alg):
     got = float(np.mean(samples))
     print(f"alg {alg}: <cos theta> {got:.4f}, L(x) {want:.4f}")
     assert abs(got - want) < 0.02, (got, want)
+
+
+def test_heun_noise_gives_the_reference_effective_temperature(O):
+    """Heun (heun_proper.f90): the drift carries dt/(1+lam^2), the noise
+    sqrt(2 D dt) dW with D = lam/(1+lam^2) kB T/(muB m) does not, so the
+    reference's scheme samples an effective temperature (1+lam^2) T.  Free
+    spins of 2 muB in 20 T at 15 K with lam = 0.3 then give
+    <cos theta> = L(x/1.09) = 0.469, not the Boltzmann 0.499; a noise
+    amplitude off by sqrt(2) would give 0.27 or 0.70."""
+    rng = np.random.default_rng(4)
+    N, m, B, T, lam = 4000, 2.0, 20.0, 15.0, 0.3
+    sim = _free_spins(O, N)
+    e0 = rng.normal(size=(1, N, 3))
+    e0 /= np.linalg.norm(e0, axis=2, keepdims=True)
+    sim.set_state(e0)
+    ext = np.zeros_like(e0)
+    ext[..., 2] = B
+    samples = []
+    for it in range(8000):
+        sim.step(4, 1e-15, lam, T, normals=rng.normal(size=(1, N, 3)), ext=ext)
+        if it >= 3000 and it % 20 == 0:
+            samples.append(sim.e[0, :, 2].mean())
+    mub = ctypes.c_double.in_dll(O.lib(), "OR_MUB").value
+    kb = ctypes.c_double.in_dll(O.lib(), "OR_K_BOLT").value
+    x = m * mub * B / (kb * T * (1.0 + lam * lam))
+    want = 1.0 / np.tanh(x) - 1.0 / x
+    got = float(np.mean(samples))
+    assert abs(got - want) < 0.01, (got, want)

@@ -149,3 +149,30 @@ def test_feco_case_files_on_gpu(tmp_path):
                               outdir=str(tmp_path), cumu_buff=10)
     ph.run(5000, lambda n: eng.sd_steps(1, n))
     check_feco_files(str(tmp_path))
+
+
+def test_heischain_af_files_on_gpu(O, tmp_path):
+    """The antiferromagnetic chain on the device through MeasurementPhase
+    with the projected-averages writer: averages, projavgs (both types) and
+    totenergy against the reference's values to 1e-8."""
+    from test_output import check_af_files
+    c = RC.read_case("HeisChainAF", "inpsd.dat")
+    lat = U.Lattice(c["cell"], c["bas"], c["atype"], c["n"], c["bc"])
+    nl, ns, nc = U.build_pair_list(lat, U.Shells.from_lines(lat, c["jlines"]), c["sym"],
+                                   c["ammom"].reshape(-1, 1))
+    latO = O.LatticeSpec(c["cell"], c["bas"], c["atype"], c["n"], c["bc"])
+    N, ncell = lat.natom, lat.natom // lat.NA
+    mm = np.tile(c["ammom"], ncell)
+    sim = RC.OracleSD(O, latO, dict(nlist=nl, nlistsize=ns, ncoup=nc), mm)
+    sim.random_init(O.mt(1), latO)          # the reference's random start
+    print(f"AF chain files N={N}", flush=True)
+    eng = U.Engine(N, 1)
+    eng.set_exchange(nl, ns, nc)
+    eng.set_moments(sim.e, mm[None])
+    eng.set_llg(0.5, 0.0, 1e-16)
+    eng.set_noise(U.UASD_NOISE_NONE)
+    proj = OUT.ProjAverages("AF_WireT", c["atype"], ncell, "Y", str(tmp_path))
+    ph = OUT.MeasurementPhase(eng, "AF_WireT", 0.0, avrg_step=100, cumu_step=50,
+                              outdir=str(tmp_path), proj=proj)
+    ph.run(5000, lambda n: eng.sd_steps(1, n))
+    check_af_files(str(tmp_path))

@@ -18,17 +18,16 @@ from uppasd_amd import output as OUT
 
 
 def read_selected(path, headers, select, extract):
-    """The row whose selected column equals the value; columns by position."""
-    (key, val), = select.items()
-    col = headers.index(key)
+    """The row whose selected columns equal the values; columns by position."""
+    cols = [(headers.index(k), float(v)) for k, v in select.items()]
     with open(path) as fh:
         for line in fh:
             t = line.split()
             if not t or t[0].startswith("#"):
                 continue
-            if float(t[col]) == float(val):
+            if all(float(t[c]) == v for c, v in cols):
                 return [float(t[headers.index(x)]) for x in extract]
-    raise KeyError(f"{key} = {val} not in {path}")
+    raise KeyError(f"{select} not in {path}")
 
 
 def test_fortran_edit_descriptors():
@@ -208,3 +207,40 @@ def test_feco_files_match_golden(O, tmp_path):
                               outdir=str(tmp_path), cumu_buff=10)
     ph.run(5000, advance)
     check_feco_files(str(tmp_path))
+
+
+def check_af_files(outdir):
+    """Every HeisChainAF golden entry (averages, both per-type rows of
+    projavgs, totenergy) read from the written files, to 1e-8."""
+    checked = 0
+    for g in RC.golden("HeisChainAF"):
+        got = read_selected(os.path.join(outdir, g["outfile"].lstrip("./")), g["headers"],
+                            g["select"], g["extract"])
+        assert np.allclose(got, g["expected"], rtol=0, atol=1e-8), (g["type"], got, g["expected"])
+        checked += 1
+    assert checked == 4
+
+
+def test_heischain_af_files_match_golden(O, tmp_path):
+    """The antiferromagnetic chain through MeasurementPhase with the
+    projected-averages writer (do_proj_avrg Y): averages, projavgs and
+    totenergy read back as the reference's harness reads them."""
+    c = RC.read_case("HeisChainAF", "inpsd.dat")
+    lat = O.LatticeSpec(c["cell"], c["bas"], c["atype"], c["n"], c["bc"])
+    pairs = O.build_pairs(lat, c["jlines"], c["sym"], ammom=c["ammom"].reshape(-1, 1))
+    sim = RC.OracleSD(O, lat, pairs, np.tile(c["ammom"], lat.natom // lat.NA))
+    sim.random_init(O.mt(1), lat)
+    run = _OracleFieldRun(O, sim, None)
+    run.get_moments = lambda: (sim.e.copy(), sim.eM.copy())
+
+    def advance(n):
+        for _ in range(n):
+            sim.step(1, 1e-16, 0.5, 0.0)
+
+    proj = OUT.ProjAverages("AF_WireT", c["atype"], lat.natom // lat.NA, "Y", str(tmp_path))
+    ph = OUT.MeasurementPhase(run, "AF_WireT", 0.0, avrg_step=100, cumu_step=50,
+                              outdir=str(tmp_path), proj=proj)
+    ph.run(5000, advance)
+    check_af_files(str(tmp_path))
+    head = open(tmp_path / "projavgs.AF_WireT.out").readline()
+    assert head.split() == ["#IterProj", "<M>", "M_{stdv}", "<M>_x", "<M>_y", "<M>_z"]

@@ -5,6 +5,7 @@
   cumulants.<simid>.json
   totenergy.<simid>.out   calc_energy (energy.f90:353-455)
   stdenergy.<simid>.out
+  projavgs.<simid>.out    prn_proj_avrg (prn_averages.f90:660-800)
 
 Columns, Fortran edit descriptors and the running means follow the reference,
 so the reference's own comparisons (fixed column positions) read these files
@@ -233,6 +234,57 @@ class TotEnergy:
         return means, devs
 
 
+class ProjAverages:
+    """prn_proj_avrg (prn_averages.f90:660-800) over the per-basis-atom sums
+    of emomM that calc_mavrg_proj buffers (prn_averages.f90:480-512).
+
+    mode 'Y' projects on atom types (atype (NA,) 1-based), 'A' on basis atoms.
+    Per ensemble the projection's average moment per cell is the sum, in
+    basis-atom order, of each basis atom's sum over the cells divided by the
+    number of cells; the row holds the ensemble means of its norm and
+    components and the ensemble deviation of the norm."""
+
+    def __init__(self, simid: str, atype, ncell: int, mode: str = "Y", outdir: str = "."):
+        if mode not in ("Y", "A"):
+            raise ValueError("mode must be 'Y' (types) or 'A' (basis atoms)")
+        self.path = os.path.join(outdir, f"projavgs.{simid.strip()}.out")
+        self.atype = np.asarray(atype, dtype=np.int64)
+        self.NA, self.ncell, self.mode = self.atype.shape[0], int(ncell), mode
+
+    def rows(self, emomM: np.ndarray):
+        """emomM (M, N, 3) in reference site order -> [(k, <M>, stdv, x, y, z)]."""
+        e = np.asarray(emomM, dtype=np.float64)
+        M = e.shape[0]
+        avg = e.reshape(M, self.ncell, self.NA, 3).sum(axis=1)      # (M, NA, 3)
+        if self.mode == "Y":
+            projs = [[b for b in range(self.NA) if self.atype[b] == k]
+                     for k in range(1, int(self.atype.max()) + 1)]
+        else:
+            projs = [[b] for b in range(self.NA)]
+        out = []
+        for k, members in enumerate(projs, 1):
+            v = np.zeros((M, 3))
+            for b in members:
+                v = v + avg[:, b] / self.ncell
+            m = np.sqrt(np.sum(v * v, axis=1))
+            mm = float(np.sum(m)) / M
+            ms = float(np.sum(m * m)) / M - mm * mm
+            ms = 0.0 if ms < 0 else float(np.sqrt(ms))
+            xm, ym, zm = (float(np.sum(v[:, c])) / M for c in range(3))
+            out.append((k, mm, ms, xm, ym, zm))
+        return out
+
+    def write(self, it: int, emomM: np.ndarray):
+        rows = self.rows(emomM)
+        with open(self.path, "a") as fh:
+            if it == 0:
+                fh.write(fmt_a("#Iter", 8) + "Proj" + "".join(fmt_a(h, 16) for h in
+                         ("<M>", "M_{stdv}", "<M>_x", "<M>_y", "<M>_z")) + "\n")
+            for k, *vals in rows:
+                fh.write(fmt_i(it) + fmt_i(k) + "".join(fmt_es(v) for v in vals) + "\n")
+        return rows
+
+
 class MeasurementPhase:
     """Resident measurement phase with the reference's cadence.
 
@@ -244,7 +296,7 @@ class MeasurementPhase:
 
     def __init__(self, eng, simid: str, temp: float, avrg_step: int = 100,
                  cumu_step: int = 50, plotenergy: bool = True, outdir: str = ".",
-                 cumu_buff: int = 1, final_measure: bool = True):
+                 cumu_buff: int = 1, final_measure: bool = True, proj=None):
         self.eng, self.temp = eng, float(temp)
         self.avrg_step, self.cumu_step, self.plotenergy = avrg_step, cumu_step, plotenergy
         self.avg = Averages(simid, outdir)
@@ -252,6 +304,7 @@ class MeasurementPhase:
         self.tot = TotEnergy(simid, outdir) if plotenergy else None
         self.energy = self.exchange = None
         self.final_measure = final_measure
+        self.proj = proj    # ProjAverages, written with the averages (engine's get_moments)
 
     def run(self, nstep: int, advance):
         """advance(n) moves the state n steps; iterations 1..nstep."""
@@ -260,6 +313,8 @@ class MeasurementPhase:
             if (mstep - 1) % self.avrg_step == 0:
                 ms, _ = self.eng.measure()
                 self.avg.write(mstep - 1, ms, self.eng.N)
+                if self.proj is not None:
+                    self.proj.write(mstep - 1, self.eng.get_moments()[1])
             if mstep % self.cumu_step == 0:
                 ms, _ = self.eng.measure()
                 self.cum.sample(ms, self.energy, exchange_per_atom=self.exchange)
@@ -284,6 +339,8 @@ class MeasurementPhase:
         if (mstep - 1) % self.avrg_step == 0:
             ms, _ = self.eng.measure()
             self.avg.write(mstep - 1, ms, self.eng.N)
+            if self.proj is not None:
+                self.proj.write(mstep - 1, self.eng.get_moments()[1])
         if mstep % self.cumu_step == 0:
             ms, _ = self.eng.measure()
             self.cum.sample(ms, self.energy, exchange_per_atom=self.exchange)

@@ -244,3 +244,27 @@ def test_heischain_af_files_match_golden(O, tmp_path):
     check_af_files(str(tmp_path))
     head = open(tmp_path / "projavgs.AF_WireT.out").readline()
     assert head.split() == ["#IterProj", "<M>", "M_{stdv}", "<M>_x", "<M>_y", "<M>_z"]
+
+
+def test_proj_averages_types_are_sums_of_basis_atoms():
+    """do_proj_avrg Y adds the per-cell averages of the basis atoms of one
+    type (prn_averages.f90:704-716), A keeps one row per basis atom: with one
+    ensemble each Y vector is the sum of its atoms' A vectors, and the norms
+    and a zero deviation follow."""
+    rng = np.random.default_rng(8)
+    atype, ncell = [1, 2, 1], 50
+    eM = rng.normal(size=(1, ncell * 3, 3))
+    ya = OUT.ProjAverages("x", atype, ncell, "A").rows(eM)
+    yy = OUT.ProjAverages("x", atype, ncell, "Y").rows(eM)
+    assert [r[0] for r in ya] == [1, 2, 3] and [r[0] for r in yy] == [1, 2]
+    va = {r[0]: np.array(r[3:]) for r in ya}
+    assert np.allclose(yy[0][3:], va[1] + va[3], rtol=1e-14)
+    assert np.allclose(yy[1][3:], va[2], rtol=1e-14)
+    for r in yy:
+        assert np.isclose(r[1], np.linalg.norm(r[3:]), rtol=1e-14) and r[2] == 0.0
+    try:
+        OUT.ProjAverages("x", atype, ncell, "C")
+    except ValueError:
+        pass
+    else:
+        raise AssertionError("mode C accepted")

@@ -172,6 +172,12 @@ def _declare(L):
                                   C.c_char, C.POINTER(MT), C.POINTER(Zig), ip, dp, dp, dp, dp]
     L.or_mc_sweep_colored.argtypes = [C.POINTER(Ham), C.c_int, C.c_int, dp, C.c_char,
                                       C.c_uint64, C.c_uint64, C.c_int, ip, ip, dp, dp, dp, dp]
+    L.or_mc_sweep_colored_ex.argtypes = [C.POINTER(Ham), C.c_int, C.c_int, dp, C.c_char,
+                                         C.c_uint64, C.c_uint64, C.c_int, ip, ip, dp, dp, dp,
+                                         dp, C.c_int]
+    L.or_trial_move_f32.argtypes = [dp, C.c_double, dp, C.c_uint64, C.c_uint64, C.c_uint32,
+                                    C.c_uint32, dp]
+    L.or_thermal3_f64.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, dp]
     L.or_calc_mavrg.argtypes = [C.c_int, C.c_int, dp]
     L.or_calc_mavrg.restype = C.c_double
     L.or_sum_moments.argtypes = [C.c_int, C.c_int, dp, dp]
@@ -365,6 +371,13 @@ def thermal3(seed, tag, step, site, ens):
     """The LLG thermal deviates of (site, ens, step): three FP32 values."""
     out = np.zeros(3, np.float32)
     lib().or_thermal3(seed, tag, step, site, ens, out)
+    return out
+
+
+def thermal3_f64(seed, step, site, ens):
+    """The FP64 engine's LLG thermal deviates of (site, ens, step)."""
+    out = np.zeros(3)
+    lib().or_thermal3_f64(seed, step, site, ens, out)
     return out
 
 

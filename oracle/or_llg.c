@@ -439,9 +439,10 @@ void or_sib_second(int alg, int N, int M, const double *landeg, double bn, const
 }
 
 /* The sd_mphase step body (sd_driver.f90:668-765) for nsteps, with the
- * thermal field drawn from the Philox counter stream the HIP kernels use
- * (noise = 1) or absent (noise = 0).  Used as the CPU baseline and for long
- * trajectory checks. */
+ * thermal field drawn from the Philox counter stream the HIP kernels use --
+ * the FP32 engine's single-precision deviates (noise = 1) or the FP64
+ * engine's double-precision ones (noise = 2) -- or absent (noise = 0).  Used
+ * as the CPU baseline and for long trajectory checks. */
 void or_sd_run(const or_ham *h, int N, int M, int alg, long nsteps, uint64_t step0, double dt,
                const double *lam, const double *temp, const double *landeg,
                const double *mmom, double *emom, double *emomM, uint64_t seed, int noise,
@@ -457,8 +458,14 @@ void or_sd_run(const or_ham *h, int N, int M, int alg, long nsteps, uint64_t ste
     for (int k = 0; k < M; ++k)
       for (int i = 0; i < N; ++i) {
         float z[3] = {0.0f, 0.0f, 0.0f};
-        if (noise == 1) or_thermal3(seed, 0x11u, step, (uint32_t)i, (uint32_t)k, z);
-        for (int c = 0; c < 3; ++c) V3(rv, c, i, k, N) = (double)z[c];
+        double zd[3] = {0.0, 0.0, 0.0};
+        if (noise == 1) {
+          or_thermal3(seed, 0x11u, step, (uint32_t)i, (uint32_t)k, z);
+          for (int c = 0; c < 3; ++c) zd[c] = (double)z[c];
+        } else if (noise == 2) {
+          or_thermal3_f64(seed, step, (uint32_t)i, (uint32_t)k, zd);
+        }
+        for (int c = 0; c < 3; ++c) V3(rv, c, i, k, N) = zd[c];
       }
     or_effective_field(h, N, M, emomM, ext, beff, NULL, NULL, NULL);
     if (alg == 1) {

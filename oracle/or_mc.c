@@ -210,10 +210,57 @@ void or_mc_sweep_ref(const or_ham *h, int N, int M, double temperature,
   free(fm); free(fg); free(newm); free(mflip); free(fa);
 }
 
+/* The trial move of the single-precision MC engine: choose_random_flip's
+ * three moves (montecarlo_common.f90:25-79) drawn in single precision from
+ * the same uniforms -- a uniform direction from z = 1 - 2 u2 and
+ * sin(theta) = sqrt(1 - z^2) with the polynomial sincos of the azimuth, the
+ * Gaussian step from the single-precision Box-Muller deviates, the flip --
+ * widened to double. */
+void or_trial_move_f32(const double e[3], double delta, const double u[4], uint64_t seed,
+                       uint64_t sweep, uint32_t site, uint32_t ens, double out[3])
+{
+  const int ftype = (int)floor(3 * u[0]);
+  if (ftype == 0) {
+    const float v1 = (float)u[1], v2 = (float)u[2];
+    const float z = 1.0f - 2.0f * v2;
+    const float st = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    float sc[2];
+    or_bm_sincos(4.0f * v1, sc);
+    out[0] = (double)(st * sc[1]);
+    out[1] = (double)(st * sc[0]);
+    out[2] = (double)z;
+  } else if (ftype == 1) {
+    float g[3];
+    or_thermal3(seed, TAG_MC_G, sweep, site, ens, g);
+    const float dl = (float)delta;
+    const float a0 = (float)e[0] + g[0] * dl, a1 = (float)e[1] + g[1] * dl,
+                a2 = (float)e[2] + g[2] * dl;
+    const float il = 1.0f / sqrtf(a0 * a0 + a1 * a1 + a2 * a2);
+    out[0] = (double)(a0 * il);
+    out[1] = (double)(a1 * il);
+    out[2] = (double)(a2 * il);
+  } else {
+    out[0] = (double)(0.0f - (float)e[0]);
+    out[1] = (double)(0.0f - (float)e[1]);
+    out[2] = (double)(0.0f - (float)e[2]);
+  }
+}
+
 void or_mc_sweep_colored(const or_ham *h, int N, int M, const double *temps, char mode,
                          uint64_t seed, uint64_t sweep, int ncolors, const int *color_ptr,
                          const int *color_sites, double *emomM, double *emom,
                          const double *mmom, const double extfield[3])
+{
+  or_mc_sweep_colored_ex(h, N, M, temps, mode, seed, sweep, ncolors, color_ptr, color_sites,
+                         emomM, emom, mmom, extfield, 0);
+}
+
+/* f32_moves = 1: Metropolis trial moves as the single-precision engine draws
+ * them (or_trial_move_f32); energies and acceptance stay in double. */
+void or_mc_sweep_colored_ex(const or_ham *h, int N, int M, const double *temps, char mode,
+                            uint64_t seed, uint64_t sweep, int ncolors, const int *color_ptr,
+                            const int *color_sites, double *emomM, double *emom,
+                            const double *mmom, const double extfield[3], int f32_moves)
 {
   for (int col = 0; col < ncolors; ++col)
     for (int p = color_ptr[col]; p < color_ptr[col + 1]; ++p) {
@@ -232,11 +279,17 @@ void or_mc_sweep_colored(const or_ham *h, int N, int M, const double *temps, cha
             V3(emomM, c, i, k, N) = m * e[c];
           }
         } else {
-          double g[3] = {0.0, 0.0, 0.0};
-          if ((int)floor(3 * u[0]) == 1) or_noise3_f64(seed, TAG_MC_G, sweep, (uint32_t)i, (uint32_t)k, g);
           double e[3] = {V3(emom, 0, i, k, N), V3(emom, 1, i, k, N), V3(emom, 2, i, k, N)};
-          double rn[3] = {u[0], u[1], u[2]}, nm[3];
-          or_choose_random_flip(e, or_mc_delta(T), rn, g, nm);
+          double nm[3];
+          if (f32_moves) {
+            or_trial_move_f32(e, or_mc_delta(T), u, seed, sweep, (uint32_t)i, (uint32_t)k, nm);
+          } else {
+            double g[3] = {0.0, 0.0, 0.0};
+            if ((int)floor(3 * u[0]) == 1)
+              or_noise3_f64(seed, TAG_MC_G, sweep, (uint32_t)i, (uint32_t)k, g);
+            double rn[3] = {u[0], u[1], u[2]};
+            or_choose_random_flip(e, or_mc_delta(T), rn, g, nm);
+          }
           double de = or_mc_energy_change(h, N, emomM, emom, mmom, i, k, nm, extfield);
           metropolis_accept(N, emom, emomM, mmom, i, k, nm, de, T, 1.0, u[3]);
         }

@@ -201,8 +201,9 @@ void or_noise3_f64(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
  * kernels): Box-Muller in single precision with polynomial ln, sin and cos,
  * as the kernels compute it, operation for operation (IEEE adds, multiplies,
  * fmaf, sqrtf, rintf; no contraction), so the deviates agree bit for bit.
- * The reference's own deviates are single precision too (r4_nor,
- * randomnumbers.f90:330-438).  Polynomials fitted by least squares on
+ * (The reference's ziggurat deviates are double precision, r4_nor,
+ * randomnumbers.f90:379,389; the FP64 engine draws or_thermal3_f64.)
+ * Polynomials fitted by least squares on
  * Chebyshev nodes (Lawson reweighting):
  *   ln(u):  u = m 2^e, m in [sqrt(1/2), sqrt(2)], x = m - 1,
  *           ln(1+x) = x + x^2 (-1/2 + x P(x))
@@ -274,6 +275,29 @@ void or_thermal3(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site, uint
   out[0] = r0 * c0;
   out[1] = r0 * s0;
   out[2] = r1 * c1;
+}
+
+/* Thermal-field deviates of the FP64 engine: Box-Muller in double precision
+ * on 52-bit uniforms u = (x + 1/2) 2^-52, x the top 52 bits of a pair of
+ * Philox words -- block 0x11 gives the radii (words 0,1 and 2,3), block 0x12
+ * the angles, 2 pi u.  The kernels take the angles by sincospi / cospi;
+ * libm's cos / sin of 2 pi u here agree with them to a few ulps. */
+static double u52(uint32_t hi, uint32_t lo)
+{
+  const uint64_t x = ((uint64_t)hi << 20) | (uint64_t)(lo >> 12);
+  return ((double)x + 0.5) * 2.220446049250313080847e-16;
+}
+
+void or_thermal3_f64(uint64_t seed, uint64_t step, uint32_t site, uint32_t ens, double out[3])
+{
+  uint32_t w[4], x[4];
+  counter_words(seed, 0x11u, step, site, ens, w);
+  counter_words(seed, 0x12u, step, site, ens, x);
+  const double r0 = sqrt(-2.0 * log(u52(w[0], w[1]))), r1 = sqrt(-2.0 * log(u52(w[2], w[3])));
+  const double t0 = 2.0 * OR_PI * u52(x[0], x[1]), t1 = 2.0 * OR_PI * u52(x[2], x[3]);
+  out[0] = r0 * cos(t0);
+  out[1] = r0 * sin(t0);
+  out[2] = r1 * cos(t1);
 }
 
 void or_uniform4(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,

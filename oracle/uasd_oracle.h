@@ -51,6 +51,8 @@ void   or_noise3_f64(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
                      uint32_t ens, double out[3]);
 void   or_thermal3(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
                    uint32_t ens, float out[3]);        /* LLG thermal deviates */
+void   or_thermal3_f64(uint64_t seed, uint64_t step, uint32_t site, uint32_t ens,
+                       double out[3]);                 /* FP64 engine's deviates */
 float  or_bm_ln(float u);
 void   or_bm_sincos(float v, float out[2]);           /* sin, cos of (pi/2) v */
 void   or_uniform4(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
@@ -203,6 +205,12 @@ void   or_mc_sweep_colored(const or_ham *h, int N, int M, const double *temps,
                            const int *color_ptr, const int *color_sites,
                            double *emomM, double *emom, const double *mmom,
                            const double extfield[3]);
+void   or_mc_sweep_colored_ex(const or_ham *h, int N, int M, const double *temps, char mode,
+                              uint64_t seed, uint64_t sweep, int ncolors, const int *color_ptr,
+                              const int *color_sites, double *emomM, double *emom,
+                              const double *mmom, const double extfield[3], int f32_moves);
+void   or_trial_move_f32(const double e[3], double delta, const double u[4], uint64_t seed,
+                         uint64_t sweep, uint32_t site, uint32_t ens, double out[3]);
 
 /* bulk fills in the reference's stream order (rng_uniformP / fill_rngarray) */
 void   or_mt_fill_real2(or_mt *s, double *out, long n);

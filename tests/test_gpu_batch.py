@@ -81,7 +81,9 @@ def test_segment_sums_match_separate_measure():
         assert np.allclose(en[r], en1, rtol=1e-13)
 
 
-@pytest.mark.parametrize("M", [4, 3])   # wide (M | 256) and strided reduction forms
+# wide (M | 256) and strided reduction forms; 128 and 256 ensembles have more
+# (ensemble, value) partials per workgroup than the workgroup has threads
+@pytest.mark.parametrize("M", [4, 3, 128, 256])
 def test_accumulators_match_host_recursion(M):
     rng = np.random.default_rng(32)
     reps = _replicas([3, 4])
@@ -113,6 +115,25 @@ def test_accumulators_match_host_recursion(M):
     eng.reset_accumulators()
     assert eng.accumulators()["samples"] == 0
     assert np.all(eng.accumulators()["W"] == 0)
+
+
+@pytest.mark.parametrize("M", [64, 128, 256, 5])
+def test_segment_sums_many_ensembles_match_host(M):
+    """measure_segments against numpy sums of the uploaded state, for ensemble
+    counts whose (ensemble, value) partials outnumber a workgroup's threads."""
+    rng = np.random.default_rng(33)
+    reps = _replicas([3, 2, 4])
+    j = Bt.join(reps)
+    eng = Bt.engine_for(j, M)
+    e = np.ascontiguousarray(CF.random_unit(j.natom, rng, M))
+    mm = np.ascontiguousarray(np.broadcast_to(j.mmom, (M, j.natom)))
+    eng.set_moments(e, mm)
+    ms, _ = eng.measure_segments()
+    eM = e * mm[..., None]
+    for r in range(len(reps)):
+        a, b = j.offsets[r], j.offsets[r + 1]
+        want = eM[:, a:b].sum(axis=1)
+        assert np.allclose(ms[r], want, rtol=0, atol=1e-12 * (b - a)), (M, r)
 
 
 def test_segments_rejected_when_malformed():

@@ -177,9 +177,9 @@ def test_resident_injected_noise_trajectory(O, alg):
 
 @pytest.mark.parametrize("alg", [4, 5, 7])
 def test_philox_noise_stream_matches_oracle(O, alg):
-    """In-kernel Philox deviates reproduce the oracle's restatement of the
-    same counter stream bit for bit (single-precision Box-Muller made of
-    IEEE operations only), so the trajectories agree to FP64 rounding."""
+    """The FP64 engine's in-kernel Philox deviates (double-precision
+    Box-Muller on 52-bit uniforms) equal the oracle's restatement of the same
+    counter stream to a few ulps, so the trajectories agree to 1e-12."""
     rng = np.random.default_rng(13)
     sysd = CF.bcc_shells(3, [1.34, 0.77])
     M, N = 2, sysd["lattice"].natom
@@ -196,7 +196,7 @@ def test_philox_noise_stream_matches_oracle(O, alg):
         normals = np.zeros((M, N, 3))
         for k in range(M):
             for i in range(N):
-                normals[k, i] = O.thermal3(seed, 0x11, step, i, k)
+                normals[k, i] = O.thermal3_f64(seed, step, i, k)
         sim.step(alg, 1e-16, 0.1, 300.0, normals=normals)
     eng.sd_steps(alg, nsteps)
     e, _ = eng.get_moments()

@@ -1,6 +1,7 @@
-"""The oracle's single-precision Box-Muller (the LLG thermal deviates of the
-HIP kernels): its polynomial ln, sin and cos against numpy's double
-precision functions, and the deviates' distribution against N(0,1)."""
+"""The oracle's restatements of the LLG thermal deviates of the HIP kernels:
+the FP32 engine's single-precision Box-Muller (its polynomial ln, sin and
+cos against numpy's double precision functions) and the FP64 engine's
+double-precision Box-Muller on 52-bit uniforms; both against N(0,1)."""
 import math
 
 import numpy as np
@@ -49,3 +50,29 @@ def test_thermal3_distribution(O):
     a = z.reshape(n, 3)
     cc = np.corrcoef(a.T)
     assert np.max(np.abs(cc - np.eye(3))) < 5 / math.sqrt(n)
+
+
+def _normal_checks(z):
+    se = 1.0 / math.sqrt(z.size)
+    assert abs(z.mean()) < 5 * se
+    assert abs(z.var() - 1.0) < 5 * math.sqrt(2.0) * se
+    assert abs(np.mean(z ** 4) - 3.0) < 5 * math.sqrt(96.0) * se
+    for t in (1.0, 2.0, 3.0):
+        p = math.erfc(t / math.sqrt(2.0))
+        assert abs(np.mean(np.abs(z) > t) - p) < 5 * math.sqrt(p * (1 - p) / z.size)
+
+
+def test_thermal3_f64_distribution_and_resolution(O):
+    """FP64 deviates: N(0,1) moments and tails, uncorrelated components,
+    and values that single precision cannot hold (the FP32 deviates are all
+    float values), with distinct low-order bits."""
+    n = 60000
+    z = np.array([O.thermal3_f64(0xC0FFEE, 7, i, 0) for i in range(n)]).ravel()
+    _normal_checks(z)
+    cc = np.corrcoef(z.reshape(n, 3).T)
+    assert np.max(np.abs(cc - np.eye(3))) < 5 / math.sqrt(n)
+    assert np.mean(z.astype(np.float32).astype(np.float64) != z) > 0.999
+    low = (z.view(np.uint64) & np.uint64(0xFFFF)).astype(np.int64)
+    assert len(np.unique(low)) > 0.6 * 65536
+    # the two Philox blocks are independent streams: no component repeats
+    assert len(np.unique(z)) == z.size

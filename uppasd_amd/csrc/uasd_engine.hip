@@ -521,7 +521,7 @@ int ready_for_llg(uasd_ctx *c) {
     return fail(UASD_ERR_STATE, "injected noise selected but no normals uploaded");
   if (c->noise_mode == UASD_NOISE_PHILOX) {
     void *old = c->nsave.p;
-    HIP_TRY(c->nsave.ensure((size_t)c->N * c->M * sizeof(float4)));
+    HIP_TRY(c->nsave.ensure((size_t)c->N * c->M * 4 * c->rsize));
     if (c->nsave.p != old) c->gen += 1;  // captured graphs hold the old address
   }
   return rebuild_classes(c);
@@ -548,7 +548,7 @@ int run_llg_half(uasd_ctx *c, int alg, int half, bool fused, bool write_e,
   }
   // a fused predictor/corrector pair draws the deviates once: the predictor
   // stores them, the corrector of the same step reads them back
-  if (fused && prm.noise_mode == UASD_NOISE_PHILOX) prm.nsave = c->nsave.as<float4>();
+  if (fused && prm.noise_mode == UASD_NOISE_PHILOX) prm.nsave = c->nsave.as<uasd::Quad<R>>();
   uasd::Quad<R> *e = quads<R>(c, c->spins, 0), *p = quads<R>(c, c->spins, 1),
                 *b2 = quads<R>(c, c->spins, 2);
   const uasd::Quad<R> *beff = fused ? nullptr : quads<R>(c, c->field);

@@ -36,7 +36,8 @@ constexpr double kMry = 2.179872325e-21;
 constexpr double kPi = 3.141592653589793;
 constexpr double kDblTolerance = (double)1e-14f;  // parameters.f90:12 (default real literal)
 
-constexpr uint32_t kTagLLG = 0x11u;  // Philox tag of the thermal field
+constexpr uint32_t kTagLLG = 0x11u;   // Philox tag of the thermal field
+constexpr uint32_t kTagLLG2 = 0x12u;  // its second block (FP64 engine: the angles)
 constexpr uint32_t kTagMCU = 0x21u;  // MC uniforms
 constexpr uint32_t kTagMCG = 0x22u;  // MC Gaussian trial step
 
@@ -155,9 +156,8 @@ __device__ __forceinline__ void normal3_f64(uint64_t seed, uint32_t tag, uint64_
   c = r1 * cos(t1);
 }
 
-// Thermal-field deviates of the LLG steps, FP32 for both precisions (the
-// reference draws them in single precision too: r4_nor,
-// randomnumbers.f90:330-438).  Box-Muller with ln, sin and cos evaluated by
+// Thermal-field deviates of the FP32 engine's LLG steps.  Box-Muller with
+// ln, sin and cos evaluated by
 // polynomials fitted here (least squares with Lawson reweighting on Chebyshev
 // nodes; relative error below 3e-7): every operation is an IEEE add,
 // multiply, fmaf, sqrtf or rint, so the oracle reproduces the deviates bit
@@ -225,6 +225,33 @@ __device__ __forceinline__ void thermal3(uint64_t seed, uint32_t tag, uint64_t s
   float s0, c0, s1, c1;
   bm::sincos_q(v1, s0, c0);
   bm::sincos_q(v3, s1, c1);
+  a = r0 * c0;
+  b = r0 * s0;
+  c = r1 * c1;
+}
+
+// Thermal-field deviates of the FP64 engine's LLG steps, at the resolution
+// of the reference's: r4_nor returns real(IEEE64) values
+// (randomnumbers.f90:379,389).  Box-Muller in double precision on 52-bit
+// uniforms, u = (x + 1/2) 2^-52 with x the top 52 bits of two Philox words
+// (exact; radius tails to 8.6 sigma): block kTagLLG gives the two radii,
+// block kTagLLG2 the two angles, taken by sincospi / cospi (exact argument
+// reduction).  The oracle restates it with libm (or_thermal3_f64; the
+// deviates agree to a few ulps).
+__device__ __forceinline__ double u52(uint32_t hi, uint32_t lo) {
+  const uint64_t x = ((uint64_t)hi << 20) | (uint64_t)(lo >> 12);
+  return ((double)x + 0.5) * 2.220446049250313080847e-16;
+}
+
+__device__ __forceinline__ void thermal3_f64(uint64_t seed, uint64_t step, uint32_t site,
+                                             uint32_t ens, double &a, double &b, double &c) {
+  uint32_t w0, w1, w2, w3, x0, x1, x2, x3;
+  counter_words(seed, kTagLLG, step, site, ens, w0, w1, w2, w3);
+  counter_words(seed, kTagLLG2, step, site, ens, x0, x1, x2, x3);
+  const double r0 = sqrt(-2.0 * log(u52(w0, w1))), r1 = sqrt(-2.0 * log(u52(w2, w3)));
+  double s0, c0;
+  sincospi(2.0 * u52(x0, x1), &s0, &c0);
+  const double c1 = cospi(2.0 * u52(x2, x3));
   a = r0 * c0;
   b = r0 * s0;
   c = r1 * c1;
@@ -515,7 +542,7 @@ struct LLGDev {
   const Quad<R> *noise;  // injected N(0,1) per q
   // Philox deviates of a fused step: stored by the predictor, read back by
   // the corrector (the same values it would draw), or nullptr
-  float4 *nsave;
+  Quad<R> *nsave;
   LamConst<R> lc0;       // lam_const(lam0, dt, bn)
   // uniform lam, temperature and moment: the noise amplitudes, computed on
   // the host by the same functions (sig_uniform = 1)
@@ -530,11 +557,15 @@ struct LLGDev {
   __device__ __forceinline__ void normals(int site, int ens, size_t q, R &a, R &b, R &c) const {
     if (noise_mode == 1) {
       const uint64_t st = step_dev ? *step_dev + step : step;
-      float fa, fb, fc;
-      thermal3(seed, kTagLLG, st, (uint32_t)site, (uint32_t)ens, fa, fb, fc);
-      a = (R)fa;
-      b = (R)fb;
-      c = (R)fc;
+      if constexpr (std::is_same<R, double>::value) {
+        thermal3_f64(seed, st, (uint32_t)site, (uint32_t)ens, a, b, c);
+      } else {
+        float fa, fb, fc;
+        thermal3(seed, kTagLLG, st, (uint32_t)site, (uint32_t)ens, fa, fb, fc);
+        a = fa;
+        b = fb;
+        c = fc;
+      }
     } else if (noise_mode == 2) {
       const Quad<R> v = noise[q];
       a = v.x;
@@ -792,15 +823,13 @@ __device__ __forceinline__ SiteIn<R> llg_fetch(const LLGDev<R> &prm, int hsite, 
   if (HALF == 1) in.pq = p[q];
   if ((ALG == 5 || ALG == 4) && HALF == 1) in.bq = b2[q];
   if (HALF == 1 && prm.nsave) {
-    const float4 v = prm.nsave[q];
-    in.n0 = (R)v.x;
-    in.n1 = (R)v.y;
-    in.n2 = (R)v.z;
+    const Quad<R> v = prm.nsave[q];
+    in.n0 = v.x;
+    in.n1 = v.y;
+    in.n2 = v.z;
   } else {
     prm.normals(hsite, ens, (size_t)q, in.n0, in.n1, in.n2);
-    // Philox deviates are FP32 values, so the conversion back is exact
-    if (HALF == 0 && prm.nsave)
-      prm.nsave[q] = make_float4((float)in.n0, (float)in.n1, (float)in.n2, 0.0f);
+    if (HALF == 0 && prm.nsave) prm.nsave[q] = Quad<R>{in.n0, in.n1, in.n2, (R)0};
   }
   return in;
 }
@@ -1615,8 +1644,9 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int v = 0; v < NV; ++v) red[threadIdx.x][v] = a[v];
   __syncthreads();
-  if ((int)threadIdx.x < M * NV) {
-    const int ens = threadIdx.x / NV, v = threadIdx.x - ens * NV;
+  // M*NV partials: more than the workgroup's threads once M > blockDim/NV
+  for (int t = (int)threadIdx.x; t < M * NV; t += (int)blockDim.x) {
+    const int ens = t / NV, v = t - ens * NV;
     double x = 0.0;
     for (int j = ens; j < (int)blockDim.x; j += M) x += red[j][v];
     partial[((size_t)blockIdx.x * M + ens) * NV + v] = x;

@@ -177,6 +177,8 @@ def _declare(L):
                                          dp, C.c_int]
     L.or_trial_move_f32.argtypes = [dp, C.c_double, dp, C.c_uint64, C.c_uint64, C.c_uint32,
                                     C.c_uint32, dp]
+    L.or_set_threads.argtypes = [C.c_int]
+    L.or_threads.restype = C.c_int
     L.or_thermal3_f64.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, dp]
     L.or_calc_mavrg.argtypes = [C.c_int, C.c_int, dp]
     L.or_calc_mavrg.restype = C.c_double

@@ -104,6 +104,8 @@ void or_effective_field(const or_ham *h, int N, int M, const double *emomM,
                         double *beff2, double *energy)
 {
   double en = 0.0;
+  /* sites are independent; the energy sum keeps its serial order */
+#pragma omp parallel for collapse(2) schedule(static) if (!energy && OR_PAR(N, M))
   for (int k = 0; k < M; ++k)
     for (int i = 0; i < N; ++i) {
       double bs[3], bq[3], be[3];
@@ -117,8 +119,9 @@ void or_effective_field(const or_ham *h, int N, int M, const double *emomM,
       }
       double tf[3];
       for (int c = 0; c < 3; ++c) tf[c] = 0.5 * (bs[c] + 2.0 * bq[c] + 2.0 * be[c] + 0.0);
-      en = en - V3(emomM, 0, i, k, N) * tf[0] - V3(emomM, 1, i, k, N) * tf[1] -
-           V3(emomM, 2, i, k, N) * tf[2];
+      if (energy)
+        en = en - V3(emomM, 0, i, k, N) * tf[0] - V3(emomM, 1, i, k, N) * tf[1] -
+             V3(emomM, 2, i, k, N) * tf[2];
     }
   if (energy) *energy = en * OR_MUB / OR_MRY;
 }

@@ -4,6 +4,7 @@
  * The reference draws three uniforms per site from state_c; a draw outside
  * the unit ball is redrawn with half the spread (x = 1*(rn-0.5)), which is
  * kept as written because it fixes the stream consumption.
+ * Also the thread control of the oracle's OpenMP site loops.
  */
 #include "uasd_oracle.h"
 #include <math.h>
@@ -39,4 +40,26 @@ void or_magninit_random(or_mt *st, int NA, int N1, int N2, int N3, int M,
             }
           }
         }
+}
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+void or_set_threads(int n)
+{
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+int or_threads(void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
 }

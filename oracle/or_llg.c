@@ -44,6 +44,7 @@ void or_depondt_first(int N, int M, const double *lambda, const double *temp,
                       double *emom2, double *emomM, double *b2eff,
                       double *thermal_field)
 {
+#pragma omp parallel for collapse(2) schedule(static) if (OR_PAR(N, M))
   for (int k = 0; k < M; ++k)
     for (int i = 0; i < N; ++i) {
       double lam = lambda[i];
@@ -74,6 +75,7 @@ void or_depondt_second(int N, int M, const double *lambda, double dt,
                        const double *beff, const double *btherm,
                        const double *b2eff, double *emom, double *emom2)
 {
+#pragma omp parallel for collapse(2) schedule(static) if (OR_PAR(N, M))
   for (int k = 0; k < M; ++k)
     for (int i = 0; i < N; ++i) {
       double lam = lambda[i];
@@ -101,6 +103,7 @@ void or_rannum_scale(int N, int M, int llg, const double *lambda, double bn,
                      const double *mmomi, const double *temp, double temprescale,
                      double *ranv)
 {
+#pragma omp parallel for collapse(2) schedule(static) if (OR_PAR(N, M))
   for (int k = 0; k < M; ++k)
     for (int i = 0; i < N; ++i) {
       double lam = lambda[i], Dk;
@@ -154,6 +157,7 @@ void or_midpoint_first(int N, int M, const double *landeg, double bn,
                        double *emom2, double *emomM, const double *mmom,
                        double deltat, const double *ranv, double *thermal_field)
 {
+#pragma omp parallel for collapse(2) schedule(static) if (OR_PAR(N, M))
   for (int i = 0; i < N; ++i)
     for (int k = 0; k < M; ++k) {
       double b[3], r[3], e[3], A[3], s1[3], et[3], detAi;
@@ -179,6 +183,7 @@ void or_midpoint_second(int N, int M, const double *landeg, double bn,
                         const double *lambda, const double *beff, const double *emom,
                         double *emom2, double deltat, const double *ranv)
 {
+#pragma omp parallel for collapse(2) schedule(static) if (OR_PAR(N, M))
   for (int i = 0; i < N; ++i)
     for (int k = 0; k < M; ++k) {
       double b[3], r[3], e[3], ep[3], A[3], et[3], detAi;
@@ -197,6 +202,7 @@ void or_midpoint_second(int N, int M, const double *landeg, double bn,
 void or_moment_update(int N, int M, const double *mmom, double *emom,
                       const double *emom2, double *emomM, double *mmomi)
 {
+#pragma omp parallel for collapse(2) schedule(static) if (OR_PAR(N, M))
   for (int k = 0; k < M; ++k)
     for (int i = 0; i < N; ++i) {
       double m = S2(mmom, i, k, N);
@@ -455,6 +461,7 @@ void or_sd_run(const or_ham *h, int N, int M, int alg, long nsteps, uint64_t ste
   for (size_t x = 0; x < n1; ++x) mmomi[x] = 1.0 / mmom[x];
   for (long s = 0; s < nsteps; ++s) {
     uint64_t step = step0 + (uint64_t)s;
+#pragma omp parallel for collapse(2) schedule(static) if (OR_PAR(N, M))
     for (int k = 0; k < M; ++k)
       for (int i = 0; i < N; ++i) {
         float z[3] = {0.0f, 0.0f, 0.0f};

@@ -20,6 +20,16 @@
 extern "C" {
 #endif
 
+/* ---------------- threads ----------------------------------------------- */
+/* The per-site loops of the field and LLG routines run on OpenMP threads
+ * when a call covers at least 4096 (site, ensemble) pairs: every site is
+ * computed by the same operations whatever the thread count, so results do
+ * not depend on it (the energy sums stay serial).  or_set_threads(n) sets
+ * the thread count (n <= 0 leaves the OpenMP default). */
+void or_set_threads(int n);
+int  or_threads(void);
+#define OR_PAR(N, M) ((long)(N) * (long)(M) >= 4096L)
+
 /* ---------------- physical constants (source/Parameters/constants.f90) --- */
 extern const double OR_GAMA, OR_K_BOLT, OR_MUB, OR_MRY, OR_PI;
 extern const double OR_DBL_TOLERANCE;   /* real(1e-14,4) widened, parameters.f90:12 */

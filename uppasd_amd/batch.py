@@ -74,6 +74,38 @@ def join(systems, per_site=()) -> Joined:
     return Joined(nl, ns, nc, mmom, offsets, extra)
 
 
+def sublattice_major(system: dict, na: int) -> dict:
+    """The system with its sites renumbered basis-major: reference site
+    i = cell*na + b becomes b*ncell + cell, so each of the na sublattices is
+    a contiguous range (a segment of a batch: per-sublattice FP64 sums and
+    accumulators on the device).  Lists, couplings and per-site arrays are
+    permuted; every site keeps its neighbours and their order."""
+    N = int(system["nlistsize"].shape[0])
+    if N % na:
+        raise ValueError("na must divide the number of sites")
+    nc = N // na
+    i = np.arange(N)
+    new = (i % na) * nc + i // na            # new index of old site i
+    old = np.empty(N, dtype=np.int64)
+    old[new] = i                             # old site at new index
+    out = dict(system)
+
+    def relist(nl):
+        nl = np.asarray(nl)[old]
+        return np.where(nl > 0, new[np.maximum(nl, 1) - 1] + 1, 0).astype(np.int32)
+
+    out["nlist"], out["nlistsize"] = relist(system["nlist"]), np.asarray(system["nlistsize"])[old]
+    out["ncoup"] = np.asarray(system["ncoup"])[old]
+    if "dmlist" in system:
+        out["dmlist"] = relist(system["dmlist"])
+        out["dmlistsize"] = np.asarray(system["dmlistsize"])[old]
+        out["dm_vect"] = np.asarray(system["dm_vect"])[old]
+    for k in ("mmom", "taniso", "eaniso", "kaniso", "sb"):
+        if k in system:
+            out[k] = np.asarray(system[k])[old]
+    return out
+
+
 def engine_for(joined: Joined, M: int, precision: str = "fp64"):
     """A context holding the joined lattice, with one segment per system."""
     from .engine import Engine

@@ -174,6 +174,17 @@ int uasd_measure_segments(uasd_ctx *ctx, double *msum, double *energy);
  * reference's ene%ene_xc(k) ... times mub/mry.  Anisotropy type 7 counts
  * 0.5*(uniaxial + sb*cubic), as calc_energy does. */
 int uasd_energy_terms(uasd_ctx *ctx, double *ene);
+/* uasd_energy_terms without any device allocation (no reference counterpart:
+ * for callers that cannot allocate, e.g. tight measurement loops or code
+ * that must not call hipMalloc).  Same kernels, same results.  The partial
+ * sums live in a fixed device array of 2^19 doubles (4 MiB) per device, held
+ * until the process ends: (ceil(n_s / 4096) summed over segments + nseg) *
+ * Mensemble * 4 must not exceed 2^19 (UASD_ERR_ARG otherwise).  The site-class
+ * tables must already be on the device: after a Hamiltonian upload that
+ * changes them, call uasd_energy_terms (or a field, step or sweep entry point)
+ * once first (UASD_ERR_STATE otherwise).  Every call on one device shares the
+ * array: calls that overlap on one device, from any context or thread, race. */
+int uasd_energy_terms_noalloc(uasd_ctx *ctx, double *ene);
 /* One cumulant sample per (segment, ensemble), kept on the device in FP64:
  * m = |sum emomM|/n and e = energy/n (mRy per site), as running means with
  * the linearly increasing weights of calc_and_print_cumulant

@@ -33,6 +33,7 @@ EXPORTS = (
     "uasd_get_coloring", "uasd_set_kernel_timing", "uasd_get_kernel_timing",
     "uasd_set_tuning", "uasd_get_tuning", "uasd_autotune_llg", "uasd_autotune_mc",
     "uasd_set_temprescale", "uasd_set_segments", "uasd_measure_segments", "uasd_energy_terms",
+    "uasd_energy_terms_noalloc",
     "uasd_accumulate", "uasd_get_accumulators", "uasd_reset_accumulators", "uasd_set_graph_steps",
     "uasd_index_bytes", "uasd_set_site_order", "uasd_set_lattice_dims", "uasd_has_stencil",
     "uasd_build_pair_list",
@@ -89,6 +90,7 @@ def lib() -> C.CDLL:
         L.uasd_measure_segments.argtypes = [vp, vp, vp]
         L.uasd_accumulate.argtypes = [vp, C.c_int]
         L.uasd_energy_terms.argtypes = [vp, vp]
+        L.uasd_energy_terms_noalloc.argtypes = [vp, vp]
         L.uasd_get_accumulators.argtypes = [vp, vp, C.POINTER(C.c_int64)]
         L.uasd_reset_accumulators.argtypes = [vp]
         L.uasd_set_graph_steps.argtypes = [vp, C.c_int]

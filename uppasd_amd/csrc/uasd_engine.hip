@@ -14,6 +14,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -81,6 +82,43 @@ struct Timing {
   int64_t launches = 0;
   double ms = 0.0;
 };
+
+// Fixed device scratch of uasd_energy_terms_noalloc: partial sums per
+// (chunk, ensemble) and the sums per (segment, ensemble), 4 doubles each.
+constexpr size_t kScratchDoubles = (size_t)1 << 19;  // 4 MiB per device
+constexpr int kMaxDevices = 64;
+
+}  // namespace
+
+// One copy per device, held until the process ends.  On the host the name is
+// only a stand-in for the symbol: kernels and copies take the device address
+// that scratch_init looks up, never `scratch` itself.
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `device`'s copy of the scratch array, looked up once per
+// device; the caller has made `device` current.
+int scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(UASD_ERR_UNSUPPORTED, "device index beyond the scratch table (64 devices)");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return UASD_OK;
+  void *p = nullptr;
+  HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return fail(UASD_ERR_HIP, "hipGetSymbolAddress returned a null scratch address");
+  g_scratch_ptr[device] = static_cast<double *>(p);
+  return UASD_OK;
+}
+
+double *scratch_of(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
 
 }  // namespace
 
@@ -365,7 +403,10 @@ void from_quads(const uasd_ctx *c, const std::vector<double> &hq, double *out, b
     }
 }
 
-int rebuild_classes(uasd_ctx *c) {
+// may_alloc = false: fail rather than grow the device tables (the no-alloc
+// entry point), i.e. when the class tables outgrow the buffers of the last
+// build.
+int rebuild_classes(uasd_ctx *c, bool may_alloc = true) {
   if (!c->classes_dirty) return UASD_OK;
   std::map<std::tuple<int, int, int>, int> ids;
   std::vector<int> cls(c->N);
@@ -382,6 +423,11 @@ int rebuild_classes(uasd_ctx *c) {
     }
     cls[i] = it->second;
   }
+  if (!may_alloc && (!c->cls.p || !c->classes.p || c->cls.bytes < cls.size() * sizeof(int) ||
+                     c->classes.bytes < table.size() * sizeof(uasd::SiteClass)))
+    return fail(UASD_ERR_STATE,
+                "the site-class tables are not on the device yet; call uasd_energy_terms (or "
+                "any field, step or sweep entry point) once after the last Hamiltonian upload");
   int r = upload_raw(c, c->cls, cls);
   if (r) return r;
   r = upload_raw(c, c->classes, table);
@@ -916,11 +962,14 @@ int uasd_ctx_create(uasd_ctx **out, int natom, int mensemble, int precision) {
     return fail(UASD_ERR_NODEVICE, std::string("device is ") + prop.gcnArchName +
                                        ", the library is built for gfx950");
   c->lds_max = (int64_t)std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
+  if (int r = scratch_init(c->device)) return r;
   HIP_TRY(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
   c->stream = c->own_stream;
   HIP_TRY(c->spins.ensure(3 * quad_bytes(c.get())));
   HIP_TRY(hipMemsetAsync(c->spins.p, 0, 3 * quad_bytes(c.get()), c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  // the whole lattice as one segment until uasd_set_segments says otherwise
+  if (int r = ensure_segments(c.get())) return r;
   *out = c.release();
   return UASD_OK;
 }
@@ -1863,35 +1912,28 @@ int uasd_measure_segments(uasd_ctx *c, double *msum, double *energy) {
   return UASD_OK;
 }
 
-int uasd_energy_terms(uasd_ctx *c, double *ene) {
-  if (int r = check_ctx(c)) return r;
-  if (!c->have_state) return fail(UASD_ERR_STATE, "moments not set");
-  if (!c->have_ex) return fail(UASD_ERR_STATE, "exchange not set");
-  if (!ene) return fail(UASD_ERR_ARG, "ene is null");
-  if (c->M > 65535) return fail(UASD_ERR_UNSUPPORTED, "more than 65535 ensembles");
-  if (int r = ensure_segments(c)) return r;
-  if (int r = rebuild_classes(c)) return r;
-  HIP_TRY(c->seg_partial.ensure((size_t)c->nchunks * c->M * 4 * sizeof(double)));
-  HIP_TRY(c->seg_out.ensure((size_t)c->nseg * c->M * 4 * sizeof(double)));
+}  // extern "C"
+
+namespace {
+
+// calc_energy's four terms per (segment, ensemble) into ene (4,M,nseg), with
+// the chunk partials in `partial` (nchunks*M*4 doubles) and the sums in `sums`
+// (nseg*M*4 doubles), both device memory.  The sums are copied straight into
+// ene and scaled there, so no host memory is taken either.
+int energy_terms_into(uasd_ctx *c, double *partial, double *sums, double *ene) {
   const int2 *chunks = c->seg_dev.as<int2>();
   const int *seg_chunk = reinterpret_cast<const int *>(chunks + c->nchunks);
   if (c->prec == UASD_FP64) {
     HIP_TRY(uasd::launch_energy_terms<double>(c->stream, ham_view<double>(c), c->nseg,
                                               c->nchunks, chunks, seg_chunk,
-                                              quads<double>(c, c->spins, 0),
-                                              c->seg_partial.as<double>(),
-                                              c->seg_out.as<double>()));
+                                              quads<double>(c, c->spins, 0), partial, sums));
   } else {
     HIP_TRY(uasd::launch_energy_terms<float>(c->stream, ham_view<float>(c), c->nseg,
                                              c->nchunks, chunks, seg_chunk,
-                                             quads<float>(c, c->spins, 0),
-                                             c->seg_partial.as<double>(),
-                                             c->seg_out.as<double>()));
+                                             quads<float>(c, c->spins, 0), partial, sums));
   }
   const size_t n = (size_t)c->nseg * c->M * 4;
-  std::vector<double> out(n);
-  HIP_TRY(hipMemcpyAsync(out.data(), c->seg_out.p, n * sizeof(double), hipMemcpyDeviceToHost,
-                         c->stream));
+  HIP_TRY(hipMemcpyAsync(ene, sums, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   // per site in mRy: energy.f90:353-366 divides by the number of atoms,
   // calculate_mean_and_deviation then scales by mub/mry
@@ -1900,10 +1942,48 @@ int uasd_energy_terms(uasd_ctx *c, double *ene) {
     for (int k = 0; k < c->M; ++k)
       for (int t = 0; t < 4; ++t) {
         const size_t i = ((size_t)sg * c->M + k) * 4 + t;
-        ene[i] = out[i] / na * (uasd::kMub / uasd::kMry);
+        ene[i] = ene[i] / na * (uasd::kMub / uasd::kMry);
       }
   }
   return UASD_OK;
+}
+
+int energy_terms_check(uasd_ctx *c, double *ene) {
+  if (int r = check_ctx(c)) return r;
+  if (!c->have_state) return fail(UASD_ERR_STATE, "moments not set");
+  if (!c->have_ex) return fail(UASD_ERR_STATE, "exchange not set");
+  if (!ene) return fail(UASD_ERR_ARG, "ene is null");
+  if (c->M > 65535) return fail(UASD_ERR_UNSUPPORTED, "more than 65535 ensembles");
+  return UASD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int uasd_energy_terms(uasd_ctx *c, double *ene) {
+  if (int r = energy_terms_check(c, ene)) return r;
+  if (int r = ensure_segments(c)) return r;
+  if (int r = rebuild_classes(c)) return r;
+  HIP_TRY(c->seg_partial.ensure((size_t)c->nchunks * c->M * 4 * sizeof(double)));
+  HIP_TRY(c->seg_out.ensure((size_t)c->nseg * c->M * 4 * sizeof(double)));
+  return energy_terms_into(c, c->seg_partial.as<double>(), c->seg_out.as<double>(), ene);
+}
+
+int uasd_energy_terms_noalloc(uasd_ctx *c, double *ene) {
+  if (int r = energy_terms_check(c, ene)) return r;
+  if (c->nseg == 0) return fail(UASD_ERR_STATE, "the context has no segments");
+  const size_t need = ((size_t)c->nchunks + (size_t)c->nseg) * (size_t)c->M * 4;
+  if (need > kScratchDoubles)
+    return fail(UASD_ERR_ARG,
+                "uasd_energy_terms_noalloc needs " + std::to_string(need) +
+                    " doubles of scratch, more than its fixed " + std::to_string(kScratchDoubles) +
+                    ": (chunks of 4096 sites + segments) * ensembles * 4 must not exceed it; "
+                    "use uasd_energy_terms");
+  double *base = scratch_of(c->device);
+  if (!base) return fail(UASD_ERR_STATE, "no scratch address for this device");
+  if (int r = rebuild_classes(c, false)) return r;
+  return energy_terms_into(c, base, base + (size_t)c->nchunks * c->M * 4, ene);
 }
 
 int uasd_accumulate(uasd_ctx *c, int with_energy) {

@@ -201,12 +201,14 @@ int uasd_get_coloring(uasd_ctx *ctx, int *ncolors, int *color_of_site);
 
 /* ---- per-kernel timing with HIP events on the context stream ----
  * enable = n > 0 records a pair of events around one launch in n of each
- * kind (1: every launch); 0 switches timing off.  Events between kernels
- * cost time of their own, so a sampled run keeps the stream close to an
- * untimed one. */
+ * kind (1: every launch) and around every uasd_sd_steps / uasd_mc_sweeps
+ * call; -1 times the calls only, with no events between kernels; 0 switches
+ * timing off.  Events between kernels cost time of their own, so a sampled
+ * run keeps the stream close to an untimed one. */
 int uasd_set_kernel_timing(uasd_ctx *ctx, int enable);
 /* kind: 0 predictor, 1 corrector, 2 field, 3 MC colour pass, 4 LLG graph
- * replay (K steps), 5 MC graph replay (K sweeps) */
+ * replay (K steps), 5 MC graph replay (K sweeps), 6 a whole uasd_sd_steps or
+ * uasd_mc_sweeps call */
 int uasd_get_kernel_timing(uasd_ctx *ctx, int kind, int64_t *launches, double *total_ms);
 
 /* Bytes per neighbour-list entry the LLG kernels read under the current

@@ -150,4 +150,29 @@ def test_sampled_kernel_timing_counts_and_leaves_results():
     assert (n0, n1) == (6, 6) and ms0 > 0.0 and ms1 > 0.0
     assert np.array_equal(eng.get_moments()[0], ref)
     with pytest.raises(U.UasdError):
-        eng.set_kernel_timing(True, every=-1)
+        eng.set_kernel_timing(True, every=-2)
+
+
+def test_call_timing_times_whole_calls_only():
+    """Events around whole sd_steps calls (kind 6) and no launch events: one
+    timed call per call, a positive duration, no per-launch samples, and the
+    trajectory of an untimed run."""
+    rng = np.random.default_rng(62)
+    sysd = CF.bcc_shells(6, CF.bcc5_couplings())
+    emom = np.ascontiguousarray(CF.random_unit(sysd["lattice"].natom, rng, 1))
+    ref, _, _ = _llg_run(sysd, emom, 4, 256, "fp64", 1, steps=24)
+    N = sysd["lattice"].natom
+    eng = U.Engine(N, 1)
+    eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
+    eng.set_moments(emom, np.broadcast_to(sysd["mmom"], (1, N)))
+    eng.set_llg(0.1, 300.0, 1e-16)
+    eng.set_noise(U.UASD_NOISE_PHILOX, 1234)
+    eng.set_tuning(eng.TUNE_LLG, 4, 256)
+    eng.set_kernel_timing(True, calls_only=True)
+    eng.sd_steps(1, 10)
+    eng.sd_steps(1, 14)
+    n6, ms6 = eng.kernel_timing(6)
+    assert n6 == 2 and ms6 > 0.0
+    assert eng.kernel_timing(0)[0] == 0 and eng.kernel_timing(1)[0] == 0
+    eng.set_kernel_timing(False)
+    assert np.array_equal(eng.get_moments()[0], ref)

@@ -227,8 +227,10 @@ struct uasd_ctx {
   // a host wait; pairs are resolved when the totals are read.
   bool timing = false;
   int timing_every = 1;       // time one launch in timing_every of each kind
-  int64_t timing_tick[6] = {};
-  Timing timers[6];  // 0 predictor, 1 corrector, 2 field, 3 MC, 4 LLG graph, 5 MC graph
+  bool timing_region = false; // time every uasd_sd_steps / uasd_mc_sweeps call (kind 6)
+  int64_t timing_tick[7] = {};
+  Timing timers[7];  // 0 predictor, 1 corrector, 2 field, 3 MC, 4 LLG graph, 5 MC graph, 6 call
+  hipEvent_t region_cur = nullptr;
   std::vector<hipEvent_t> ev_pool;  // free events
   struct Pending {
     hipEvent_t a, b;
@@ -245,6 +247,9 @@ int make_tile(uasd_ctx *c, int k, uasd::TileDev &t);
 namespace {
 
 void destroy_graphs(uasd_ctx *c);
+int sd_steps_run(uasd_ctx *c, int alg, int64_t nsteps);
+int mc_sweeps_run(uasd_ctx *c, char mode, const uasd::MCParams &mp, int64_t nsweeps,
+                  uint64_t seed);
 
 // Reference site held by device slot d, and the slot of reference site i.
 inline int site_of(const uasd_ctx *c, int d) {
@@ -511,6 +516,25 @@ int toc(uasd_ctx *c, int kind) {
   HIP_TRY(hipEventRecord(b, c->stream));
   c->pending.push_back({c->cur, b, kind});
   c->cur = nullptr;
+  return UASD_OK;
+}
+
+// Event pair around a whole stepping call (kind 6), on the context stream.
+int region_tic(uasd_ctx *c) {
+  if (!c->timing_region || c->capturing) return UASD_OK;
+  c->region_cur = take_event(c);
+  if (!c->region_cur) return fail(UASD_ERR_HIP, "hipEventCreate failed");
+  HIP_TRY(hipEventRecord(c->region_cur, c->stream));
+  return UASD_OK;
+}
+
+int region_toc(uasd_ctx *c) {
+  if (!c->region_cur) return UASD_OK;
+  hipEvent_t b = take_event(c);
+  if (!b) return fail(UASD_ERR_HIP, "hipEventCreate failed");
+  HIP_TRY(hipEventRecord(b, c->stream));
+  c->pending.push_back({c->region_cur, b, 6});
+  c->region_cur = nullptr;
   return UASD_OK;
 }
 
@@ -1492,6 +1516,16 @@ int uasd_sd_steps(uasd_ctx *c, int alg, int64_t nsteps) {
     return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint), 4 (Heun), 5 (Depondt), 6 (SIB) or 7 (spherical midpoint)");
   if (nsteps < 0) return fail(UASD_ERR_ARG, "nsteps < 0");
   if (int r = ready_for_llg(c)) return r;
+  if (int r = region_tic(c)) return r;
+  if (int r = sd_steps_run(c, alg, nsteps)) return r;
+  return region_toc(c);
+}
+
+}  // extern "C"
+
+namespace {
+
+int sd_steps_run(uasd_ctx *c, int alg, int64_t nsteps) {
   int64_t s = 0;
   const int K = c->graph_steps;
   if (K > 0) {
@@ -1522,6 +1556,10 @@ int uasd_sd_steps(uasd_ctx *c, int alg, int64_t nsteps) {
   }
   return UASD_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int uasd_effective_field(uasd_ctx *c, double *beff, double *energy) {
   if (int r = check_ctx(c)) return r;
@@ -1618,6 +1656,17 @@ int uasd_mc_sweeps(uasd_ctx *c, char mode, const double *temps, int64_t nsweeps,
   if (nsweeps < 0) return fail(UASD_ERR_ARG, "nsweeps < 0");
   uasd::MCParams mp;
   if (int r = mc_prepare(c, mode, temps, seed, mp)) return r;
+  if (int r = region_tic(c)) return r;
+  if (int r = mc_sweeps_run(c, mode, mp, nsweeps, seed)) return r;
+  return region_toc(c);
+}
+
+}  // extern "C"
+
+namespace {
+
+int mc_sweeps_run(uasd_ctx *c, char mode, const uasd::MCParams &mp, int64_t nsweeps,
+                  uint64_t seed) {
   int64_t s = 0;
   const int K = c->graph_steps;
   if (K > 0) {
@@ -1648,6 +1697,10 @@ int uasd_mc_sweeps(uasd_ctx *c, char mode, const double *temps, int64_t nsweeps,
   }
   return mc_run(c, mp, nsweeps - s);
 }
+
+}  // namespace
+
+extern "C" {
 
 // ---------------------------------------------------------------- tuning
 int uasd_index_bytes(uasd_ctx *c, int *exchange, int *dm) {
@@ -2039,9 +2092,10 @@ int uasd_get_coloring(uasd_ctx *c, int *ncolors, int *color_of_site) {
 int uasd_set_kernel_timing(uasd_ctx *c, int enable) {
   if (int r = check_ctx(c)) return r;
   if (int r = resolve_timing(c)) return r;
-  if (enable < 0) return fail(UASD_ERR_ARG, "enable must be >= 0");
+  if (enable < -1) return fail(UASD_ERR_ARG, "enable must be >= -1");
   c->timing = enable > 0;
   c->timing_every = enable > 0 ? enable : 1;
+  c->timing_region = enable != 0;
   for (auto &t : c->timers) t = Timing{};
   for (auto &k : c->timing_tick) k = 0;
   return UASD_OK;
@@ -2049,7 +2103,7 @@ int uasd_set_kernel_timing(uasd_ctx *c, int enable) {
 
 int uasd_get_kernel_timing(uasd_ctx *c, int kind, int64_t *launches, double *total_ms) {
   if (int r = check_ctx(c)) return r;
-  if (kind < 0 || kind > 5) return fail(UASD_ERR_ARG, "kind must be 0..5");
+  if (kind < 0 || kind > 6) return fail(UASD_ERR_ARG, "kind must be 0..6");
   if (int r = resolve_timing(c)) return r;
   if (launches) *launches = c->timers[kind].launches;
   if (total_ms) *total_ms = c->timers[kind].ms;

@@ -209,10 +209,12 @@ class Engine:
         B.check(B.lib().uasd_get_coloring(self._h, C.byref(nc), B.ptr(col)), "uasd_get_coloring")
         return nc.value, col
 
-    def set_kernel_timing(self, enable: bool, every: int = 1):
-        """HIP events around one launch in `every` of each kernel kind."""
-        B.check(B.lib().uasd_set_kernel_timing(self._h, int(every) if enable else 0),
-                "uasd_set_kernel_timing")
+    def set_kernel_timing(self, enable: bool, every: int = 1, calls_only: bool = False):
+        """HIP events around one launch in `every` of each kernel kind and
+        around every sd_steps / mc_sweeps call (kind 6); calls_only times the
+        calls alone, with no events between kernels."""
+        code = (-1 if calls_only else int(every)) if enable else 0
+        B.check(B.lib().uasd_set_kernel_timing(self._h, code), "uasd_set_kernel_timing")
 
     def kernel_timing(self, kind: int):
         n = C.c_int64(0)

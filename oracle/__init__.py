@@ -387,3 +387,105 @@ def noise3_f64(seed, tag, step, site, ens):
     out = np.zeros(3)
     lib().or_noise3_f64(seed, tag, step, site, ens, out)
     return out
+
+
+# ------------------------------------------------------- the reference itself
+# _ref/libuppasd_ref.so: the reference's own field / Depondt / midpoint / RNG
+# / moment-update routines compiled from its sources (make target `ref`,
+# amdflang), with the C entry points of ref_driver.f90.  Built here when the
+# reference sources are present; the built library travels with the tree.
+_REF_PATH = os.path.join(_HERE, "_ref", "libuppasd_ref.so")
+REF_SOURCES = "/root/reference/source"
+_ref = None
+
+
+def build_ref(force: bool = False):
+    """Compile _ref/libuppasd_ref.so; None when the reference sources or
+    amdflang are absent (the tests that need it then skip)."""
+    if not os.path.isdir(REF_SOURCES) or not os.path.exists("/opt/rocm/bin/amdflang"):
+        return None
+    args = ["make", "-s", "-C", _HERE, "ref"]
+    if force:
+        args.insert(1, "-B")
+    subprocess.run(args, check=True)
+    return _REF_PATH
+
+
+def ref_lib():
+    """The reference's compiled routines, or None when not built."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(_REF_PATH):
+            return None
+        L = C.CDLL(_REF_PATH)
+        vp = C.c_void_p
+        L.ref_set_threads.argtypes = [C.c_int]
+        L.ref_threads.restype = C.c_int
+        L.ref_setup.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip, dp, C.c_int, ip, ip, dp,
+                                C.c_int, ip, dp, dp, dp]
+        L.ref_set_external_field.argtypes = [vp]
+        L.ref_field.argtypes = [dp, dp, dp, C.POINTER(C.c_double)]
+        L.ref_seed.argtypes = [C.c_int, C.c_int]
+        L.ref_sd_run.argtypes = [C.c_int, C.c_int, C.c_double, dp, dp, dp, dp, dp]
+        L.ref_sd_run.restype = C.c_double
+        _ref = L
+    return _ref
+
+
+class RefSD:
+    """One system in the reference's own module state (one at a time: the
+    reference keeps ham_t and its work arrays in modules).  Arrays as for
+    OracleHam: (M, N, 3) C order == Fortran (3, N, M); lists (N, W)."""
+
+    def __init__(self, natom, mensemble, nlist, nlistsize, ncoup, dmlist=None, dmlistsize=None,
+                 dm_vect=None, taniso=None, eaniso=None, kaniso=None, sb=None):
+        L = ref_lib()
+        if L is None:
+            raise RuntimeError(f"{_REF_PATH} is not built (oracle.build_ref())")
+        self.L, self.N, self.M = L, int(natom), int(mensemble)
+        nl = np.ascontiguousarray(nlist, dtype=np.int32)
+        zi, zd = np.zeros(1, np.int32), np.zeros(3)
+        if dmlist is not None:
+            dl = np.ascontiguousarray(dmlist, dtype=np.int32)
+            ds = np.ascontiguousarray(dmlistsize, dtype=np.int32)
+            dv = np.ascontiguousarray(dm_vect, dtype=np.float64)
+            mdm = dl.shape[1]
+        else:
+            dl, ds, dv, mdm = zi, zi, zd, 0
+        if taniso is not None:
+            ta = np.ascontiguousarray(taniso, dtype=np.int32)
+            ea = np.ascontiguousarray(eaniso, dtype=np.float64)
+            ka = np.ascontiguousarray(kaniso, dtype=np.float64)
+            sbv = np.ascontiguousarray(sb, dtype=np.float64)
+        else:
+            ta, ea, ka, sbv = zi, zd, zd, zd
+        L.ref_setup(self.N, self.M, nl.shape[1], nl,
+                    np.ascontiguousarray(nlistsize, dtype=np.int32),
+                    np.ascontiguousarray(ncoup, dtype=np.float64), mdm, dl, ds, dv,
+                    1 if taniso is not None else 0, ta, ea, ka, sbv)
+        L.ref_set_external_field(None)
+
+    def set_external_field(self, ext):
+        """(M, N, 3) external_field, or None."""
+        self._ext = None if ext is None else np.ascontiguousarray(
+            np.broadcast_to(ext, (self.M, self.N, 3)), dtype=np.float64)
+        self.L.ref_set_external_field(None if self._ext is None else self._ext.ctypes.data)
+
+    def field(self, emomM, mmom):
+        """effective_field_full: beff (M, N, 3) and the energy in mRy."""
+        b = np.zeros((self.M, self.N, 3))
+        en = C.c_double(0.0)
+        self.L.ref_field(np.ascontiguousarray(emomM, dtype=np.float64),
+                         np.ascontiguousarray(mmom, dtype=np.float64), b, C.byref(en))
+        return b, en.value
+
+    def seed(self, tseed: int, ziggurat: bool = True):
+        self.L.ref_seed(int(tseed), 1 if ziggurat else 0)
+
+    def run(self, alg, nsteps, dt, lam, temp, mmom, emom, emomM):
+        """nsteps of the sd_mphase body (SDEalgh 1 or 5); emom, emomM and mmom
+        ((M, N, 3), (M, N)) are advanced in place.  Returns the last field
+        energy (mRy)."""
+        lam = np.ascontiguousarray(np.broadcast_to(lam, (self.N,)), dtype=np.float64)
+        temp = np.ascontiguousarray(np.broadcast_to(temp, (self.N,)), dtype=np.float64)
+        return self.L.ref_sd_run(int(alg), int(nsteps), float(dt), lam, temp, mmom, emom, emomM)

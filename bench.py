@@ -35,65 +35,72 @@ HBM_PEAK_GBS = 8000.0
 SOLVER_NAMES = {1: "midpoint", 4: "Heun", 5: "Depondt", 6: "SIB", 7: "spherical midpoint"}
 
 
-def kernel_bytes(W: int, s: int, ib: int = 4, cls: int = 4, nsave: int = 16,
+def kernel_bytes(W: int, s: int, ib: int = 4, cls: int = 4, noise: str = "philox",
                  b2: bool = False) -> dict:
-    """Algorithmic HBM bytes per site of the fused LLG kernels.
+    """Algorithmic HBM bytes per site of the fused LLG kernels as built.
 
     Spins are stored as quads {x, y, z, mmom} of s-byte reals; neighbour
     entries take ib bytes (4 for int32 indices, 2 for int16 differences, 0 for
     the LDS-tiled stencil kernels, which read no lists); cls bytes of site
     class per site (0 for the tiled kernels on a one-species lattice, whose
-    classes come with the launch); nsave bytes of thermal deviates the
-    predictor stores for its corrector (a float4; 0 without Philox noise).
-    predictor: gather e_j quads (4s), index stream ib*W, class, write p (4s),
-               write deviates
-    corrector: gather p_j quads (4s), own e (4s), index ib*W, class, write e (4s),
-               read deviates
+    classes come with the launch).
+    predictor: gather e_j quads (4s), index stream ib*W, class, write p (4s)
+    corrector: gather p_j quads (4s), own e (4s), index ib*W, class, write e (4s)
+    noise: "philox" -- the predictor stores its deviates as a quad of reals
+    for the corrector (4s written, 4s read); "injected" -- both halves read the
+    caller's N(0,1) quads (4s each); "none" -- nothing.
     Each site's own spin is one of its neighbours' gathers, so the gathered
     quads count once per site (perfect reuse through L2 / MALL).  b2: the
     solver keeps a field quad between the halves (Depondt's bdup, Heun's
     b2eff), written by the predictor and read by the corrector (4s each).
     """
     q = 4 * s if b2 else 0
-    pred = 4 * s + ib * W + cls + 4 * s + nsave + q
-    corr = 4 * s + 4 * s + ib * W + cls + 4 * s + nsave + q
+    nz = 0 if noise == "none" else 4 * s
+    pred = 4 * s + ib * W + cls + 4 * s + nz + q
+    corr = 4 * s + 4 * s + ib * W + cls + 4 * s + nz + q
     return {"predictor": pred, "corrector": corr, "step": pred + corr}
 
 
-def profile_path(order: int, variant: str) -> str:
-    """Committed rocprofv3 summary of the FP64 midpoint workload for a device
-    site order and kernel variant (idx4 / idx2 list gathers, tileK stencil with
-    one tile per workgroup, tileKr with resident workgroups looping over tiles)."""
-    return os.path.join(ROOT, "profiles",
-                        f"k_llg_fp64_midpoint_bcc64_58nn_order{order}_{variant}.md")
+def survey_bytes(W: int, s: int, injected: bool = False) -> dict:
+    """Algorithmic bytes per spin-update of the hot path's two-pass LLG step,
+    as the byte model of the workload counts them: pass 1 gathers the moments
+    and writes the predicted moments and field (9 s + 4 W), pass 2 gathers the
+    predicted moments, reads the moments and the predicted field and writes
+    the new moments (12 s + 4 W); 4-byte neighbour indices, couplings held per
+    basis site, noise from counters (an injected N(0,1) stream adds 3 s to each
+    pass).  21 s + 8 W per step: 632 B in FP64 at W = 58."""
+    extra = 3 * s if injected else 0
+    p1 = 9 * s + 4 * W + extra
+    p2 = 12 * s + 4 * W + extra
+    return {"predictor": p1, "corrector": p2, "step": p1 + p2}
 
 
-def profiled_traffic(kernel: str, order: int, variant: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary of
-    the same configuration (FETCH_SIZE + WRITE_SIZE scaled by a plain copy),
-    or None when there is none for it."""
-    path = profile_path(order, variant)
-    if not os.path.exists(path):
-        return None
-    in_table = False          # rows of the FETCH_SIZE / WRITE_SIZE table only
-    for line in open(path):
-        cells = [c.strip() for c in line.split("|")]
-        if len(cells) > 3 and cells[1] == "kernel":
-            in_table = len(cells) > 3 and cells[2].startswith("FETCH_SIZE")
-            continue
-        if in_table and len(cells) > 6 and cells[1] == f"`{kernel}`":
-            return (float(cells[4]) + float(cells[5])) * 1e6
-    return None
-
-
-def tile_plane(sysd, tile: int) -> int:
-    """Component plane size the tile kernel launches with (its template
-    argument): the smallest of 1160, 2312, 3080, 3848 holding NA * HV, for the
-    bcc five-shell stencil (reach 2 cells each way)."""
-    from uppasd_amd.engine import TILE_SHAPES
-    tx, ty, tz = TILE_SHAPES[tile - 1]
-    nahv = sysd["lattice"].NA * (tx + 4) * (ty + 4) * (tz + 4)
-    return next(p for p in (1160, 2312, 3080, 3848) if nahv <= p)
+def timed_steps(eng, solver: int, steps: int, timing_every: int):
+    """Two runs of `steps` steps on the context stream.  The first carries HIP
+    events around the whole call only (no events between kernels): its wall
+    time gives the value and its event time the step time.  The second
+    records events around both halves of one step in `timing_every`, which
+    gives each half's share of the step; the halves' durations are those
+    shares of the first run's step time (events between kernels stretch the
+    sampled durations, the shares are what they are used for)."""
+    eng.set_kernel_timing(True, calls_only=True)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.sd_steps(solver, steps)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    _, call_ms = eng.kernel_timing(6)
+    eng.set_kernel_timing(True, timing_every)
+    eng.sd_steps(solver, steps)
+    eng.sync()
+    n_pred, ms_pred = eng.kernel_timing(0)
+    n_corr, ms_corr = eng.kernel_timing(1)
+    eng.set_kernel_timing(False)
+    sp, sc = ms_pred / max(n_pred, 1), ms_corr / max(n_corr, 1)
+    step_s = call_ms * 1e-3 / steps
+    return {"wall_s": wall, "step_s": step_s,
+            "pred_s": step_s * sp / (sp + sc), "corr_s": step_s * sc / (sp + sc),
+            "sampled_ms": [round(sp, 4), round(sc, 4)], "sampled_launches": [n_pred, n_corr]}
 
 
 def build_c2(L: int):
@@ -101,30 +108,83 @@ def build_c2(L: int):
     return CF.bcc_shells(L, CF.bcc5_couplings(), moment=2.2)
 
 
+def _ref_rate(O, sysd, alg, temp, threads, seconds_target, lam=0.1, dt=1e-16):
+    """Spin-updates/s of the reference's own routines (oracle/_ref, the
+    sd_mphase loop body: field, evolve_first, field, evolve_second,
+    moment_update, thermal noise from its ziggurat stream) on `threads` host
+    threads, over about `seconds_target` seconds of steps."""
+    L = O.ref_lib()
+    L.ref_set_threads(threads)
+    used = L.ref_threads()
+    N = sysd["lattice"].natom
+    ref = O.RefSD(N, 1, sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
+    ref.seed(1)
+    rng = np.random.default_rng(1)
+    e = np.ascontiguousarray(CF_random_unit(N, rng))
+    mm = np.ascontiguousarray(np.broadcast_to(sysd["mmom"], (1, N)), dtype=np.float64)
+    eM = np.ascontiguousarray(e * mm[..., None])
+    t0 = time.perf_counter()
+    ref.run(alg, 1, dt, lam, temp, mm, e, eM)
+    one = time.perf_counter() - t0
+    nsteps = max(1, int(seconds_target / max(one, 1e-6)))
+    t0 = time.perf_counter()
+    ref.run(alg, nsteps, dt, lam, temp, mm, e, eM)
+    el = time.perf_counter() - t0
+    return N * nsteps / el, used, nsteps, el
+
+
+def CF_random_unit(N, rng):
+    e = rng.normal(size=(1, N, 3))
+    return e / np.linalg.norm(e, axis=2, keepdims=True)
+
+
 def cpu_baseline(sysd, seconds_target: float = 12.0):
-    """Oracle (plain C, FP64, 1 thread) on the same Hamiltonian."""
+    """The reference's own CPU path on the host cores: its routines compiled
+    from its sources (oracle/_ref/libuppasd_ref.so, OpenMP) on the headline
+    workload (midpoint, 300 K, its ziggurat noise) with every core the
+    process may use, plus the C1 configuration (bcc 20^3 x 2, two shells,
+    Depondt, 300 K) on one thread and on all.  Where that library is not
+    built, the C oracle (OpenMP, FP64, the FP64 engine's Philox deviates)
+    stands in, kind "port"."""
     import ctypes
     import oracle as O
-    L = O.lib()
+    n_cores = len(os.sched_getaffinity(0))
     N = sysd["lattice"].natom
+    if O.ref_lib() is not None:
+        v, used, nst, el = _ref_rate(O, sysd, 1, 300.0, n_cores, seconds_target)
+        from uppasd_amd import configs as CF
+        c1 = CF.bcc_shells(20, [1.34, 0.77], moment=2.2)     # J1 to 8 NN, J2 to 6 NNN
+        c1_1, _, n1, e1 = _ref_rate(O, c1, 5, 300.0, 1, 3.0)
+        c1_n, usedn, nn_, en_ = _ref_rate(O, c1, 5, 300.0, n_cores, 3.0)
+        return {"value": v, "unit": "spin-updates/s", "cores": used, "kind": "reference",
+                "host_cores_available": n_cores,
+                "sample": f"the reference's field/midpoint/rannum/moment_update routines "
+                          f"(compiled from its sources, -O2, OpenMP) on the same workload "
+                          f"({N} spins, 58 neighbours, 300 K ziggurat noise): {nst} steps in "
+                          f"{el:.1f} s on {used} threads",
+                "c1_depondt_bcc20": {"spins": c1["lattice"].natom,
+                                     "one_thread": c1_1, "all_threads": c1_n,
+                                     "threads": usedn, "steps": [n1, nn_]}}
+    L = O.lib()
+    L.or_set_threads(n_cores)
+    used = L.or_threads()
     ham = O.OracleHam(N, sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
     rng = np.random.default_rng(1)
-    e = rng.normal(size=(1, N, 3))
-    e /= np.linalg.norm(e, axis=2, keepdims=True)
-    e = np.ascontiguousarray(e)
+    e = np.ascontiguousarray(CF_random_unit(N, rng))
     mm = np.ascontiguousarray(np.broadcast_to(sysd["mmom"], (1, N)))
     eM = np.ascontiguousarray(e * mm[..., None])
     lam, T, lg = np.full(N, 0.1), np.full(N, 300.0), np.ones(N)
     t0 = time.perf_counter()
-    L.or_sd_run(ctypes.byref(ham.h), N, 1, 1, 1, 0, 1e-16, lam, T, lg, mm, e, eM, 7, 1, None)
+    L.or_sd_run(ctypes.byref(ham.h), N, 1, 1, 1, 0, 1e-16, lam, T, lg, mm, e, eM, 7, 2, None)
     one = time.perf_counter() - t0
     nsteps = max(1, int(seconds_target / max(one, 1e-6)))
     t0 = time.perf_counter()
-    L.or_sd_run(ctypes.byref(ham.h), N, 1, 1, nsteps, 1, 1e-16, lam, T, lg, mm, e, eM, 7, 1, None)
+    L.or_sd_run(ctypes.byref(ham.h), N, 1, 1, nsteps, 1, 1e-16, lam, T, lg, mm, e, eM, 7, 2, None)
     dt = time.perf_counter() - t0
-    return {"value": N * nsteps / dt, "unit": "spin-updates/s", "cores": 1, "kind": "port",
-            "sample": f"same workload ({N} spins, 58 neighbours, midpoint FP64, Philox noise "
-                      f"at 300 K), {nsteps} steps in {dt:.1f} s on one host core"}
+    return {"value": N * nsteps / dt, "unit": "spin-updates/s", "cores": used, "kind": "port",
+            "host_cores_available": n_cores,
+            "sample": f"same workload ({N} spins, 58 neighbours, midpoint FP64, FP64 Philox "
+                      f"noise at 300 K), {nsteps} steps in {dt:.1f} s on {used} host threads"}
 
 
 def copy_calibration(mib: int = 512, reps: int = 5):
@@ -169,8 +229,10 @@ def form_c3(U, CF, steps=5000):
     pert = e0 + 1e-7 * rng.normal(size=e0.shape)
     pert = np.ascontiguousarray(pert / np.linalg.norm(pert, axis=2, keepdims=True))
     engs = {}
+    lat = sysd["lattice"]
     for prec in ("fp64", "fp32", "fp64_perturbed"):
         eng = U.Engine(N, 1, prec[:4])
+        eng.set_lattice_dims(lat.NA, *lat.n)    # the LDS-tiled kernels are tuning candidates
         eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
         eng.set_dm(sysd["dmlist"], sysd["dmlistsize"], sysd["dm_vect"])
         eng.set_anisotropy(sysd["taniso"], sysd["eaniso"], sysd["kaniso"], sysd["sb"])
@@ -228,69 +290,158 @@ def _c4_replicas(CF, n_rep=64):
     return reps
 
 
-def form_c4(U, CF, sweeps=40):
+def _batch_means(x, nblk=8):
+    """Mean and standard error of a time series by batch means over nblk
+    blocks (x: (T, ...), the time axis first)."""
+    n = x.shape[0] // nblk * nblk
+    b = x[:n].reshape(nblk, n // nblk, *x.shape[1:]).mean(axis=1)
+    return b.mean(axis=0), b.std(axis=0, ddof=1) / np.sqrt(nblk)
+
+
+def _binder_blocks(m, nblk=8):
+    """U4 = 1 - <m^4>/(3<m^2>^2) of a series and its jackknife error over
+    nblk blocks."""
+    n = m.shape[0] // nblk * nblk
+    b2 = (m[:n] ** 2).reshape(nblk, n // nblk, *m.shape[1:]).mean(axis=1)
+    b4 = (m[:n] ** 4).reshape(nblk, n // nblk, *m.shape[1:]).mean(axis=1)
+    u = 1.0 - b4.mean(axis=0) / (3.0 * b2.mean(axis=0) ** 2)
+    jk = np.stack([1.0 - np.delete(b4, i, 0).mean(0) / (3.0 * np.delete(b2, i, 0).mean(0) ** 2)
+                   for i in range(nblk)])
+    err = np.sqrt((nblk - 1) / nblk * ((jk - jk.mean(0)) ** 2).sum(0))
+    return u, err
+
+
+def _z_stats(a, sa, b, sb):
+    z = np.abs(a - b) / np.sqrt(sa ** 2 + sb ** 2 + 1e-300)
+    return {"within_2sigma": float(np.mean(z <= 2.0)), "max_z": float(np.max(z)),
+            "median_z": float(np.median(z))}
+
+
+def form_c4(U, CF, sweeps=100, stat_sweeps=600, every=5, modes=("M", "H"), plain=True):
     """Batched MC: random binary bcc alloy, 64 replicas of 8^3..48^3 cells
     joined into one lattice, 32 temperatures 200..1600 K as ensembles,
-    checkerboard Metropolis; FP32 spins with FP64 observables (one cumulant
-    sample per sweep).  The plain path is the FP64 engine run one replica at
-    a time (all 32 temperatures per launch), on a bounded sample."""
+    checkerboard Metropolis and heat bath; FP32 spins with FP64 observables.
+
+    Throughput: `sweeps` sweeps in graphs of 10, one cumulant sample (energy
+    included) per 10 sweeps, FP32 and FP64, against the plain FP64 engine run
+    one replica at a time with all 32 temperatures per launch (graphs of 10,
+    every 8th replica).  Observables: a further `stat_sweeps` sweeps from the
+    ordered state, the first sixth discarded, |m| and energy per spin of every
+    (replica, temperature) measured every `every` sweeps; <E>, <|m|> and U4
+    with 2-sigma errors by 8 batch means (jackknife for U4), FP32 against
+    FP64, per temperature for the largest replica and as z-scores over all
+    2048 (replica, temperature) pairs."""
     from uppasd_amd import batch as Bt
     reps = _c4_replicas(CF)
     j = Bt.join(reps)
     M = 32
     temps = np.linspace(200.0, 1600.0, M)
     Ntot = j.natom
-    rng = np.random.default_rng(44)
-    out = {"workload": f"bcc alloy 64 replicas (8^3..48^3 cells, {Ntot} spins) x 32 T, Metropolis",
-           "unit": "spin-updates/s"}
-    for prec in ("fp32", "fp64"):
-        eng = Bt.engine_for(j, M, prec)
-        e0 = np.ascontiguousarray(CF.random_unit(Ntot, rng, M))
-        eng.set_moments(e0, np.ascontiguousarray(np.broadcast_to(j.mmom, (M, Ntot))))
-        eng.autotune_mc("M", temps, sweeps=1)
-        eng.set_graph_steps(10)
-        eng.mc_sweeps("M", temps, 10, 7)
-        eng.sync()
+    nrep = len(reps)
+    sizes = np.diff(j.offsets).astype(np.float64)
+    out = {"workload": f"bcc alloy 64 replicas (8^3..48^3 cells, {Ntot} spins) x 32 T "
+                       f"200..1600 K, Metropolis and heat bath",
+           "unit": "spin-updates/s", "sweeps_timed": sweeps, "stat_sweeps": stat_sweeps}
+    up = np.zeros((M, Ntot, 3))
+    up[..., 2] = 1.0
+    mm = np.ascontiguousarray(np.broadcast_to(j.mmom, (M, Ntot)))
+    series = {}
+    for mode in modes:
+        for prec in ("fp32", "fp64"):
+            eng = Bt.engine_for(j, M, prec)
+            eng.set_moments(np.ascontiguousarray(up), mm)
+            eng.autotune_mc(mode, temps, sweeps=1)
+            eng.set_graph_steps(10)
 
-        def run(eng=eng):
-            for _ in range(sweeps // 10):
-                eng.mc_sweeps("M", temps, 10, 7)
-                eng.accumulate(with_energy=True)
+            def run(eng=eng):
+                for _ in range(sweeps // 10):
+                    eng.mc_sweeps(mode, temps, 10, 7)
+                    eng.accumulate(with_energy=True)
+                eng.sync()
+            t = _sync_time(run)
+            out[f"{mode}_batched_{prec}_value"] = Ntot * M * sweeps / t
+            if not stat_sweeps:
+                del eng
+                continue
+            eng.set_moments(np.ascontiguousarray(up), mm)
+            ms, es = [], []
+            for k in range(stat_sweeps // every):
+                eng.mc_sweeps(mode, temps, every, 11)
+                if k >= stat_sweeps // every // 6:
+                    msum, en = eng.measure_segments(with_energy=True)
+                    ms.append(np.linalg.norm(msum, axis=2) / sizes[:, None])
+                    es.append(en / sizes[:, None])
+            series[(mode, prec)] = (np.array(ms), np.array(es))
+            del eng
+        out[f"{mode}_batched_fp32_over_fp64"] = (out[f"{mode}_batched_fp32_value"]
+                                                 / out[f"{mode}_batched_fp64_value"])
+        if not stat_sweeps:
+            continue
+        # FP32 against FP64 observables
+        m32, e32 = series[(mode, "fp32")]
+        m64, e64 = series[(mode, "fp64")]
+        am32, sm32 = _batch_means(m32)
+        am64, sm64 = _batch_means(m64)
+        ae32, se32 = _batch_means(e32)
+        ae64, se64 = _batch_means(e64)
+        u32, su32 = _binder_blocks(m32)
+        u64, su64 = _binder_blocks(m64)
+        big = nrep - 1
+        out[f"{mode}_observables"] = {
+            "samples": int(m32.shape[0]),
+            "z_all_pairs": {"E": _z_stats(ae32, se32, ae64, se64),
+                            "abs_m": _z_stats(am32, sm32, am64, sm64),
+                            "U4": _z_stats(u32, su32, u64, su64)},
+            "largest_replica_per_T": {
+                "T": [float(x) for x in temps],
+                "E_fp32": [[float(a), 2 * float(b)] for a, b in zip(ae32[big], se32[big])],
+                "E_fp64": [[float(a), 2 * float(b)] for a, b in zip(ae64[big], se64[big])],
+                "abs_m_fp32": [[float(a), 2 * float(b)] for a, b in zip(am32[big], sm32[big])],
+                "abs_m_fp64": [[float(a), 2 * float(b)] for a, b in zip(am64[big], sm64[big])],
+                "U4_fp32": [[float(a), 2 * float(b)] for a, b in zip(u32[big], su32[big])],
+                "U4_fp64": [[float(a), 2 * float(b)] for a, b in zip(u64[big], su64[big])]},
+            "units": "E in mRy per spin, |m| in muB per spin; pairs [value, 2 sigma]"}
+    # plain FP64: replica by replica (all temperatures per launch), graphs of 10
+    for mode in modes if plain else ():
+        t_plain, n_plain = 0.0, 0
+        for r in range(0, nrep, 8):
+            rep = reps[r]
+            N = rep["lattice"].natom
+            eng = U.Engine(N, M, "fp64")
+            eng.set_exchange(rep["nlist"], rep["nlistsize"], rep["ncoup"])
+            upr = np.zeros((M, N, 3))
+            upr[..., 2] = 1.0
+            eng.set_moments(upr, np.ascontiguousarray(np.broadcast_to(rep["mmom"], (M, N))))
+            eng.set_graph_steps(10)
+            eng.mc_sweeps(mode, temps, 10, 7)
             eng.sync()
-        t = _sync_time(run)
-        out[f"batched_{prec}_value"] = Ntot * M * sweeps / t
-        if prec == "fp32":
-            acc = eng.accumulators()
-            out["U4_first_replica_lowT_highT"] = [float(acc["U4"][0, 0]), float(acc["U4"][0, -1])]
-        del eng
-    # plain FP64: replica by replica, eager launches, bounded to 8 replicas
-    t_plain, n_plain = 0.0, 0
-    for r in range(0, len(reps), 8):
-        rep = reps[r]
-        N = rep["lattice"].natom
-        eng = U.Engine(N, M, "fp64")
-        eng.set_exchange(rep["nlist"], rep["nlistsize"], rep["ncoup"])
-        eng.set_moments(np.ascontiguousarray(CF.random_unit(N, rng, M)),
-                        np.ascontiguousarray(np.broadcast_to(rep["mmom"], (M, N))))
-        eng.mc_sweeps("M", temps, 2, 7)
-        eng.sync()
 
-        def run(eng=eng):
-            eng.mc_sweeps("M", temps, 10, 7)
-            eng.sync()
-        t_plain += _sync_time(run)
-        n_plain += N * M * 10
-    out["plain_fp64_value"] = n_plain / t_plain
-    out["batched_fp32_over_plain_fp64"] = out["batched_fp32_value"] / out["plain_fp64_value"]
+            def run(eng=eng):
+                for _ in range(3):
+                    eng.mc_sweeps(mode, temps, 10, 7)
+                    eng.accumulate(with_energy=True)
+                eng.sync()
+            t_plain += _sync_time(run)
+            n_plain += N * M * 30
+            del eng
+        out[f"{mode}_plain_fp64_value"] = n_plain / t_plain
+        out[f"{mode}_batched_fp32_over_plain_fp64"] = (out[f"{mode}_batched_fp32_value"]
+                                                       / out[f"{mode}_plain_fp64_value"])
     return out
 
 
-def form_c5(U, CF, steps=2000):
+def form_c5(U, CF, steps=10000, graph=50, oracle_check=True):
     """Batched LLG: kagome antiferromagnet (J1 -1 mRy, Dz 0.08 mRy), 256
     ensembles of 24^2..96^2 cells at log-spaced 1..300 K joined into one
-    lattice, FP32 Depondt with hipGraph-captured steps, alpha 0.2, dt 5e-17.
-    Plain path: the FP64 engine one ensemble at a time with eager launches
-    (bounded sample)."""
+    lattice (each ensemble's sites numbered sublattice by sublattice, so its
+    three sublattices are segments), Depondt with hipGraph-captured steps,
+    alpha 0.2, dt 5e-17, `steps` steps; FP64 time-averaged energy and
+    sublattice magnetisation from the device accumulators, one sample per
+    graph of `graph` steps.  FP32 and FP64 batched; the plain FP64 path is
+    the FP64 engine one ensemble at a time with the same graphs and samples
+    (every 32nd ensemble, 1000 steps).  Error: the smallest ensemble (1 K)
+    integrated by the FP64 oracle on the FP32 engine's own deviates, energy
+    per spin at the end and time-averaged over the same samples."""
     from uppasd_amd import batch as Bt
     Ls = np.round(np.linspace(24, 96, 256)).astype(int)
     Ts = np.logspace(0.0, np.log10(300.0), 256)
@@ -298,36 +449,58 @@ def form_c5(U, CF, steps=2000):
     reps = []
     for L, T in zip(Ls, Ts):
         if L not in cache:
-            cache[L] = CF.kagome_afm(int(L), J=-1.0, Dz=0.08)
+            cache[L] = Bt.sublattice_major(CF.kagome_afm(int(L), J=-1.0, Dz=0.08), 3)
         d = dict(cache[L])
         d["temp"] = T
         reps.append(d)
     j = Bt.join(reps, per_site=("temp",))
     Ntot = j.natom
+    seg = np.concatenate([[a + k * (b - a) // 3 for k in range(3)]
+                          for a, b in zip(j.offsets[:-1], j.offsets[1:])] + [[Ntot]])
     rng = np.random.default_rng(55)
     e0 = np.ascontiguousarray(CF.random_unit(Ntot, rng, 1))
     out = {"workload": f"kagome AFM+DMI, 256 ensembles (24^2..96^2 cells, {Ntot} spins), "
-                       f"T 1..300 K, Depondt alpha 0.2, dt 5e-17, {steps} steps",
+                       f"T 1..300 K, Depondt alpha 0.2, dt 5e-17, {steps} steps, time-averaged "
+                       f"energy and sublattice |m| per ensemble every {graph} steps",
            "unit": "spin-updates/s"}
     lam = np.full(Ntot, 0.2)
-    for prec, graph in (("fp32", 50), ("fp64", 50)):
+    avg = {}
+    for prec in ("fp32", "fp64"):
         eng = Bt.engine_for(j, 1, prec)
+        eng.set_segments(seg)
         eng.set_moments(e0, j.mmom[None])
         eng.set_llg(lam, j.extra["temp"], 5e-17)
         eng.set_noise(U.UASD_NOISE_PHILOX, 5)
         eng.autotune_llg(5, steps=2)
         eng.set_graph_steps(graph)
-        eng.sd_steps(5, graph)
-        eng.sync()
 
         def run(eng=eng):
             for _ in range(steps // graph):
                 eng.sd_steps(5, graph)
-                eng.accumulate(with_energy=False)
+                eng.accumulate(with_energy=True)
             eng.sync()
         t = _sync_time(run)
         out[f"batched_{prec}_value"] = Ntot * steps / t
+        acc = eng.accumulators()
+        e_ens = acc["e"][:, 0].reshape(256, 3).mean(axis=1)      # equal sublattices
+        avg[prec] = (e_ens, acc["m"][:, 0].reshape(256, 3))
+        if prec == "fp32":
+            # ensemble 0 = segments 0..2 (its sublattices, equal in size)
+            ene_final = float(eng.energy_terms()[0:3, 0].sum(axis=1).mean())
         del eng
+    out["batched_fp32_over_fp64"] = out["batched_fp32_value"] / out["batched_fp64_value"]
+    out["time_avg_energy_per_spin_mRy"] = {"T_1K": float(avg["fp32"][0][0]),
+                                           "T_300K": float(avg["fp32"][0][-1])}
+    out["time_avg_sublattice_m"] = {"T_1K": [float(x) for x in avg["fp32"][1][0]],
+                                    "T_300K": [float(x) for x in avg["fp32"][1][-1]]}
+    out["fp32_vs_fp64_time_avg_energy_rel_diff"] = {
+        "median": float(np.median(np.abs(avg["fp32"][0] / avg["fp64"][0] - 1.0))),
+        "max": float(np.max(np.abs(avg["fp32"][0] / avg["fp64"][0] - 1.0))),
+        "note": "independent thermal streams (FP32 and FP64 deviates): a statistical difference"}
+    if oracle_check:
+        out["oracle_error_smallest_ensemble"] = _c5_oracle_error(reps[0], e0[:, :j.offsets[1]],
+                                                                 steps, graph, ene_final,
+                                                                 avg["fp32"][0][0])
     t_plain, n_plain = 0.0, 0
     for r in range(0, 256, 32):
         rep = reps[r]
@@ -335,52 +508,131 @@ def form_c5(U, CF, steps=2000):
         eng = U.Engine(N, 1, "fp64")
         eng.set_exchange(rep["nlist"], rep["nlistsize"], rep["ncoup"])
         eng.set_dm(rep["dmlist"], rep["dmlistsize"], rep["dm_vect"])
+        eng.set_segments(np.array([0, N // 3, 2 * N // 3, N]))
         eng.set_moments(np.ascontiguousarray(CF.random_unit(N, rng, 1)), rep["mmom"][None])
         eng.set_llg(0.2, rep["temp"], 5e-17)
         eng.set_noise(U.UASD_NOISE_PHILOX, 5)
-        eng.sd_steps(5, 5)
+        eng.set_graph_steps(graph)
+        eng.sd_steps(5, graph)
         eng.sync()
 
         def run(eng=eng):
-            eng.sd_steps(5, 100)
+            for _ in range(1000 // graph):
+                eng.sd_steps(5, graph)
+                eng.accumulate(with_energy=True)
             eng.sync()
         t_plain += _sync_time(run)
-        n_plain += N * 100
+        n_plain += N * 1000
+        del eng
     out["plain_fp64_value"] = n_plain / t_plain
     out["batched_fp32_over_plain_fp64"] = out["batched_fp32_value"] / out["plain_fp64_value"]
     return out
 
 
-def list_kernel_compare(U, sysd, args, order, e0, steps=300):
-    """The same workload on the list-gather kernels (lattice not declared, so
-    no stencil), autotuned: its corrector time and roofline beside the tiled
-    kernel's.  Bounded to a few hundred steps."""
-    N = sysd["lattice"].natom
+def _c5_oracle_error(rep, e0, steps, graph, e_final_dev, e_avg_dev):
+    """The smallest C5 ensemble (first in the batch, so its device sites and
+    Philox keys are 0..N-1) integrated by the FP64 oracle on the FP32
+    engine's deviates: relative error of the FP32 device run's final and
+    time-averaged energy per spin."""
+    import ctypes
+    import oracle as O
+    N = rep["lattice"].natom
+    ham = O.OracleHam(N, rep["nlist"], rep["nlistsize"], rep["ncoup"], dmlist=rep["dmlist"],
+                      dmlistsize=rep["dmlistsize"], dm_vect=rep["dm_vect"])
+    e = np.ascontiguousarray(e0.copy())
+    mm = np.ascontiguousarray(rep["mmom"][None].astype(np.float64))
+    eM = np.ascontiguousarray(e * mm[..., None])
+    lam, T, lg = np.full(N, 0.2), np.full(N, float(rep["temp"])), np.ones(N)
+    fc = 9.274009994e-24 / 2.179872325e-21
+    samples = []
+    for k in range(steps // graph):
+        O.lib().or_sd_run(ctypes.byref(ham.h), N, 1, 5, graph, k * graph, 5e-17, lam, T, lg, mm,
+                          e, eM, 5, 1, None)
+        samples.append(ham.energy_terms(eM)[0].sum() * fc)
+    w = np.arange(1, len(samples) + 1, dtype=np.float64)
+    e_avg = float(np.sum(w * np.array(samples)) / np.sum(w))
+    return {"N": N, "T_K": float(rep["temp"]), "steps": steps,
+            "final_energy_rel_err": abs(e_final_dev - samples[-1]) / abs(samples[-1]),
+            "time_avg_energy_rel_err": abs(e_avg_dev - e_avg) / abs(e_avg)}
+
+
+def c2_variant(U, args, label: str, sysd=None, L: int = 64, temp: float = 300.0,
+               noise: str = "philox", stencil: bool = True, steps: int = 400,
+               warmup: int = 40):
+    """The plain FP64 workload under another noise source, size or kernel
+    family, bounded to a few hundred steps: value, step time and roofline
+    fractions by the workload's byte model and by the kernels' own bytes,
+    with the counter traffic of the committed profile of the same run where
+    there is one.  noise: "philox" (in-kernel), "injected" (one N(0,1) array
+    uploaded once and read by both halves of every step; a drop-in caller
+    would upload a fresh one per step over PCIe), "none" (T = 0)."""
+    if sysd is None:
+        sysd = build_c2(L)
+    lat = sysd["lattice"]
+    N = lat.natom
     W = int(sysd["nlistsize"].max())
-    eng = U.Engine(N, 1, args.precision, site_order=order)
+    eng = U.Engine(N, 1, args.precision, site_order=lat.NA)
+    if stencil:
+        eng.set_lattice_dims(lat.NA, *lat.n)
     eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
+    rng = np.random.default_rng(2)
+    e0 = rng.normal(size=(1, N, 3))
+    e0 /= np.linalg.norm(e0, axis=2, keepdims=True)
     eng.set_moments(np.ascontiguousarray(e0), sysd["mmom"][None])
-    eng.set_llg(0.1, 300.0, 1e-16)
-    eng.set_noise(U.UASD_NOISE_PHILOX, 0xC2C2C2C2)
+    eng.set_llg(0.1, temp if noise != "none" else 0.0, 1e-16)
+    if noise == "philox":
+        eng.set_noise(U.UASD_NOISE_PHILOX, 0xC2C2C2C2)
+    elif noise == "injected":
+        eng.set_noise(U.UASD_NOISE_INJECTED, 0, np.ascontiguousarray(rng.normal(size=(1, N, 3))))
+    else:
+        eng.set_noise(U.UASD_NOISE_NONE)
     eng.autotune_llg(args.solver, steps=3)
-    eng.sd_steps(args.solver, 20)
+    eng.sd_steps(args.solver, warmup)
     eng.sync()
-    eng.set_kernel_timing(True, args.timing_every)
-    t0 = time.perf_counter()
-    eng.sd_steps(args.solver, steps)
-    eng.sync()
-    elapsed = time.perf_counter() - t0
-    n, ms = eng.kernel_timing(1)
-    t = ms / max(n, 1) * 1e-3
-    s = 8 if args.precision == "fp64" else 4
-    ib = eng.index_bytes()[0]
-    kb = kernel_bytes(W, s, ib, b2=args.solver in (4, 5))
-    ach = kb["corrector"] * N / t / 1e9
+    tm = timed_steps(eng, args.solver, steps, args.timing_every)
     g, b = eng.tuning(eng.TUNE_LLG)
-    return {"value": N * steps / elapsed, "corrector_ms": round(t * 1e3, 4),
-            "bytes_per_site": kb, "index_bytes": ib, "gather": g, "block": b & 0xFFFF,
-            "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": ach / HBM_PEAK_GBS}}
+    tile, pipe = (b >> 18) & 0xF, (b >> 22) & 1
+    s = 8 if args.precision == "fp64" else 4
+    ib = 0 if tile else eng.index_bytes()[0]
+    kb = kernel_bytes(W, s, ib, cls=0 if tile else 4, noise=noise, b2=args.solver in (4, 5))
+    sb = survey_bytes(W, s, injected=noise == "injected")
+    del eng
+    variant = f"tile{tile}{'r' if pipe else ''}" if tile else f"idx{ib}"
+    tr = profiled_traffic(label, variant)
+    traffic = tr["predictor"] + tr["corrector"] if tr else None
+    out = {"workload": f"bcc {lat.n[0]}^3 x 2 ({N} spins), {W} neighbours, "
+                       f"{SOLVER_NAMES[args.solver]}, noise {noise}"
+                       f"{'' if noise == 'none' else f', T {temp:g} K'}",
+           "value": N * steps / tm["wall_s"], "ms_per_step": tm["step_s"] * 1e3,
+           "kernel_ms": [round(tm["pred_s"] * 1e3, 4), round(tm["corr_s"] * 1e3, 4)],
+           "kernel": variant,
+           "bytes_per_update_model": sb["step"], "own_bytes_per_site": kb["step"],
+           "frac_model": sb["step"] * N / tm["step_s"] / 1e9 / HBM_PEAK_GBS,
+           "frac_own_bytes": kb["step"] * N / tm["step_s"] / 1e9 / HBM_PEAK_GBS,
+           "traffic_per_step": traffic, "steps": steps}
+    if traffic:
+        out["traffic_over_own_bytes"] = traffic / (kb["step"] * N)
+    return out
+
+
+def profiled_traffic(label: str, variant: str) -> dict:
+    """HBM bytes per launch of each LLG kernel half from the committed
+    rocprofv3 summary `profiles/k_llg_<label>_<variant>.md` (its FETCH_SIZE /
+    WRITE_SIZE table, already scaled by a plain copy of known size):
+    {"predictor": B, "corrector": B}, or {} when there is no such summary."""
+    path = os.path.join(ROOT, "profiles", f"k_llg_{label}_{variant}.md")
+    if not os.path.exists(path):
+        return {}
+    in_table, out = False, {}
+    for line in open(path):
+        cells = [c.strip() for c in line.split("|")]
+        if len(cells) > 3 and cells[1] == "kernel":
+            in_table = cells[2].startswith("FETCH_SIZE")
+            continue
+        if in_table and len(cells) > 6 and cells[1].startswith("`uasd::k_llg"):
+            half = int(cells[1].split(",")[2])      # k_llg[_tile]<R, ALG, HALF, ...>
+            out["corrector" if half else "predictor"] = (float(cells[4]) + float(cells[5])) * 1e6
+    return out if len(out) == 2 else {}
 
 
 def main():
@@ -407,6 +659,17 @@ def main():
                     help="do not declare the lattice (list-gather kernels only)")
     ap.add_argument("--site-order", type=int, default=0,
                     help="device site order: 0 = basis-major (NA of the lattice), 1 = reference")
+    ap.add_argument("--variants", default="T0,injected,L128",
+                    help="comma list of runs of the plain workload beside the headline: T0 "
+                         "(no noise), injected (caller's N(0,1) stream), L128 (bcc 128^3 x 2, "
+                         "above the MALL), or 'none'")
+    ap.add_argument("--c4-sweeps", type=int, default=100,
+                    help="timed sweeps of each batched MC run of the c4 form")
+    ap.add_argument("--c4-stat-sweeps", type=int, default=600,
+                    help="sweeps of the c4 form's FP32-vs-FP64 observables (0: none)")
+    ap.add_argument("--c4-modes", default="M,H", help="MC modes of the c4 form")
+    ap.add_argument("--no-c4-plain", action="store_true",
+                    help="skip the c4 form's plain FP64 replica-by-replica runs")
     ap.add_argument("--forms", default="c3,c4,c5",
                     help="comma list of extra forms to run (c3,c4,c5) or 'none'")
     args = ap.parse_args()
@@ -452,34 +715,24 @@ def main():
 
     eng.sd_steps(args.solver, args.warmup)
     eng.sync()
-    eng.set_kernel_timing(True, args.timing_every)
-    eng.sync()
-    t0 = time.perf_counter()
-    eng.sd_steps(args.solver, args.steps)
-    eng.sync()
-    elapsed = time.perf_counter() - t0
-    n_corr, ms_corr = eng.kernel_timing(1)
-    n_pred, ms_pred = eng.kernel_timing(0)
-    eng.set_kernel_timing(False)
+    tm = timed_steps(eng, args.solver, args.steps, args.timing_every)
+    elapsed = tm["wall_s"]
+    t_pred, t_corr = tm["pred_s"], tm["corr_s"]
 
     s = 8 if args.precision == "fp64" else 4
     ib = 0 if tile else eng.index_bytes()[0]     # the tile kernel reads no lists
     kb = kernel_bytes(W, s, ib, cls=0 if tile else 4,   # one species: classes per basis
                       b2=args.solver in (4, 5))
-    t_corr = ms_corr / max(n_corr, 1) * 1e-3
-    t_pred = ms_pred / max(n_pred, 1) * 1e-3
-    achieved = kb["corrector"] * N / t_corr / 1e9
+    sb = survey_bytes(W, s)
+    # the dominant kernel is the corrector: the workload's algorithmic bytes
+    # per update of its pass over every site, per its duration
+    achieved = sb["corrector"] * N / t_corr / 1e9
+    achieved_own = kb["corrector"] * N / t_corr / 1e9
     value = N * args.steps / elapsed
     traffic = None
-    if args.precision == "fp64" and args.L == 64 and args.solver == 1 and not nt:
-        if tile:
-            plane = tile_plane(sysd, tile)
-            # exchange-only Hamiltonian: the kernel without DM/anisotropy/Zeeman terms
-            traffic = profiled_traffic(f"uasd::k_llg_tile<double, 1, 1, {plane}, false>", order,
-                                       f"tile{tile}{'r' if pipe else ''}")
-        else:
-            traffic = profiled_traffic(f"uasd::k_llg<double, 1, 1, true, {gather}>", order,
-                                       f"idx{ib}")
+    if args.precision == "fp64" and args.L == 64 and args.solver == 1 and order == 2:
+        variant = f"tile{tile}{'r' if pipe else ''}" if tile else f"idx{ib}"
+        traffic = profiled_traffic("fp64_midpoint_bcc64_58nn_philox", variant).get("corrector")
     _, eM = eng.get_moments()
     norm_drift = float(np.max(np.abs(np.linalg.norm(eM[0], axis=1) / sysd["mmom"] - 1.0)))
 
@@ -490,7 +743,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": tm["step_s"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -509,9 +762,13 @@ def main():
                        "resident_tile_loop": bool(pipe),
                        "site_order": order,
                        "autotuned": not args.gather, "tune_s": round(tune_s, 3)},
+            "wall_ms_per_step": elapsed / args.steps * 1e3,
             "kernel_ms": {"predictor": round(t_pred * 1e3, 4), "corrector": round(t_corr * 1e3, 4),
-                          "timed_launches": [n_pred, n_corr],
-                          "method": f"HIP events around one step in {args.timing_every}"},
+                          "sampled": tm["sampled_ms"], "sampled_launches": tm["sampled_launches"],
+                          "method": ("step time from HIP events around the timed call, split "
+                                     "by the halves' shares of events around one step in "
+                                     f"{args.timing_every} of a second run")},
+            "bytes_per_update_model": sb,
             "bytes_per_site": kb,
             "index_bytes": ib,
             "max_norm_drift": norm_drift,
@@ -523,18 +780,48 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "bytes_per_update": sb["corrector"],
+            "own_bytes_per_site": kb["corrector"],
+            "own_bytes_achieved": achieved_own,
+            "own_bytes_frac": achieved_own / HBM_PEAK_GBS,
+            "step_frac": sb["step"] * N / tm["step_s"] / 1e9 / HBM_PEAK_GBS,
         },
     }
     del eng
+    variants = [v for v in args.variants.split(",") if v and v != "none"]
     if tile and not args.no_list_compare:
-        out["list_kernel"] = list_kernel_compare(U, sysd, args, order, e0)
+        variants = ["list"] + variants
+    if variants:
+        out["variants"] = {}
+    for v in variants:
+        t0 = time.perf_counter()
+        if v == "list":      # the same workload on the list-gather kernels
+            res = c2_variant(U, args, "fp64_midpoint_bcc64_58nn_philox", sysd=sysd,
+                             stencil=False)
+        elif v == "T0":
+            res = c2_variant(U, args, "fp64_midpoint_bcc64_58nn_T0", sysd=sysd, noise="none")
+        elif v == "injected":
+            res = c2_variant(U, args, "fp64_midpoint_bcc64_58nn_injected", sysd=sysd,
+                             noise="injected")
+        elif v == "L128":    # above the 256 MB MALL
+            res = c2_variant(U, args, "fp64_midpoint_bcc128_58nn_philox", L=128, steps=200,
+                             warmup=20)
+        else:
+            raise SystemExit(f"unknown variant {v}")
+        res["wall_s"] = round(time.perf_counter() - t0, 1)
+        out["variants"][v] = res
+        print(f"variant {v} done in {res['wall_s']} s", file=sys.stderr, flush=True)
     forms = [f for f in args.forms.split(",") if f and f != "none"]
     if forms:
         from uppasd_amd import configs as CF
         out["forms"] = {}
         for f in forms:
             t0 = time.perf_counter()
-            res = {"c3": form_c3, "c4": form_c4, "c5": form_c5}[f](U, CF)
+            if f == "c4":
+                res = form_c4(U, CF, sweeps=args.c4_sweeps, stat_sweeps=args.c4_stat_sweeps,
+                              modes=tuple(args.c4_modes.split(",")), plain=not args.no_c4_plain)
+            else:
+                res = {"c3": form_c3, "c5": form_c5}[f](U, CF)
             res["wall_s"] = round(time.perf_counter() - t0, 1)
             out["forms"][f] = res
             print(f"form {f} done in {res['wall_s']} s", file=sys.stderr, flush=True)

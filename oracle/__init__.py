@@ -175,8 +175,11 @@ def _declare(L):
     L.or_mc_sweep_colored_ex.argtypes = [C.POINTER(Ham), C.c_int, C.c_int, dp, C.c_char,
                                          C.c_uint64, C.c_uint64, C.c_int, ip, ip, dp, dp, dp,
                                          dp, C.c_int]
-    L.or_trial_move_f32.argtypes = [dp, C.c_double, dp, C.c_uint64, C.c_uint64, C.c_uint32,
-                                    C.c_uint32, dp]
+    L.or_trial_move_f32.argtypes = [dp, C.c_double, np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"), dp]
+    L.or_counter_words.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32,
+                                   np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")]
+    L.or_u24.argtypes = [C.c_uint32]
+    L.or_u24.restype = C.c_float
     L.or_set_threads.argtypes = [C.c_int]
     L.or_threads.restype = C.c_int
     L.or_thermal3_f64.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, dp]
@@ -489,3 +492,67 @@ class RefSD:
         lam = np.ascontiguousarray(np.broadcast_to(lam, (self.N,)), dtype=np.float64)
         temp = np.ascontiguousarray(np.broadcast_to(temp, (self.N,)), dtype=np.float64)
         return self.L.ref_sd_run(int(alg), int(nsteps), float(dt), lam, temp, mmom, emom, emomM)
+
+
+def shells_and_map(lat: LatticeSpec, lines, sym: int, hdim: int = 1, use_nntype: bool = True,
+                   nchmax: int = 1, do_ralloy: int = 0) -> dict:
+    """read_exchange (restated) and the oracle's setup_nm: the shell arrays
+    (nn, redcoord (NT,max_no_shells,3) Fortran order, nntype) and the map nm
+    (max_no_equiv, max_no_shells, Natom) C order, nmdim (Natom,
+    max_no_shells) C order, with max_no_neigh and max_no_equiv."""
+    L = lib()
+    ls = lat.c_struct()
+    isite = np.ascontiguousarray([int(x[0]) for x in lines], dtype=np.int32)
+    jsite = np.ascontiguousarray([int(x[1]) for x in lines], dtype=np.int32)
+    ichem = np.ascontiguousarray([int(x[2]) for x in lines], dtype=np.int32)
+    jchem = np.ascontiguousarray([int(x[3]) for x in lines], dtype=np.int32)
+    r = np.ascontiguousarray([[float(v) for v in x[4:7]] for x in lines]).reshape(-1)
+    val = np.ascontiguousarray([[float(v) for v in x[7:7 + hdim]] for x in lines]).reshape(-1)
+    sh = Shells()
+    if L.or_read_exchange_lines(C.byref(ls), do_ralloy, nchmax, hdim, len(lines), isite, jsite,
+                                ichem, jchem, r, val, C.byref(sh)) != 0:
+        raise RuntimeError("or_read_exchange_lines failed")
+    nm = NMap()
+    L.or_setup_nm(C.byref(ls), sym, C.byref(sh), 1 if use_nntype else 0, C.byref(nm))
+    N, S, E = nm.Natom, nm.max_no_shells, nm.max_no_equiv
+    out = {"NT": sh.NT, "max_no_shells": sh.max_no_shells,
+           "nn": _np_from(sh.nn, sh.NT, np.int32).copy(),
+           "redcoord": _np_from(sh.redcoord, sh.NT * sh.max_no_shells * 3, np.float64).copy(),
+           "nntype": _np_from(sh.nntype, sh.NT * sh.max_no_shells, np.int32).copy(),
+           "nm": _np_from(nm.nm, N * S * E, np.int32).reshape(E, S, N).copy(),
+           "nmdim": _np_from(nm.nmdim, N * S, np.int32).reshape(N, S).copy(),
+           "max_no_neigh": nm.max_no_neigh, "max_no_equiv": E}
+    L.or_nmap_free(C.byref(nm))
+    L.or_shells_free(C.byref(sh))
+    return out
+
+
+def ref_setup_nm(lat: LatticeSpec, shells: dict, sym: int, use_nntype: bool = True) -> dict:
+    """The reference's own setup_nm (oracle/_ref) on the same lattice and
+    shell arrays; nm and nmdim in the layouts of shells_and_map."""
+    L = ref_lib()
+    if L is None:
+        raise RuntimeError(f"{_REF_PATH} is not built (oracle.build_ref())")
+    L.ref_setup_nm.argtypes = [C.c_int] * 6 + [dp, dp, dp, C.c_char_p, ip, dp, C.c_int, ip, dp,
+                                               C.c_int, C.c_int, ip, C.POINTER(C.c_int),
+                                               C.POINTER(C.c_int)]
+    L.ref_nm_shape.argtypes = [ip]
+    L.ref_nm_copy.argtypes = [ip, ip]
+    N = lat.natom
+    cells = N // lat.NA
+    atype = np.ascontiguousarray(np.tile(lat.atype, cells), dtype=np.int32)   # i = b + NA*cell
+    bas = np.ascontiguousarray(lat.bas, dtype=np.float64)                     # (NA,3) = (3,NA) F
+    mnn, meq = C.c_int(0), C.c_int(0)
+    L.ref_setup_nm(N, shells["NT"], lat.NA, *lat.n, np.ascontiguousarray(lat.cell[0]),
+                   np.ascontiguousarray(lat.cell[1]), np.ascontiguousarray(lat.cell[2]),
+                   "".join(lat.bc).encode(), atype, bas, shells["max_no_shells"], shells["nn"],
+                   shells["redcoord"], sym, 1 if use_nntype else 0, shells["nntype"],
+                   C.byref(mnn), C.byref(meq))
+    d = np.zeros(5, np.int32)
+    L.ref_nm_shape(d)
+    nm = np.zeros(int(d[0]) * int(d[1]) * int(d[2]), np.int32)
+    nmdim = np.zeros(int(d[3]) * int(d[4]), np.int32)
+    L.ref_nm_copy(nm, nmdim)
+    return {"nm": nm.reshape(int(d[2]), int(d[1]), int(d[0])),
+            "nmdim": nmdim.reshape(int(d[4]), int(d[3])),
+            "max_no_neigh": mnn.value, "max_no_equiv": meq.value}

@@ -28,6 +28,7 @@ module uasd_ref_driver
    use RandomNumbers, only : setup_rng_hb, rannum, allocate_randomwork, ranv, rng_defaults, &
       use_vsl
    use UpdateMoments, only : moment_update
+   use NeighbourMap, only : setup_nm
    use omp_lib
    implicit none
 
@@ -44,6 +45,8 @@ module uasd_ref_driver
    integer, allocatable :: maxNoConstl(:), unitCellType(:,:), constellationsNeighType(:,:,:)
    integer, allocatable :: cell_index(:), macro_nlistsize(:)
    real(dblprec), allocatable :: constlNCoup(:,:,:), constellations(:,:,:), emomM_macro(:,:,:)
+   ! the last neighbour map of ref_setup_nm
+   integer, allocatable :: nm_keep(:,:,:), nmdim_keep(:,:)
 
 contains
 
@@ -236,5 +239,60 @@ contains
          call moment_update(nat, mens, mmom, mmom0, mmom2, emom, emom2, emomM, mmomi, 0, 'N')
       end do
    end function ref_sd_run
+
+   !> setup_nm (neighbourmap.f90:32-359) for a periodic crystal with basis:
+   !> reference atom i = b + NA*(x + N1*(y + N2*z)) (1-based as the
+   !> reference numbers them), atype (natom) the type of every atom, bc the
+   !> three boundary characters, shells as read_exchange leaves them (nn (NT),
+   !> redcoord (NT,max_no_shells,3), nntype (NT,max_no_shells) when
+   !> use_nntype /= 0).  Keeps nm / nmdim for ref_nm_copy; returns the
+   !> dimensions.  Block size 1, no random alloy.
+   subroutine ref_setup_nm(natom, nt, na, n1, n2, n3, c1, c2, c3, bc, atype, bas, &
+         max_no_shells, nn, redcoord, sym, use_nntype, nntype, max_no_neigh, max_no_equiv) &
+         bind(C, name='ref_setup_nm')
+      integer(c_int), value :: natom, nt, na, n1, n2, n3, max_no_shells, sym, use_nntype
+      real(c_double), intent(in) :: c1(3), c2(3), c3(3), bas(3, na)
+      real(c_double), intent(in) :: redcoord(nt, max_no_shells, 3)
+      character(kind=c_char), intent(in) :: bc(3)
+      integer(c_int), intent(in) :: atype(natom), nn(nt), nntype(nt, max_no_shells)
+      integer(c_int), intent(out) :: max_no_neigh, max_no_equiv
+      integer :: mnn, meq
+      real(dblprec) :: cc1(3), cc2(3), cc3(3), bb(3, na), rc(nt, max_no_shells, 3)
+
+      if (allocated(nm_keep)) deallocate(nm_keep)
+      if (allocated(nmdim_keep)) deallocate(nmdim_keep)
+      cc1 = c1
+      cc2 = c2
+      cc3 = c3
+      bb = bas
+      rc = redcoord
+      if (use_nntype /= 0) then
+         call setup_nm(natom, nt, na, n1, n2, n3, cc1, cc2, cc3, bc(1), bc(2), bc(3), 1, atype, &
+            bb, mnn, max_no_shells, meq, sym, nn, rc, nm_keep, nmdim_keep, 0, natom, &
+            nntype=nntype)
+      else
+         call setup_nm(natom, nt, na, n1, n2, n3, cc1, cc2, cc3, bc(1), bc(2), bc(3), 1, atype, &
+            bb, mnn, max_no_shells, meq, sym, nn, rc, nm_keep, nmdim_keep, 0, natom)
+      end if
+      max_no_neigh = mnn
+      max_no_equiv = meq
+   end subroutine ref_setup_nm
+
+   !> The kept neighbour map: nm (natom,max_no_shells,max_no_equiv) and nmdim
+   !> (max_no_shells,natom) as setup_nm allocated them.
+   subroutine ref_nm_copy(nm_out, nmdim_out) bind(C, name='ref_nm_copy')
+      integer(c_int), intent(out) :: nm_out(size(nm_keep, 1), size(nm_keep, 2), size(nm_keep, 3))
+      integer(c_int), intent(out) :: nmdim_out(size(nmdim_keep, 1), size(nmdim_keep, 2))
+      nm_out = nm_keep
+      nmdim_out = nmdim_keep
+   end subroutine ref_nm_copy
+
+   !> Dimensions of the kept map: (natom, max_no_shells, max_no_equiv) of nm
+   !> and (rows, cols) of nmdim.
+   subroutine ref_nm_shape(d) bind(C, name='ref_nm_shape')
+      integer(c_int), intent(out) :: d(5)
+      d(1:3) = shape(nm_keep)
+      d(4:5) = shape(nmdim_keep)
+   end subroutine ref_nm_shape
 
 end module uasd_ref_driver

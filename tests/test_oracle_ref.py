@@ -169,3 +169,41 @@ def test_stochastic_depondt_on_the_reference_ziggurat_stream(O, R):
     d = np.max(np.abs(sim.e - er))
     print(f"stochastic Depondt: max |e_oracle - e_reference| = {d:.2e}")
     assert d <= 1e-12
+
+
+def _nm_cases():
+    import refcase as RC
+    import uppasd_amd as U
+    from uppasd_amd.lattice import BCC_SHELLS
+    bcc5 = [(1, 1, 1, 1, *BCC_SHELLS[k], j) for k, j in enumerate(CF.bcc5_couplings())]
+    out = []
+    for n, bc in (((6, 6, 6), "PPP"), ((4, 5, 3), "PPP"), ((4, 4, 4), "P0P"), ((3, 1, 7), "00P")):
+        out.append((f"bcc5_{n[0]}{n[1]}{n[2]}_{bc}", U.bcc(n, tuple(bc)), bcc5, 1, 1, True))
+    t = CF.triangular_dmi(8)
+    out.append(("triangular_J_sym0", t["lattice"], t["jlines"], 0, 1, True))
+    out.append(("triangular_DM_sym0", t["lattice"], t["dmlines"], 0, 3, False))
+    out.append(("triangular_hexagonal_sym3", t["lattice"], [(1, 1, 1, 1, 1.0, 0.0, 0.0, 1.0)], 3,
+                1, True))
+    k = CF.kagome_afm(5)
+    out.append(("kagome_sym0", k["lattice"], k["jlines"], 0, 1, True))
+    c = RC.read_case("Regression", "inpsd.dat")
+    out.append(("regression_maptype2", U.Lattice(c["cell"], c["bas"], c["atype"], c["n"], c["bc"]),
+                c["jlines"], c["sym"], 1, True))
+    return out
+
+
+@pytest.mark.parametrize("case", _nm_cases(), ids=lambda c: c[0])
+def test_neighbour_map_equals_the_reference_setup_nm(O, R, case):
+    """setup_nm (neighbourmap.f90:32-359) of the reference, compiled from its
+    source, and the oracle's restatement give the same neighbour map, entry
+    for entry (atom, shell, equivalent site, in the reference's symmetry
+    order): the order the neighbour lists and so the FP64 exchange sums
+    follow.  The product's builder equals the oracle's lists bit for bit
+    (test_builder.py)."""
+    _, lat, lines, sym, hdim, use_nntype = case
+    latO = O.LatticeSpec(lat.cell, lat.bas, lat.atype, lat.n, lat.bc)
+    a = O.shells_and_map(latO, lines, sym, hdim=hdim, use_nntype=use_nntype)
+    b = O.ref_setup_nm(latO, a, sym, use_nntype)
+    assert (a["max_no_neigh"], a["max_no_equiv"]) == (b["max_no_neigh"], b["max_no_equiv"])
+    assert a["nm"].shape == b["nm"].shape and np.array_equal(a["nm"], b["nm"])
+    assert np.array_equal(a["nmdim"], b["nmdim"])

@@ -240,6 +240,15 @@ int uasd_get_tuning(uasd_ctx *ctx, int kind, int *gather, int *block);
 int uasd_autotune_llg(uasd_ctx *ctx, int sdealgh, int steps, double *best_ms);
 int uasd_autotune_mc(uasd_ctx *ctx, char mode, const double *temps, int sweeps,
                      double *best_ms);
+/* Autotuner cache (no reference counterpart).  enable = 1 keeps every
+ * autotuner winner for the process, keyed by the workload (kernel family,
+ * solver or mode, precision, sizes, list widths, terms, site order, lattice
+ * dimensions), and later uasd_autotune_* calls on the same workload take the
+ * kept shape without timing (best_ms is then the kept time).  path (may be
+ * NULL) names a text file whose entries are loaded now and to which new
+ * winners are appended.  enable = 0 switches the cache off and forgets it
+ * (the default: every call times). */
+int uasd_set_tune_cache(int enable, const char *path);
 
 /* ---- host set-up: the reference neighbour map, bit-exact ----
  * setup_nm + setup_neighbour_hamiltonian (neighbourmap.f90:32-359,

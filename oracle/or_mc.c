@@ -210,18 +210,21 @@ void or_mc_sweep_ref(const or_ham *h, int N, int M, double temperature,
   free(fm); free(fg); free(newm); free(mflip); free(fa);
 }
 
-/* The trial move of the single-precision MC engine: choose_random_flip's
- * three moves (montecarlo_common.f90:25-79) drawn in single precision from
- * the same uniforms -- a uniform direction from z = 1 - 2 u2 and
- * sin(theta) = sqrt(1 - z^2) with the polynomial sincos of the azimuth, the
- * Gaussian step from the single-precision Box-Muller deviates, the flip --
- * widened to double. */
-void or_trial_move_f32(const double e[3], double delta, const double u[4], uint64_t seed,
-                       uint64_t sweep, uint32_t site, uint32_t ens, double out[3])
+/* The Metropolis trial move of the single-precision MC engine, from one
+ * Philox block w (tag TAG_MC_U): choose_random_flip's three moves
+ * (montecarlo_common.f90:25-79) drawn in single precision.  v_i = or_u24(w_i)
+ * (top 24 bits); the move is (3 b) >> 16 of b = (w0 & 0xff) << 8 | (w1 & 0xff),
+ * equal weights to 2^-16; a uniform direction from z = 1 - 2 v2, sin(theta)
+ * = sqrt(1 - z^2) and the azimuth 2 pi v1 by the polynomial sincos; the
+ * Gaussian step from Box-Muller with the polynomial ln and sincos: (g0, g1)
+ * = sqrt(-2 ln v1) (cos, sin)(2 pi v2), g2 = sqrt(-2 ln v0) cos(2 pi a) with
+ * a the 16-bit angle ((w2 & 0xff) << 8 | (w3 & 0xff) + 1/2) 2^-16; the flip.
+ * Widened to double.  The engine accepts by v3 = or_u24(w3). */
+void or_trial_move_f32(const double e[3], double delta, const uint32_t w[4], double out[3])
 {
-  const int ftype = (int)floor(3 * u[0]);
+  const float v0 = or_u24(w[0]), v1 = or_u24(w[1]), v2 = or_u24(w[2]);
+  const int ftype = (int)((3u * (((w[0] & 0xffu) << 8) | (w[1] & 0xffu))) >> 16);
   if (ftype == 0) {
-    const float v1 = (float)u[1], v2 = (float)u[2];
     const float z = 1.0f - 2.0f * v2;
     const float st = sqrtf(fmaxf(0.0f, 1.0f - z * z));
     float sc[2];
@@ -230,11 +233,15 @@ void or_trial_move_f32(const double e[3], double delta, const double u[4], uint6
     out[1] = (double)(st * sc[0]);
     out[2] = (double)z;
   } else if (ftype == 1) {
-    float g[3];
-    or_thermal3(seed, TAG_MC_G, sweep, site, ens, g);
+    const float r01 = sqrtf(-2.0f * or_bm_ln(v1)), r2 = sqrtf(-2.0f * or_bm_ln(v0));
+    const float a16 = ((float)(((w[2] & 0xffu) << 8) | (w[3] & 0xffu)) + 0.5f) *
+                      1.52587890625e-05f;
+    float sc01[2], sc2[2];
+    or_bm_sincos(4.0f * v2, sc01);
+    or_bm_sincos(4.0f * a16, sc2);
     const float dl = (float)delta;
-    const float a0 = (float)e[0] + g[0] * dl, a1 = (float)e[1] + g[1] * dl,
-                a2 = (float)e[2] + g[2] * dl;
+    const float a0 = (float)e[0] + (r01 * sc01[1]) * dl, a1 = (float)e[1] + (r01 * sc01[0]) * dl,
+                a2 = (float)e[2] + (r2 * sc2[1]) * dl;
     const float il = 1.0f / sqrtf(a0 * a0 + a1 * a1 + a2 * a2);
     out[0] = (double)(a0 * il);
     out[1] = (double)(a1 * il);
@@ -281,8 +288,12 @@ void or_mc_sweep_colored_ex(const or_ham *h, int N, int M, const double *temps, 
         } else {
           double e[3] = {V3(emom, 0, i, k, N), V3(emom, 1, i, k, N), V3(emom, 2, i, k, N)};
           double nm[3];
+          double uacc = u[3];
           if (f32_moves) {
-            or_trial_move_f32(e, or_mc_delta(T), u, seed, sweep, (uint32_t)i, (uint32_t)k, nm);
+            uint32_t w[4];
+            or_counter_words(seed, TAG_MC_U, sweep, (uint32_t)i, (uint32_t)k, w);
+            or_trial_move_f32(e, or_mc_delta(T), w, nm);
+            uacc = (double)or_u24(w[3]);
           } else {
             double g[3] = {0.0, 0.0, 0.0};
             if ((int)floor(3 * u[0]) == 1)
@@ -291,7 +302,7 @@ void or_mc_sweep_colored_ex(const or_ham *h, int N, int M, const double *temps, 
             or_choose_random_flip(e, or_mc_delta(T), rn, g, nm);
           }
           double de = or_mc_energy_change(h, N, emomM, emom, mmom, i, k, nm, extfield);
-          metropolis_accept(N, emom, emomM, mmom, i, k, nm, de, T, 1.0, u[3]);
+          metropolis_accept(N, emom, emomM, mmom, i, k, nm, de, T, 1.0, uacc);
         }
       }
     }

@@ -180,6 +180,16 @@ static void counter_words(uint64_t seed, uint32_t tag, uint64_t step, uint32_t s
   or_philox4x32_10(ctr, key, out);
 }
 
+void or_counter_words(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site, uint32_t ens,
+                      uint32_t out[4])
+{
+  counter_words(seed, tag, step, site, ens, out);
+}
+
+/* Uniform on (0, 1) from the top 24 bits of a word, exact in single
+ * precision (the FP32 MC engine's uniforms). */
+float or_u24(uint32_t w) { return ((float)(w >> 8) + 0.5f) * 5.9604644775390625e-08f; }
+
 /* Three N(0,1) deviates per (site, ensemble, step): Box-Muller on the four
  * Philox words, (u0,u1) -> z0,z1 and (u2,u3) -> z2. */
 void or_noise3_f64(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,

@@ -56,6 +56,9 @@ double or_zig_nor(or_zig *z);
 
 /* ---------------- counter-based RNG used by the HIP kernels -------------- */
 void   or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void   or_counter_words(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
+                        uint32_t ens, uint32_t out[4]);
+float  or_u24(uint32_t w);
 double or_u32_to_open01(uint32_t x);                       /* (x+0.5)*2^-32 */
 void   or_noise3_f64(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
                      uint32_t ens, double out[3]);
@@ -219,8 +222,7 @@ void   or_mc_sweep_colored_ex(const or_ham *h, int N, int M, const double *temps
                               uint64_t seed, uint64_t sweep, int ncolors, const int *color_ptr,
                               const int *color_sites, double *emomM, double *emom,
                               const double *mmom, const double extfield[3], int f32_moves);
-void   or_trial_move_f32(const double e[3], double delta, const double u[4], uint64_t seed,
-                         uint64_t sweep, uint32_t site, uint32_t ens, double out[3]);
+void   or_trial_move_f32(const double e[3], double delta, const uint32_t w[4], double out[3]);
 
 /* bulk fills in the reference's stream order (rng_uniformP / fill_rngarray) */
 void   or_mt_fill_real2(or_mt *s, double *out, long n);

@@ -176,3 +176,47 @@ def test_call_timing_times_whole_calls_only():
     assert eng.kernel_timing(0)[0] == 0 and eng.kernel_timing(1)[0] == 0
     eng.set_kernel_timing(False)
     assert np.array_equal(eng.get_moments()[0], ref)
+
+
+def test_tune_cache_reuses_winners_and_persists(tmp_path):
+    """With the cache on, a second context of the same workload takes the
+    first one's winner without timing (the kept time comes back), the file
+    holds one line per workload, and a fresh load of that file serves the
+    next context; switching the cache off restores timing."""
+    rng = np.random.default_rng(63)
+    sysd = CF.bcc_shells(6, CF.bcc5_couplings())
+    N = sysd["lattice"].natom
+    emom = np.ascontiguousarray(CF.random_unit(N, rng, 1))
+    path = tmp_path / "tune.txt"
+
+    def ctx():
+        eng = U.Engine(N, 1)
+        eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
+        eng.set_moments(emom, np.broadcast_to(sysd["mmom"], (1, N)))
+        eng.set_llg(0.1, 300.0, 1e-16)
+        eng.set_noise(U.UASD_NOISE_PHILOX, 1234)
+        return eng
+
+    try:
+        U.set_tune_cache(True, str(path))
+        a = ctx()
+        ms_a = a.autotune_llg(1, steps=2)
+        b = ctx()
+        ms_b = b.autotune_llg(1, steps=2)
+        assert ms_b == ms_a and b.tuning(b.TUNE_LLG) == a.tuning(a.TUNE_LLG)
+        lines = path.read_text().splitlines()
+        assert len(lines) == 1 and lines[0].split()[0].startswith("v1/gfx950/llg/a1/fp64/")
+        U.set_tune_cache(False)
+        U.set_tune_cache(True, str(path))      # reload from the file alone
+        c = ctx()
+        assert abs(c.autotune_llg(1, steps=2) - ms_a) <= 1e-5     # kept to 6 decimals
+        assert c.tuning(c.TUNE_LLG) == a.tuning(a.TUNE_LLG)
+        e_c = (c.sd_steps(1, 5), c.get_moments()[0])[1]
+        U.set_tune_cache(False)
+        d = ctx()
+        d.autotune_llg(1, steps=2)            # timed again: a fresh measurement
+        assert len(path.read_text().splitlines()) == 1
+        d.sd_steps(1, 5)
+        assert np.array_equal(d.get_moments()[0], e_c)
+    finally:
+        U.set_tune_cache(False)

@@ -141,3 +141,47 @@ def test_setup_anisotropy_scaling_gives_energy_in_mry(O, m):
             else:
                 want = uni + sbv * m ** 2 * (K1 * c4 + K2 * c6)
             assert got == pytest.approx(want, rel=1e-12, abs=1e-15), (ty, m)
+
+
+def test_fp32_trial_moves_from_one_philox_block(O):
+    """The FP32 engine's Metropolis update draws everything from one Philox
+    block (or_trial_move_f32 restates it).  Over 60,000 (site, ensemble)
+    draws: the three moves come in thirds; the uniform-direction move is
+    uniform on the sphere (<z> = 0, <z^2> = 1/3, azimuth uniform); the
+    Gaussian step from e = 0 points in an isotropic direction (<n> = 0,
+    <n_i n_j> = delta_ij / 3, so g2 with its 16-bit angle matches g0 and g1);
+    the flip reverses; the acceptance uniform v3 is uniform within each move
+    type (mean 1/2, variance 1/12), i.e. independent of the move choice."""
+    L = O.lib()
+    n = 60000
+    e = np.array([0.6, 0.0, 0.8])
+    kinds = {0: [], 1: [], 2: []}
+    acc = {0: [], 1: [], 2: []}
+    w = np.zeros(4, np.uint32)
+    out = np.zeros(3)
+    for t in range(n):
+        L.or_counter_words(77, 0x21, 5, t // 8, t % 8, w)
+        b = (int(w[0] & 0xff) << 8) | int(w[1] & 0xff)
+        k = (3 * b) >> 16
+        L.or_trial_move_f32(np.zeros(3) if k == 1 else e, 1.0, w, out)
+        kinds[k].append(out.copy())
+        acc[k].append(L.or_u24(int(w[3])))
+    counts = np.array([len(kinds[k]) for k in range(3)])
+    assert np.all(np.abs(counts - n / 3) < 4 * np.sqrt(n * (1 / 3) * (2 / 3))), counts
+    u = np.array(kinds[0])
+    sig = 4 / np.sqrt(len(u))
+    assert np.allclose(np.linalg.norm(u, axis=1), 1.0, atol=1e-6)
+    assert abs(u[:, 2].mean()) < sig and abs((u[:, 2] ** 2).mean() - 1 / 3) < sig
+    phi = np.arctan2(u[:, 1], u[:, 0])
+    assert abs(np.cos(phi).mean()) < sig and abs(np.sin(phi).mean()) < sig
+    g = np.array(kinds[1])
+    assert np.allclose(np.linalg.norm(g, axis=1), 1.0, atol=1e-6)
+    assert np.all(np.abs(g.mean(axis=0)) < sig)
+    cov = g.T @ g / len(g)
+    assert np.allclose(cov, np.eye(3) / 3, atol=sig), cov
+    f = np.array(kinds[2])
+    assert np.allclose(f, -e[None, :], atol=1e-7)
+    for k in range(3):
+        a = np.array(acc[k])
+        s = 4 / np.sqrt(len(a))
+        assert abs(a.mean() - 0.5) < s and abs(a.var() - 1 / 12) < s, (k, a.mean(), a.var())

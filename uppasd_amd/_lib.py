@@ -36,7 +36,7 @@ EXPORTS = (
     "uasd_energy_terms_noalloc",
     "uasd_accumulate", "uasd_get_accumulators", "uasd_reset_accumulators", "uasd_set_graph_steps",
     "uasd_index_bytes", "uasd_set_site_order", "uasd_set_lattice_dims", "uasd_has_stencil",
-    "uasd_build_pair_list",
+    "uasd_build_pair_list", "uasd_set_tune_cache",
 )
 
 
@@ -101,6 +101,7 @@ def lib() -> C.CDLL:
         L.uasd_get_tuning.argtypes = [vp, C.c_int, ip, ip]
         L.uasd_autotune_llg.argtypes = [vp, C.c_int, C.c_int, dp]
         L.uasd_autotune_mc.argtypes = [vp, C.c_char, vp, C.c_int, dp]
+        L.uasd_set_tune_cache.argtypes = [C.c_int, C.c_char_p]
         L.uasd_build_pair_list.argtypes = [C.c_int] * 5 + [vp] * 6 + [C.c_int, C.c_int] + \
             [vp, vp, vp] + [C.c_int, C.c_int] + [vp, vp, vp] + [ip, vp, vp, vp]
         _lib = L

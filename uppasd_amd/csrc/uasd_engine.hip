@@ -282,6 +282,7 @@ uasd::HamDev<R> ham_view(const uasd_ctx *c) {
   uasd::HamDev<R> h{};
   h.N = c->N;
   h.M = c->M;
+  h.mdiv = uasd::make_fast_div((uint32_t)c->M);
   h.W = c->W;
   h.ex_rows = c->ex_rows;
   h.nbr = c->nbr.as<int>();
@@ -1776,6 +1777,50 @@ struct EventPair {
 
 // Times `run` (asynchronous launches on c->stream) for every candidate shape,
 // once untimed and once timed, and leaves the fastest in *slot.
+// ------------------------------------------------------ autotuner cache
+// Winners of uasd_autotune_* per workload, kept for the process once
+// uasd_set_tune_cache has switched the cache on, and read from / appended to
+// its text file when it names one: one line per workload, "key gather block
+// nt idx32 tile pipe best_ms".  The key holds what the launch shapes' speed
+// depends on: the kernel family, solver or mode, precision, sizes, list
+// widths, terms, site order, lattice dimensions and list entry width.
+std::mutex g_tune_mu;
+bool g_tune_on = false;
+std::string g_tune_path;
+std::map<std::string, std::pair<uasd::Tune, double>> g_tune;
+
+std::string tune_key(const uasd_ctx *c, const char *kind, int alg) {
+  char buf[320];
+  std::snprintf(buf, sizeof buf,
+                "v1/gfx950/%s/a%d/%s/N%d/M%d/seg%d/W%d/Wd%d/an%d/ext%d/order%d/lat%d,%d,%d,%d/i16%d%d",
+                kind, alg, c->prec == UASD_FP64 ? "fp64" : "fp32", c->N, c->M, c->nseg, c->W,
+                c->Wd, c->an_rows > 0 ? 1 : 0, c->ext_mode, c->order_na, c->lat_na, c->lat_n1,
+                c->lat_n2, c->lat_n3, c->use16 ? 1 : 0, c->use16_dm ? 1 : 0);
+  return buf;
+}
+
+bool tune_lookup(const std::string &key, uasd::Tune &t, double &ms) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  if (!g_tune_on) return false;
+  auto it = g_tune.find(key);
+  if (it == g_tune.end()) return false;
+  t = it->second.first;
+  ms = it->second.second;
+  return true;
+}
+
+void tune_store(const std::string &key, const uasd::Tune &t, double ms) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  if (!g_tune_on) return;
+  g_tune[key] = {t, ms};
+  if (g_tune_path.empty()) return;
+  if (FILE *f = std::fopen(g_tune_path.c_str(), "a")) {
+    std::fprintf(f, "%s %d %d %d %d %d %d %.6f\n", key.c_str(), t.gather, t.block, t.nt, t.idx32,
+                 t.tile, t.pipe, ms);
+    std::fclose(f);
+  }
+}
+
 template <typename Run>
 int tune_search(uasd_ctx *c, uasd::Tune *slot, Run &&run, double *best_ms) {
   EventPair ev;
@@ -1828,8 +1873,21 @@ int uasd_autotune_llg(uasd_ctx *c, int alg, int steps, double *best_ms) {
     return fail(UASD_ERR_UNSUPPORTED, "SDEalgh must be 1 (midpoint), 4 (Heun), 5 (Depondt), 6 (SIB) or 7 (spherical midpoint)");
   if (steps <= 0) return fail(UASD_ERR_ARG, "steps must be > 0");
   if (int r = ready_for_llg(c)) return r;
+  build_stencil(c);
+  const std::string key = tune_key(c, "llg", alg);
+  uasd::Tune cached;
+  double cached_ms = 0.0;
+  if (tune_lookup(key, cached, cached_ms) && uasd::tune_valid(cached) &&
+      (cached.tile == 0 || tile_fits(c, cached.tile))) {
+    c->tune[0] = c->tune[1] = cached;
+    c->tune[1].tile = c->tune[1].pipe = 0;
+    if (best_ms) *best_ms = cached_ms;
+    c->gen += 1;
+    return UASD_OK;
+  }
   StateGuard g(c);
   if (g.err) return g.err;
+  double ms_found = 0.0;
   int r = tune_search(c, &c->tune[0], [&]() -> int {
     for (int s = 0; s < steps; ++s) {
       int e = c->prec == UASD_FP64 ? run_llg_half<double>(c, alg, 0, true, false)
@@ -1841,8 +1899,10 @@ int uasd_autotune_llg(uasd_ctx *c, int alg, int steps, double *best_ms) {
       c->step += 1;
     }
     return UASD_OK;
-  }, best_ms);
+  }, &ms_found);
   if (r) return r;
+  if (best_ms) *best_ms = ms_found;
+  tune_store(key, c->tune[0], ms_found);
   c->tune[1] = c->tune[0];  // the field kernel gathers the same way
   c->gen += 1;
   return g.restore();
@@ -1853,12 +1913,51 @@ int uasd_autotune_mc(uasd_ctx *c, char mode, const double *temps, int sweeps, do
   if (sweeps <= 0) return fail(UASD_ERR_ARG, "sweeps must be > 0");
   uasd::MCParams mp;
   if (int r = mc_prepare(c, mode, temps, 0x7u, mp)) return r;
+  const std::string key = tune_key(c, "mc", (int)mode);
+  uasd::Tune cached;
+  double cached_ms = 0.0;
+  if (tune_lookup(key, cached, cached_ms) && uasd::tune_valid(cached) && cached.tile == 0) {
+    c->tune[2] = cached;
+    if (best_ms) *best_ms = cached_ms;
+    c->gen += 1;
+    return UASD_OK;
+  }
   StateGuard g(c);
   if (g.err) return g.err;
-  int r = tune_search(c, &c->tune[2], [&]() -> int { return mc_run(c, mp, sweeps); }, best_ms);
+  double ms_found = 0.0;
+  int r = tune_search(c, &c->tune[2], [&]() -> int { return mc_run(c, mp, sweeps); }, &ms_found);
   if (r) return r;
+  if (best_ms) *best_ms = ms_found;
+  tune_store(key, c->tune[2], ms_found);
   c->gen += 1;
   return g.restore();
+}
+
+int uasd_set_tune_cache(int enable, const char *path) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune_on = enable != 0;
+  g_tune_path = (enable && path) ? path : "";
+  if (!g_tune_on) {
+    g_tune.clear();
+    return UASD_OK;
+  }
+  if (g_tune_path.empty()) return UASD_OK;
+  FILE *f = std::fopen(g_tune_path.c_str(), "r");
+  if (!f) return UASD_OK;  // a new file: created by the first result
+  char key[320];
+  uasd::Tune t;
+  double ms = 0.0;
+  int loaded = 0;
+  while (std::fscanf(f, "%319s %d %d %d %d %d %d %lf", key, &t.gather, &t.block, &t.nt, &t.idx32,
+                     &t.tile, &t.pipe, &ms) == 8) {
+    if (uasd::tune_valid(t)) {
+      g_tune[key] = {t, ms};
+      ++loaded;
+    }
+  }
+  std::fclose(f);
+  (void)loaded;
+  return UASD_OK;
 }
 
 int uasd_measure(uasd_ctx *c, double *msum, double *energy) {

@@ -52,9 +52,30 @@ struct SiteClass {
   int ex_row, dm_row, an_row, pad;
 };
 
+// Division of an index q < 2^31 by a runtime divisor d >= 1 as a multiply and
+// a shift (Granlund and Montgomery, 1994, theorem 4.2 with N = 31): with
+// l = ceil(log2 d) and m = floor(2^(31+l) / d) + 1, which fits in 32 bits,
+// q / d = (q * m) >> (31 + l).  make_fast_div computes (m, 31 + l) on the host.
+struct FastDiv {
+  uint32_t m;
+  int s;
+};
+
+inline FastDiv make_fast_div(uint32_t d) {
+  int l = 0;
+  while ((1ull << l) < d) ++l;
+  const uint64_t m = ((uint64_t)1 << (31 + l)) / d + 1;
+  return FastDiv{(uint32_t)m, 31 + l};
+}
+
+__host__ __device__ __forceinline__ uint32_t fast_div(uint32_t q, FastDiv f) {
+  return (uint32_t)(((uint64_t)q * f.m) >> f.s);
+}
+
 template <typename R>
 struct HamDev {
   int N, M;
+  FastDiv mdiv;  // division by M
   // exchange
   int W, ex_rows;
   const int *__restrict__ nbr;
@@ -92,6 +113,7 @@ struct HamDev {
 
 template <typename R>
 __device__ __forceinline__ uint32_t host_site(const HamDev<R> &h, int d) {
+  if (h.order_na == 1) return (uint32_t)d;  // reference order: no division
   return (uint32_t)((d % h.order_nc) * h.order_na + d / h.order_nc);
 }
 
@@ -257,6 +279,11 @@ __device__ __forceinline__ void thermal3_f64(uint64_t seed, uint64_t step, uint3
   c = r1 * c1;
 }
 
+// Uniform on (0, 1) from the top 24 bits of a Philox word: exact in FP32.
+__device__ __forceinline__ float u24(uint32_t w) {
+  return ((float)(w >> 8) + 0.5f) * 5.9604644775390625e-08f;
+}
+
 __device__ __forceinline__ void uniform4(uint64_t seed, uint32_t tag, uint64_t step, uint32_t site,
                                          uint32_t ens, double &u0, double &u1, double &u2,
                                          double &u3) {
@@ -323,8 +350,13 @@ struct NbDec {
   template <typename R>
   __device__ __forceinline__ NbDec(const HamDev<R> &h, int d)
       : nc(h.order_nc), na(h.order_na), kb(h.idx_kb) {
-    b = d / nc;
-    c = d - b * nc;
+    if (na == 1) {  // reference order: one block of nc = N sites
+      b = 0;
+      c = d;
+    } else {
+      b = d / nc;
+      c = d - b * nc;
+    }
   }
   __device__ __forceinline__ int operator()(int v) const { return v; }
   __device__ __forceinline__ int operator()(int16_t v) const {
@@ -758,7 +790,7 @@ __global__ void __launch_bounds__(256) k_field(HamDev<R> h, const Quad<R> *__res
   const int64_t total = (int64_t)h.N * h.M;
   const int64_t q = xcd_block() * blockDim.x + threadIdx.x;
   if (q >= total) return;
-  const int site = (int)(q / h.M), ens = (int)(q - (int64_t)site * h.M);
+  const int site = (int)fast_div((uint32_t)q, h.mdiv), ens = (int)q - site * h.M;
   R s0, s1, s2, q0, q1, q2, x0, x1, x2;
   site_field<R, G>(h, src, site, ens, s0, s1, s2, q0, q1, q2, x0, x1, x2);
   Quad<R> b;
@@ -974,7 +1006,7 @@ __global__ void __launch_bounds__(256)
   const int64_t total = (int64_t)h.N * h.M;
   const int64_t q = xcd_block() * blockDim.x + threadIdx.x;
   if (q >= total) return;
-  const int site = (int)(q / h.M), ens = (int)(q - (int64_t)site * h.M);
+  const int site = (int)fast_div((uint32_t)q, h.mdiv), ens = (int)q - site * h.M;
   R b0, b1, b2v;
   if (FUSED) {
     total_field<R, G>(h, HALF == 0 ? e : (const Quad<R> *)p, site, ens, b0, b1, b2v);
@@ -1489,20 +1521,21 @@ struct MCParams {
 
 template <typename R, int G>
 __global__ void __launch_bounds__(256) k_mc_colour(HamDev<R> h, MCParams mp, Quad<R> *s) {
-  const int64_t total = (int64_t)mp.nsites * h.M;
-  const int64_t t = xcd_block() * blockDim.x + threadIdx.x;
+  // (site of the colour, ensemble) pairs number fewer than 2^31
+  const uint32_t total = (uint32_t)mp.nsites * (uint32_t)h.M;
+  const uint32_t t = (uint32_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const int pidx = (int)(t / h.M), ens = (int)(t - (int64_t)pidx * h.M);
+  const int pidx = (int)fast_div(t, h.mdiv), ens = (int)t - pidx * h.M;
   const int site = mp.sites[pidx];
   const size_t q = (size_t)site * h.M + ens;
-  double u0, u1, u2, u3;
   const uint64_t sweep = mp.sweep_dev ? *mp.sweep_dev + mp.sweep : mp.sweep;
   const uint32_t hs = host_site(h, site);
-  uniform4(mp.seed, kTagMCU, sweep, hs, (uint32_t)ens, u0, u1, u2, u3);
   const Quad<R> me = s[q];
   const R x0 = (R)mp.ext[3 * ens], x1 = (R)mp.ext[3 * ens + 1], x2 = (R)mp.ext[3 * ens + 2];
   R o0, o1, o2;
   if (mp.heat_bath) {
+    double u0, u1, u2, u3;
+    uniform4(mp.seed, kTagMCU, sweep, hs, (uint32_t)ens, u0, u1, u2, u3);
     R s0, s1, s2, q0, q1, q2, y0, y1, y2;
     site_field<R, G>(h, (const Quad<R> *)s, site, ens, s0, s1, s2, q0, q1, q2, y0, y1, y2);
     const R f0 = (R)0 + s0 + (q0 + x0 + (R)0);
@@ -1510,13 +1543,19 @@ __global__ void __launch_bounds__(256) k_mc_colour(HamDev<R> h, MCParams mp, Qua
     const R f2 = (R)0 + s2 + (q2 + x2 + (R)0);
     heat_bath<R>(mp.hbeta[ens], me.w, f0, f1, f2, u1, u0, o0, o1, o2);
   } else if (std::is_same<R, float>::value) {
-    // FP32 engine: trial moves and acceptance in single precision from the
-    // same uniforms (24 bits each).  Uniform direction by z = 1 - 2 u2,
-    // sin(theta) = sqrt(1 - z^2) (the same distribution as acos / sin), the
-    // azimuth by the polynomial sincos and the Gaussian step by the thermal
-    // deviates' Box-Muller, both of the thermal field.
-    const float v1 = (float)u1, v2 = (float)u2, v3 = (float)u3;
-    const int ftype = (int)floor(3 * u0);
+    // FP32 engine: the whole Metropolis update from one Philox block, in
+    // single precision.  Each word w_i gives a 24-bit uniform v_i from its top
+    // bits; the 32 low bits left over pick the move and the third Gaussian's
+    // angle.  The move (montecarlo_common.f90:25-79's three, with equal
+    // weights to 2^-16): a uniform direction by z = 1 - 2 v2, sin(theta) =
+    // sqrt(1 - z^2) and the azimuth 2 pi v1 (the distribution of acos /
+    // sin); the Gaussian step by Box-Muller, (g0, g1) from (v1, v2) and g2
+    // from radius v0 and a 16-bit angle; the flip.  Acceptance compares v3
+    // with expf.  The polynomial ln and sincos are the thermal field's.
+    uint32_t w0, w1, w2, w3;
+    counter_words(mp.seed, kTagMCU, sweep, hs, (uint32_t)ens, w0, w1, w2, w3);
+    const float v0 = u24(w0), v1 = u24(w1), v2 = u24(w2), v3 = u24(w3);
+    const int ftype = (int)((3u * (((w0 & 0xffu) << 8) | (w1 & 0xffu))) >> 16);
     float n0, n1, n2;
     if (ftype == 0) {
       const float z = 1.0f - 2.0f * v2;
@@ -1527,10 +1566,14 @@ __global__ void __launch_bounds__(256) k_mc_colour(HamDev<R> h, MCParams mp, Qua
       n1 = st * sp;
       n2 = z;
     } else if (ftype == 1) {
-      float g0, g1, g2;
-      thermal3(mp.seed, kTagMCG, sweep, hs, (uint32_t)ens, g0, g1, g2);
+      const float r01 = sqrtf(-2.0f * bm::ln(v1)), r2 = sqrtf(-2.0f * bm::ln(v0));
+      const float a16 = ((float)(((w2 & 0xffu) << 8) | (w3 & 0xffu)) + 0.5f) * 1.52587890625e-05f;
+      float s01, c01, s2, c2;
+      bm::sincos_q(4.0f * v2, s01, c01);
+      bm::sincos_q(4.0f * a16, s2, c2);
       const float dl = (float)mp.delta[ens];
-      const float a0 = (float)me.x + g0 * dl, a1 = (float)me.y + g1 * dl, a2 = (float)me.z + g2 * dl;
+      const float a0 = (float)me.x + (r01 * c01) * dl, a1 = (float)me.y + (r01 * s01) * dl,
+                  a2 = (float)me.z + (r2 * c2) * dl;
       const float il = 1.0f / sqrtf(a0 * a0 + a1 * a1 + a2 * a2);
       n0 = a0 * il;
       n1 = a1 * il;
@@ -1546,6 +1589,8 @@ __global__ void __launch_bounds__(256) k_mc_colour(HamDev<R> h, MCParams mp, Qua
     o1 = (R)n1;
     o2 = (R)n2;
   } else {
+    double u0, u1, u2, u3;
+    uniform4(mp.seed, kTagMCU, sweep, hs, (uint32_t)ens, u0, u1, u2, u3);
     double g0 = 0.0, g1 = 0.0, g2 = 0.0;
     if ((int)floor(3 * u0) == 1)
       normal3_f64(mp.seed, kTagMCG, sweep, hs, (uint32_t)ens, g0, g1, g2);

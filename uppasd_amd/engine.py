@@ -311,6 +311,15 @@ class Engine:
         return ms.value
 
 
+def set_tune_cache(enable: bool = True, path: str | None = None):
+    """Keep autotuner winners per workload for the process (and in `path`,
+    loaded now and appended to); later autotune calls on the same workload
+    reuse them without timing.  enable=False switches the cache off."""
+    B.check(B.lib().uasd_set_tune_cache(1 if enable else 0,
+                                        None if path is None else str(path).encode()),
+            "uasd_set_tune_cache")
+
+
 def moment_update(mmom, emom, emom2, emomM):
     """copym (updatemoments.f90:48-95): emom <- emom2, emomM <- emom2*mmom (in place)."""
     emom[...] = emom2

@@ -637,6 +637,9 @@ def profiled_traffic(label: str, variant: str) -> dict:
 
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--tune-cache", default=os.path.join(ROOT, "uppasd_amd", "tune_gfx950.txt"),
+                    help="autotuner cache file: winners kept there are reused without timing, "
+                         "new ones appended; 'none' times every candidate")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain copy kernel of known size (for counter scaling)")
     ap.add_argument("--gpus", type=int, default=1)
@@ -659,6 +662,9 @@ def main():
                     help="do not declare the lattice (list-gather kernels only)")
     ap.add_argument("--site-order", type=int, default=0,
                     help="device site order: 0 = basis-major (NA of the lattice), 1 = reference")
+    ap.add_argument("--noise", choices=["philox", "injected", "none"], default="philox",
+                    help="thermal noise of the main run: in-kernel Philox at 300 K, an injected "
+                         "N(0,1) array at 300 K, or none (T = 0)")
     ap.add_argument("--variants", default="T0,injected,L128",
                     help="comma list of runs of the plain workload beside the headline: T0 "
                          "(no noise), injected (caller's N(0,1) stream), L128 (bcc 128^3 x 2, "
@@ -683,6 +689,8 @@ def main():
         sys.exit(2)
 
     import uppasd_amd as U
+    if args.tune_cache != "none":
+        U.set_tune_cache(True, args.tune_cache)
 
     t_setup = time.perf_counter()
     sysd = build_c2(args.L)
@@ -698,8 +706,13 @@ def main():
     e0 = rng.normal(size=(1, N, 3))
     e0 /= np.linalg.norm(e0, axis=2, keepdims=True)
     eng.set_moments(np.ascontiguousarray(e0), sysd["mmom"][None])
-    eng.set_llg(0.1, 300.0, 1e-16)
-    eng.set_noise(U.UASD_NOISE_PHILOX, 0xC2C2C2C2)
+    eng.set_llg(0.1, 0.0 if args.noise == "none" else 300.0, 1e-16)
+    if args.noise == "philox":
+        eng.set_noise(U.UASD_NOISE_PHILOX, 0xC2C2C2C2)
+    elif args.noise == "injected":
+        eng.set_noise(U.UASD_NOISE_INJECTED, 0, np.ascontiguousarray(rng.normal(size=(1, N, 3))))
+    else:
+        eng.set_noise(U.UASD_NOISE_NONE)
     setup_s = time.perf_counter() - t_setup
 
     t_tune = time.perf_counter()
@@ -722,17 +735,19 @@ def main():
     s = 8 if args.precision == "fp64" else 4
     ib = 0 if tile else eng.index_bytes()[0]     # the tile kernel reads no lists
     kb = kernel_bytes(W, s, ib, cls=0 if tile else 4,   # one species: classes per basis
-                      b2=args.solver in (4, 5))
-    sb = survey_bytes(W, s)
+                      noise=args.noise, b2=args.solver in (4, 5))
+    sb = survey_bytes(W, s, injected=args.noise == "injected")
     # the dominant kernel is the corrector: the workload's algorithmic bytes
     # per update of its pass over every site, per its duration
     achieved = sb["corrector"] * N / t_corr / 1e9
     achieved_own = kb["corrector"] * N / t_corr / 1e9
     value = N * args.steps / elapsed
     traffic = None
-    if args.precision == "fp64" and args.L == 64 and args.solver == 1 and order == 2:
+    if args.precision == "fp64" and args.solver == 1 and order == 2:
         variant = f"tile{tile}{'r' if pipe else ''}" if tile else f"idx{ib}"
-        traffic = profiled_traffic("fp64_midpoint_bcc64_58nn_philox", variant).get("corrector")
+        noise_label = {"philox": "philox", "injected": "injected", "none": "T0"}[args.noise]
+        traffic = profiled_traffic(f"fp64_midpoint_bcc{args.L}_58nn_{noise_label}",
+                                   variant).get("corrector")
     _, eM = eng.get_moments()
     norm_drift = float(np.max(np.abs(np.linalg.norm(eM[0], axis=1) / sysd["mmom"] - 1.0)))
 
@@ -751,9 +766,11 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": (f"bcc {args.L}^3 x 2 ({N} spins), 5 exchange shells ({W} neighbours), "
-                         f"{SOLVER_NAMES[args.solver]} LLG, alpha 0.1, "
-                         f"dt 1e-16 s, T 300 K Philox thermal field, {args.precision}, "
-                         "device-resident"),
+                         f"{SOLVER_NAMES[args.solver]} LLG, alpha 0.1, dt 1e-16 s, "
+                         + {"philox": "T 300 K Philox thermal field",
+                            "injected": "T 300 K injected N(0,1) thermal field",
+                            "none": "T 0"}[args.noise]
+                         + f", {args.precision}, device-resident"),
             "spins": N, "neighbours": W, "solver": args.solver, "precision": args.precision,
             "setup_s": round(setup_s, 2),
             "launch": {"gather": gather, "block": block, "nt_index_loads": bool(nt),
@@ -761,7 +778,9 @@ def main():
                        "tile_shape": U.engine.TILE_SHAPES[tile - 1] if tile else None,
                        "resident_tile_loop": bool(pipe),
                        "site_order": order,
-                       "autotuned": not args.gather, "tune_s": round(tune_s, 3)},
+                       "autotuned": not args.gather, "tune_s": round(tune_s, 3),
+                       "tune_cache": None if args.tune_cache == "none" else
+                       os.path.relpath(args.tune_cache, ROOT)},
             "wall_ms_per_step": elapsed / args.steps * 1e3,
             "kernel_ms": {"predictor": round(t_pred * 1e3, 4), "corrector": round(t_corr * 1e3, 4),
                           "sampled": tm["sampled_ms"], "sampled_launches": tm["sampled_launches"],

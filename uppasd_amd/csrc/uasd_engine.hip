@@ -1783,7 +1783,8 @@ struct EventPair {
 // its text file when it names one: one line per workload, "key gather block
 // nt idx32 tile pipe best_ms".  The key holds what the launch shapes' speed
 // depends on: the kernel family, solver or mode, precision, sizes, list
-// widths, terms, site order, lattice dimensions and list entry width.
+// widths, terms, site order, lattice dimensions, list entry width and noise
+// source (none at a uniform T = 0).
 std::mutex g_tune_mu;
 bool g_tune_on = false;
 std::string g_tune_path;
@@ -1792,10 +1793,12 @@ std::map<std::string, std::pair<uasd::Tune, double>> g_tune;
 std::string tune_key(const uasd_ctx *c, const char *kind, int alg) {
   char buf[320];
   std::snprintf(buf, sizeof buf,
-                "v1/gfx950/%s/a%d/%s/N%d/M%d/seg%d/W%d/Wd%d/an%d/ext%d/order%d/lat%d,%d,%d,%d/i16%d%d",
+                "v1/gfx950/%s/a%d/%s/N%d/M%d/seg%d/W%d/Wd%d/an%d/ext%d/order%d/lat%d,%d,%d,%d/i16%d%d/"
+                "noise%d",
                 kind, alg, c->prec == UASD_FP64 ? "fp64" : "fp32", c->N, c->M, c->nseg, c->W,
                 c->Wd, c->an_rows > 0 ? 1 : 0, c->ext_mode, c->order_na, c->lat_na, c->lat_n1,
-                c->lat_n2, c->lat_n3, c->use16 ? 1 : 0, c->use16_dm ? 1 : 0);
+                c->lat_n2, c->lat_n3, c->use16 ? 1 : 0, c->use16_dm ? 1 : 0,
+                c->temp_uniform && c->temp0 == 0.0 ? 0 : c->noise_mode);
   return buf;
 }
 

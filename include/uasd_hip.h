@@ -229,7 +229,10 @@ int uasd_index_bytes(uasd_ctx *ctx, int *exchange, int *dm);
  * (64,4,1) (8,4,4) (4,4,4) (8,8,2) (16,8,1) (8,8,1) (16,8,4) (8,16,4) (8,8,8),
  * one thread per site (cells x basis atoms <= 1024); plus 0x400000 for
  * resident workgroups that each loop over a run of adjacent tiles, with the
- * next tile's halo loads in flight during the current tile's computation.
+ * next tile's halo loads in flight during the current tile's computation;
+ * plus 0x800000 (any kind, for the whole context) to gather DM neighbours
+ * separately even where every site's DM bonds are its exchange bonds (at
+ * most 8), which the list kernels otherwise gather once for both sums.
  * Results do not depend on any of them. */
 int uasd_set_tuning(uasd_ctx *ctx, int kind, int gather, int block);
 int uasd_get_tuning(uasd_ctx *ctx, int kind, int *gather, int *block);

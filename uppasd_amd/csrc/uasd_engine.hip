@@ -163,6 +163,8 @@ struct uasd_ctx {
 
   bool have_ex = false, have_state = false, have_llg = false;
   bool classes_dirty = true;
+  bool dm_shared = false;  // DM bonds = exchange bonds at every site (fused gather)
+  bool tune_no_fuse = false;  // keep the separate DM gather (a tuning choice)
 
   // LLG parameters
   double dt = 1e-16, temprescale = 1.0, bn = 1.0;
@@ -294,6 +296,7 @@ uasd::HamDev<R> ham_view(const uasd_ctx *c) {
   h.dnbr = c->dnbr.as<int>();
   h.dnbr16 = c->use16_dm ? c->dnbr16.as<int16_t>() : nullptr;
   h.dm_len = c->dm_len.as<int>();
+  h.dm_shared = c->dm_shared && !c->tune_no_fuse ? 1 : 0;
   if (c->Wd > 0) {
     size_t n = (size_t)c->Wd * c->dm_rows;
     h.dvx = c->dv.as<R>();
@@ -409,6 +412,25 @@ void from_quads(const uasd_ctx *c, const std::vector<double> &hq, double *out, b
     }
 }
 
+// Whether every site's DM neighbours are its exchange neighbours, in the same
+// order and number, at most kFusedW of them (or there is no DM term): the
+// field kernels then gather each neighbour once for both sums.
+bool dm_lists_shared(const uasd_ctx *c) {
+  if (c->Wd == 0 || c->dm_row_of_site.empty()) return false;
+  if (c->W > uasd::kFusedW) return false;
+  const int N = c->N;
+  for (int i = 0; i < N; ++i) {
+    const int dr = c->dm_row_of_site[i];
+    if (dr < 0) continue;
+    const int er = c->ex_row_of_site.empty() ? 0 : c->ex_row_of_site[i];
+    const int L = c->ex_len_host[er];
+    if (c->dm_len_host[dr] != L) return false;
+    for (int j = 0; j < L; ++j)
+      if (c->dnbr_host[(size_t)j * N + i] != c->nbr_host[(size_t)j * N + i]) return false;
+  }
+  return true;
+}
+
 // may_alloc = false: fail rather than grow the device tables (the no-alloc
 // entry point), i.e. when the class tables outgrow the buffers of the last
 // build.
@@ -429,6 +451,7 @@ int rebuild_classes(uasd_ctx *c, bool may_alloc = true) {
     }
     cls[i] = it->second;
   }
+  c->dm_shared = dm_lists_shared(c);
   if (!may_alloc && (!c->cls.p || !c->classes.p || c->cls.bytes < cls.size() * sizeof(int) ||
                      c->classes.bytes < table.size() * sizeof(uasd::SiteClass)))
     return fail(UASD_ERR_STATE,
@@ -1718,12 +1741,14 @@ int uasd_set_tuning(uasd_ctx *c, int kind, int gather, int block) {
   if (kind < 0 || kind > 2) return fail(UASD_ERR_ARG, "kind must be 0 (LLG), 1 (field) or 2 (MC)");
   // block may carry flags: 0x10000 non-temporal list loads, 0x20000 int32
   // entries, (k << 18) LDS-tiled stencil shape k and 0x400000 resident
-  // workgroups looping over tiles (LLG only)
+  // workgroups looping over tiles (LLG only), 0x800000 a separate DM gather
+  // where the DM bonds are the exchange bonds (context-wide)
   uasd::Tune t{gather, block & 0xffff, (block >> 16) & 1, (block >> 17) & 1, (block >> 18) & 0xf,
                (block >> 22) & 1};
-  if (!uasd::tune_valid(t) || (block >> 23) != 0 || (kind != 0 && t.tile != 0))
+  if (!uasd::tune_valid(t) || (block >> 24) != 0 || (kind != 0 && t.tile != 0))
     return fail(UASD_ERR_ARG, "gather must be 2, 4, 8 or 16 and block 64, 128 or 256");
   c->tune[kind] = t;
+  c->tune_no_fuse = ((block >> 23) & 1) != 0;
   return UASD_OK;
 }
 
@@ -1733,7 +1758,8 @@ int uasd_get_tuning(uasd_ctx *c, int kind, int *gather, int *block) {
   if (gather) *gather = c->tune[kind].gather;
   if (block)
     *block = c->tune[kind].block | (c->tune[kind].nt << 16) | (c->tune[kind].idx32 << 17) |
-             (c->tune[kind].tile << 18) | (c->tune[kind].pipe << 22);
+             (c->tune[kind].tile << 18) | (c->tune[kind].pipe << 22) |
+             ((c->tune_no_fuse ? 1 : 0) << 23);
   return UASD_OK;
 }
 

@@ -84,6 +84,9 @@ struct HamDev {
   const R *__restrict__ cpl;
   // DM
   int Wd, dm_rows;
+  // every site's DM neighbours are its exchange neighbours in the same order
+  // and number at most kFusedW: one gather serves both sums
+  int dm_shared;
   const int *__restrict__ dnbr;
   const int16_t *__restrict__ dnbr16;
   const int *__restrict__ dm_len;
@@ -466,6 +469,64 @@ __device__ __forceinline__ void dm_sum(const HamDev<R> &h, const I *__restrict__
   }
 }
 
+// Exchange and DM sums of a site whose DM bonds are its exchange bonds
+// (HamDev::dm_shared): each neighbour is gathered once, its moment m_j kept
+// in registers for the DM terms.  The exchange terms are added in list
+// order as exchange_sum adds them, then the DM terms in list order as dm_sum
+// adds them, so the field is bit-identical to the two separate sums.
+constexpr int kFusedW = 8;
+
+template <typename R, int G, bool NT, typename I>
+__device__ __forceinline__ void exdm_sum(const HamDev<R> &h, const I *__restrict__ nb,
+                                         const Quad<R> *__restrict__ src, int site, int ens,
+                                         int exrow, int dmrow, R &s0, R &s1, R &s2) {
+  const int M = h.M, N = h.N;
+  const NbDec dec(h, site);
+  const int len = h.ex_len[exrow];
+  const R *cp = h.cpl + exrow;
+  const size_t rs = (size_t)h.ex_rows;
+  R c0[kFusedW], c1[kFusedW], c2[kFusedW];
+#pragma unroll
+  for (int j0 = 0; j0 < kFusedW; j0 += G) {
+    int n[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u)
+      n[u] = j0 + u < len ? dec(nb_load<NT>(nb + (size_t)(j0 + u) * N)) : site;
+    Quad<R> v[G];
+    R J[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      if (j0 + u < len) {
+        v[u] = src[(size_t)n[u] * M + ens];
+        J[u] = cp[(size_t)(j0 + u) * rs];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      if (j0 + u < kFusedW && j0 + u < len) {
+        c0[j0 + u] = v[u].x * v[u].w;
+        c1[j0 + u] = v[u].y * v[u].w;
+        c2[j0 + u] = v[u].z * v[u].w;
+        s0 = fma(J[u], c0[j0 + u], s0);
+        s1 = fma(J[u], c1[j0 + u], s1);
+        s2 = fma(J[u], c2[j0 + u], s2);
+      }
+    }
+  }
+  if (dmrow < 0) return;
+  const size_t drs = (size_t)h.dm_rows;
+#pragma unroll
+  for (int j = 0; j < kFusedW; ++j) {
+    if (j < len) {
+      const size_t r = (size_t)j * drs + dmrow;
+      const R D0 = h.dvx[r], D1 = h.dvy[r], D2 = h.dvz[r];
+      s0 = s0 + D2 * c1[j] - D1 * c2[j];
+      s1 = s1 + D0 * c2[j] - D2 * c0[j];
+      s2 = s2 + D1 * c0[j] - D0 * c1[j];
+    }
+  }
+}
+
 // G = neighbours whose loads are issued together (a tuning parameter).
 template <typename R, int G>
 __device__ __forceinline__ void site_field(const HamDev<R> &h, const Quad<R> *__restrict__ src,
@@ -475,7 +536,20 @@ __device__ __forceinline__ void site_field(const HamDev<R> &h, const Quad<R> *__
   qf0 = qf1 = qf2 = (R)0;
   const SiteClass sc = h.classes[h.cls[site]];
   const int M = h.M;
-  if (h.nbr16) {
+  if (h.dm_shared) {
+    const int dr = h.Wd > 0 ? sc.dm_row : -1;
+    if (h.nbr16) {
+      if (h.nt)
+        exdm_sum<R, G, true>(h, h.nbr16 + site, src, site, ens, sc.ex_row, dr, s0, s1, s2);
+      else
+        exdm_sum<R, G, false>(h, h.nbr16 + site, src, site, ens, sc.ex_row, dr, s0, s1, s2);
+    } else {
+      if (h.nt)
+        exdm_sum<R, G, true>(h, h.nbr + site, src, site, ens, sc.ex_row, dr, s0, s1, s2);
+      else
+        exdm_sum<R, G, false>(h, h.nbr + site, src, site, ens, sc.ex_row, dr, s0, s1, s2);
+    }
+  } else if (h.nbr16) {
     if (h.nt)
       exchange_sum<R, G, true>(h, h.nbr16 + site, src, site, ens, sc.ex_row, s0, s1, s2);
     else
@@ -486,7 +560,7 @@ __device__ __forceinline__ void site_field(const HamDev<R> &h, const Quad<R> *__
     else
       exchange_sum<R, G, false>(h, h.nbr + site, src, site, ens, sc.ex_row, s0, s1, s2);
   }
-  if (h.Wd > 0 && sc.dm_row >= 0) {
+  if (!h.dm_shared && h.Wd > 0 && sc.dm_row >= 0) {
     if (h.dnbr16)
       dm_sum<R, G>(h, h.dnbr16 + site, src, site, ens, sc.dm_row, s0, s1, s2);
     else
@@ -1103,7 +1177,50 @@ __device__ __forceinline__ void tile_field(const HamDev<R> &h, const TileDev &t,
     sc = h.classes[h.cls[site]];
   }
   R s0 = 0, s1 = 0, s2 = 0, qf0 = 0, qf1 = 0, qf2 = 0;
-  {
+  if (FULL && h.dm_shared) {
+    // the DM bonds are the exchange bonds (at most kFusedW): one LDS read of
+    // each neighbour serves both sums, the terms added in the order of the
+    // separate loops below
+    const int len = h.ex_len[sc.ex_row];
+    const R *cp = h.cpl + sc.ex_row;
+    const int *off = t.off + b * t.W;
+    R c0[kFusedW], c1[kFusedW], c2[kFusedW];
+#pragma unroll
+    for (int j0 = 0; j0 < kFusedW; j0 += 4) {
+      R J[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (j0 + u < len) {
+          const R *v = sm + (hbase + cload(off + j0 + u));
+          c0[j0 + u] = v[0];
+          c1[j0 + u] = v[PL];
+          c2[j0 + u] = v[2 * PL];
+          J[u] = UCLS ? cload(cp + (size_t)(j0 + u) * h.ex_rows) : cp[(size_t)(j0 + u) * h.ex_rows];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (j0 + u < len) {
+          s0 = fma(J[u], c0[j0 + u], s0);
+          s1 = fma(J[u], c1[j0 + u], s1);
+          s2 = fma(J[u], c2[j0 + u], s2);
+        }
+      }
+    }
+    if (h.Wd > 0 && sc.dm_row >= 0) {
+#pragma unroll
+      for (int j = 0; j < kFusedW; ++j) {
+        if (j < len) {
+          const size_t r = (size_t)j * h.dm_rows + sc.dm_row;
+          const R D0 = UCLS ? cload(h.dvx + r) : h.dvx[r], D1 = UCLS ? cload(h.dvy + r) : h.dvy[r],
+                  D2 = UCLS ? cload(h.dvz + r) : h.dvz[r];
+          s0 = s0 + D2 * c1[j] - D1 * c2[j];
+          s1 = s1 + D0 * c2[j] - D2 * c0[j];
+          s2 = s2 + D1 * c0[j] - D0 * c1[j];
+        }
+      }
+    }
+  } else {
     // four neighbours per round: LDS reads first, then the terms in order
     const int len = h.ex_len[sc.ex_row];
     const R *cp = h.cpl + sc.ex_row;
@@ -1134,7 +1251,7 @@ __device__ __forceinline__ void tile_field(const HamDev<R> &h, const TileDev &t,
       s2 = fma(J, v[2 * PL], s2);
     }
   }
-  if (FULL && h.Wd > 0 && sc.dm_row >= 0) {
+  if (FULL && !h.dm_shared && h.Wd > 0 && sc.dm_row >= 0) {
     const int len = h.dm_len[sc.dm_row];
     const int *off = t.doff + b * t.Wd;
     for (int j = 0; j < len; ++j) {

@@ -15,7 +15,9 @@ pytestmark = pytest.mark.gpu
 
 NT = 0x10000   # non-temporal neighbour-list loads
 I32 = 0x20000  # int32 entries although packed int16 entries exist
+PAIR = 0x1000000  # MC: two adjacent ensembles per thread
 SHAPES = [(2, 64), (4, 128 | NT), (8, 256), (8, 256 | NT | I32), (16, 64 | I32), (16, 256 | NT)]
+MC_SHAPES = SHAPES + [(2, 64 | PAIR), (4, 256 | PAIR | I32), (8, 128 | PAIR | NT)]
 
 
 def _llg_run(sysd, emom, gather, block, precision, alg, steps=20):
@@ -60,7 +62,7 @@ def test_mc_results_independent_of_launch_shape():
     N = sysd["lattice"].natom
     emom = np.ascontiguousarray(CF.random_unit(N, rng, 2))
     out = []
-    for g, b in [(4, 256)] + SHAPES:
+    for g, b in [(4, 256)] + MC_SHAPES:
         eng = U.Engine(N, 2, "fp64")
         eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
         eng.set_moments(emom, np.broadcast_to(sysd["mmom"], (2, N)))
@@ -68,6 +70,42 @@ def test_mc_results_independent_of_launch_shape():
         eng.mc_sweeps("M", [300.0, 900.0], 5, 77)
         eng.mc_sweeps("H", [300.0, 900.0], 5, 78)
         out.append(eng.get_moments()[0])
+    for x in out[1:]:
+        assert np.array_equal(x, out[0])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("M", [4, 3])
+def test_mc_pair_of_ensembles_per_thread_is_bit_identical(precision, M):
+    """Two adjacent ensembles per thread (shared neighbour entries and
+    couplings) against one per thread, on a Hamiltonian with every MC term
+    (exchange, DM with |m| != 1, anisotropy types 1, 2 and 7, Zeeman):
+    Metropolis and heat bath give the same moments bit for bit.  With M odd,
+    and in heat bath, the flag leaves one ensemble per thread."""
+    rng = np.random.default_rng(64)
+    sysd = CF.triangular_dmi(12, J=1.0, D=0.2, K=0.0, B=0.0, moment=2.2)
+    N = sysd["lattice"].natom
+    taniso = rng.choice([1, 2, 7], size=N).astype(np.int32)
+    ea = rng.normal(size=(N, 3))
+    ea /= np.linalg.norm(ea, axis=1, keepdims=True)
+    kan = np.ascontiguousarray(rng.uniform(-0.02, 0.02, size=(N, 2)))
+    sb = rng.uniform(0.2, 1.5, N)
+    temps = np.linspace(100.0, 900.0, M)
+    emom = np.ascontiguousarray(CF.random_unit(N, rng, M))
+    out = []
+    for g, b in [(4, 256), (4, 256 | PAIR), (2, 64 | PAIR | I32), (8, 128 | PAIR | NT)]:
+        eng = U.Engine(N, M, precision)
+        eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
+        eng.set_dm(sysd["dmlist"], sysd["dmlistsize"], sysd["dm_vect"])
+        eng.set_anisotropy(taniso, np.ascontiguousarray(ea), kan, sb)
+        eng.set_external_field(np.array([0.4, -0.3, 1.1]))
+        eng.set_moments(emom, np.full((M, N), 2.2))
+        eng.set_tuning(eng.TUNE_MC, g, b)
+        assert eng.tuning(eng.TUNE_MC) == (g, b)
+        eng.mc_sweeps("M", temps, 4, 321)
+        eng.mc_sweeps("H", temps, 3, 322)
+        out.append(eng.get_moments()[0])
+    assert np.abs(out[0] - emom).max() > 0.1      # the sweeps moved the spins
     for x in out[1:]:
         assert np.array_equal(x, out[0])
 
@@ -81,6 +119,10 @@ def test_bad_shape_rejected():
         eng.set_tuning(eng.TUNE_LLG, 4, 512)
     with pytest.raises(U.UasdError):
         eng.set_tuning(7, 4, 256)
+    with pytest.raises(U.UasdError):
+        eng.set_tuning(eng.TUNE_LLG, 4, 256 | PAIR)   # MC only
+    with pytest.raises(U.UasdError):
+        eng.set_tuning(eng.TUNE_MC, 4, 256 | (PAIR << 1))
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
@@ -205,7 +247,7 @@ def test_tune_cache_reuses_winners_and_persists(tmp_path):
         ms_b = b.autotune_llg(1, steps=2)
         assert ms_b == ms_a and b.tuning(b.TUNE_LLG) == a.tuning(a.TUNE_LLG)
         lines = path.read_text().splitlines()
-        assert len(lines) == 1 and lines[0].split()[0].startswith("v1/gfx950/llg/a1/fp64/")
+        assert len(lines) == 1 and lines[0].split()[0].startswith("v2/gfx950/llg/a1/fp64/")
         U.set_tune_cache(False)
         U.set_tune_cache(True, str(path))      # reload from the file alone
         c = ctx()

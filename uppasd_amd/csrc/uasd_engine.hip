@@ -136,6 +136,7 @@ struct uasd_ctx {
   std::vector<int> ex_row_of_site;  // (N)
   std::vector<int> nbr_host;        // (W*N) slot-major, 0-based
   std::vector<int> ex_len_host;     // (ex_rows)
+  bool ex_full = false;             // every exchange row holds W neighbours
   int Wd = 0, dm_rows = 0;
   std::vector<int> dm_row_of_site, dnbr_host, dm_len_host;
   std::vector<int> an_row_of_site;
@@ -287,6 +288,7 @@ uasd::HamDev<R> ham_view(const uasd_ctx *c) {
   h.mdiv = uasd::make_fast_div((uint32_t)c->M);
   h.W = c->W;
   h.ex_rows = c->ex_rows;
+  h.ex_full = c->ex_full ? 1 : 0;
   h.nbr = c->nbr.as<int>();
   h.nbr16 = c->use16 ? c->nbr16.as<int16_t>() : nullptr;
   h.ex_len = c->ex_len.as<int>();
@@ -1090,6 +1092,7 @@ int uasd_set_exchange(uasd_ctx *c, int mnn, int nham, const int *nlist, const in
   c->ex_rows = rows;
   c->ex_row_of_site = row_dev;
   c->ex_len_host = rs.len;
+  c->ex_full = W > 0 && std::all_of(rs.len.begin(), rs.len.end(), [W](int l) { return l == W; });
   c->nbr_host = nb;
   if (int r = upload_raw(c, c->nbr, nb)) return r;
   std::vector<int16_t> d16;
@@ -1742,10 +1745,12 @@ int uasd_set_tuning(uasd_ctx *c, int kind, int gather, int block) {
   // block may carry flags: 0x10000 non-temporal list loads, 0x20000 int32
   // entries, (k << 18) LDS-tiled stencil shape k and 0x400000 resident
   // workgroups looping over tiles (LLG only), 0x800000 a separate DM gather
-  // where the DM bonds are the exchange bonds (context-wide)
+  // where the DM bonds are the exchange bonds (context-wide), 0x1000000 two
+  // adjacent ensembles per thread (MC only)
   uasd::Tune t{gather, block & 0xffff, (block >> 16) & 1, (block >> 17) & 1, (block >> 18) & 0xf,
-               (block >> 22) & 1};
-  if (!uasd::tune_valid(t) || (block >> 24) != 0 || (kind != 0 && t.tile != 0))
+               (block >> 22) & 1, (block >> 24) & 1};
+  if (!uasd::tune_valid(t) || (block >> 25) != 0 || (kind != 0 && t.tile != 0) ||
+      (kind != 2 && t.pair != 0))
     return fail(UASD_ERR_ARG, "gather must be 2, 4, 8 or 16 and block 64, 128 or 256");
   c->tune[kind] = t;
   c->tune_no_fuse = ((block >> 23) & 1) != 0;
@@ -1759,7 +1764,7 @@ int uasd_get_tuning(uasd_ctx *c, int kind, int *gather, int *block) {
   if (block)
     *block = c->tune[kind].block | (c->tune[kind].nt << 16) | (c->tune[kind].idx32 << 17) |
              (c->tune[kind].tile << 18) | (c->tune[kind].pipe << 22) |
-             ((c->tune_no_fuse ? 1 : 0) << 23);
+             ((c->tune_no_fuse ? 1 : 0) << 23) | (c->tune[kind].pair << 24);
   return UASD_OK;
 }
 
@@ -1819,7 +1824,7 @@ std::map<std::string, std::pair<uasd::Tune, double>> g_tune;
 std::string tune_key(const uasd_ctx *c, const char *kind, int alg) {
   char buf[320];
   std::snprintf(buf, sizeof buf,
-                "v1/gfx950/%s/a%d/%s/N%d/M%d/seg%d/W%d/Wd%d/an%d/ext%d/order%d/lat%d,%d,%d,%d/i16%d%d/"
+                "v2/gfx950/%s/a%d/%s/N%d/M%d/seg%d/W%d/Wd%d/an%d/ext%d/order%d/lat%d,%d,%d,%d/i16%d%d/"
                 "noise%d",
                 kind, alg, c->prec == UASD_FP64 ? "fp64" : "fp32", c->N, c->M, c->nseg, c->W,
                 c->Wd, c->an_rows > 0 ? 1 : 0, c->ext_mode, c->order_na, c->lat_na, c->lat_n1,
@@ -1844,8 +1849,8 @@ void tune_store(const std::string &key, const uasd::Tune &t, double ms) {
   g_tune[key] = {t, ms};
   if (g_tune_path.empty()) return;
   if (FILE *f = std::fopen(g_tune_path.c_str(), "a")) {
-    std::fprintf(f, "%s %d %d %d %d %d %d %.6f\n", key.c_str(), t.gather, t.block, t.nt, t.idx32,
-                 t.tile, t.pipe, ms);
+    std::fprintf(f, "%s %d %d %d %d %d %d %d %.6f\n", key.c_str(), t.gather, t.block, t.nt,
+                 t.idx32, t.tile, t.pipe, t.pair, ms);
     std::fclose(f);
   }
 }
@@ -1864,7 +1869,10 @@ int tune_search(uasd_ctx *c, uasd::Tune *slot, Run &&run, double *best_ms) {
   for (int ix = 0; ix < nidx; ++ix)
     for (int nt = 0; nt < 2; ++nt)
       for (int g : gathers)
-        for (int bl : blocks) cand.push_back(uasd::Tune{g, bl, nt, ix, 0});
+        for (int bl : blocks) {
+          cand.push_back(uasd::Tune{g, bl, nt, ix, 0});
+          if (slot == &c->tune[2] && c->M % 2 == 0) cand.push_back(uasd::Tune{g, bl, nt, ix, 0, 0, 1});
+        }
   if (slot == &c->tune[0]) {
     build_stencil(c);
     for (int k = 1; k <= uasd::kNumTileShapes; ++k)
@@ -1906,7 +1914,7 @@ int uasd_autotune_llg(uasd_ctx *c, int alg, int steps, double *best_ms) {
   const std::string key = tune_key(c, "llg", alg);
   uasd::Tune cached;
   double cached_ms = 0.0;
-  if (tune_lookup(key, cached, cached_ms) && uasd::tune_valid(cached) &&
+  if (tune_lookup(key, cached, cached_ms) && uasd::tune_valid(cached) && cached.pair == 0 &&
       (cached.tile == 0 || tile_fits(c, cached.tile))) {
     c->tune[0] = c->tune[1] = cached;
     c->tune[1].tile = c->tune[1].pipe = 0;
@@ -1977,8 +1985,8 @@ int uasd_set_tune_cache(int enable, const char *path) {
   uasd::Tune t;
   double ms = 0.0;
   int loaded = 0;
-  while (std::fscanf(f, "%319s %d %d %d %d %d %d %lf", key, &t.gather, &t.block, &t.nt, &t.idx32,
-                     &t.tile, &t.pipe, &ms) == 8) {
+  while (std::fscanf(f, "%319s %d %d %d %d %d %d %d %lf", key, &t.gather, &t.block, &t.nt,
+                     &t.idx32, &t.tile, &t.pipe, &t.pair, &ms) == 9) {
     if (uasd::tune_valid(t)) {
       g_tune[key] = {t, ms};
       ++loaded;

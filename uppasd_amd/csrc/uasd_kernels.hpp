@@ -78,6 +78,7 @@ struct HamDev {
   FastDiv mdiv;  // division by M
   // exchange
   int W, ex_rows;
+  int ex_full;  // every exchange row holds W neighbours
   const int *__restrict__ nbr;
   const int16_t *__restrict__ nbr16;  // (n - site) mod N differences, or nullptr
   const int *__restrict__ ex_len;
@@ -384,13 +385,15 @@ __device__ __forceinline__ I nb_load(const I *p) {
 // loads of a batch are issued, then all spin and coupling loads, then the
 // terms are added in neighbour order (the same order, hence the same
 // rounding, as one at a time).  A last partial batch masks the slots past len.
-template <typename R, int G, bool NT, typename I>
-__device__ __forceinline__ void exchange_sum(const HamDev<R> &h, const I *__restrict__ nb,
-                                             const Quad<R> *__restrict__ src, int site, int ens,
-                                             int row, R &s0, R &s1, R &s2) {
+// U > 1 sums the fields of the U adjacent ensembles e0 .. e0+U-1 of the site
+// together: each neighbour entry and coupling is read once for all of them,
+// and each field is summed exactly as it is for U = 1.
+template <typename R, int G, bool NT, typename I, int U>
+__device__ __forceinline__ void exchange_run(const HamDev<R> &h, const I *__restrict__ nb,
+                                             const Quad<R> *__restrict__ src, int site, int e0,
+                                             int row, int len, R (&f)[U][3]) {
   const int M = h.M, N = h.N;
   const NbDec dec(h, site);
-  const int len = h.ex_len[row];
   const R *cp = h.cpl + row;
   const size_t rs = (size_t)h.ex_rows;
   int j = 0;
@@ -398,18 +401,22 @@ __device__ __forceinline__ void exchange_sum(const HamDev<R> &h, const I *__rest
     int n[G];
 #pragma unroll
     for (int u = 0; u < G; ++u) n[u] = dec(nb_load<NT>(nb + (size_t)(j + u) * N));
-    Quad<R> v[G];
+    Quad<R> v[G][U];
     R J[G];
 #pragma unroll
     for (int u = 0; u < G; ++u) {
-      v[u] = src[(size_t)n[u] * M + ens];
+#pragma unroll
+      for (int k = 0; k < U; ++k) v[u][k] = src[(size_t)n[u] * M + e0 + k];
       J[u] = cp[(size_t)(j + u) * rs];
     }
 #pragma unroll
     for (int u = 0; u < G; ++u) {
-      s0 = fma(J[u], v[u].x * v[u].w, s0);
-      s1 = fma(J[u], v[u].y * v[u].w, s1);
-      s2 = fma(J[u], v[u].z * v[u].w, s2);
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        f[k][0] = fma(J[u], v[u][k].x * v[u][k].w, f[k][0]);
+        f[k][1] = fma(J[u], v[u][k].y * v[u][k].w, f[k][1]);
+        f[k][2] = fma(J[u], v[u][k].z * v[u][k].w, f[k][2]);
+      }
     }
   }
   if (j < len) {
@@ -417,24 +424,51 @@ __device__ __forceinline__ void exchange_sum(const HamDev<R> &h, const I *__rest
 #pragma unroll
     for (int u = 0; u < G; ++u)
       n[u] = j + u < len ? dec(nb_load<NT>(nb + (size_t)(j + u) * N)) : site;
-    Quad<R> v[G];
+    Quad<R> v[G][U];
     R J[G];
 #pragma unroll
     for (int u = 0; u < G; ++u) {
       if (j + u < len) {
-        v[u] = src[(size_t)n[u] * M + ens];
+#pragma unroll
+        for (int k = 0; k < U; ++k) v[u][k] = src[(size_t)n[u] * M + e0 + k];
         J[u] = cp[(size_t)(j + u) * rs];
       }
     }
 #pragma unroll
     for (int u = 0; u < G; ++u) {
       if (j + u < len) {
-        s0 = fma(J[u], v[u].x * v[u].w, s0);
-        s1 = fma(J[u], v[u].y * v[u].w, s1);
-        s2 = fma(J[u], v[u].z * v[u].w, s2);
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+          f[k][0] = fma(J[u], v[u][k].x * v[u][k].w, f[k][0]);
+          f[k][1] = fma(J[u], v[u][k].y * v[u][k].w, f[k][1]);
+          f[k][2] = fma(J[u], v[u][k].z * v[u][k].w, f[k][2]);
+        }
       }
     }
   }
+}
+
+// When every row holds W neighbours the loop runs to the launch argument W,
+// the same for every lane: its control and the batch tail are scalar.
+template <typename R, int G, bool NT, typename I, int U>
+__device__ __forceinline__ void exchange_fields(const HamDev<R> &h, const I *__restrict__ nb,
+                                                const Quad<R> *__restrict__ src, int site, int e0,
+                                                int row, R (&f)[U][3]) {
+  if (h.ex_full)
+    exchange_run<R, G, NT, I, U>(h, nb, src, site, e0, row, h.W, f);
+  else
+    exchange_run<R, G, NT, I, U>(h, nb, src, site, e0, row, h.ex_len[row], f);
+}
+
+template <typename R, int G, bool NT, typename I>
+__device__ __forceinline__ void exchange_sum(const HamDev<R> &h, const I *__restrict__ nb,
+                                             const Quad<R> *__restrict__ src, int site, int ens,
+                                             int row, R &s0, R &s1, R &s2) {
+  R f[1][3] = {{s0, s1, s2}};
+  exchange_fields<R, G, NT, I, 1>(h, nb, src, site, ens, row, f);
+  s0 = f[0][0];
+  s1 = f[0][1];
+  s2 = f[0][2];
 }
 
 // DM sum of site_field: field += m_j x D_ij (hamiltonianactions.f90:565-578).
@@ -477,12 +511,11 @@ __device__ __forceinline__ void dm_sum(const HamDev<R> &h, const I *__restrict__
 constexpr int kFusedW = 8;
 
 template <typename R, int G, bool NT, typename I>
-__device__ __forceinline__ void exdm_sum(const HamDev<R> &h, const I *__restrict__ nb,
+__device__ __forceinline__ void exdm_run(const HamDev<R> &h, const I *__restrict__ nb,
                                          const Quad<R> *__restrict__ src, int site, int ens,
-                                         int exrow, int dmrow, R &s0, R &s1, R &s2) {
+                                         int exrow, int dmrow, int len, R &s0, R &s1, R &s2) {
   const int M = h.M, N = h.N;
   const NbDec dec(h, site);
-  const int len = h.ex_len[exrow];
   const R *cp = h.cpl + exrow;
   const size_t rs = (size_t)h.ex_rows;
   R c0[kFusedW], c1[kFusedW], c2[kFusedW];
@@ -525,6 +558,17 @@ __device__ __forceinline__ void exdm_sum(const HamDev<R> &h, const I *__restrict
       s2 = s2 + D1 * c0[j] - D0 * c1[j];
     }
   }
+}
+
+// As exchange_sum: with full rows every guard compares with the launch's W.
+template <typename R, int G, bool NT, typename I>
+__device__ __forceinline__ void exdm_sum(const HamDev<R> &h, const I *__restrict__ nb,
+                                         const Quad<R> *__restrict__ src, int site, int ens,
+                                         int exrow, int dmrow, R &s0, R &s1, R &s2) {
+  if (h.ex_full)
+    exdm_run<R, G, NT, I>(h, nb, src, site, ens, exrow, dmrow, h.W, s0, s1, s2);
+  else
+    exdm_run<R, G, NT, I>(h, nb, src, site, ens, exrow, dmrow, h.ex_len[exrow], s0, s1, s2);
 }
 
 // G = neighbours whose loads are issued together (a tuning parameter).
@@ -1518,89 +1562,102 @@ __device__ __forceinline__ R sq(R x) {
 }
 
 // calculate_energy (montecarlo_common.f90:431-865), exchange + anisotropy +
-// DM + external field; returns mub*(E_trial - E_current).
-template <typename R, int G>
-__device__ __forceinline__ R mc_delta_e(const HamDev<R> &h, const Quad<R> *__restrict__ s,
-                                        int site, int ens, R n0, R n1, R n2, R x0, R x1, R x2) {
+// DM + external field, for the U adjacent ensembles e0 .. e0+U-1 of one site:
+// de[k] = mub*(E_trial - E_current) of ensemble e0+k, whose current moment is
+// me[k], trial direction tr[k] and uniform field x[k].  The site's class, its
+// neighbour entries and its couplings are read once for the U ensembles.
+template <typename R, int G, int U>
+__device__ __forceinline__ void mc_delta_e(const HamDev<R> &h, const Quad<R> *__restrict__ s,
+                                           int site, int e0, const Quad<R> (&me)[U],
+                                           const R (&tr)[U][3], const R (&x)[U][3],
+                                           R (&de)[U]) {
   const SiteClass sc = h.classes[h.cls[site]];
   const int M = h.M, N = h.N;
-  const size_t qi = (size_t)site * M + ens;
-  const Quad<R> me = s[qi];
-  const R m = me.w;
-  const R c0 = me.x * m, c1 = me.y * m, c2 = me.z * m;
-  const R t0 = n0 * m, t1 = n1 * m, t2 = n2 * m;
-  R e_c = 0, e_t = 0;
-  {
-    // batched loads, terms added in neighbour order (as in site_field)
-    const int len = h.ex_len[sc.ex_row];
-    for (int j = 0; j < len; j += G) {
-      int nb[G];
+  // The exchange energy of the current and the trial moment, -sum_j J_j
+  // m.m_j (montecarlo_common.f90:512-520), is formed as -m.f with the
+  // exchange field f = sum_j J_j m_j of site_field: the same quantity with a
+  // third of the arithmetic, equal to the term-by-term sums to rounding.
+  R f[U][3];
 #pragma unroll
-      for (int u = 0; u < G; ++u)
-        nb[u] = j + u >= len ? site
-                : h.nbr16 ? NbDec(h, site)(h.nbr16[(size_t)(j + u) * N + site])
-                          : h.nbr[(size_t)(j + u) * N + site];
-      Quad<R> v[G];
-      R J[G];
+  for (int k = 0; k < U; ++k) f[k][0] = f[k][1] = f[k][2] = (R)0;
+  if (h.nbr16) {
+    if (h.nt)
+      exchange_fields<R, G, true, int16_t, U>(h, h.nbr16 + site, s, site, e0, sc.ex_row, f);
+    else
+      exchange_fields<R, G, false, int16_t, U>(h, h.nbr16 + site, s, site, e0, sc.ex_row, f);
+  } else {
+    if (h.nt)
+      exchange_fields<R, G, true, int, U>(h, h.nbr + site, s, site, e0, sc.ex_row, f);
+    else
+      exchange_fields<R, G, false, int, U>(h, h.nbr + site, s, site, e0, sc.ex_row, f);
+  }
+  R c0[U], c1[U], c2[U], t0[U], t1[U], t2[U], e_c[U], e_t[U];
 #pragma unroll
-      for (int u = 0; u < G; ++u) {
-        if (j + u < len) {
-          v[u] = s[(size_t)nb[u] * M + ens];
-          J[u] = h.cpl[(size_t)(j + u) * h.ex_rows + sc.ex_row];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < G; ++u) {
-        if (j + u < len) {
-          const R a0 = v[u].x * v[u].w, a1 = v[u].y * v[u].w, a2 = v[u].z * v[u].w;
-          e_c = e_c - J[u] * (c0 * a0 + c1 * a1 + c2 * a2);
-          e_t = e_t - J[u] * (t0 * a0 + t1 * a1 + t2 * a2);
-        }
-      }
-    }
+  for (int k = 0; k < U; ++k) {
+    const R m = me[k].w;
+    c0[k] = me[k].x * m;
+    c1[k] = me[k].y * m;
+    c2[k] = me[k].z * m;
+    t0[k] = tr[k][0] * m;
+    t1[k] = tr[k][1] * m;
+    t2[k] = tr[k][2] * m;
+    e_c[k] = (R)0 - (c0[k] * f[k][0] + c1[k] * f[k][1] + c2[k] * f[k][2]);
+    e_t[k] = (R)0 - (t0[k] * f[k][0] + t1[k] * f[k][1] + t2[k] * f[k][2]);
   }
   if (sc.an_row >= 0) {
     const int ty = h.an_type[sc.an_row];
     const R k1 = h.an_k1[sc.an_row], k2 = h.an_k2[sc.an_row];
     const R ea0 = h.an_ex[sc.an_row], ea1 = h.an_ey[sc.an_row], ea2 = h.an_ez[sc.an_row];
-    if (ty == 1 || ty == 7) {
-      R tta = c0 * ea0 + c1 * ea1 + c2 * ea2;
-      R ttb = t0 * ea0 + t1 * ea1 + t2 * ea2;
-      e_c = e_c + k1 * sq(tta) + k2 * sq(sq(tta));
-      e_t = e_t + k1 * sq(ttb) + k2 * sq(sq(ttb));
-    } else if (ty == 2) {
-      e_c = e_c - k1 * (sq(c0) * sq(c1) + sq(c1) * sq(c2) + sq(c2) * sq(c0)) -
-            k2 * (sq(c0) * sq(c1) * sq(c2));
-      e_t = e_t - k1 * (sq(t0) * sq(t1) + sq(t1) * sq(t2) + sq(t2) * sq(t0)) -
-            k2 * (sq(t0) * sq(t1) * sq(t2));
-    }
-    if (ty == 7) {
-      const R sb = h.an_sb[sc.an_row];
-      const R aw1 = k1 * sb, aw2 = k2 * sb;
-      e_c = e_c + aw1 * (sq(c0) * sq(c1) + sq(c1) * sq(c2) + sq(c2) * sq(c0)) +
-            aw2 * (sq(c0) * sq(c1) * sq(c2));
-      e_t = e_t + aw1 * (sq(t0) * sq(t1) + sq(t1) * sq(t2) + sq(t2) * sq(t0)) +
-            aw2 * (sq(t0) * sq(t1) * sq(t2));
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      if (ty == 1 || ty == 7) {
+        R tta = c0[k] * ea0 + c1[k] * ea1 + c2[k] * ea2;
+        R ttb = t0[k] * ea0 + t1[k] * ea1 + t2[k] * ea2;
+        e_c[k] = e_c[k] + k1 * sq(tta) + k2 * sq(sq(tta));
+        e_t[k] = e_t[k] + k1 * sq(ttb) + k2 * sq(sq(ttb));
+      } else if (ty == 2) {
+        e_c[k] = e_c[k] - k1 * (sq(c0[k]) * sq(c1[k]) + sq(c1[k]) * sq(c2[k]) + sq(c2[k]) * sq(c0[k])) -
+                 k2 * (sq(c0[k]) * sq(c1[k]) * sq(c2[k]));
+        e_t[k] = e_t[k] - k1 * (sq(t0[k]) * sq(t1[k]) + sq(t1[k]) * sq(t2[k]) + sq(t2[k]) * sq(t0[k])) -
+                 k2 * (sq(t0[k]) * sq(t1[k]) * sq(t2[k]));
+      }
+      if (ty == 7) {
+        const R sb = h.an_sb[sc.an_row];
+        const R aw1 = k1 * sb, aw2 = k2 * sb;
+        e_c[k] = e_c[k] + aw1 * (sq(c0[k]) * sq(c1[k]) + sq(c1[k]) * sq(c2[k]) + sq(c2[k]) * sq(c0[k])) +
+                 aw2 * (sq(c0[k]) * sq(c1[k]) * sq(c2[k]));
+        e_t[k] = e_t[k] + aw1 * (sq(t0[k]) * sq(t1[k]) + sq(t1[k]) * sq(t2[k]) + sq(t2[k]) * sq(t0[k])) +
+                 aw2 * (sq(t0[k]) * sq(t1[k]) * sq(t2[k]));
+      }
     }
   }
   if (h.Wd > 0 && sc.dm_row >= 0) {
-    const R u0 = me.x, u2 = me.z;  // unit-vector terms as in montecarlo_common.f90:612,615
     const int len = h.dm_len[sc.dm_row];
     for (int j = 0; j < len; ++j) {
       const int nb = h.dnbr16 ? NbDec(h, site)(h.dnbr16[(size_t)j * N + site])
                               : h.dnbr[(size_t)j * N + site];
       const size_t r = (size_t)j * h.dm_rows + sc.dm_row;
       const R D0 = h.dvx[r], D1 = h.dvy[r], D2 = h.dvz[r];
-      R a0, a1, a2;
-      moment(s, (size_t)nb * M + ens, a0, a1, a2);
-      e_c = e_c - D0 * (c1 * a2 - u2 * a1) - D1 * (c2 * a0 - c0 * a2) - D2 * (u0 * a1 - c1 * a0);
-      e_t = e_t - D0 * (t1 * a2 - t2 * a1) - D1 * (t2 * a0 - t0 * a2) - D2 * (t0 * a1 - t1 * a0);
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        // unit-vector terms as in montecarlo_common.f90:612,615
+        const R u0 = me[k].x, u2 = me[k].z;
+        R a0, a1, a2;
+        moment(s, (size_t)nb * M + e0 + k, a0, a1, a2);
+        e_c[k] = e_c[k] - D0 * (c1[k] * a2 - u2 * a1) - D1 * (c2[k] * a0 - c0[k] * a2) -
+                 D2 * (u0 * a1 - c1[k] * a0);
+        e_t[k] = e_t[k] - D0 * (t1[k] * a2 - t2[k] * a1) - D1 * (t2[k] * a0 - t0[k] * a2) -
+                 D2 * (t0[k] * a1 - t1[k] * a0);
+      }
     }
   }
-  e_c = e_c - x0 * c0 - x1 * c1 - x2 * c2;
-  e_t = e_t - x0 * t0 - x1 * t1 - x2 * t2;
-  const R tt = e_t - e_c;
-  return (R)kMub * tt;
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    e_c[k] = e_c[k] - x[k][0] * c0[k] - x[k][1] * c1[k] - x[k][2] * c2[k];
+    e_t[k] = e_t[k] - x[k][0] * t0[k] - x[k][1] * t1[k] - x[k][2] * t2[k];
+    const R tt = e_t[k] - e_c[k];
+    de[k] = (R)kMub * tt;
+  }
 }
 
 // flip_h (montecarlo_common.f90:371-422)
@@ -1634,92 +1691,132 @@ struct MCParams {
   uint64_t seed, sweep;      // sweep counter, or offset from *sweep_dev
   const uint64_t *sweep_dev;  // device-held sweep base (graph replays) or nullptr
   int heat_bath;
+  FastDiv pdiv;  // division by M / U (the threads of one site)
 };
 
-template <typename R, int G>
+// FP32 engine: the whole Metropolis trial of one (site, ensemble) from one
+// Philox block, in single precision.  Each word w_i gives a 24-bit uniform
+// v_i from its top bits; the 32 low bits left over pick the move and the
+// third Gaussian's angle.  The move (montecarlo_common.f90:25-79's three, with
+// equal weights to 2^-16): a uniform direction by z = 1 - 2 v2, sin(theta) =
+// sqrt(1 - z^2) and the azimuth 2 pi v1 (the distribution of acos / sin); the
+// Gaussian step by Box-Muller, (g0, g1) from (v1, v2) and g2 from radius v0
+// and a 16-bit angle; the flip.  acc = v3 is compared with expf.  The
+// polynomial ln and sincos are the thermal field's.
+__device__ __forceinline__ void trial_f32(const MCParams &mp, uint64_t sweep, uint32_t hs,
+                                          int ens, float e0, float e1, float e2, float &n0,
+                                          float &n1, float &n2, float &acc) {
+  uint32_t w0, w1, w2, w3;
+  counter_words(mp.seed, kTagMCU, sweep, hs, (uint32_t)ens, w0, w1, w2, w3);
+  const float v0 = u24(w0), v1 = u24(w1), v2 = u24(w2);
+  acc = u24(w3);
+  const int ftype = (int)((3u * (((w0 & 0xffu) << 8) | (w1 & 0xffu))) >> 16);
+  if (ftype == 0) {
+    const float z = 1.0f - 2.0f * v2;
+    const float st = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    float sp, cp;
+    bm::sincos_q(4.0f * v1, sp, cp);
+    n0 = st * cp;
+    n1 = st * sp;
+    n2 = z;
+  } else if (ftype == 1) {
+    const float r01 = sqrtf(-2.0f * bm::ln(v1)), r2 = sqrtf(-2.0f * bm::ln(v0));
+    const float a16 = ((float)(((w2 & 0xffu) << 8) | (w3 & 0xffu)) + 0.5f) * 1.52587890625e-05f;
+    float s01, c01, s2, c2;
+    bm::sincos_q(4.0f * v2, s01, c01);
+    bm::sincos_q(4.0f * a16, s2, c2);
+    const float dl = (float)mp.delta[ens];
+    const float a0 = e0 + (r01 * c01) * dl, a1 = e1 + (r01 * s01) * dl, a2 = e2 + (r2 * c2) * dl;
+    const float il = 1.0f / sqrtf(a0 * a0 + a1 * a1 + a2 * a2);
+    n0 = a0 * il;
+    n1 = a1 * il;
+    n2 = a2 * il;
+  } else {
+    n0 = 0.0f - e0;
+    n1 = 0.0f - e1;
+    n2 = 0.0f - e2;
+  }
+}
+
+// One colour of a sweep.  HB selects heat bath (mp.heat_bath) or Metropolis
+// at compile time, so each mode's registers are allocated for its own code
+// alone.  A Metropolis thread updates the U adjacent ensembles e0 .. e0+U-1
+// of one site of the colour (U = 1, or 2 when M is even): the trial moves and
+// acceptances are those of U = 1, drawn from the same counters, and the
+// energy changes share the site's neighbour entries and couplings.
+template <typename R, int G, int U, bool HB>
 __global__ void __launch_bounds__(256) k_mc_colour(HamDev<R> h, MCParams mp, Quad<R> *s) {
   // (site of the colour, ensemble) pairs number fewer than 2^31
-  const uint32_t total = (uint32_t)mp.nsites * (uint32_t)h.M;
+  const uint32_t per = (uint32_t)h.M / U;
+  const uint32_t total = (uint32_t)mp.nsites * per;
   const uint32_t t = (uint32_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const int pidx = (int)fast_div(t, h.mdiv), ens = (int)t - pidx * h.M;
+  const int pidx = (int)fast_div(t, U == 1 ? h.mdiv : mp.pdiv);
+  const int e0 = ((int)t - pidx * (int)per) * U;
   const int site = mp.sites[pidx];
-  const size_t q = (size_t)site * h.M + ens;
+  const size_t q0 = (size_t)site * h.M + e0;
   const uint64_t sweep = mp.sweep_dev ? *mp.sweep_dev + mp.sweep : mp.sweep;
   const uint32_t hs = host_site(h, site);
-  const Quad<R> me = s[q];
-  const R x0 = (R)mp.ext[3 * ens], x1 = (R)mp.ext[3 * ens + 1], x2 = (R)mp.ext[3 * ens + 2];
-  R o0, o1, o2;
-  if (mp.heat_bath) {
-    double u0, u1, u2, u3;
-    uniform4(mp.seed, kTagMCU, sweep, hs, (uint32_t)ens, u0, u1, u2, u3);
-    R s0, s1, s2, q0, q1, q2, y0, y1, y2;
-    site_field<R, G>(h, (const Quad<R> *)s, site, ens, s0, s1, s2, q0, q1, q2, y0, y1, y2);
-    const R f0 = (R)0 + s0 + (q0 + x0 + (R)0);
-    const R f1 = (R)0 + s1 + (q1 + x1 + (R)0);
-    const R f2 = (R)0 + s2 + (q2 + x2 + (R)0);
-    heat_bath<R>(mp.hbeta[ens], me.w, f0, f1, f2, u1, u0, o0, o1, o2);
-  } else if (std::is_same<R, float>::value) {
-    // FP32 engine: the whole Metropolis update from one Philox block, in
-    // single precision.  Each word w_i gives a 24-bit uniform v_i from its top
-    // bits; the 32 low bits left over pick the move and the third Gaussian's
-    // angle.  The move (montecarlo_common.f90:25-79's three, with equal
-    // weights to 2^-16): a uniform direction by z = 1 - 2 v2, sin(theta) =
-    // sqrt(1 - z^2) and the azimuth 2 pi v1 (the distribution of acos /
-    // sin); the Gaussian step by Box-Muller, (g0, g1) from (v1, v2) and g2
-    // from radius v0 and a 16-bit angle; the flip.  Acceptance compares v3
-    // with expf.  The polynomial ln and sincos are the thermal field's.
-    uint32_t w0, w1, w2, w3;
-    counter_words(mp.seed, kTagMCU, sweep, hs, (uint32_t)ens, w0, w1, w2, w3);
-    const float v0 = u24(w0), v1 = u24(w1), v2 = u24(w2), v3 = u24(w3);
-    const int ftype = (int)((3u * (((w0 & 0xffu) << 8) | (w1 & 0xffu))) >> 16);
-    float n0, n1, n2;
-    if (ftype == 0) {
-      const float z = 1.0f - 2.0f * v2;
-      const float st = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-      float sp, cp;
-      bm::sincos_q(4.0f * v1, sp, cp);
-      n0 = st * cp;
-      n1 = st * sp;
-      n2 = z;
-    } else if (ftype == 1) {
-      const float r01 = sqrtf(-2.0f * bm::ln(v1)), r2 = sqrtf(-2.0f * bm::ln(v0));
-      const float a16 = ((float)(((w2 & 0xffu) << 8) | (w3 & 0xffu)) + 0.5f) * 1.52587890625e-05f;
-      float s01, c01, s2, c2;
-      bm::sincos_q(4.0f * v2, s01, c01);
-      bm::sincos_q(4.0f * a16, s2, c2);
-      const float dl = (float)mp.delta[ens];
-      const float a0 = (float)me.x + (r01 * c01) * dl, a1 = (float)me.y + (r01 * s01) * dl,
-                  a2 = (float)me.z + (r2 * c2) * dl;
-      const float il = 1.0f / sqrtf(a0 * a0 + a1 * a1 + a2 * a2);
-      n0 = a0 * il;
-      n1 = a1 * il;
-      n2 = a2 * il;
-    } else {
-      n0 = 0.0f - (float)me.x;
-      n1 = 0.0f - (float)me.y;
-      n2 = 0.0f - (float)me.z;
-    }
-    const R de = mc_delta_e<R, G>(h, (const Quad<R> *)s, site, ens, (R)n0, (R)n1, (R)n2, x0, x1, x2);
-    if (!(de <= (R)0 || v3 < expf(-(float)mp.beta[ens] * (float)de))) return;
-    o0 = (R)n0;
-    o1 = (R)n1;
-    o2 = (R)n2;
-  } else {
-    double u0, u1, u2, u3;
-    uniform4(mp.seed, kTagMCU, sweep, hs, (uint32_t)ens, u0, u1, u2, u3);
-    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-    if ((int)floor(3 * u0) == 1)
-      normal3_f64(mp.seed, kTagMCG, sweep, hs, (uint32_t)ens, g0, g1, g2);
-    R n0, n1, n2;
-    trial_move<R>(me.x, me.y, me.z, mp.delta[ens], u0, u1, u2, g0, g1, g2, n0, n1, n2);
-    const R de = mc_delta_e<R, G>(h, (const Quad<R> *)s, site, ens, n0, n1, n2, x0, x1, x2);
-    if (!(de <= (R)0 || u3 < exp(-mp.beta[ens] * (double)de))) return;
-    o0 = n0;
-    o1 = n1;
-    o2 = n2;
+  Quad<R> me[U];
+  R x[U][3];
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    me[k] = s[q0 + k];
+    x[k][0] = (R)mp.ext[3 * (e0 + k)];
+    x[k][1] = (R)mp.ext[3 * (e0 + k) + 1];
+    x[k][2] = (R)mp.ext[3 * (e0 + k) + 2];
   }
-  s[q] = Quad<R>{o0, o1, o2, me.w};
+  if constexpr (HB) {
+    // heat bath: one ensemble per thread (the launch never pairs them)
+    static_assert(U == 1, "heat-bath launches take one ensemble per thread");
+    double u0, u1, u2, u3;
+    uniform4(mp.seed, kTagMCU, sweep, hs, (uint32_t)e0, u0, u1, u2, u3);
+    R s0, s1, s2, q0f, q1f, q2f, y0, y1, y2;
+    site_field<R, G>(h, (const Quad<R> *)s, site, e0, s0, s1, s2, q0f, q1f, q2f, y0, y1, y2);
+    const R f0 = (R)0 + s0 + (q0f + x[0][0] + (R)0);
+    const R f1 = (R)0 + s1 + (q1f + x[0][1] + (R)0);
+    const R f2 = (R)0 + s2 + (q2f + x[0][2] + (R)0);
+    R o0, o1, o2;
+    heat_bath<R>(mp.hbeta[e0], me[0].w, f0, f1, f2, u1, u0, o0, o1, o2);
+    s[q0] = Quad<R>{o0, o1, o2, me[0].w};
+  } else {
+    R tr[U][3];
+    double acc[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int ens = e0 + k;
+      if (std::is_same<R, float>::value) {
+        float n0, n1, n2, a;
+        trial_f32(mp, sweep, hs, ens, (float)me[k].x, (float)me[k].y, (float)me[k].z, n0, n1, n2,
+                  a);
+        tr[k][0] = (R)n0;
+        tr[k][1] = (R)n1;
+        tr[k][2] = (R)n2;
+        acc[k] = a;
+      } else {
+        double u0, u1, u2, u3;
+        uniform4(mp.seed, kTagMCU, sweep, hs, (uint32_t)ens, u0, u1, u2, u3);
+        double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+        if ((int)floor(3 * u0) == 1)
+          normal3_f64(mp.seed, kTagMCG, sweep, hs, (uint32_t)ens, g0, g1, g2);
+        trial_move<R>(me[k].x, me[k].y, me[k].z, mp.delta[ens], u0, u1, u2, g0, g1, g2, tr[k][0],
+                      tr[k][1], tr[k][2]);
+        acc[k] = u3;
+      }
+    }
+    R de[U];
+    mc_delta_e<R, G, U>(h, (const Quad<R> *)s, site, e0, me, tr, x, de);
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int ens = e0 + k;
+      bool take;
+      if (std::is_same<R, float>::value)
+        take = de[k] <= (R)0 || (float)acc[k] < expf(-(float)mp.beta[ens] * (float)de[k]);
+      else
+        take = de[k] <= (R)0 || acc[k] < exp(-mp.beta[ens] * (double)de[k]);
+      if (take) s[q0 + k] = Quad<R>{tr[k][0], tr[k][1], tr[k][2], me[k].w};
+    }
+  }
 }
 
 // ---------------------------------------------------------- FP64 reductions

@@ -27,13 +27,15 @@ struct Tune {
   int idx32 = 0;
   int tile = 0;
   int pipe = 0;  // tile kernel: 1 = resident workgroups loop over several tiles
+  int pair = 0;  // MC colour pass: 1 = two adjacent ensembles per thread (M even)
 };
 
 inline bool tune_valid(const Tune &t) {
   return (t.gather == 2 || t.gather == 4 || t.gather == 8 || t.gather == 16) &&
          (t.block == 64 || t.block == 128 || t.block == 256) && (t.nt == 0 || t.nt == 1) &&
          (t.idx32 == 0 || t.idx32 == 1) && t.tile >= 0 && t.tile <= kNumTileShapes &&
-         (t.pipe == 0 || (t.pipe == 1 && t.tile > 0));
+         (t.pipe == 0 || (t.pipe == 1 && t.tile > 0)) &&
+         (t.pair == 0 || (t.pair == 1 && t.tile == 0));
 }
 
 // The Hamiltonian view a launch uses under a tuning choice.

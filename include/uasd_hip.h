@@ -216,6 +216,11 @@ int uasd_get_kernel_timing(uasd_ctx *ctx, int kind, int64_t *launches, double *t
  * entries (every cell difference fits; periodic lattices numbered cell by
  * cell) and the tuning uses them, else 4. */
 int uasd_index_bytes(uasd_ctx *ctx, int *exchange, int *dm);
+/* The kernel the fused LLG step launches under the current tuning (no
+ * reference counterpart): *tile = the LDS-tiled stencil shape k (1..15), or 0
+ * for the list-gather kernels (also when a requested shape does not fit the
+ * context: no stencil, LDS, thread count). */
+int uasd_llg_tile(uasd_ctx *ctx, int *tile);
 
 /* ---- launch shapes of the gathering kernels (no reference counterpart:
  * the reference's OpenMP loops have no launch shape) ----
@@ -227,9 +232,10 @@ int uasd_index_bytes(uasd_ctx *ctx, int *exchange, int *dm);
  * shape k = 1..15 (kind 0 only; needs uasd_set_lattice_dims and a stencil):
  * cells (x, y, z) = (8,8,4) (16,8,2) (32,4,2) (16,4,4) (16,16,1) (32,8,1)
  * (64,4,1) (8,4,4) (4,4,4) (8,8,2) (16,8,1) (8,8,1) (16,8,4) (8,16,4) (8,8,8),
- * one thread per site (cells x basis atoms <= 1024); plus 0x400000 for
- * resident workgroups that each loop over a run of adjacent tiles, with the
- * next tile's halo loads in flight during the current tile's computation;
+ * one thread per site (cells x basis atoms <= 1024), work units (tile,
+ * ensemble); plus 0x400000 for resident workgroups that each loop over a run
+ * of adjacent units, with the next unit's halo loads in flight during the
+ * current unit's computation;
  * plus 0x800000 (any kind, for the whole context) to gather DM neighbours
  * separately even where every site's DM bonds are its exchange bonds (at
  * most 8), which the list kernels otherwise gather once for both sums;

@@ -15,9 +15,9 @@ pytestmark = pytest.mark.gpu
 TILE = lambda k, pipe=0: 256 | (k << 18) | (pipe << 22)
 
 
-def _eng(sysd, order, precision="fp64"):
+def _eng(sysd, order, precision="fp64", M=1):
     lat = sysd["lattice"]
-    eng = U.Engine(lat.natom, 1, precision, site_order=order)
+    eng = U.Engine(lat.natom, M, precision, site_order=order)
     eng.set_lattice_dims(lat.NA, *lat.n)
     eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
     if "dmlist" in sysd:
@@ -30,8 +30,9 @@ def _eng(sysd, order, precision="fp64"):
 
 
 def _run(sysd, e0, order, tile, alg, precision="fp64", steps=12, pipe=0):
-    eng = _eng(sysd, order, precision)
-    eng.set_moments(e0, sysd["mmom"][None])
+    M = e0.shape[0]
+    eng = _eng(sysd, order, precision, M)
+    eng.set_moments(e0, np.broadcast_to(sysd["mmom"], (M, sysd["lattice"].natom)))
     eng.set_llg(0.1, 200.0, 1e-16)
     eng.set_noise(U.UASD_NOISE_PHILOX, 17)
     eng.set_tuning(eng.TUNE_LLG, 4, TILE(tile, pipe) if tile else 256)
@@ -63,6 +64,34 @@ def test_tile_equals_list_kernels(case, alg):
             for pipe in (0, 1):
                 got, _ = _run(sysd, e0, order, k, alg, pipe=pipe)
                 assert np.array_equal(got, ref), (case, order, k, pipe)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("M", [2, 3])
+@pytest.mark.parametrize("alg", [1, 5])
+def test_tile_several_ensembles_equals_list_kernels(alg, M, precision):
+    """M ensembles of one lattice: the tile kernel runs (tile, ensemble)
+    units, staging each ensemble's halo from the interleaved quads, with the
+    per-ensemble noise and fields; every shape, one unit per workgroup and
+    the resident loop (whose runs of units cross from one ensemble and tile
+    to the next), both site orders, give the list kernels' moments bit for
+    bit."""
+    for case in ("bcc5_L10", "tri_dmi"):
+        sysd = CASES[case]()
+        lat = sysd["lattice"]
+        rng = np.random.default_rng(95 + M)
+        e0 = np.ascontiguousarray(CF.random_unit(lat.natom, rng, M))
+        ref, eng = _run(sysd, e0, 1, 0, alg, precision)
+        assert eng.stencil()[0], case
+        assert np.abs(ref[0] - ref[1]).max() > 1e-3      # the ensembles differ
+        shapes = [k for k, sh in enumerate(U.engine.TILE_SHAPES, 1)
+                  if (lat.n[2] > 1) == (sh[2] > 1)][:3]
+        for order in (1, lat.NA):
+            for k in shapes:
+                for pipe in (0, 1):
+                    got, e = _run(sysd, e0, order, k, alg, precision, pipe=pipe)
+                    assert e.llg_tile() == k      # the tile kernel ran
+                    assert np.array_equal(got, ref), (case, order, k, pipe)
 
 
 @pytest.mark.parametrize("alg", [1, 4, 5, 6, 7])
@@ -97,7 +126,8 @@ def test_no_stencil_with_vacuum_boundary():
     e0 = np.ascontiguousarray(CF.random_unit(sysd["lattice"].natom, rng, 1))
     ref, eng = _run(sysd, e0, 1, 0, 1)
     assert not eng.stencil()[0]
-    got, _ = _run(sysd, e0, 1, 1, 1)   # tile requested: falls back to the list kernels
+    got, e = _run(sysd, e0, 1, 1, 1)   # tile requested: falls back to the list kernels
+    assert e.llg_tile() == 0
     assert np.array_equal(got, ref)
 
 

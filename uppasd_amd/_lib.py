@@ -35,7 +35,8 @@ EXPORTS = (
     "uasd_set_temprescale", "uasd_set_segments", "uasd_measure_segments", "uasd_energy_terms",
     "uasd_energy_terms_noalloc",
     "uasd_accumulate", "uasd_get_accumulators", "uasd_reset_accumulators", "uasd_set_graph_steps",
-    "uasd_index_bytes", "uasd_set_site_order", "uasd_set_lattice_dims", "uasd_has_stencil",
+    "uasd_index_bytes", "uasd_llg_tile", "uasd_set_site_order", "uasd_set_lattice_dims",
+    "uasd_has_stencil",
     "uasd_build_pair_list", "uasd_set_tune_cache",
 )
 
@@ -95,6 +96,7 @@ def lib() -> C.CDLL:
         L.uasd_reset_accumulators.argtypes = [vp]
         L.uasd_set_graph_steps.argtypes = [vp, C.c_int]
         L.uasd_index_bytes.argtypes = [vp, ip, ip]
+        L.uasd_llg_tile.argtypes = [vp, ip]
         L.uasd_set_site_order.argtypes = [vp, C.c_int]
         L.uasd_set_lattice_dims.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.uasd_has_stencil.argtypes = [vp, ip, vp]

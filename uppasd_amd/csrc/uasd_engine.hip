@@ -841,7 +841,9 @@ void build_stencil(uasd_ctx *c) {
 
 // Tile shape k (1-based) usable for this lattice and precision?
 bool tile_fits(const uasd_ctx *c, int k) {
-  if (!c->stencil_ok || c->M != 1 || k < 1 || k > uasd::kNumTileShapes) return false;
+  if (!c->stencil_ok || k < 1 || k > uasd::kNumTileShapes) return false;
+  // quad indices of the staging tables and the (tile, ensemble) units in int
+  if ((int64_t)c->N * c->M >= (int64_t)1 << 31) return false;
   if (c->order_na != 1 && c->order_na != c->lat_na) return false;  // tile kernel's slot formula
   const int *sh = uasd::kTileShapes[k - 1];
   if ((c->lat_n3 > 1) != (sh[2] > 1)) return false;
@@ -1735,6 +1737,17 @@ int uasd_index_bytes(uasd_ctx *c, int *exchange, int *dm) {
   const bool i32 = c->tune[0].idx32 != 0;  // as the LLG kernels are tuned
   if (exchange) *exchange = c->use16 && !i32 ? 2 : 4;
   if (dm) *dm = c->use16_dm && !i32 ? 2 : 4;
+  return UASD_OK;
+}
+
+int uasd_llg_tile(uasd_ctx *c, int *tile) {
+  if (int r = check_ctx(c)) return r;
+  if (!tile) return fail(UASD_ERR_ARG, "tile must not be NULL");
+  *tile = 0;
+  if (c->tune[0].tile > 0) {
+    build_stencil(c);
+    if (tile_fits(c, c->tune[0].tile)) *tile = c->tune[0].tile;
+  }
   return UASD_OK;
 }
 

@@ -1208,7 +1208,8 @@ constexpr int kTilePlanes[kNumTilePlanes] = {1160, 2312, 3080, 3848};
 // kernel holds fewer launch arguments live (exchange-only Hamiltonians).
 template <typename R, int PL, bool UCLS, bool FULL>
 __device__ __forceinline__ void tile_field(const HamDev<R> &h, const TileDev &t, const R *sm,
-                                           int hbase, int b, int site, R &b0, R &b1, R &b2v) {
+                                           int hbase, int b, int site, int ens, int64_t q, R &b0,
+                                           R &b1, R &b2v) {
   SiteClass sc;
   if (UCLS) {
     // wave-uniform values kept in scalar registers
@@ -1329,11 +1330,11 @@ __device__ __forceinline__ void tile_field(const HamDev<R> &h, const TileDev &t,
   }
   R x0 = 0, x1 = 0, x2 = 0;
   if (FULL && h.ext_mode == 1) {
-    x0 = h.ext_u[0];
-    x1 = h.ext_u[1];
-    x2 = h.ext_u[2];
+    x0 = h.ext_u[3 * ens];
+    x1 = h.ext_u[3 * ens + 1];
+    x2 = h.ext_u[3 * ens + 2];
   } else if (FULL && h.ext_mode == 2) {
-    const Quad<R> v = h.ext_q[site];
+    const Quad<R> v = h.ext_q[q];
     x0 = v.x;
     x1 = v.y;
     x2 = v.z;
@@ -1367,10 +1368,11 @@ __device__ __forceinline__ void tile_origin(const TileDev &t, int tile, int &X0,
 // The staging table of a tile: tab[hx] is the x term of halo column hx,
 // tab[HX + row] the device slot of the row's basis atom at its wrapped (y, z)
 // and x = 0, so that slot = tab[hx] + tab[HX + row] (periodic images of any
-// multiplicity).
+// multiplicity).  Both terms are scaled by M, so that the sum is the quad
+// index of the slot's ensemble 0 (N*M < 2^31).
 template <typename R>
 __device__ __forceinline__ void halo_table(const HamDev<R> &h, const TileDev &t, int X0, int Y0,
-                                           int Z0, int ncell, int *tab) {
+                                           int Z0, int ncell, int M, int *tab) {
   const int nr = t.NA * t.HY * t.HZ, hyz = t.HY * t.HZ;
   for (int i = threadIdx.x; i < t.HX + nr; i += blockDim.x) {
     int v;
@@ -1386,7 +1388,7 @@ __device__ __forceinline__ void halo_table(const HamDev<R> &h, const TileDev &t,
       const int cyz = t.N1 * (y + t.N2 * z);
       v = h.order_na == 1 ? hb + t.NA * cyz : hb * ncell + cyz;
     }
-    tab[i] = v;
+    tab[i] = v * M;
   }
 }
 
@@ -1445,24 +1447,28 @@ __device__ __forceinline__ void halo_rest(const TileDev &t, const int *tab, cons
   }
 }
 
-// Persistent tiles: workgroup w takes the tiles [w*per, (w+1)*per) of the
-// XCD-aware block order, adjacent in x.  The halo loads of its next tile are
-// issued before the current tile is computed and stored after it, so their
-// latency is covered by that computation.
-template <typename R, int ALG, int HALF, int PL, bool FULL>
+// Persistent tiles: the work units are (tile, ensemble) pairs, unit u =
+// tile*M + ens, so the M ensembles of a tile follow one another (their quads
+// share cache lines).  Workgroup w takes the units [w*per, (w+1)*per) of the
+// XCD-aware block order.  The halo loads of its next unit are issued before
+// the current unit is computed and stored after it, so their latency is
+// covered by that computation.  ENS = false is the one-ensemble instance, in
+// which every unit term folds away at compile time.
+template <typename R, int ALG, int HALF, int PL, bool FULL, bool ENS>
 __global__ void __launch_bounds__(1024)
     k_llg_tile(HamDev<R> h, LLGDev<R> prm, TileDev t, const Quad<R> *e, Quad<R> *p,
                Quad<R> *b2, Quad<R> *out) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   R *sm = reinterpret_cast<R *>(lds_raw);
   const Quad<R> *src = HALF == 0 ? e : (const Quad<R> *)p;
-  const int ntiles = t.ntx * t.nty * t.ntz;
+  const int M = ENS ? h.M : 1;
+  const int nunits = t.ntx * t.nty * t.ntz * M;
   const int first = (int)xcd_block() * t.per;
-  const int last = min(ntiles, first + t.per);
+  const int last = min(nunits, first + t.per);
   if (first >= last) return;  // whole workgroups exit before any barrier
   const int ncell = t.N1 * t.N2 * t.N3;
   const int nw = t.HX + t.NA * t.HY * t.HZ;
-  int *tabs = reinterpret_cast<int *>(sm + 3 * PL);  // two tables, alternating by tile
+  int *tabs = reinterpret_cast<int *>(sm + 3 * PL);  // two tables, alternating by unit
   // this thread's site in every tile: cell (lx, ly, lz), basis b.  blockDim.x
   // = tc * NA with tc a multiple of 64, so b is the same for all lanes of a
   // wavefront and the stencil offsets (and, for one species, the couplings)
@@ -1475,49 +1481,54 @@ __global__ void __launch_bounds__(1024)
   const int ly = fdiv(lxy, t.rTX), lx = lxy - ly * t.TX;
   const int hcell = (lz + t.RZ) * t.HX * t.HY + (ly + t.RY) * t.HX + (lx + t.RX);
   const HaloCol col = halo_col(t);
-  // the first tile: table, halo
+  // the first unit: table, halo
+  int tile = first / M, ens = first - tile * M;
   int X0, Y0, Z0;
-  tile_origin(t, first, X0, Y0, Z0);
-  halo_table(h, t, X0, Y0, Z0, ncell, tabs);
+  tile_origin(t, tile, X0, Y0, Z0);
+  halo_table(h, t, X0, Y0, Z0, ncell, M, tabs);
   __syncthreads();
   Quad<R> v[kTilePre];
   int item[kTilePre];
-  halo_load<R, kTilePre>(t, tabs, col, 0, src, v, item);
+  halo_load<R, kTilePre>(t, tabs, col, 0, src + ens, v, item);
 #pragma unroll
   for (int u = 0; u < kTilePre; ++u)
     if (item[u] >= 0) halo_put<R, PL>(sm, item[u], v[u]);
-  halo_rest<R, PL>(t, tabs, col, src, sm);
-  for (int tile = first;;) {
-    const int k = tile - first;
+  halo_rest<R, PL>(t, tabs, col, src + ens, sm);
+  for (int unit = first;;) {
+    const int k = unit - first;
     const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
     const bool valid = x < t.N1 && y < t.N2 && z < t.N3;
     const int cell = x + t.N1 * (y + t.N2 * z);
     const int site = h.order_na == 1 ? b + t.NA * cell : b * ncell + cell;
-    const int next = tile + 1 < last ? tile + 1 : -1;
+    const int64_t q = (int64_t)site * M + ens;
+    const int next = unit + 1 < last ? unit + 1 : -1;
+    const int ntile = ens + 1 < M ? tile : tile + 1, nens = ens + 1 < M ? ens + 1 : 0;
     int *tnext = tabs + ((k + 1) & 1) * nw;
     int nX0 = 0, nY0 = 0, nZ0 = 0;
     if (next >= 0) {
-      tile_origin(t, next, nX0, nY0, nZ0);
-      halo_table(h, t, nX0, nY0, nZ0, ncell, tnext);
+      tile_origin(t, ntile, nX0, nY0, nZ0);
+      halo_table(h, t, nX0, nY0, nZ0, ncell, M, tnext);
     }
-    __syncthreads();  // this tile's halo and the next tile's table are in place
-    if (next >= 0) halo_load<R, kTilePre>(t, tnext, col, 0, src, v, item);
+    __syncthreads();  // this unit's halo and the next unit's table are in place
+    if (next >= 0) halo_load<R, kTilePre>(t, tnext, col, 0, src + nens, v, item);
     if (valid) {
       R b0, b1, b2v;
       if (t.ucls)
-        tile_field<R, PL, true, FULL>(h, t, sm, hcell, b, site, b0, b1, b2v);
+        tile_field<R, PL, true, FULL>(h, t, sm, hcell, b, site, ens, q, b0, b1, b2v);
       else
-        tile_field<R, PL, false, FULL>(h, t, sm, hcell, b, site, b0, b1, b2v);
-      const SiteIn<R> in = llg_fetch<R, ALG, HALF>(prm, b + t.NA * cell, 0, site, e, p, b2);
-      llg_apply<R, ALG, HALF>(prm, site, site, b0, b1, b2v, in, p, b2, out);
+        tile_field<R, PL, false, FULL>(h, t, sm, hcell, b, site, ens, q, b0, b1, b2v);
+      const SiteIn<R> in = llg_fetch<R, ALG, HALF>(prm, b + t.NA * cell, ens, q, e, p, b2);
+      llg_apply<R, ALG, HALF>(prm, site, q, b0, b1, b2v, in, p, b2, out);
     }
     if (next < 0) break;
-    __syncthreads();  // every read of this tile's halo is done
+    __syncthreads();  // every read of this unit's halo is done
 #pragma unroll
     for (int u = 0; u < kTilePre; ++u)
       if (item[u] >= 0) halo_put<R, PL>(sm, item[u], v[u]);
-    halo_rest<R, PL>(t, tnext, col, src, sm);
-    tile = next;
+    halo_rest<R, PL>(t, tnext, col, src + nens, sm);
+    unit = next;
+    tile = ntile;
+    ens = nens;
     X0 = nX0;
     Y0 = nY0;
     Z0 = nZ0;
@@ -1711,22 +1722,27 @@ __device__ __forceinline__ void trial_f32(const MCParams &mp, uint64_t sweep, ui
   const float v0 = u24(w0), v1 = u24(w1), v2 = u24(w2);
   acc = u24(w3);
   const int ftype = (int)((3u * (((w0 & 0xffu) << 8) | (w1 & 0xffu))) >> 16);
-  if (ftype == 0) {
-    const float z = 1.0f - 2.0f * v2;
-    const float st = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-    float sp, cp;
-    bm::sincos_q(4.0f * v1, sp, cp);
-    n0 = st * cp;
-    n1 = st * sp;
+  // The lanes of a wave choose their moves independently, so a wave runs
+  // every move's arithmetic.  The uniform direction's sincos(2 pi v1) and
+  // sqrt(1 - z^2) and the Gaussian pair's sincos(2 pi v2) and sqrt(-2 ln v1)
+  // are therefore one sincos and one sqrt of a per-lane argument: each lane
+  // computes exactly the values of its own move.
+  const bool uni = ftype == 0;
+  const float z = 1.0f - 2.0f * v2;
+  float sA, cA;
+  bm::sincos_q(4.0f * (uni ? v1 : v2), sA, cA);
+  const float rA = sqrtf(uni ? fmaxf(0.0f, 1.0f - z * z) : -2.0f * bm::ln(v1));
+  if (uni) {
+    n0 = rA * cA;
+    n1 = rA * sA;
     n2 = z;
   } else if (ftype == 1) {
-    const float r01 = sqrtf(-2.0f * bm::ln(v1)), r2 = sqrtf(-2.0f * bm::ln(v0));
+    const float r2 = sqrtf(-2.0f * bm::ln(v0));
     const float a16 = ((float)(((w2 & 0xffu) << 8) | (w3 & 0xffu)) + 0.5f) * 1.52587890625e-05f;
-    float s01, c01, s2, c2;
-    bm::sincos_q(4.0f * v2, s01, c01);
+    float s2, c2;
     bm::sincos_q(4.0f * a16, s2, c2);
     const float dl = (float)mp.delta[ens];
-    const float a0 = e0 + (r01 * c01) * dl, a1 = e1 + (r01 * s01) * dl, a2 = e2 + (r2 * c2) * dl;
+    const float a0 = e0 + (rA * cA) * dl, a1 = e1 + (rA * sA) * dl, a2 = e2 + (r2 * c2) * dl;
     const float il = 1.0f / sqrtf(a0 * a0 + a1 * a1 + a2 * a2);
     n0 = a0 * il;
     n1 = a1 * il;

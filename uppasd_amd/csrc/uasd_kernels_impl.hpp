@@ -123,19 +123,25 @@ template <typename R>
 hipError_t launch_llg_tile(hipStream_t s, TileDev t, int pipe, int alg, int half,
                            const HamDev<R> &h, const LLGDev<R> &prm, const Quad<R> *e,
                            Quad<R> *p, Quad<R> *b2, Quad<R> *out) {
-  const int ntiles = t.ntx * t.nty * t.ntz;
-  if (ntiles == 0) return hipSuccess;
+  // work units: (tile, ensemble) pairs
+  const int64_t nunits64 = (int64_t)t.ntx * t.nty * t.ntz * h.M;
+  if (nunits64 == 0) return hipSuccess;
+  if (nunits64 >= (int64_t)1 << 31 || (int64_t)h.N * h.M >= (int64_t)1 << 31)
+    return hipErrorInvalidValue;
+  const int nunits = (int)nunits64;
   const int threads = t.TX * t.TY * t.TZ * t.NA;
-  if (threads > 1024 || h.M != 1) return hipErrorInvalidValue;
+  if (threads > 1024) return hipErrorInvalidValue;
   const int plane = tile_plane(t.NA * t.HV);
   // DM, anisotropy or Zeeman terms present: the full kernel
   const bool full = h.Wd > 0 || h.an_type != nullptr || h.ext_mode != 0;
+  // several ensembles: the (tile, ensemble) unit instance
+  const bool ens = h.M > 1;
   if (plane == 0) return hipErrorInvalidValue;
   if (t.rs < 1) return hipErrorInvalidValue;  // at least one halo row per round
   const size_t lds = tile_lds_bytes(sizeof(R), plane, t.HX + t.NA * t.HY * t.HZ);
-  // one tile per workgroup, or (pipe) as many workgroups as are resident at
+  // one unit per workgroup, or (pipe) as many workgroups as are resident at
   // once (occupancy x CUs, cached per kernel and shape), each looping over a
-  // run of consecutive tiles
+  // run of consecutive units
   auto tiles_per_wg = [&](const void *kern, size_t &c_lds, int &c_thr, int &c_slots) -> hipError_t {
     t.per = 1;
     if (!pipe) return hipSuccess;
@@ -151,30 +157,34 @@ hipError_t launch_llg_tile(hipStream_t s, TileDev t, int pipe, int alg, int half
       c_thr = threads;
       c_slots = std::max(1, occ) * std::max(1, ncu);
     }
-    t.per = (ntiles + c_slots - 1) / c_slots;
+    t.per = (nunits + c_slots - 1) / c_slots;
     return hipSuccess;
   };
-#define UASD_TILE_CASE(A, H, PL, F)                                                             \
-  if (alg == A && half == H && plane == PL && full == F) {                                                   \
+#define UASD_TILE_CASE(A, H, PL, F, E)                                                          \
+  if (alg == A && half == H && plane == PL && full == F && ens == E) {                          \
     static size_t lds_set = 0; /* dynamic LDS above 64 KiB must be requested */                \
     if (lds > lds_set) {                                                                        \
-      hipError_t err = hipFuncSetAttribute((const void *)&k_llg_tile<R, A, H, PL, F>,              \
+      hipError_t err = hipFuncSetAttribute((const void *)&k_llg_tile<R, A, H, PL, F, E>,        \
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
       if (err != hipSuccess) return err;                                                        \
       lds_set = lds;                                                                            \
     }                                                                                           \
     static size_t c_lds = 0;                                                                    \
     static int c_thr = 0, c_slots = 0;                                                          \
-    hipError_t gerr = tiles_per_wg((const void *)&k_llg_tile<R, A, H, PL, F>, c_lds, c_thr, c_slots); \
+    hipError_t gerr = tiles_per_wg((const void *)&k_llg_tile<R, A, H, PL, F, E>, c_lds, c_thr,  \
+                                   c_slots);                                                    \
     if (gerr != hipSuccess) return gerr;                                                        \
-    const int nwg = (ntiles + t.per - 1) / t.per;                                               \
+    const int nwg = (nunits + t.per - 1) / t.per;                                               \
     const dim3 grid((nwg + kXCD - 1) / kXCD * kXCD), block(threads);                            \
-    hipLaunchKernelGGL((k_llg_tile<R, A, H, PL, F>), grid, block, lds, s, h, prm, t, e, p, b2, out); \
+    hipLaunchKernelGGL((k_llg_tile<R, A, H, PL, F, E>), grid, block, lds, s, h, prm, t, e, p, b2, \
+                       out);                                                                    \
     return hipGetLastError();                                                                   \
   }
 #define UASD_TILE_FULL(A, H, PL)                                                                \
-  UASD_TILE_CASE(A, H, PL, false)                                                               \
-  UASD_TILE_CASE(A, H, PL, true)
+  UASD_TILE_CASE(A, H, PL, false, false)                                                        \
+  UASD_TILE_CASE(A, H, PL, true, false)                                                         \
+  UASD_TILE_CASE(A, H, PL, false, true)                                                         \
+  UASD_TILE_CASE(A, H, PL, true, true)
 #define UASD_TILE_PLANES(A, H)                                                                  \
   UASD_TILE_FULL(A, H, kTilePlanes[0])                                                          \
   UASD_TILE_FULL(A, H, kTilePlanes[1])                                                          \

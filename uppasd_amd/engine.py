@@ -242,6 +242,13 @@ class Engine:
         B.check(B.lib().uasd_index_bytes(self._h, C.byref(a), C.byref(b)), "uasd_index_bytes")
         return a.value, b.value
 
+    def llg_tile(self):
+        """Tile shape k the fused LLG step launches under the current tuning,
+        0 for the list-gather kernels."""
+        k = C.c_int(0)
+        B.check(B.lib().uasd_llg_tile(self._h, C.byref(k)), "uasd_llg_tile")
+        return k.value
+
     def set_graph_steps(self, k: int):
         """Replay hipGraphs of k LLG steps / MC sweeps (0 = launch one by one)."""
         B.check(B.lib().uasd_set_graph_steps(self._h, int(k)), "uasd_set_graph_steps")

@@ -558,32 +558,33 @@ def _c5_oracle_error(rep, e0, steps, graph, e_final_dev, e_avg_dev):
 
 def c2_variant(U, args, label: str, sysd=None, L: int = 64, temp: float = 300.0,
                noise: str = "philox", stencil: bool = True, steps: int = 400,
-               warmup: int = 40):
+               warmup: int = 40, M: int = 1):
     """The plain FP64 workload under another noise source, size or kernel
     family, bounded to a few hundred steps: value, step time and roofline
     fractions by the workload's byte model and by the kernels' own bytes,
     with the counter traffic of the committed profile of the same run where
     there is one.  noise: "philox" (in-kernel), "injected" (one N(0,1) array
     uploaded once and read by both halves of every step; a drop-in caller
-    would upload a fresh one per step over PCIe), "none" (T = 0)."""
+    would upload a fresh one per step over PCIe), "none" (T = 0).  M > 1: M
+    ensembles of the lattice, each with its own start and noise."""
     if sysd is None:
         sysd = build_c2(L)
     lat = sysd["lattice"]
     N = lat.natom
     W = int(sysd["nlistsize"].max())
-    eng = U.Engine(N, 1, args.precision, site_order=lat.NA)
+    eng = U.Engine(N, M, args.precision, site_order=lat.NA)
     if stencil:
         eng.set_lattice_dims(lat.NA, *lat.n)
     eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
     rng = np.random.default_rng(2)
-    e0 = rng.normal(size=(1, N, 3))
+    e0 = rng.normal(size=(M, N, 3))
     e0 /= np.linalg.norm(e0, axis=2, keepdims=True)
-    eng.set_moments(np.ascontiguousarray(e0), sysd["mmom"][None])
+    eng.set_moments(np.ascontiguousarray(e0), np.broadcast_to(sysd["mmom"], (M, N)))
     eng.set_llg(0.1, temp if noise != "none" else 0.0, 1e-16)
     if noise == "philox":
         eng.set_noise(U.UASD_NOISE_PHILOX, 0xC2C2C2C2)
     elif noise == "injected":
-        eng.set_noise(U.UASD_NOISE_INJECTED, 0, np.ascontiguousarray(rng.normal(size=(1, N, 3))))
+        eng.set_noise(U.UASD_NOISE_INJECTED, 0, np.ascontiguousarray(rng.normal(size=(M, N, 3))))
     else:
         eng.set_noise(U.UASD_NOISE_NONE)
     eng.autotune_llg(args.solver, steps=3)
@@ -591,7 +592,7 @@ def c2_variant(U, args, label: str, sysd=None, L: int = 64, temp: float = 300.0,
     eng.sync()
     tm = timed_steps(eng, args.solver, steps, args.timing_every)
     g, b = eng.tuning(eng.TUNE_LLG)
-    tile, pipe = (b >> 18) & 0xF, (b >> 22) & 1
+    tile, pipe = eng.llg_tile(), (b >> 22) & 1     # the kernel the step launches
     s = 8 if args.precision == "fp64" else 4
     ib = 0 if tile else eng.index_bytes()[0]
     kb = kernel_bytes(W, s, ib, cls=0 if tile else 4, noise=noise, b2=args.solver in (4, 5))
@@ -600,18 +601,19 @@ def c2_variant(U, args, label: str, sysd=None, L: int = 64, temp: float = 300.0,
     variant = f"tile{tile}{'r' if pipe else ''}" if tile else f"idx{ib}"
     tr = profiled_traffic(label, variant)
     traffic = tr["predictor"] + tr["corrector"] if tr else None
-    out = {"workload": f"bcc {lat.n[0]}^3 x 2 ({N} spins), {W} neighbours, "
-                       f"{SOLVER_NAMES[args.solver]}, noise {noise}"
+    NM = N * M
+    out = {"workload": f"bcc {lat.n[0]}^3 x 2 ({N} spins{f' x {M} ensembles' if M > 1 else ''}), "
+                       f"{W} neighbours, {SOLVER_NAMES[args.solver]}, noise {noise}"
                        f"{'' if noise == 'none' else f', T {temp:g} K'}",
-           "value": N * steps / tm["wall_s"], "ms_per_step": tm["step_s"] * 1e3,
+           "value": NM * steps / tm["wall_s"], "ms_per_step": tm["step_s"] * 1e3,
            "kernel_ms": [round(tm["pred_s"] * 1e3, 4), round(tm["corr_s"] * 1e3, 4)],
            "kernel": variant,
            "bytes_per_update_model": sb["step"], "own_bytes_per_site": kb["step"],
-           "frac_model": sb["step"] * N / tm["step_s"] / 1e9 / HBM_PEAK_GBS,
-           "frac_own_bytes": kb["step"] * N / tm["step_s"] / 1e9 / HBM_PEAK_GBS,
+           "frac_model": sb["step"] * NM / tm["step_s"] / 1e9 / HBM_PEAK_GBS,
+           "frac_own_bytes": kb["step"] * NM / tm["step_s"] / 1e9 / HBM_PEAK_GBS,
            "traffic_per_step": traffic, "steps": steps}
     if traffic:
-        out["traffic_over_own_bytes"] = traffic / (kb["step"] * N)
+        out["traffic_over_own_bytes"] = traffic / (kb["step"] * NM)
     return out
 
 
@@ -665,10 +667,10 @@ def main():
     ap.add_argument("--noise", choices=["philox", "injected", "none"], default="philox",
                     help="thermal noise of the main run: in-kernel Philox at 300 K, an injected "
                          "N(0,1) array at 300 K, or none (T = 0)")
-    ap.add_argument("--variants", default="T0,injected,L128",
+    ap.add_argument("--variants", default="T0,injected,L128,M8",
                     help="comma list of runs of the plain workload beside the headline: T0 "
                          "(no noise), injected (caller's N(0,1) stream), L128 (bcc 128^3 x 2, "
-                         "above the MALL), or 'none'")
+                         "above the MALL), M8 (eight ensembles of bcc 32^3 x 2), or 'none'")
     ap.add_argument("--c4-sweeps", type=int, default=100,
                     help="timed sweeps of each batched MC run of the c4 form")
     ap.add_argument("--c4-stat-sweeps", type=int, default=600,
@@ -722,7 +724,7 @@ def main():
         eng.autotune_llg(args.solver, steps=3)
     tune_s = time.perf_counter() - t_tune
     gather, block = eng.tuning(eng.TUNE_LLG)
-    nt, idx32, tile = (block >> 16) & 1, (block >> 17) & 1, (block >> 18) & 0xF
+    nt, idx32, tile = (block >> 16) & 1, (block >> 17) & 1, eng.llg_tile()
     pipe = (block >> 22) & 1
     block &= 0xFFFF
 
@@ -825,6 +827,8 @@ def main():
         elif v == "L128":    # above the 256 MB MALL
             res = c2_variant(U, args, "fp64_midpoint_bcc128_58nn_philox", L=128, steps=200,
                              warmup=20)
+        elif v == "M8":      # the same spin count as eight ensembles of bcc 32^3 x 2
+            res = c2_variant(U, args, "fp64_midpoint_bcc32_58nn_M8_philox", L=32, M=8)
         else:
             raise SystemExit(f"unknown variant {v}")
         res["wall_s"] = round(time.perf_counter() - t0, 1)

@@ -122,7 +122,7 @@ def test_bad_shape_rejected():
     with pytest.raises(U.UasdError):
         eng.set_tuning(eng.TUNE_LLG, 4, 256 | PAIR)   # MC only
     with pytest.raises(U.UasdError):
-        eng.set_tuning(eng.TUNE_MC, 4, 256 | (PAIR << 1))
+        eng.set_tuning(eng.TUNE_MC, 4, 256 | (PAIR << 1))   # no such flag
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])

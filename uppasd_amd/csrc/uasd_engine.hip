@@ -224,6 +224,9 @@ struct uasd_ctx {
   // colouring
   int ncolors = 0;
   std::vector<int> color_of_site, color_ptr, color_sites;
+  // per colour: its slots are color_start + i * color_stride (stride 0: not
+  // an arithmetic progression, the kernels read color_sites)
+  std::vector<int> color_start, color_stride;
   bool coloring_dirty = true;
 
   // Per-kernel timing: an event pair is recorded around each launch without
@@ -750,6 +753,18 @@ int build_coloring(uasd_ctx *c) {
   c->color_sites.assign(N, 0);
   std::vector<int> fillp(c->color_ptr.begin(), c->color_ptr.end() - 1);
   for (int i = 0; i < N; ++i) c->color_sites[fillp[col[i]]++] = i;
+  c->color_start.assign(ncol, 0);
+  c->color_stride.assign(ncol, 0);
+  for (int k = 0; k < ncol; ++k) {
+    const int p0 = c->color_ptr[k], n = c->color_ptr[k + 1] - p0;
+    if (n == 0) continue;
+    const int a0 = c->color_sites[p0], d = n > 1 ? c->color_sites[p0 + 1] - a0 : 1;
+    bool ap = d > 0;
+    for (int i = 2; ap && i < n; ++i)
+      ap = c->color_sites[p0 + i] - c->color_sites[p0 + i - 1] == d;
+    c->color_start[k] = a0;
+    c->color_stride[k] = ap ? d : 0;
+  }
   c->coloring_dirty = false;
   c->gen += 1;
   return UASD_OK;
@@ -1665,6 +1680,8 @@ static int mc_run(uasd_ctx *c, uasd::MCParams mp, int64_t nsweeps) {
     for (int col = 0; col < c->ncolors; ++col) {
       mp.sites = sites_dev + c->color_ptr[col];
       mp.nsites = c->color_ptr[col + 1] - c->color_ptr[col];
+      mp.site0 = c->color_start[col];
+      mp.sstride = c->color_stride[col];
       tic(c, 3);
       if (c->prec == UASD_FP64) {
         HIP_TRY(uasd::launch_mc<double>(c->stream, c->tune[2], ham_view<double>(c), mp,
@@ -1707,6 +1724,8 @@ int mc_sweeps_run(uasd_ctx *c, char mode, const uasd::MCParams &mp, int64_t nswe
         for (int col = 0; col < c->ncolors; ++col) {
           g.sites = mp.sites + c->color_ptr[col];
           g.nsites = c->color_ptr[col + 1] - c->color_ptr[col];
+          g.site0 = c->color_start[col];
+          g.sstride = c->color_stride[col];
           if (c->prec == UASD_FP64) {
             HIP_TRY(uasd::launch_mc<double>(c->stream, c->tune[2], ham_view<double>(c), g,
                                             quads<double>(c, c->spins, 0)));
@@ -1869,7 +1888,8 @@ void tune_store(const std::string &key, const uasd::Tune &t, double ms) {
 }
 
 template <typename Run>
-int tune_search(uasd_ctx *c, uasd::Tune *slot, Run &&run, double *best_ms) {
+// pair: also time two ensembles per thread (the Metropolis colour pass, M even)
+int tune_search(uasd_ctx *c, uasd::Tune *slot, Run &&run, double *best_ms, bool pair = false) {
   EventPair ev;
   HIP_TRY(hipEventCreate(&ev.a));
   HIP_TRY(hipEventCreate(&ev.b));
@@ -1884,7 +1904,7 @@ int tune_search(uasd_ctx *c, uasd::Tune *slot, Run &&run, double *best_ms) {
       for (int g : gathers)
         for (int bl : blocks) {
           cand.push_back(uasd::Tune{g, bl, nt, ix, 0});
-          if (slot == &c->tune[2] && c->M % 2 == 0) cand.push_back(uasd::Tune{g, bl, nt, ix, 0, 0, 1});
+          if (pair && c->M % 2 == 0) cand.push_back(uasd::Tune{g, bl, nt, ix, 0, 0, 1});
         }
   if (slot == &c->tune[0]) {
     build_stencil(c);
@@ -1975,7 +1995,8 @@ int uasd_autotune_mc(uasd_ctx *c, char mode, const double *temps, int sweeps, do
   StateGuard g(c);
   if (g.err) return g.err;
   double ms_found = 0.0;
-  int r = tune_search(c, &c->tune[2], [&]() -> int { return mc_run(c, mp, sweeps); }, &ms_found);
+  int r = tune_search(c, &c->tune[2], [&]() -> int { return mc_run(c, mp, sweeps); }, &ms_found,
+                      mode == 'M');
   if (r) return r;
   if (best_ms) *best_ms = ms_found;
   tune_store(key, c->tune[2], ms_found);

@@ -1695,6 +1695,7 @@ __device__ __forceinline__ void heat_bath(double beta, R m, R f0, R f1, R f2, do
 struct MCParams {
   const int *sites;     // sites of the colour being updated
   int nsites;
+  int site0, sstride;   // sstride > 0: the colour's i-th site is site0 + i*sstride
   const double *delta;  // (M) Gaussian step width per ensemble
   const double *beta;   // (M) Metropolis 1/(k_B (T*temprescale + 1e-15))
   const double *hbeta;  // (M) heat-bath 1/k_B/(temprescale*T)
@@ -1769,7 +1770,7 @@ __global__ void __launch_bounds__(256) k_mc_colour(HamDev<R> h, MCParams mp, Qua
   if (t >= total) return;
   const int pidx = (int)fast_div(t, U == 1 ? h.mdiv : mp.pdiv);
   const int e0 = ((int)t - pidx * (int)per) * U;
-  const int site = mp.sites[pidx];
+  const int site = mp.sstride ? mp.site0 + pidx * mp.sstride : mp.sites[pidx];
   const size_t q0 = (size_t)site * h.M + e0;
   const uint64_t sweep = mp.sweep_dev ? *mp.sweep_dev + mp.sweep : mp.sweep;
   const uint32_t hs = host_site(h, site);

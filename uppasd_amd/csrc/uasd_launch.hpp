@@ -65,7 +65,8 @@ template <typename R>
 hipError_t launch_mc(hipStream_t s, const Tune &t, const HamDev<R> &h, const MCParams &mp,
                      Quad<R> *spins);
 
-// Fused LLG half-step with the LDS-tiled stencil kernel (M = 1).
+// Fused LLG half-step with the LDS-tiled stencil kernel, work units (tile,
+// ensemble).
 template <typename R>
 hipError_t launch_llg_tile(hipStream_t s, TileDev t, int pipe, int alg, int half,
                            const HamDev<R> &h, const LLGDev<R> &prm, const Quad<R> *e,

@@ -387,8 +387,11 @@ __device__ __forceinline__ I nb_load(const I *p) {
 // rounding, as one at a time).  A last partial batch masks the slots past len.
 // U > 1 sums the fields of the U adjacent ensembles e0 .. e0+U-1 of the site
 // together: each neighbour entry and coupling is read once for all of them,
-// and each field is summed exactly as it is for U = 1.
-template <typename R, int G, bool NT, typename I, int U>
+// and each field is summed exactly as it is for U = 1.  JW: each term as
+// (J m_j) e_j instead of J (e_j m_j), one multiply fewer per neighbour and
+// equal to rounding (the MC energy differences, not the fields of the LLG
+// step, which keep the reference's emomM products).
+template <typename R, int G, bool NT, typename I, int U, bool JW = false>
 __device__ __forceinline__ void exchange_run(const HamDev<R> &h, const I *__restrict__ nb,
                                              const Quad<R> *__restrict__ src, int site, int e0,
                                              int row, int len, R (&f)[U][3]) {
@@ -413,9 +416,16 @@ __device__ __forceinline__ void exchange_run(const HamDev<R> &h, const I *__rest
     for (int u = 0; u < G; ++u) {
 #pragma unroll
       for (int k = 0; k < U; ++k) {
-        f[k][0] = fma(J[u], v[u][k].x * v[u][k].w, f[k][0]);
-        f[k][1] = fma(J[u], v[u][k].y * v[u][k].w, f[k][1]);
-        f[k][2] = fma(J[u], v[u][k].z * v[u][k].w, f[k][2]);
+        if (JW) {
+          const R jw = J[u] * v[u][k].w;
+          f[k][0] = fma(jw, v[u][k].x, f[k][0]);
+          f[k][1] = fma(jw, v[u][k].y, f[k][1]);
+          f[k][2] = fma(jw, v[u][k].z, f[k][2]);
+        } else {
+          f[k][0] = fma(J[u], v[u][k].x * v[u][k].w, f[k][0]);
+          f[k][1] = fma(J[u], v[u][k].y * v[u][k].w, f[k][1]);
+          f[k][2] = fma(J[u], v[u][k].z * v[u][k].w, f[k][2]);
+        }
       }
     }
   }
@@ -439,9 +449,16 @@ __device__ __forceinline__ void exchange_run(const HamDev<R> &h, const I *__rest
       if (j + u < len) {
 #pragma unroll
         for (int k = 0; k < U; ++k) {
-          f[k][0] = fma(J[u], v[u][k].x * v[u][k].w, f[k][0]);
-          f[k][1] = fma(J[u], v[u][k].y * v[u][k].w, f[k][1]);
-          f[k][2] = fma(J[u], v[u][k].z * v[u][k].w, f[k][2]);
+          if (JW) {
+            const R jw = J[u] * v[u][k].w;
+            f[k][0] = fma(jw, v[u][k].x, f[k][0]);
+            f[k][1] = fma(jw, v[u][k].y, f[k][1]);
+            f[k][2] = fma(jw, v[u][k].z, f[k][2]);
+          } else {
+            f[k][0] = fma(J[u], v[u][k].x * v[u][k].w, f[k][0]);
+            f[k][1] = fma(J[u], v[u][k].y * v[u][k].w, f[k][1]);
+            f[k][2] = fma(J[u], v[u][k].z * v[u][k].w, f[k][2]);
+          }
         }
       }
     }
@@ -450,14 +467,14 @@ __device__ __forceinline__ void exchange_run(const HamDev<R> &h, const I *__rest
 
 // When every row holds W neighbours the loop runs to the launch argument W,
 // the same for every lane: its control and the batch tail are scalar.
-template <typename R, int G, bool NT, typename I, int U>
+template <typename R, int G, bool NT, typename I, int U, bool JW = false>
 __device__ __forceinline__ void exchange_fields(const HamDev<R> &h, const I *__restrict__ nb,
                                                 const Quad<R> *__restrict__ src, int site, int e0,
                                                 int row, R (&f)[U][3]) {
   if (h.ex_full)
-    exchange_run<R, G, NT, I, U>(h, nb, src, site, e0, row, h.W, f);
+    exchange_run<R, G, NT, I, U, JW>(h, nb, src, site, e0, row, h.W, f);
   else
-    exchange_run<R, G, NT, I, U>(h, nb, src, site, e0, row, h.ex_len[row], f);
+    exchange_run<R, G, NT, I, U, JW>(h, nb, src, site, e0, row, h.ex_len[row], f);
 }
 
 template <typename R, int G, bool NT, typename I>
@@ -1543,18 +1560,24 @@ __device__ __forceinline__ void trial_move(R e0, R e1, R e2, double delta, doubl
                                           R &n1, R &n2) {
   const int ftype = (int)floor(3 * u0);
   if (ftype == 0) {
-    double phi = u1 * 2 * kPi;
-    double theta = acos(1 - 2 * u2);
-    double st = sin(theta);
-    n0 = (R)(st * cos(phi));
-    n1 = (R)(st * sin(phi));
-    n2 = (R)cos(theta);
+    // phi = 2 pi u1, theta = acos(1 - 2 u2) (montecarlo_common.f90:55-59):
+    // cos(theta) = z and sin(theta) = sqrt(1 - z^2) >= 0 on [0, pi], and
+    // sincospi(2 u1) is sin and cos of phi, all equal to the library forms to
+    // rounding without the acos, sin and cos of theta
+    const double z = 1 - 2 * u2;
+    const double st = sqrt(fmax(0.0, 1 - z * z));
+    double sp, cp;
+    sincospi(2 * u1, &sp, &cp);
+    n0 = (R)(st * cp);
+    n1 = (R)(st * sp);
+    n2 = (R)z;
   } else if (ftype == 1) {
     R a0 = e0 + (R)(g0 * delta), a1 = e1 + (R)(g1 * delta), a2 = e2 + (R)(g2 * delta);
-    R l = fsqrt(a0 * a0 + a1 * a1 + a2 * a2);
-    n0 = a0 / l;
-    n1 = a1 / l;
-    n2 = a2 / l;
+    // one reciprocal for the three divisions by the length (equal to rounding)
+    const R il = (R)1 / fsqrt(a0 * a0 + a1 * a1 + a2 * a2);
+    n0 = a0 * il;
+    n1 = a1 * il;
+    n2 = a2 * il;
   } else {
     // Spin flip, newmom = -emom.  Written as a subtraction: hipcc (clang 22,
     // -O1 and above) drops a bare negation of a loaded value when it merges
@@ -1591,16 +1614,17 @@ __device__ __forceinline__ void mc_delta_e(const HamDev<R> &h, const Quad<R> *__
   R f[U][3];
 #pragma unroll
   for (int k = 0; k < U; ++k) f[k][0] = f[k][1] = f[k][2] = (R)0;
+  const int row = sc.ex_row;
   if (h.nbr16) {
     if (h.nt)
-      exchange_fields<R, G, true, int16_t, U>(h, h.nbr16 + site, s, site, e0, sc.ex_row, f);
+      exchange_fields<R, G, true, int16_t, U, true>(h, h.nbr16 + site, s, site, e0, row, f);
     else
-      exchange_fields<R, G, false, int16_t, U>(h, h.nbr16 + site, s, site, e0, sc.ex_row, f);
+      exchange_fields<R, G, false, int16_t, U, true>(h, h.nbr16 + site, s, site, e0, row, f);
   } else {
     if (h.nt)
-      exchange_fields<R, G, true, int, U>(h, h.nbr + site, s, site, e0, sc.ex_row, f);
+      exchange_fields<R, G, true, int, U, true>(h, h.nbr + site, s, site, e0, row, f);
     else
-      exchange_fields<R, G, false, int, U>(h, h.nbr + site, s, site, e0, sc.ex_row, f);
+      exchange_fields<R, G, false, int, U, true>(h, h.nbr + site, s, site, e0, row, f);
   }
   R c0[U], c1[U], c2[U], t0[U], t1[U], t2[U], e_c[U], e_t[U];
 #pragma unroll

@@ -240,9 +240,8 @@ int uasd_llg_tile(uasd_ctx *ctx, int *tile);
  * separately even where every site's DM bonds are its exchange bonds (at
  * most 8), which the list kernels otherwise gather once for both sums;
  * plus 0x1000000 (kind 2 only) for two adjacent ensembles of a site per
- * Metropolis thread, sharing its neighbour entries and couplings (M even;
- * one per thread otherwise and in heat bath).  Results do not depend on any
- * of them. */
+ * thread, sharing its neighbour entries and couplings (M even; one per
+ * thread otherwise).  Results do not depend on any of them. */
 int uasd_set_tuning(uasd_ctx *ctx, int kind, int gather, int block);
 int uasd_get_tuning(uasd_ctx *ctx, int kind, int *gather, int *block);
 /* Autotuner: times every shape on the context's own lattice and state

@@ -80,8 +80,8 @@ def test_mc_pair_of_ensembles_per_thread_is_bit_identical(precision, M):
     """Two adjacent ensembles per thread (shared neighbour entries and
     couplings) against one per thread, on a Hamiltonian with every MC term
     (exchange, DM with |m| != 1, anisotropy types 1, 2 and 7, Zeeman):
-    Metropolis and heat bath give the same moments bit for bit.  With M odd,
-    and in heat bath, the flag leaves one ensemble per thread."""
+    Metropolis and heat bath give the same moments bit for bit.  With M odd
+    the flag leaves one ensemble per thread."""
     rng = np.random.default_rng(64)
     sysd = CF.triangular_dmi(12, J=1.0, D=0.2, K=0.0, B=0.0, moment=2.2)
     N = sysd["lattice"].natom

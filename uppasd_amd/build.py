@@ -9,6 +9,7 @@ plain C++ with OpenMP, built by g++.
 """
 from __future__ import annotations
 
+import hashlib
 import os
 import subprocess
 import sys
@@ -43,6 +44,10 @@ def _run(cmd, verbose):
 def build(verbose: bool = False) -> str:
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
+    # hipcc reads the sources twice per unit (device, then host pass): a
+    # source edited while it runs would leave host launch code for kernels
+    # the device code does not have, so the build refuses to finish then
+    before = _source_digest()
     # the translation units are independent: compile them side by side
     cmds, objs = [], []
     for src, extra in UNITS:
@@ -59,8 +64,24 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run(["g++", "-shared", "-o", tmp, *objs, f"-L{ROCM}/lib", "-lamdhip64", "-fopenmp",
           f"-Wl,-rpath,{ROCM}/lib"], verbose)
+    if _source_digest() != before:
+        os.remove(tmp)
+        raise RuntimeError("sources changed while the library was being built; build again")
     os.replace(tmp, LIB)
     return LIB
+
+
+def _source_digest() -> str:
+    """SHA-256 over the sources the library is built from."""
+    h = hashlib.sha256()
+    files = sorted(os.listdir(CSRC)) + [os.path.join("..", "..", "include", "uasd_hip.h")]
+    for name in files:
+        path = os.path.normpath(os.path.join(CSRC, name))
+        if os.path.isfile(path):
+            h.update(name.encode())
+            with open(path, "rb") as f:
+                h.update(f.read())
+    return h.hexdigest()
 
 
 FLANG = os.path.join(ROCM, "bin", "amdflang")

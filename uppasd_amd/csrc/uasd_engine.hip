@@ -1888,7 +1888,7 @@ void tune_store(const std::string &key, const uasd::Tune &t, double ms) {
 }
 
 template <typename Run>
-// pair: also time two ensembles per thread (the Metropolis colour pass, M even)
+// pair: also time two ensembles per thread (the MC colour pass, M even)
 int tune_search(uasd_ctx *c, uasd::Tune *slot, Run &&run, double *best_ms, bool pair = false) {
   EventPair ev;
   HIP_TRY(hipEventCreate(&ev.a));
@@ -1996,7 +1996,7 @@ int uasd_autotune_mc(uasd_ctx *c, char mode, const double *temps, int sweeps, do
   if (g.err) return g.err;
   double ms_found = 0.0;
   int r = tune_search(c, &c->tune[2], [&]() -> int { return mc_run(c, mp, sweeps); }, &ms_found,
-                      mode == 'M');
+                      true);
   if (r) return r;
   if (best_ms) *best_ms = ms_found;
   tune_store(key, c->tune[2], ms_found);

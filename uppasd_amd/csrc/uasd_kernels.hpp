@@ -661,6 +661,63 @@ __device__ __forceinline__ void site_field(const HamDev<R> &h, const Quad<R> *__
   }
 }
 
+// The pair, DM and anisotropy parts of site_field for the U adjacent
+// ensembles e0 .. e0+U-1 of one site: the exchange terms of all U from one
+// read of the neighbour entries and couplings, then each ensemble's DM and
+// anisotropy terms.  The terms are added in site_field's order (the fused
+// exchange/DM gather adds them in the order of the separate sums), so each
+// field equals site_field's bit for bit.
+template <typename R, int G, int U>
+__device__ __forceinline__ void site_field_u(const HamDev<R> &h, const Quad<R> *__restrict__ src,
+                                             int site, int e0, R (&sf)[U][3], R (&qf)[U][3]) {
+  const SiteClass sc = h.classes[h.cls[site]];
+  const int M = h.M, row = sc.ex_row;
+#pragma unroll
+  for (int k = 0; k < U; ++k) sf[k][0] = sf[k][1] = sf[k][2] = qf[k][0] = qf[k][1] = qf[k][2] = (R)0;
+  if (h.nbr16) {
+    if (h.nt)
+      exchange_fields<R, G, true, int16_t, U>(h, h.nbr16 + site, src, site, e0, row, sf);
+    else
+      exchange_fields<R, G, false, int16_t, U>(h, h.nbr16 + site, src, site, e0, row, sf);
+  } else {
+    if (h.nt)
+      exchange_fields<R, G, true, int, U>(h, h.nbr + site, src, site, e0, row, sf);
+    else
+      exchange_fields<R, G, false, int, U>(h, h.nbr + site, src, site, e0, row, sf);
+  }
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    if (h.Wd > 0 && sc.dm_row >= 0) {
+      if (h.dnbr16)
+        dm_sum<R, G>(h, h.dnbr16 + site, src, site, e0 + k, sc.dm_row, sf[k][0], sf[k][1],
+                     sf[k][2]);
+      else
+        dm_sum<R, G>(h, h.dnbr + site, src, site, e0 + k, sc.dm_row, sf[k][0], sf[k][1],
+                     sf[k][2]);
+    }
+    if (sc.an_row >= 0) {
+      const int t = h.an_type[sc.an_row];
+      if (t == 1 || t == 2 || t == 7) {
+        R m0, m1, m2;
+        moment(src, (size_t)site * M + e0 + k, m0, m1, m2);
+        if (t == 1) {
+          uniaxial(h, sc.an_row, m0, m1, m2, sf[k][0], sf[k][1], sf[k][2]);
+        } else if (t == 2) {
+          cubic(h, sc.an_row, m0, m1, m2, sf[k][0], sf[k][1], sf[k][2]);
+        } else {
+          R t0 = 0, t1 = 0, t2 = 0;
+          uniaxial(h, sc.an_row, m0, m1, m2, sf[k][0], sf[k][1], sf[k][2]);
+          cubic(h, sc.an_row, m0, m1, m2, t0, t1, t2);
+          const R sb = h.an_sb[sc.an_row];
+          qf[k][0] = qf[k][0] + t0 * sb;
+          qf[k][1] = qf[k][1] + t1 * sb;
+          qf[k][2] = qf[k][2] + t2 * sb;
+        }
+      }
+    }
+  }
+}
+
 // beff = 0 + beff1 + beff2, beff2 = beff_q + ext + 0  (hamiltonianactions.f90:240-243)
 template <typename R, int G>
 __device__ __forceinline__ void total_field(const HamDev<R> &h, const Quad<R> *__restrict__ src,
@@ -1703,14 +1760,21 @@ __device__ __forceinline__ void heat_bath(double beta, R m, R f0, R f1, R f2, do
   double z1 = beta * (double)f1 * kMub * (double)m;
   double z2 = beta * (double)f2 * kMub * (double)m;
   double zarg = sqrt(z0 * z0 + z1 * z1 + z2 * z2);
-  double zct = z2 / zarg;
+  // the reference's four divisions by zarg and zarg*zst as two reciprocals,
+  // its exp(-2 zarg) once, and cos / sin of phi = pi (2 flipprob - 1) by
+  // sincospi: equal to the reference's expressions to rounding
+  const double rz = 1.0 / zarg;
+  double zct = z2 * rz;
   double zst = sqrt(1.0 - zct * zct) + kDblTolerance;
-  double zcp = z0 / (zarg * zst);
-  double zsp = z1 / (zarg * zst);
-  double ct = 1.0 + (1.0 / zarg) * log((1.0 - exp(-2.0 * zarg)) * qr + exp(-2.0 * zarg) + kDblTolerance);
+  const double rzs = rz / zst;
+  double zcp = z0 * rzs;
+  double zsp = z1 * rzs;
+  const double ex = exp(-2.0 * zarg);
+  double ct = 1.0 + rz * log((1.0 - ex) * qr + ex + kDblTolerance);
   double stv = sqrt(1.0 - ct * ct);
-  double phi = kPi * (2.0 * flipprob - 1.0);
-  double s0 = stv * cos(phi), s1 = stv * sin(phi), s2 = ct;
+  double sp, cp;
+  sincospi(2.0 * flipprob - 1.0, &sp, &cp);
+  double s0 = stv * cp, s1 = stv * sp, s2 = ct;
   o0 = (R)(zcp * zct * s0 - zsp * s1 + zcp * zst * s2);
   o1 = (R)(zsp * zct * s0 + zcp * s1 + zsp * zst * s2);
   o2 = (R)(-zst * s0 + zct * s2);
@@ -1781,10 +1845,11 @@ __device__ __forceinline__ void trial_f32(const MCParams &mp, uint64_t sweep, ui
 
 // One colour of a sweep.  HB selects heat bath (mp.heat_bath) or Metropolis
 // at compile time, so each mode's registers are allocated for its own code
-// alone.  A Metropolis thread updates the U adjacent ensembles e0 .. e0+U-1
-// of one site of the colour (U = 1, or 2 when M is even): the trial moves and
-// acceptances are those of U = 1, drawn from the same counters, and the
-// energy changes share the site's neighbour entries and couplings.
+// alone.  A thread updates the U adjacent ensembles e0 .. e0+U-1 of one site
+// of the colour (U = 1, or 2 when M is even): the trial moves, acceptances
+// and heat-bath draws are those of U = 1, from the same counters, and the
+// energy changes or local fields share the site's neighbour entries and
+// couplings.
 template <typename R, int G, int U, bool HB>
 __global__ void __launch_bounds__(256) k_mc_colour(HamDev<R> h, MCParams mp, Quad<R> *s) {
   // (site of the colour, ensemble) pairs number fewer than 2^31
@@ -1808,18 +1873,26 @@ __global__ void __launch_bounds__(256) k_mc_colour(HamDev<R> h, MCParams mp, Qua
     x[k][2] = (R)mp.ext[3 * (e0 + k) + 2];
   }
   if constexpr (HB) {
-    // heat bath: one ensemble per thread (the launch never pairs them)
-    static_assert(U == 1, "heat-bath launches take one ensemble per thread");
-    double u0, u1, u2, u3;
-    uniform4(mp.seed, kTagMCU, sweep, hs, (uint32_t)e0, u0, u1, u2, u3);
-    R s0, s1, s2, q0f, q1f, q2f, y0, y1, y2;
-    site_field<R, G>(h, (const Quad<R> *)s, site, e0, s0, s1, s2, q0f, q1f, q2f, y0, y1, y2);
-    const R f0 = (R)0 + s0 + (q0f + x[0][0] + (R)0);
-    const R f1 = (R)0 + s1 + (q1f + x[0][1] + (R)0);
-    const R f2 = (R)0 + s2 + (q2f + x[0][2] + (R)0);
-    R o0, o1, o2;
-    heat_bath<R>(mp.hbeta[e0], me[0].w, f0, f1, f2, u1, u0, o0, o1, o2);
-    s[q0] = Quad<R>{o0, o1, o2, me[0].w};
+    // the local field (effective_field_single) of each of the U ensembles
+    R sf[U][3], qf[U][3];
+    if (U == 1) {
+      R y0, y1, y2;
+      site_field<R, G>(h, (const Quad<R> *)s, site, e0, sf[0][0], sf[0][1], sf[0][2], qf[0][0],
+                       qf[0][1], qf[0][2], y0, y1, y2);
+    } else {
+      site_field_u<R, G, U>(h, (const Quad<R> *)s, site, e0, sf, qf);
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      double u0, u1, u2, u3;
+      uniform4(mp.seed, kTagMCU, sweep, hs, (uint32_t)(e0 + k), u0, u1, u2, u3);
+      const R f0 = (R)0 + sf[k][0] + (qf[k][0] + x[k][0] + (R)0);
+      const R f1 = (R)0 + sf[k][1] + (qf[k][1] + x[k][1] + (R)0);
+      const R f2 = (R)0 + sf[k][2] + (qf[k][2] + x[k][2] + (R)0);
+      R o0, o1, o2;
+      heat_bath<R>(mp.hbeta[e0 + k], me[k].w, f0, f1, f2, u1, u0, o0, o1, o2);
+      s[q0 + k] = Quad<R>{o0, o1, o2, me[k].w};
+    }
   } else {
     R tr[U][3];
     double acc[U];

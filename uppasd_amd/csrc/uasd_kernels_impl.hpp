@@ -100,15 +100,18 @@ hipError_t launch_mc(hipStream_t s, const Tune &t, const HamDev<R> &h, const MCP
   if (n == 0) return hipSuccess;
   if (!tune_valid(t)) return hipErrorInvalidValue;
   const HamDev<R> hh = tuned(h, t);
-  // two adjacent ensembles per Metropolis thread where M is even
-  const bool two = t.pair && h.M % 2 == 0 && !mp.heat_bath;
+  // two adjacent ensembles per thread where M is even
+  const bool two = t.pair && h.M % 2 == 0;
   MCParams m = mp;
   m.pdiv = make_fast_div((uint32_t)(two ? h.M / 2 : h.M));
   return with_gather(t, [&](auto g) {
     constexpr int G = decltype(g)::value;
     const dim3 grid(grid_for(two ? n / 2 : n, t.block)), block(t.block);
     if (mp.heat_bath) {
-      hipLaunchKernelGGL((k_mc_colour<R, G, 1, true>), grid, block, 0, s, hh, m, spins);
+      if (two)
+        hipLaunchKernelGGL((k_mc_colour<R, G, 2, true>), grid, block, 0, s, hh, m, spins);
+      else
+        hipLaunchKernelGGL((k_mc_colour<R, G, 1, true>), grid, block, 0, s, hh, m, spins);
     } else {
       if (two)
         hipLaunchKernelGGL((k_mc_colour<R, G, 2, false>), grid, block, 0, s, hh, m, spins);

@@ -869,6 +869,8 @@ bool tile_fits(const uasd_ctx *c, int k) {
   if (sh[0] * sh[1] * sh[2] * c->lat_na > 1024 || (sh[0] * sh[1] * sh[2]) % 64 != 0) return false;
   const int plane = uasd::tile_plane(hv * c->lat_na);
   if (plane == 0) return false;
+  // halo coordinates in FP32 reciprocal divisions: dividends below 2^22
+  if (c->lat_n1 >= (1 << 20) || c->lat_n2 >= (1 << 20) || c->lat_n3 >= (1 << 20)) return false;
   // tile index arithmetic in FP32 reciprocals: fewer than 2^22 tiles
   if ((int64_t)((c->lat_n1 + sh[0] - 1) / sh[0]) * ((c->lat_n2 + sh[1] - 1) / sh[1]) *
           ((c->lat_n3 + sh[2] - 1) / sh[2]) >= (1 << 22))
@@ -906,6 +908,14 @@ int make_tile(uasd_ctx *c, int k, uasd::TileDev &t) {
   t.ntz = (t.N3 + t.TZ - 1) / t.TZ;
   t.rntx = 1.0f / (float)t.ntx;
   t.rntxy = 1.0f / (float)(t.ntx * t.nty);
+  t.rHY = 1.0f / (float)t.HY;
+  t.rHYZ = 1.0f / (float)(t.HY * t.HZ);
+  t.rN1 = 1.0f / (float)t.N1;
+  t.rN2 = 1.0f / (float)t.N2;
+  t.rN3 = 1.0f / (float)t.N3;
+  t.ox = t.N1 * ((t.RX + t.N1 - 1) / t.N1);
+  t.oy = t.N2 * ((t.RY + t.N2 - 1) / t.N2);
+  t.oz = t.N3 * ((t.RZ + t.N3 - 1) / t.N3);
   t.W = c->st_ex.W;
   t.Wd = c->Wd > 0 ? c->st_dm.W : 0;
   if (c->tile_off_shape != k) {

@@ -1234,6 +1234,11 @@ struct TileDev {
   // 1/d in FP32 for the small divisors d of the index arithmetic:
   // fdiv(n, 1/d) = n/d exactly for 0 <= n < 2^22
   float rTX, rTXY, rTC, rHX, rntx, rntxy;
+  // the same for the halo table: 1/HY, 1/(HY*HZ), 1/N1, 1/N2, 1/N3, and the
+  // multiples of N1, N2, N3 at least RX, RY, RZ that keep wrapped
+  // coordinates non-negative
+  float rHY, rHYZ, rN1, rN2, rN3;
+  int ox, oy, oz;
   int rs;                 // halo rows staged per round: blockDim.x / HX
   int per;                // tiles per workgroup (consecutive in the XCD-aware order)
   int ntx, nty, ntz;      // tiles per direction
@@ -1448,17 +1453,21 @@ template <typename R>
 __device__ __forceinline__ void halo_table(const HamDev<R> &h, const TileDev &t, int X0, int Y0,
                                            int Z0, int ncell, int M, int *tab) {
   const int nr = t.NA * t.HY * t.HZ, hyz = t.HY * t.HZ;
+  // wrapped coordinates by the reciprocal divisions (exact: every dividend
+  // is non-negative and below 2^22)
   for (int i = threadIdx.x; i < t.HX + nr; i += blockDim.x) {
     int v;
     if (i < t.HX) {
-      const int x = ((X0 + i - t.RX) % t.N1 + t.N1) % t.N1;
+      const int n = X0 + i - t.RX + t.ox;
+      const int x = n - t.N1 * fdiv(n, t.rN1);
       v = h.order_na == 1 ? t.NA * x : x;
     } else {
       const int r = i - t.HX;
-      const int hb = r / hyz, rz = r - hb * hyz;
-      const int hz = rz / t.HY, hy = rz - hz * t.HY;
-      const int y = ((Y0 + hy - t.RY) % t.N2 + t.N2) % t.N2;
-      const int z = ((Z0 + hz - t.RZ) % t.N3 + t.N3) % t.N3;
+      const int hb = fdiv(r, t.rHYZ), rz = r - hb * hyz;
+      const int hz = fdiv(rz, t.rHY), hy = rz - hz * t.HY;
+      const int ny = Y0 + hy - t.RY + t.oy, nz = Z0 + hz - t.RZ + t.oz;
+      const int y = ny - t.N2 * fdiv(ny, t.rN2);
+      const int z = nz - t.N3 * fdiv(nz, t.rN3);
       const int cyz = t.N1 * (y + t.N2 * z);
       v = h.order_na == 1 ? hb + t.NA * cyz : hb * ncell + cyz;
     }

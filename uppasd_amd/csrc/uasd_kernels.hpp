@@ -375,6 +375,21 @@ struct NbDec {
   }
 };
 
+// NbDec for the one-block site order (order_na = 1, so kb = 0): the entry is
+// the wrapped difference of the slots, decoded without the basis shift.
+struct NbDec1 {
+  int c, nc;
+  template <typename R>
+  __device__ __forceinline__ NbDec1(const HamDev<R> &h, int d) : c(d), nc(h.order_nc) {}
+  __device__ __forceinline__ int operator()(int v) const { return v; }
+  __device__ __forceinline__ int operator()(int16_t v) const {
+    int c2 = c + (int)v;
+    c2 += c2 < 0 ? nc : 0;
+    c2 -= c2 >= nc ? nc : 0;
+    return c2;
+  }
+};
+
 template <bool NT, typename I>
 __device__ __forceinline__ I nb_load(const I *p) {
   if constexpr (NT) return __builtin_nontemporal_load(p);
@@ -391,12 +406,12 @@ __device__ __forceinline__ I nb_load(const I *p) {
 // (J m_j) e_j instead of J (e_j m_j), one multiply fewer per neighbour and
 // equal to rounding (the MC energy differences, not the fields of the LLG
 // step, which keep the reference's emomM products).
-template <typename R, int G, bool NT, typename I, int U, bool JW = false>
+template <typename R, int G, bool NT, typename I, int U, bool JW = false, typename D = NbDec>
 __device__ __forceinline__ void exchange_run(const HamDev<R> &h, const I *__restrict__ nb,
                                              const Quad<R> *__restrict__ src, int site, int e0,
                                              int row, int len, R (&f)[U][3]) {
   const int M = h.M, N = h.N;
-  const NbDec dec(h, site);
+  const D dec(h, site);
   const R *cp = h.cpl + row;
   const size_t rs = (size_t)h.ex_rows;
   int j = 0;
@@ -475,6 +490,23 @@ __device__ __forceinline__ void exchange_fields(const HamDev<R> &h, const I *__r
     exchange_run<R, G, NT, I, U, JW>(h, nb, src, site, e0, row, h.W, f);
   else
     exchange_run<R, G, NT, I, U, JW>(h, nb, src, site, e0, row, h.ex_len[row], f);
+}
+
+// The MC energy gather: as exchange_fields with JW, and int16 entries of the
+// one-block site order decoded by NbDec1 (the choice made once, outside the
+// neighbour loop).
+template <typename R, int G, bool NT, typename I, int U>
+__device__ __forceinline__ void mc_exchange_fields(const HamDev<R> &h, const I *__restrict__ nb,
+                                                   const Quad<R> *__restrict__ src, int site,
+                                                   int e0, int row, R (&f)[U][3]) {
+  if (std::is_same<I, int16_t>::value && h.order_na == 1) {
+    if (h.ex_full)
+      exchange_run<R, G, NT, I, U, true, NbDec1>(h, nb, src, site, e0, row, h.W, f);
+    else
+      exchange_run<R, G, NT, I, U, true, NbDec1>(h, nb, src, site, e0, row, h.ex_len[row], f);
+  } else {
+    exchange_fields<R, G, NT, I, U, true>(h, nb, src, site, e0, row, f);
+  }
 }
 
 template <typename R, int G, bool NT, typename I>
@@ -1683,14 +1715,14 @@ __device__ __forceinline__ void mc_delta_e(const HamDev<R> &h, const Quad<R> *__
   const int row = sc.ex_row;
   if (h.nbr16) {
     if (h.nt)
-      exchange_fields<R, G, true, int16_t, U, true>(h, h.nbr16 + site, s, site, e0, row, f);
+      mc_exchange_fields<R, G, true, int16_t, U>(h, h.nbr16 + site, s, site, e0, row, f);
     else
-      exchange_fields<R, G, false, int16_t, U, true>(h, h.nbr16 + site, s, site, e0, row, f);
+      mc_exchange_fields<R, G, false, int16_t, U>(h, h.nbr16 + site, s, site, e0, row, f);
   } else {
     if (h.nt)
-      exchange_fields<R, G, true, int, U, true>(h, h.nbr + site, s, site, e0, row, f);
+      mc_exchange_fields<R, G, true, int, U>(h, h.nbr + site, s, site, e0, row, f);
     else
-      exchange_fields<R, G, false, int, U, true>(h, h.nbr + site, s, site, e0, row, f);
+      mc_exchange_fields<R, G, false, int, U>(h, h.nbr + site, s, site, e0, row, f);
   }
   R c0[U], c1[U], c2[U], t0[U], t1[U], t2[U], e_c[U], e_t[U];
 #pragma unroll

@@ -110,6 +110,26 @@ def test_resident_workgroups_loop_over_tiles(alg):
         assert np.array_equal(got, ref), k
 
 
+@pytest.mark.parametrize("alg", [1, 5])
+def test_resident_loop_over_units_of_several_ensembles(alg):
+    """Three ensembles of bcc 40^3 x 2 (250 tiles of (8,8,4) cells or 300
+    of (16,4,4), partial at the upper x edge; 750 or 900 (tile, ensemble)
+    units against 512 resident workgroups, two per CU as the FP64 tiles'
+    LDS allows): runs of two consecutive units, so a workgroup moves from
+    one ensemble to the next of a tile, or from a tile's last ensemble to
+    the next tile's first, prefetching that unit's halo; the moments equal
+    the list kernels' bit for bit."""
+    sysd = CF.bcc_shells(40, CF.bcc5_couplings())
+    lat = sysd["lattice"]
+    rng = np.random.default_rng(96)
+    e0 = np.ascontiguousarray(CF.random_unit(lat.natom, rng, 3))
+    ref, _ = _run(sysd, e0, lat.NA, 0, alg, steps=3)
+    for k in (1, 4):
+        got, e = _run(sysd, e0, lat.NA, k, alg, steps=3, pipe=1)
+        assert e.llg_tile() == k
+        assert np.array_equal(got, ref), k
+
+
 @pytest.mark.parametrize("alg", [1, 4, 5, 6, 7])
 def test_tile_fp32_equals_list_fp32(alg):
     sysd = CF.triangular_dmi(24)

@@ -12,8 +12,9 @@ resident in HBM.
 
 Prints one JSON line (see the keys below).  The roofline object is for the
 dominant kernel (the corrector), timed with HIP events on the stream it runs
-on; the cpu_baseline object times the FP64 C oracle (one thread) on a bounded
-sample of the same workload.
+on; the cpu_baseline object times the reference's own routines, compiled from
+its sources, on every host core over a bounded sample of the same workload
+(the FP64 C oracle stands in where that library is not built).
 """
 from __future__ import annotations
 
@@ -61,7 +62,7 @@ def kernel_bytes(W: int, s: int, ib: int = 4, cls: int = 4, noise: str = "philox
     return {"predictor": pred, "corrector": corr, "step": pred + corr}
 
 
-def survey_bytes(W: int, s: int, injected: bool = False) -> dict:
+def model_bytes(W: int, s: int, injected: bool = False) -> dict:
     """Algorithmic bytes per spin-update of the hot path's two-pass LLG step,
     as the byte model of the workload counts them: pass 1 gathers the moments
     and writes the predicted moments and field (9 s + 4 W), pass 2 gathers the
@@ -596,7 +597,7 @@ def c2_variant(U, args, label: str, sysd=None, L: int = 64, temp: float = 300.0,
     s = 8 if args.precision == "fp64" else 4
     ib = 0 if tile else eng.index_bytes()[0]
     kb = kernel_bytes(W, s, ib, cls=0 if tile else 4, noise=noise, b2=args.solver in (4, 5))
-    sb = survey_bytes(W, s, injected=noise == "injected")
+    sb = model_bytes(W, s, injected=noise == "injected")
     del eng
     variant = f"tile{tile}{'r' if pipe else ''}" if tile else f"idx{ib}"
     tr = profiled_traffic(label, variant)
@@ -738,7 +739,7 @@ def main():
     ib = 0 if tile else eng.index_bytes()[0]     # the tile kernel reads no lists
     kb = kernel_bytes(W, s, ib, cls=0 if tile else 4,   # one species: classes per basis
                       noise=args.noise, b2=args.solver in (4, 5))
-    sb = survey_bytes(W, s, injected=args.noise == "injected")
+    sb = model_bytes(W, s, injected=args.noise == "injected")
     # the dominant kernel is the corrector: the workload's algorithmic bytes
     # per update of its pass over every site, per its duration
     achieved = sb["corrector"] * N / t_corr / 1e9

@@ -3,7 +3,7 @@ tests/golden/ref_cases/ (data only: input files and expected numbers).
 
 Run in the build container, where the reference is mounted:
     python tests/golden/make_fixtures.py /root/reference
-The GPU box never reads the reference; it uses the committed copies.
+Runs on the GPU never read the reference; they use the committed copies.
 """
 import json
 import os

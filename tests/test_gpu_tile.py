@@ -68,7 +68,7 @@ def test_tile_equals_list_kernels(case, alg):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("M", [2, 3])
-@pytest.mark.parametrize("alg", [1, 5])
+@pytest.mark.parametrize("alg", [1, 4, 5, 6, 7])
 def test_tile_several_ensembles_equals_list_kernels(alg, M, precision):
     """M ensembles of one lattice: the tile kernel runs (tile, ensemble)
     units, staging each ensemble's halo from the interleaved quads, with the
@@ -110,7 +110,7 @@ def test_resident_workgroups_loop_over_tiles(alg):
         assert np.array_equal(got, ref), k
 
 
-@pytest.mark.parametrize("alg", [1, 5])
+@pytest.mark.parametrize("alg", [1, 4, 5, 6, 7])
 def test_resident_loop_over_units_of_several_ensembles(alg):
     """Three ensembles of bcc 40^3 x 2 (250 tiles of (8,8,4) cells or 300
     of (16,4,4), partial at the upper x edge; 750 or 900 (tile, ensemble)

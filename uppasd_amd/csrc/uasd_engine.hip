@@ -48,7 +48,7 @@ struct DevBuf {
   DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);  // teardown: no caller to report to
     p = nullptr;
     bytes = 0;
   }
@@ -535,7 +535,11 @@ void tic(uasd_ctx *c, int kind) {
   if (!c->timing || c->capturing) return;
   if (c->timing_tick[kind]++ % c->timing_every != 0) return;
   c->cur = take_event(c);
-  if (c->cur) hipEventRecord(c->cur, c->stream);
+  // a start event that could not be recorded times nothing: back to the pool
+  if (c->cur && hipEventRecord(c->cur, c->stream) != hipSuccess) {
+    c->ev_pool.push_back(c->cur);
+    c->cur = nullptr;
+  }
 }
 
 int toc(uasd_ctx *c, int kind) {
@@ -553,7 +557,12 @@ int region_tic(uasd_ctx *c) {
   if (!c->timing_region || c->capturing) return UASD_OK;
   c->region_cur = take_event(c);
   if (!c->region_cur) return fail(UASD_ERR_HIP, "hipEventCreate failed");
-  HIP_TRY(hipEventRecord(c->region_cur, c->stream));
+  const hipError_t err = hipEventRecord(c->region_cur, c->stream);
+  if (err != hipSuccess) {  // no region opened: the event goes back to the pool
+    c->ev_pool.push_back(c->region_cur);
+    c->region_cur = nullptr;
+  }
+  HIP_TRY(err);
   return UASD_OK;
 }
 
@@ -1053,13 +1062,22 @@ int uasd_ctx_create(uasd_ctx **out, int natom, int mensemble, int precision) {
 
 int uasd_ctx_destroy(uasd_ctx *c) {
   if (!c) return UASD_OK;
-  if (c->stream) hipStreamSynchronize(c->stream);
-  resolve_timing(c);
+  // the context is released whatever happens; a failure of the work still
+  // queued on its stream is reported, the release calls' own are not
+  const hipError_t serr = c->stream ? hipStreamSynchronize(c->stream) : hipSuccess;
+  for (const auto &pe : c->pending) {  // timings not yet read are dropped
+    (void)hipEventDestroy(pe.a);
+    (void)hipEventDestroy(pe.b);
+  }
   destroy_graphs(c);
-  if (c->cur) hipEventDestroy(c->cur);
-  for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
-  if (c->own_stream) hipStreamDestroy(c->own_stream);
+  if (c->cur) (void)hipEventDestroy(c->cur);
+  if (c->region_cur) (void)hipEventDestroy(c->region_cur);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
+  if (serr != hipSuccess)
+    return fail(UASD_ERR_HIP, std::string("uasd_ctx_destroy: queued work failed: ") +
+                                  hipGetErrorString(serr));
   return UASD_OK;
 }
 
@@ -1487,8 +1505,8 @@ namespace {
 
 void destroy_graphs(uasd_ctx *c) {
   for (auto &kv : c->graphs) {
-    if (kv.second.exec) hipGraphExecDestroy(kv.second.exec);
-    if (kv.second.g) hipGraphDestroy(kv.second.g);
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    if (kv.second.g) (void)hipGraphDestroy(kv.second.g);
   }
   c->graphs.clear();
 }
@@ -1503,7 +1521,7 @@ int capture_graph(uasd_ctx *c, F &&body, uasd_ctx::Graph &out) {
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (r) {
-    if (g) hipGraphDestroy(g);
+    if (g) (void)hipGraphDestroy(g);
     return r;
   }
   if (e != hipSuccess)
@@ -1511,7 +1529,7 @@ int capture_graph(uasd_ctx *c, F &&body, uasd_ctx::Graph &out) {
   hipGraphExec_t x = nullptr;
   const hipError_t e2 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
   if (e2 != hipSuccess) {
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     return fail(UASD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
   }
   out.g = g;
@@ -1843,8 +1861,8 @@ struct StateGuard {
 struct EventPair {
   hipEvent_t a = nullptr, b = nullptr;
   ~EventPair() {
-    if (a) hipEventDestroy(a);
-    if (b) hipEventDestroy(b);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
   }
 };
 

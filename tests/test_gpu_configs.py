@@ -4,7 +4,9 @@ FP64 oracle on the same inputs).
   C1  bcc 20^3 x 2 (16,000 spins), J1 1.34 / J2 0.77 mRy, Depondt alpha 0.1,
       dt 1e-16, 300 K on an injected N(0,1) stream: 40 steps, <= 1e-12
   C2  bcc 64^3 x 2 (524,288 spins), five shells (58 neighbours), midpoint:
-      T = 0, 6 steps, and 300 K on an injected stream, 3 steps, <= 1e-12
+      T = 0, 6 steps, and 300 K on an injected stream, 3 steps, <= 1e-12;
+      and as benchmarked (tile kernel, basis-major order, 300 K on the FP64
+      Philox stream), 3 steps, <= 1e-12
   C3  triangular 512^2 (262,144 spins), J + DMI + Kz + Bz, Depondt alpha 0.05:
       T = 0, 8 steps, <= 1e-12
   C4  random binary bcc alloy replica (16^3 cells, species-pair couplings and
@@ -85,6 +87,37 @@ def test_c2_full_size_midpoint(O):
     assert sysd["lattice"].natom == 524288 and int(sysd["nlistsize"].max()) == 58
     assert _run_pair(O, sysd, 1, 0.1, 0.0, 6, np.random.default_rng(71), False) <= 1e-12
     assert _run_pair(O, sysd, 1, 0.1, 300.0, 3, np.random.default_rng(72), True) <= 1e-12
+
+
+def test_c2_as_benchmarked_tile_kernel_philox(O):
+    """The headline run as the benchmark makes it: bcc 64^3 x 2, basis-major
+    site order, lattice declared, the LDS-tiled kernel with (16,4,4)-cell
+    tiles and the resident loop, midpoint at 300 K on the FP64 Philox stream.
+    Three steps equal the FP64 oracle's on the same counter stream
+    (or_sd_run, noise 2) within 1e-12."""
+    import ctypes
+    sysd = CF.bcc_shells(64, CF.bcc5_couplings())
+    lat = sysd["lattice"]
+    N = lat.natom
+    e0 = np.ascontiguousarray(CF.random_unit(N, np.random.default_rng(77), 1))
+    seed, steps = 0xC2C2C2C2, 3
+    eng = U.Engine(N, 1, "fp64", site_order=lat.NA)
+    eng.set_lattice_dims(lat.NA, *lat.n)
+    eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
+    eng.set_moments(e0, sysd["mmom"][None])
+    eng.set_llg(0.1, 300.0, 1e-16)
+    eng.set_noise(U.UASD_NOISE_PHILOX, seed)
+    eng.set_tuning(eng.TUNE_LLG, 4, 256 | (4 << 18) | (1 << 22))   # tile (16,4,4), resident
+    eng.sd_steps(1, steps)
+    assert eng.llg_tile() == 4
+    e, _ = eng.get_moments()
+    ham = O.OracleHam(N, sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
+    mm = np.ascontiguousarray(np.broadcast_to(sysd["mmom"], (1, N)), dtype=np.float64)
+    eo = e0.copy()
+    eMo = np.ascontiguousarray(eo * mm[..., None])
+    O.lib().or_sd_run(ctypes.byref(ham.h), N, 1, 1, steps, 0, 1e-16, np.full(N, 0.1),
+                      np.full(N, 300.0), np.ones(N), mm, eo, eMo, seed, 2, None)
+    assert np.max(np.abs(e - eo)) <= 1e-12
 
 
 def test_c3_full_size_depondt(O):

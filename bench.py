@@ -557,6 +557,50 @@ def _c5_oracle_error(rep, e0, steps, graph, e_final_dev, e_avg_dev):
             "time_avg_energy_rel_err": abs(e_avg_dev - e_avg) / abs(e_avg)}
 
 
+def host_halves_variant(U, sysd, steps: int = 20, warmup: int = 3) -> dict:
+    """The headline workload through the host-array drop-in halves, in the
+    order a reference driver calls them every step (sd_driver.f90:517-836):
+    effective_field -> evolve_first -> effective_field -> evolve_second on
+    host arrays (each call uploads its inputs and downloads its outputs over
+    PCIe), then moment_update on the host.  FP64, midpoint, 300 K, the
+    in-kernel Philox stream.  A PCIe-inclusive rate beside the
+    device-resident headline, never the metric."""
+    N = sysd["lattice"].natom
+    eng = U.Engine(N, 1, "fp64")
+    eng.set_exchange(sysd["nlist"], sysd["nlistsize"], sysd["ncoup"])
+    rng = np.random.default_rng(1)
+    e = np.ascontiguousarray(CF_random_unit(N, rng))
+    mm = np.ascontiguousarray(np.broadcast_to(sysd["mmom"], (1, N)), dtype=np.float64)
+    eng.set_moments(e, mm)
+    eng.set_llg(0.1, 300.0, 1e-16)
+    eng.set_noise(U.UASD_NOISE_PHILOX, 0xC2C2C2C2)
+    eM = np.ascontiguousarray(e * mm[..., None])
+    e2 = e.copy()
+    b2 = np.zeros_like(e)            # unused by the midpoint solver
+
+    def step():
+        beff, _ = eng.effective_field(eM)
+        eng.evolve_first(1, beff, e, e2, eM)
+        beff, _ = eng.effective_field(eM)
+        eng.evolve_second(1, beff, b2, e, e2)
+        U.moment_update(mm, e, e2, eM)
+
+    for _ in range(warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    dt = time.perf_counter() - t0
+    drift = float(np.max(np.abs(np.linalg.norm(e, axis=2) - 1.0)))
+    del eng
+    return {"workload": f"bcc 64^3 x 2 ({N} spins), 58 neighbours, midpoint, noise philox, "
+                        f"T 300 K, host arrays through the drop-in halves",
+            "value": N * steps / dt, "ms_per_step": dt / steps * 1e3, "steps": steps,
+            "max_norm_drift": drift,
+            "note": "PCIe-inclusive: every call moves its arrays between host and device; "
+                    "moment_update runs on the host; not the metric"}
+
+
 def c2_variant(U, args, label: str, sysd=None, L: int = 64, temp: float = 300.0,
                noise: str = "philox", stencil: bool = True, steps: int = 400,
                warmup: int = 40, M: int = 1):
@@ -668,10 +712,11 @@ def main():
     ap.add_argument("--noise", choices=["philox", "injected", "none"], default="philox",
                     help="thermal noise of the main run: in-kernel Philox at 300 K, an injected "
                          "N(0,1) array at 300 K, or none (T = 0)")
-    ap.add_argument("--variants", default="T0,injected,L128,M8",
+    ap.add_argument("--variants", default="T0,injected,L128,M8,host",
                     help="comma list of runs of the plain workload beside the headline: T0 "
                          "(no noise), injected (caller's N(0,1) stream), L128 (bcc 128^3 x 2, "
-                         "above the MALL), M8 (eight ensembles of bcc 32^3 x 2), or 'none'")
+                         "above the MALL), M8 (eight ensembles of bcc 32^3 x 2), host (host "
+                         "arrays through the drop-in halves, PCIe-inclusive), or 'none'")
     ap.add_argument("--c4-sweeps", type=int, default=100,
                     help="timed sweeps of each batched MC run of the c4 form")
     ap.add_argument("--c4-stat-sweeps", type=int, default=600,
@@ -830,6 +875,8 @@ def main():
                              warmup=20)
         elif v == "M8":      # the same spin count as eight ensembles of bcc 32^3 x 2
             res = c2_variant(U, args, "fp64_midpoint_bcc32_58nn_M8_philox", L=32, M=8)
+        elif v == "host":    # host arrays through the drop-in halves (PCIe-inclusive)
+            res = host_halves_variant(U, sysd)
         else:
             raise SystemExit(f"unknown variant {v}")
         res["wall_s"] = round(time.perf_counter() - t0, 1)

@@ -1386,12 +1386,37 @@ int uasd_get_moments(uasd_ctx *c, double *emom, double *emomM) {
 // ---------------------------------------------------------- drop-in (host)
 // Host vec3 arrays are uploaded as quads; spin quads keep the resident mmom
 // in w, the others carry a constant.
+// dst[q] = (src[q].xyz, wsrc[q].w): uploaded directions merged with the
+// moment magnitudes already on the device
+extern "C++" {
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_merge_xyz(const uasd::Quad<R> *__restrict__ src, const uasd::Quad<R> *wsrc,
+                uasd::Quad<R> *dst, size_t n) {  // wsrc and dst may be the same array
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const uasd::Quad<R> v = src[q];
+  dst[q] = uasd::Quad<R>{v.x, v.y, v.z, wsrc[q].w};
+}
+}
+
+// quad array k of the spins <- the (3,N,M) array a, with w = mmom of the
+// state (quad array 0): a goes to the scratch quads, the merge runs on the
+// device, so the magnitudes are not read back to the host
 static int upload_vec_keep_w(uasd_ctx *c, int k, const double *a) {
-  std::vector<double> cur;
-  if (int r = download_quads(c, c->spins, 0, cur)) return r;  // w = mmom of the state
-  std::vector<double> hq = to_quads(c, a, nullptr, 0.0);
-  for (size_t q = 0; q < (size_t)c->N * c->M; ++q) hq[4 * q + 3] = cur[4 * q + 3];
-  return upload_quads(c, c->spins, k, hq);
+  if (int r = upload_quads(c, c->scratch, 0, to_quads(c, a, nullptr, 0.0))) return r;
+  const size_t n = (size_t)c->N * c->M;
+  const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+  if (c->prec == UASD_FP64)
+    hipLaunchKernelGGL(k_merge_xyz<double>, grid, block, 0, c->stream,
+                       quads<double>(c, c->scratch), quads<double>(c, c->spins, 0),
+                       quads<double>(c, c->spins, k), n);
+  else
+    hipLaunchKernelGGL(k_merge_xyz<float>, grid, block, 0, c->stream,
+                       quads<float>(c, c->scratch), quads<float>(c, c->spins, 0),
+                       quads<float>(c, c->spins, k), n);
+  HIP_TRY(hipGetLastError());
+  return UASD_OK;
 }
 
 int uasd_effective_field_host(uasd_ctx *c, const double *emomM, double *beff, double *beff1,
